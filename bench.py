@@ -1,0 +1,206 @@
+"""Benchmark: batched forward HH cable/tree population on one MI355X.
+
+Workload (one "step" = one full forward simulation of the population):
+8192 unbranched HH cables with compartment counts uniform in [21, 801] plus
+1024 random binary trees of 200-4000 compartments, float32 state, 4000 time
+steps of 0.005 ms with an extracellular point-source pulse (see
+dendra_amd/workloads.py).  Inputs are resident in HBM before the timed region.
+
+Printed JSON line: metric compartment-steps/s of the persistent path (cable
+and tree kernels on two streams), with
+  * roofline: the dominant kernel (the persistent cable kernel) -- algorithmic
+    bytes 48 B per compartment-step (read+write V,m,h,n and read four
+    coefficient streams, float32) times its compartment-steps per launch,
+    over its average launch time measured with HIP events on its stream;
+  * plain: the per-step GPU path (2 launches per cable step, 2 per tree step)
+    timed on the same workload;
+  * cpu_baseline: the binary64 C oracle (OpenMP) on a bounded sample of the
+    same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CS = 48.0
+
+
+def _parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cables", type=int, default=8192)
+    ap.add_argument("--trees", type=int, default=1024)
+    ap.add_argument("--time-steps", type=int, default=4000)
+    ap.add_argument("--plain", type=int, default=1, help="plain-path runs to time (0 = skip)")
+    ap.add_argument("--cpu-sample", type=float, default=0.125,
+                    help="fraction of cells in the CPU oracle sample (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=800)
+    ap.add_argument("--waves-per-block", type=int, default=4)
+    ap.add_argument("--only", choices=["all", "cables", "trees"], default="all")
+    return ap.parse_args()
+
+
+def main():
+    args = _parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(cells are independent: see DESIGN.md for how the population shards)",
+              file=sys.stderr)
+        return 2
+    import torch
+
+    from dendra_amd import _lib
+    from dendra_amd.model import HH
+    from dendra_amd.population import CablePopulation, TreePopulation
+    from dendra_amd.workloads import uneven_population
+
+    _lib.load()
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    work = uneven_population(args.cables, args.trees, args.time_steps)
+    print(f"# workload built in {time.time() - t0:.1f}s: {work.cables.total} cable + "
+          f"{work.trees.total} tree compartments, {work.n_steps} steps", file=sys.stderr)
+    hh = HH(celsius=6.3)
+    use_c = args.only in ("all", "cables")
+    use_t = args.only in ("all", "trees")
+    cab = CablePopulation(work.cables, work.dt, hh, torch.float32, dev) if use_c else None
+    tre = TreePopulation(work.trees, work.dt, hh, torch.float32, dev) if use_t else None
+    wave = work.wave
+    main_stream = torch.cuda.current_stream(dev)
+    side = torch.cuda.Stream(dev)
+    ev = {k: [] for k in ("c0", "c1", "t0", "t1")}
+
+    def one_run(record=False):
+        if cab is not None:
+            cab.reset()
+        if tre is not None:
+            tre.reset()
+        side.wait_stream(main_stream)
+        if tre is not None:
+            with torch.cuda.stream(side):
+                if record:
+                    e = torch.cuda.Event(enable_timing=True); e.record(side); ev["t0"].append(e)
+                tre.run(work.n_steps, wave=wave, stream=side.cuda_stream)
+                if record:
+                    e = torch.cuda.Event(enable_timing=True); e.record(side); ev["t1"].append(e)
+        if cab is not None:
+            if record:
+                e = torch.cuda.Event(enable_timing=True); e.record(main_stream); ev["c0"].append(e)
+            cab.run(work.n_steps, wave=wave, waves_per_block=args.waves_per_block)
+            if record:
+                e = torch.cuda.Event(enable_timing=True); e.record(main_stream); ev["c1"].append(e)
+        main_stream.wait_stream(side)
+
+    for _ in range(args.warmup):
+        one_run()
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        one_run(record=True)
+        print(f"# run {i + 1}/{args.steps} enqueued", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t_start) / max(args.steps, 1)
+    comps = (work.cables.total if use_c else 0) + (work.trees.total if use_t else 0)
+    cs_per_run = comps * work.n_steps
+    value = cs_per_run / wall
+    c_ms = [a.elapsed_time(b) for a, b in zip(ev["c0"], ev["c1"])]
+    t_ms = [a.elapsed_time(b) for a, b in zip(ev["t0"], ev["t1"])]
+    out = {
+        "metric": "compartment-steps/s (batched forward, uneven HH population)",
+        "value": value,
+        "unit": "compartment-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded counter RNG)",
+        "config": {
+            "workload": "uneven HH population: 8192 cables K~U[21,801] + 1024 binary "
+                        "trees K~U[200,4000], 4000 steps dt 0.005 ms, point-source Ve",
+            "cables": args.cables if use_c else 0, "trees": args.trees if use_t else 0,
+            "compartments": comps, "time_steps": work.n_steps,
+            "compartment_steps_per_run": cs_per_run, "path": "persistent",
+        },
+    }
+    if c_ms:
+        c_avg = float(np.mean(c_ms)) * 1e-3
+        c_bytes = BYTES_PER_CS * work.cables.total * work.n_steps
+        ach = c_bytes / c_avg / 1e9
+        out["roofline"] = {
+            "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "hh_cable_run_kernel<float>",
+            "kernel_ms": c_avg * 1e3,
+            "algorithmic_bytes_per_launch": c_bytes,
+        }
+        out["cable_cs_per_s"] = work.cables.total * work.n_steps / c_avg
+    if t_ms:
+        t_avg = float(np.mean(t_ms)) * 1e-3
+        out["tree_kernel_ms"] = t_avg * 1e3
+        out["tree_cs_per_s"] = work.trees.total * work.n_steps / t_avg
+    if args.plain > 0:
+        torch.cuda.synchronize()
+        n_pl = work.n_steps
+        if cab is not None:
+            cab.reset(); cab.run(20, wave=wave, path="plain"); cab.reset()
+        if tre is not None:
+            tre.reset(); tre.run(20, wave=wave, path="plain"); tre.reset()
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        for _ in range(args.plain):
+            if cab is not None:
+                cab.reset(); cab.run(n_pl, wave=wave, path="plain")
+            if tre is not None:
+                tre.reset(); tre.run(n_pl, wave=wave, path="plain")
+        torch.cuda.synchronize()
+        plain_wall = (time.perf_counter() - tp) / args.plain
+        out["plain"] = {"value": cs_per_run / plain_wall, "ms_per_run": plain_wall * 1e3,
+                        "speedup_persistent_over_plain": plain_wall / wall}
+    if args.cpu_sample > 0:
+        out["cpu_baseline"] = cpu_baseline(work, hh, args.cpu_sample, args.cpu_steps)
+    print(json.dumps(out))
+    return 0
+
+
+def cpu_baseline(work, hh, frac, n_steps):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+
+    threads = oracle.cpu_threads()
+    stride = max(1, int(round(1.0 / frac)))
+    cs = work.cables.subset(np.arange(0, work.cables.n_cells, stride))
+    ts = work.trees.subset(np.arange(0, work.trees.n_trees, stride))
+    wave = work.wave[:n_steps].astype(np.float64)
+    t = time.perf_counter()
+    oracle.cable_run(cs, hh.array(), work.dt, n_steps, ve_wave=wave, threads=threads)
+    oracle.tree_run(ts, hh.array(), work.dt, n_steps, ve_wave=wave, threads=threads)
+    dt = time.perf_counter() - t
+    n_cs = (cs.total + ts.total) * n_steps
+    return {
+        "value": n_cs / dt, "unit": "compartment-steps/s", "cores": threads,
+        "kind": "port",
+        "sample": f"every {stride}th cable and tree ({cs.n_cells} cables, {ts.n_trees} "
+                  f"trees, {cs.total + ts.total} compartments) x {n_steps} steps, float64 "
+                  f"C oracle, OpenMP {threads} threads, {dt:.1f}s",
+    }
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,131 @@
+"""ctypes binding of the gfx950 C-ABI library ``libdnhip.so``.
+
+The library is the only compute path of this package: if it is missing or
+cannot be loaded, every entry point raises instead of falling back to a
+CPU or PyTorch implementation.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "libdnhip.so"
+
+_lock = threading.Lock()
+_lib = None
+
+_ERRORS = {
+    -1: "invalid argument",
+    -2: "kernel launch failed",
+    -3: "size out of range for this kernel",
+    -4: "unsupported configuration",
+}
+
+p = C.c_void_p
+i32 = C.c_int32
+i64 = C.c_int64
+f32 = C.c_float
+f64 = C.c_double
+
+_CABLE_RUN = [i32, p, p, p, p, p, p, p, p, p, p, p, p, p, p, p, p, i64, i64]
+_TREE_PLAN = [i32, p, p, p, p, p, p, p, p, p]
+
+_SIGNATURES = {
+    "dn_tridiag_scratch_elems": (i64, [i64, i64, i32, i32]),
+    "dn_tridiag_solve_f32": (i32, [p, p, p, p, p, i64, i64, i32, p, i64, p]),
+    "dn_tridiag_solve_f64": (i32, [p, p, p, p, p, i64, i64, i32, p, i64, p]),
+    "dn_tree_dhs_scratch_elems": (i64, [i64, i64, i32]),
+    "dn_tree_dhs_f32": (i32, [p, p, p, p, p, p, p, i64, i64, i64, p, i64, p]),
+    "dn_tree_dhs_f64": (i32, [p, p, p, p, p, p, p, i64, i64, i64, p, i64, p]),
+    "dn_hh_cable_rmax": (i32, []),
+    "dn_hh_cable_run_f32": (i32, _CABLE_RUN + [f32, p, p, i64, i64, i32, p]),
+    "dn_hh_cable_run_f64": (i32, _CABLE_RUN + [f64, p, p, i64, i64, i32, p]),
+    "dn_hh_cable_step_f32": (
+        i32,
+        [i64, p, i64, i32, p, p, p, p, p, p, p, p, p, p, f32, f32, p, p, p, p],
+    ),
+    "dn_hh_cable_step_f64": (
+        i32,
+        [i64, p, i64, i32, p, p, p, p, p, p, p, p, p, p, f64, f64, p, p, p, p],
+    ),
+    "dn_hh_tree_slots": (i32, []),
+    "dn_hh_tree_run_f32": (
+        i32,
+        _TREE_PLAN
+        + [p, i32, i32, i32, p, p, p, p, p, p, p, p, p, p, p, i64, i64, f32, p, p]
+        + [i64, i64, p],
+    ),
+    "dn_hh_tree_run_f64": (
+        i32,
+        _TREE_PLAN
+        + [p, i32, i32, i32, p, p, p, p, p, p, p, p, p, p, p, i64, i64, f64, p, p]
+        + [i64, i64, p],
+    ),
+    "dn_hh_tree_step_f32": (
+        i32,
+        _TREE_PLAN + [p, i32, i64, p, p, p, p, p, p, p, p, p, p, f32, f32, p, p, p, p],
+    ),
+    "dn_hh_tree_step_f64": (
+        i32,
+        _TREE_PLAN + [p, i32, i64, p, p, p, p, p, p, p, p, p, p, f64, f64, p, p, p, p],
+    ),
+}
+
+
+def _bind(lib) -> None:
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+
+
+def exported_symbols() -> list[str]:
+    """Names this binding knows how to call (every declared C-ABI symbol)."""
+    return sorted(_SIGNATURES)
+
+
+def load():
+    """Load ``libdnhip.so`` once; raise ``RuntimeError`` if it is unavailable."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(
+                    f"{LIB_PATH.name} is not built; run "
+                    "`python -c 'import __graft_entry__ as g; g.build()'` first"
+                )
+            lib = C.CDLL(str(LIB_PATH))
+            _bind(lib)
+            _lib = lib
+    return _lib
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        msg = _ERRORS.get(int(status), f"error {status}")
+        raise RuntimeError(f"{what}: {msg}")
+
+
+def ptr(t) -> int | None:
+    """Raw device (or host) address of a tensor, None for None."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_handle(device=None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def require_gpu(*tensors) -> None:
+    for t in tensors:
+        if t is not None and t.device.type != "cuda":
+            raise ValueError("dendra_amd kernels require tensors on a ROCm GPU device")

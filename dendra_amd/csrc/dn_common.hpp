@@ -1,0 +1,197 @@
+// dn_common.hpp -- shared device code for the gfx950 cable hot path.
+//
+// Hodgkin-Huxley rates, cnexp gate update and the exprelr ("vtrap") series,
+// written for 64-lane CDNA4 wavefronts.  Float32 follows the reference's
+// float32 branches (series degrees and switch points), float64 its float64
+// branches:
+//   rates               dendra/models/mod/HH.py:24-56
+//   affine current      dendra/models/mod/HH.py:109-116
+//   cnexp step          dendra/models/mechanisms/_cnexp.py:46-132
+//   exprelr / vtrap     dendra/models/mechanisms/ops.py:13-128
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define DN_OK 0
+#define DN_EINVAL -1
+#define DN_ELAUNCH -2
+#define DN_ERANGE -3
+#define DN_EUNSUPPORTED -4
+
+#define DN_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace dn {
+
+// ---------------------------------------------------------------------------
+// scalar helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float fexp(float x) { return __expf(x); }
+__device__ __forceinline__ double fexp(double x) { return exp(x); }
+// 1-ulp hardware reciprocal in float32; exact division in float64.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float fabs_(float x) { return fabsf(x); }
+__device__ __forceinline__ double fabs_(double x) { return fabs(x); }
+
+template <typename T> struct HHParams {
+  T gnabar, gkbar, gl, ena, ek, el, q10;
+};
+
+// x / (exp(x/10) - 1) with the local even Bernoulli series near x = 0.
+__device__ __forceinline__ float vtrap10(float x) {
+  float q = x * 0.1f;
+  float q2 = q * q;
+  float e = fmaf(q2, -691.0f / 1307674368000.0f, 1.0f / 47900160.0f);
+  e = fmaf(e, q2, -1.0f / 1209600.0f);
+  e = fmaf(e, q2, 1.0f / 30240.0f);
+  e = fmaf(e, q2, -1.0f / 720.0f);
+  e = fmaf(e, q2, 1.0f / 12.0f);
+  float series = (10.0f - x * 0.5f) + x * (q * e);
+  float direct = x * frcp(fexp(q) - 1.0f);
+  return fabsf(q) <= 1.0f ? series : direct;
+}
+__device__ __forceinline__ double vtrap10(double x) {
+  double q = x / 10.0;
+  double q2 = q * q;
+  double e = q2 * (43867.0 / 5109094217170944000.0) - 3617.0 / 10670622842880000.0;
+  e = e * q2 + 1.0 / 74724249600.0;
+  e = e * q2 - 691.0 / 1307674368000.0;
+  e = e * q2 + 1.0 / 47900160.0;
+  e = e * q2 - 1.0 / 1209600.0;
+  e = e * q2 + 1.0 / 30240.0;
+  e = e * q2 - 1.0 / 720.0;
+  e = e * q2 + 1.0 / 12.0;
+  double series = (10.0 - x / 2.0) + x * (q * e);
+  double direct = x / (exp(q) - 1.0);
+  return fabs(q) <= 0.5 ? series : direct;
+}
+
+// phi1(z) = expm1(z)/z on |z| <= 0.5 (Taylor, degree 9 / 16).
+__device__ __forceinline__ float phi1(float z) {
+  float v = fmaf(z, 1.0f / 3628800.0f, 1.0f / 362880.0f);
+  v = fmaf(v, z, 1.0f / 40320.0f);
+  v = fmaf(v, z, 1.0f / 5040.0f);
+  v = fmaf(v, z, 1.0f / 720.0f);
+  v = fmaf(v, z, 1.0f / 120.0f);
+  v = fmaf(v, z, 1.0f / 24.0f);
+  v = fmaf(v, z, 1.0f / 6.0f);
+  v = fmaf(v, z, 0.5f);
+  v = fmaf(v, z, 1.0f);
+  return v;
+}
+__device__ __forceinline__ double phi1(double z) {
+  double v = z * (1.0 / 355687428096000.0) + 1.0 / 20922789888000.0;
+  v = v * z + 1.0 / 1307674368000.0;
+  v = v * z + 1.0 / 87178291200.0;
+  v = v * z + 1.0 / 6227020800.0;
+  v = v * z + 1.0 / 479001600.0;
+  v = v * z + 1.0 / 39916800.0;
+  v = v * z + 1.0 / 3628800.0;
+  v = v * z + 1.0 / 362880.0;
+  v = v * z + 1.0 / 40320.0;
+  v = v * z + 1.0 / 5040.0;
+  v = v * z + 1.0 / 720.0;
+  v = v * z + 1.0 / 120.0;
+  v = v * z + 1.0 / 24.0;
+  v = v * z + 1.0 / 6.0;
+  v = v * z + 1.0 / 2.0;
+  v = v * z + 1.0;
+  return v;
+}
+
+// Exact affine step of x' = xa - xb x over dt (ms).
+template <typename T>
+__device__ __forceinline__ T cnexp(T x, T xa, T xb, T dt) {
+  T z = -xb * dt;
+  T inc = x + dt * (xa - xb * x) * phi1(z);
+  T eq = xa * frcp(xb);
+  T rel = eq + (x - eq) * fexp(z);
+  return fabs_(z) <= T(0.5) ? inc : rel;
+}
+
+// Gate advance at v_old, then the affine current terms from the new gates.
+// sna/sk scale gNa/gK per compartment (1 when the model has no scales).
+template <typename T>
+__device__ __forceinline__ void hh_advance(T v, T dt, const HHParams<T>& p, T sna,
+                                           T sk, T& m, T& h, T& n, T& G, T& J) {
+  T am = T(0.1) * vtrap10(-(v + T(40)));
+  T bm = T(4) * fexp((v + T(65)) * T(-1.0 / 18.0));
+  T ah = T(0.07) * fexp((v + T(65)) * T(-1.0 / 20.0));
+  T bh = frcp(fexp((v + T(35)) * T(-0.1)) + T(1));
+  T an = T(0.01) * vtrap10(-(v + T(55)));
+  T bn = T(0.125) * fexp((v + T(65)) * T(-1.0 / 80.0));
+  m = cnexp(m, p.q10 * am, p.q10 * (am + bm), dt);
+  h = cnexp(h, p.q10 * ah, p.q10 * (ah + bh), dt);
+  n = cnexp(n, p.q10 * an, p.q10 * (an + bn), dt);
+  T gna = p.gnabar * sna * (m * m * m) * h;
+  T n2 = n * n;
+  T gk = p.gkbar * sk * (n2 * n2);
+  G = p.gl + gna + gk;
+  J = p.gl * p.el + gna * p.ena + gk * p.ek;
+}
+
+// ---------------------------------------------------------------------------
+// cross-lane moves inside a segment of W lanes (W a power of two <= 64).
+// Out-of-segment sources return the caller's own value, which the
+// eliminations below rely on (the corresponding couplings are exactly 0).
+// ---------------------------------------------------------------------------
+template <int W>
+__device__ __forceinline__ float shfl_up_seg(float x, int s) {
+  int lane = __lane_id();
+  int seg_lane = lane & (W - 1);
+  int src = seg_lane >= s ? lane - s : lane;
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(x)));
+}
+template <int W>
+__device__ __forceinline__ float shfl_down_seg(float x, int s) {
+  int lane = __lane_id();
+  int seg_lane = lane & (W - 1);
+  int src = seg_lane + s < W ? lane + s : lane;
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(x)));
+}
+template <int W>
+__device__ __forceinline__ double shfl_up_seg(double x, int s) {
+  int lane = __lane_id();
+  int seg_lane = lane & (W - 1);
+  int src = (seg_lane >= s ? lane - s : lane) << 2;
+  int2 u = *reinterpret_cast<int2*>(&x);
+  u.x = __builtin_amdgcn_ds_bpermute(src, u.x);
+  u.y = __builtin_amdgcn_ds_bpermute(src, u.y);
+  return *reinterpret_cast<double*>(&u);
+}
+template <int W>
+__device__ __forceinline__ double shfl_down_seg(double x, int s) {
+  int lane = __lane_id();
+  int seg_lane = lane & (W - 1);
+  int src = (seg_lane + s < W ? lane + s : lane) << 2;
+  int2 u = *reinterpret_cast<int2*>(&x);
+  u.x = __builtin_amdgcn_ds_bpermute(src, u.x);
+  u.y = __builtin_amdgcn_ds_bpermute(src, u.y);
+  return *reinterpret_cast<double*>(&u);
+}
+
+// ---------------------------------------------------------------------------
+// Parallel cyclic reduction of one tridiagonal row per lane across a
+// W-lane segment:  A x_{l-1} + B x_l + C x_{l+1} = D.  Rows at the segment
+// ends must carry A = 0 (first lane) and C = 0 (last lane).  Returns x_l.
+// ---------------------------------------------------------------------------
+template <typename T, int W>
+__device__ __forceinline__ T pcr_lanes(T A, T B, T C, T D) {
+#pragma unroll
+  for (int s = 1; s < W; s <<= 1) {
+    T Al = shfl_up_seg<W>(A, s), Bl = shfl_up_seg<W>(B, s);
+    T Cl = shfl_up_seg<W>(C, s), Dl = shfl_up_seg<W>(D, s);
+    T Ar = shfl_down_seg<W>(A, s), Br = shfl_down_seg<W>(B, s);
+    T Cr = shfl_down_seg<W>(C, s), Dr = shfl_down_seg<W>(D, s);
+    T al = -A * frcp(Bl);
+    T be = -C * frcp(Br);
+    T nA = al * Al;
+    T nB = B + al * Cl + be * Ar;
+    T nC = be * Cr;
+    T nD = D + al * Dl + be * Dr;
+    A = nA; B = nB; C = nC; D = nD;
+  }
+  return D * frcp(B);
+}
+
+}  // namespace dn

@@ -1,0 +1,140 @@
+// dn_tridiag.hip -- batched tridiagonal solves for gfx950.
+//
+// Drop-in for the reference's thomas_solve_cuda_t / pcr_solve_cuda_t
+// (dendra/models/integrators/triton/t_kernel_thomas.py:83-130,218-232 and
+// pcr_kernel_thomas.py:99-195,224-226): a (B,K-1) sub-diagonal, b (B,K)
+// diagonal, c (B,K-1) super-diagonal, d (B,K) right-hand side, x (B,K)
+// output.  Inputs are never modified.  The transposed (adjoint) solve of
+// ThomasSolve.backward (t_kernel_thomas.py:147-189) is the same call with
+// a and c exchanged.
+//
+// Two kernels:
+//  * thomas_lane: one lane per system, sequential Thomas sweep.  Best for
+//    many short systems.  Its elimination workspace lives in x (dp) and in a
+//    caller scratch of B*K values (cp).
+//  * wave_hybrid: one 64-lane wavefront per system.  Each lane eliminates a
+//    contiguous chunk of nr rows (local Thomas with a spike column), the 64
+//    two-row interfaces are reduced to one row per lane by a cyclic-reduction
+//    step and solved by wavefront PCR through ds_bpermute, then every lane
+//    back-substitutes its chunk.  The system is staged in LDS (coalesced loads)
+//    when 4*64*nr values fit, else it runs in the caller's global scratch.
+#include "dn_wave.hpp"
+
+namespace dn {
+
+template <typename T>
+__global__ __launch_bounds__(64) void tridiag_wave_kernel(
+    const T* __restrict__ a, const T* __restrict__ b, const T* __restrict__ c,
+    const T* __restrict__ d, T* __restrict__ x, int64_t K, int nr, int use_lds,
+    T* __restrict__ scratch) {
+  extern __shared__ unsigned char dn_smem_raw[];
+  const int64_t r = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t KP = 64 * (int64_t)nr;
+  T* base = use_lds ? reinterpret_cast<T*>(dn_smem_raw) : scratch + r * 4 * KP;
+  T *sa = base, *sb = base + KP, *sc = base + 2 * KP, *sd = base + 3 * KP;
+  const T* ar = a + r * (K - 1);
+  const T* br = b + r * K;
+  const T* cr = c + r * (K - 1);
+  const T* dr = d + r * K;
+  for (int64_t j = lane; j < KP; j += 64) {
+    bool in = j < K;
+    sa[j] = (in && j >= 1) ? ar[j - 1] : T(0);
+    sb[j] = in ? br[j] : T(1);
+    sc[j] = (j + 1 < K) ? cr[j] : T(0);
+    sd[j] = in ? dr[j] : T(0);
+  }
+  __syncthreads();
+  wave_hybrid_solve<T>(sa, sb, sc, sd, nr);
+  __syncthreads();
+  T* xr = x + r * K;
+  for (int64_t j = lane; j < K; j += 64) xr[j] = sd[j];
+}
+
+// Lane-per-system Thomas: restates the sweep of tridiag/pcr.py:125-183.
+template <typename T>
+__global__ __launch_bounds__(256) void tridiag_thomas_lane_kernel(
+    const T* __restrict__ a, const T* __restrict__ b, const T* __restrict__ c,
+    const T* __restrict__ d, T* __restrict__ x, int64_t B, int64_t K,
+    T* __restrict__ cp) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= B) return;
+  const T* ar = a + r * (K - 1);
+  const T* br = b + r * K;
+  const T* cr = c + r * (K - 1);
+  const T* dr = d + r * K;
+  T* xr = x + r * K;
+  T* cpr = cp + r * K;
+  if (K == 1) { xr[0] = dr[0] / br[0]; return; }
+  T den = br[0];
+  T cprev = cr[0] / den;
+  T dprev = dr[0] / den;
+  cpr[0] = cprev;
+  xr[0] = dprev;
+  for (int64_t i = 1; i < K; ++i) {
+    T ai = ar[i - 1];
+    T dn_ = br[i] - ai * cprev;
+    if (i < K - 1) cprev = cr[i] / dn_;
+    dprev = (dr[i] - ai * dprev) / dn_;
+    cpr[i] = cprev;
+    xr[i] = dprev;
+  }
+  T xn = xr[K - 1];
+  for (int64_t i = K - 2; i >= 0; --i) {
+    xn = xr[i] - cpr[i] * xn;
+    xr[i] = xn;
+  }
+}
+
+template <typename T>
+static int tridiag_launch(const T* a, const T* b, const T* c, const T* d, T* x,
+                          int64_t B, int64_t K, int method, T* scratch,
+                          int64_t scratch_elems, hipStream_t stream) {
+  if (!b || !d || !x || B < 1 || K < 1) return DN_EINVAL;
+  if (K > 1 && (!a || !c)) return DN_EINVAL;
+  if (method == 0) method = (K <= 32 && B >= 4096) ? 1 : 2;
+  if (method == 1) {
+    if (!scratch || scratch_elems < B * K) return DN_ERANGE;
+    int blocks = (int)((B + 255) / 256);
+    hipLaunchKernelGGL(tridiag_thomas_lane_kernel<T>, dim3(blocks), dim3(256), 0,
+                       stream, a, b, c, d, x, B, K, scratch);
+  } else {
+    int nr = (int)((K + 63) / 64);
+    int64_t KP = 64 * (int64_t)nr;
+    size_t lds = (size_t)4 * KP * sizeof(T);
+    int use_lds = lds <= 64 * 1024;
+    if (!use_lds && (!scratch || scratch_elems < B * 4 * KP)) return DN_ERANGE;
+    if (B > 0x7fffffff) return DN_ERANGE;
+    hipLaunchKernelGGL(tridiag_wave_kernel<T>, dim3((unsigned)B), dim3(64),
+                       use_lds ? lds : 0, stream, a, b, c, d, x, K, nr, use_lds,
+                       scratch);
+  }
+  return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+}
+
+}  // namespace dn
+
+// Scratch elements the caller must provide (0 when none is needed).
+DN_EXPORT int64_t dn_tridiag_scratch_elems(int64_t B, int64_t K, int method,
+                                           int elem_bytes) {
+  if (method == 0) method = (K <= 32 && B >= 4096) ? 1 : 2;
+  if (method == 1) return B * K;
+  int64_t KP = 64 * ((K + 63) / 64);
+  return (size_t)4 * KP * elem_bytes <= 64 * 1024 ? 0 : B * 4 * KP;
+}
+
+DN_EXPORT int dn_tridiag_solve_f32(const float* a, const float* b, const float* c,
+                                   const float* d, float* x, int64_t B, int64_t K,
+                                   int method, float* scratch, int64_t scratch_elems,
+                                   hipStream_t stream) {
+  return dn::tridiag_launch<float>(a, b, c, d, x, B, K, method, scratch,
+                                   scratch_elems, stream);
+}
+
+DN_EXPORT int dn_tridiag_solve_f64(const double* a, const double* b, const double* c,
+                                   const double* d, double* x, int64_t B, int64_t K,
+                                   int method, double* scratch, int64_t scratch_elems,
+                                   hipStream_t stream) {
+  return dn::tridiag_launch<double>(a, b, c, d, x, B, K, method, scratch,
+                                    scratch_elems, stream);
+}

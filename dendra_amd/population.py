@@ -1,0 +1,343 @@
+"""GPU populations: the host drivers of the fused cable and tree kernels.
+
+``CablePopulation.run`` and ``TreePopulation.run`` are the counterparts of
+the reference's ``Population.run`` hot loop (dendra/models/core.py:4555-4614)
+over the ``_bwd_euler_ub`` / ``_dhs`` integrators: state lives on the GPU in
+plan order (structure-of-arrays), and a run is either
+
+* ``path="persistent"``: the multi-step kernels, each wavefront / workgroup
+  advancing its cells for ``steps_per_launch`` steps with state on chip, or
+* ``path="plain"``: one gating+assembly launch and one solve launch per
+  time step, state in HBM (the structure of the reference's GPU path).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .model import (
+    HH,
+    CableSpec,
+    TreeSpec,
+    cable_workspace,
+    hh_steady_state,
+    plan_cables,
+    tree_workspace,
+)
+from .topology import section_plan
+
+_DT = {torch.float32: np.float32, torch.float64: np.float64}
+
+
+def _dev(a, dtype, device):
+    if a is None:
+        return None
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)
+
+
+class CablePopulation:
+    """Ragged batch of unbranched HH cables (backward Euler, increment form)."""
+
+    def __init__(self, spec: CableSpec, dt: float, hh: HH | None = None,
+                 dtype=torch.float32, device="cuda", rmax: int | None = None,
+                 max_lanes: int = 64):
+        lib = _lib.load()
+        self.spec, self.dt, self.hh = spec, float(dt), hh or HH()
+        self.dtype, self.device = dtype, torch.device(device)
+        _lib.require_gpu(torch.empty(0, device=self.device))
+        self.rmax = rmax or lib.dn_hh_cable_rmax()
+        self.plan = plan_cables(spec.sizes, self.rmax, max_lanes)
+        sizes = spec.sizes
+        # flat permutation user -> plan order
+        self.perm = np.concatenate(
+            [np.arange(spec.off[c], spec.off[c + 1]) for c in self.plan.order]
+        ) if spec.n_cells else np.zeros(0, dtype=np.int64)
+        ws = cable_workspace(spec)
+        P = self.perm
+        dev, dt_ = self.device, dtype
+        self.gL = _dev(ws.gL[P], dt_, dev)
+        self.gRp = _dev(ws.gRp[P], dt_, dev)
+        self.scale = _dev(ws.scale[P], dt_, dev)
+        self.drive = _dev(None if ws.drive is None else ws.drive[P], dt_, dev)
+        self.sna = _dev(None if spec.sna is None else spec.sna[P], dt_, dev)
+        self.sk = _dev(None if spec.sk is None else spec.sk[P], dt_, dev)
+        self.off = torch.as_tensor(self.plan.off, dtype=torch.int64, device=dev)
+        i32 = lambda a: torch.as_tensor(a, dtype=torch.int32, device=dev)
+        self.task_cell0 = i32(self.plan.task_cell0)
+        self.task_ncell = i32(self.plan.task_ncell)
+        self.task_nr = i32(self.plan.task_nr)
+        self.task_p = i32(self.plan.task_p)
+        self.max_nr_plain = int(-(-int(sizes.max()) // 64)) if len(sizes) else 1
+        self.hh_arr = self.hh.array().astype(_DT[dtype])
+        self.reset()
+
+    @property
+    def total(self) -> int:
+        return self.spec.total
+
+    def reset(self, v_init: float | None = None):
+        v0 = self.spec.v_init if v_init is None else v_init
+        m0, h0, n0 = hh_steady_state(v0)
+        n = self.total
+        mk = lambda x: torch.full((n,), float(x), dtype=self.dtype, device=self.device)
+        self.v, self.m, self.h, self.n = mk(v0), mk(m0), mk(h0), mk(n0)
+        self.t_step = 0
+
+    def set_state(self, v, m, h, n):
+        P = torch.as_tensor(self.perm, device=self.device)
+        for name, val in (("v", v), ("m", m), ("h", h), ("n", n)):
+            t = torch.as_tensor(np.asarray(val), dtype=self.dtype, device=self.device)
+            getattr(self, name).copy_(t[P])
+
+    def state(self) -> dict:
+        """State in user (spec) order as float64 numpy arrays."""
+        out = {}
+        for name in ("v", "m", "h", "n"):
+            a = np.empty(self.total)
+            a[self.perm] = getattr(self, name).double().cpu().numpy()
+            out[name] = a
+        return out
+
+    def unpermute(self, rec: torch.Tensor) -> np.ndarray:
+        r = rec.double().cpu().numpy()
+        out = np.empty_like(r)
+        out[:, self.perm] = r
+        return out
+
+    def run(self, n_steps: int, wave=None, record_every: int = 0,
+            path: str = "persistent", waves_per_block: int = 4,
+            steps_per_launch: int | None = None, stream=None):
+        """Advance n_steps.  ``wave`` is the per-step drive waveform (length
+        >= t_step + n_steps) or None.  Returns the V record [n_rec, total] in
+        plan order when record_every > 0, else None."""
+        lib = _lib.load()
+        f64 = self.dtype == torch.float64
+        wave_t = None
+        if wave is not None:
+            wave_t = torch.as_tensor(np.asarray(wave), dtype=self.dtype, device=self.device)
+            if wave_t.numel() < self.t_step + n_steps:
+                raise ValueError("waveform shorter than the run")
+        rec = None
+        if record_every > 0:
+            n_rec = (self.t_step + n_steps) // record_every - self.t_step // record_every
+            rec_base = self.t_step // record_every
+            rec = torch.empty((n_rec, self.total), dtype=self.dtype, device=self.device)
+        strm = stream if stream is not None else _lib.stream_handle(self.device)
+        if path == "persistent":
+            fn = lib.dn_hh_cable_run_f64 if f64 else lib.dn_hh_cable_run_f32
+            chunk = steps_per_launch or n_steps
+            done = 0
+            while done < n_steps:
+                k = min(chunk, n_steps - done)
+                rec_ptr = None
+                if rec is not None:
+                    # the kernel indexes rows by absolute step; shift the base
+                    rec_ptr = rec.data_ptr() - rec_base * self.total * rec.element_size()
+                st = fn(
+                    len(self.plan.task_p), _lib.ptr(self.task_cell0),
+                    _lib.ptr(self.task_ncell), _lib.ptr(self.task_nr),
+                    _lib.ptr(self.task_p), _lib.ptr(self.off),
+                    _lib.ptr(self.v), _lib.ptr(self.m), _lib.ptr(self.h), _lib.ptr(self.n),
+                    _lib.ptr(self.gL), _lib.ptr(self.gRp), _lib.ptr(self.scale),
+                    _lib.ptr(self.drive), _lib.ptr(self.sna), _lib.ptr(self.sk),
+                    _lib.ptr(wave_t), self.t_step, k, self.dt,
+                    self.hh_arr.ctypes.data, rec_ptr, max(record_every, 1), self.total,
+                    waves_per_block, strm,
+                )
+                _lib.check(st, "persistent cable run")
+                self.t_step += k
+                done += k
+        elif path == "plain":
+            fn = lib.dn_hh_cable_step_f64 if f64 else lib.dn_hh_cable_step_f32
+            mainv = torch.empty_like(self.v)
+            rhs = torch.empty_like(self.v)
+            w_host = None if wave is None else np.asarray(wave, dtype=np.float64)
+            for s in range(n_steps):
+                w = 0.0 if w_host is None else float(w_host[self.t_step])
+                st = fn(
+                    self.spec.n_cells, _lib.ptr(self.off), self.total, self.max_nr_plain,
+                    _lib.ptr(self.v), _lib.ptr(self.m), _lib.ptr(self.h), _lib.ptr(self.n),
+                    _lib.ptr(self.gL), _lib.ptr(self.gRp), _lib.ptr(self.scale),
+                    _lib.ptr(self.drive), _lib.ptr(self.sna), _lib.ptr(self.sk),
+                    w, self.dt, self.hh_arr.ctypes.data, _lib.ptr(mainv), _lib.ptr(rhs),
+                    strm,
+                )
+                _lib.check(st, "plain cable step")
+                self.t_step += 1
+                if rec is not None and self.t_step % record_every == 0:
+                    rec[self.t_step // record_every - 1 - rec_base].copy_(self.v)
+        else:
+            raise ValueError(f"unknown path {path!r}")
+        return rec
+
+
+@dataclass
+class _TreeBucket:
+    order: torch.Tensor
+    n: int
+    kmax: int
+    threads: int
+
+
+class TreePopulation:
+    """Ragged batch of HH trees (backward Euler in current units, DHS)."""
+
+    def __init__(self, spec: TreeSpec, dt: float, hh: HH | None = None,
+                 dtype=torch.float32, device="cuda", bucket_edges=(1024, 2048, 4096)):
+        lib = _lib.load()
+        self.spec, self.dt, self.hh = spec, float(dt), hh or HH()
+        self.dtype, self.device = dtype, torch.device(device)
+        dev = self.device
+        slots = lib.dn_hh_tree_slots()
+        ws = tree_workspace(spec, dt)
+        perm, parent, ch_ptr, ch_idx = [], [], [0], []
+        sec_off, sec_head, sec_tail, lvl_off, lvl_ptr = [0], [], [], [0], []
+        node_tree = []
+        for t in range(spec.n_trees):
+            o, e = int(spec.off[t]), int(spec.off[t + 1])
+            sp = section_plan(spec.parent[o:e])
+            perm.append(sp.perm + o)
+            parent.append(sp.parent)
+            ch_ptr.extend((sp.ch_ptr[1:] + len(ch_idx)).tolist())
+            ch_idx.extend(sp.ch_idx.tolist())
+            sec_head.append(sp.sec_head)
+            sec_tail.append(sp.sec_tail)
+            sec_off.append(sec_off[-1] + len(sp.sec_head))
+            lvl_ptr.append(sp.lvl_ptr)
+            lvl_off.append(lvl_off[-1] + len(sp.lvl_ptr))
+            node_tree.append(np.full(e - o, t))
+        self.perm = np.concatenate(perm)
+        P = self.perm
+        i32 = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.int32, device=dev)
+        self.node_off = torch.as_tensor(spec.off, dtype=torch.int64, device=dev)
+        self.parent = i32(np.concatenate(parent))
+        self.ch_ptr = i32(np.asarray(ch_ptr))
+        self.ch_idx = i32(np.asarray(ch_idx) if ch_idx else np.zeros(1))
+        self.sec_off = i32(sec_off)
+        self.sec_head = i32(np.concatenate(sec_head))
+        self.sec_tail = i32(np.concatenate(sec_tail))
+        self.lvl_off = i32(lvl_off)
+        self.lvl_ptr = i32(np.concatenate(lvl_ptr))
+        self.node_tree = i32(np.concatenate(node_tree))
+        self.n_levels = np.diff(np.asarray(lvl_off)) - 1
+        dt_ = dtype
+        self.area = _dev(ws.area[P], dt_, dev)
+        self.cm_dt = _dev(ws.cm_dt[P], dt_, dev)
+        self.g = _dev(ws.g[P], dt_, dev)
+        self.drive = _dev(None if ws.drive is None else ws.drive[P], dt_, dev)
+        self.sna = _dev(None if spec.sna is None else spec.sna[P], dt_, dev)
+        self.sk = _dev(None if spec.sk is None else spec.sk[P], dt_, dev)
+        self.hh_arr = self.hh.array().astype(_DT[dtype])
+        sizes = spec.sizes
+        self.kmax = int(sizes.max())
+        # launch buckets by size, largest trees first inside each bucket
+        self.buckets = []
+        lo = 0
+        for hi in bucket_edges:
+            sel = np.flatnonzero((sizes > lo) & (sizes <= hi))
+            lo = hi
+            if len(sel) == 0:
+                continue
+            sel = sel[np.argsort(-sizes[sel], kind="stable")]
+            kmax = int(sizes[sel].max())
+            threads = int(min(512, max(64, -(-kmax // (slots * 64)) * 64)))
+            self.buckets.append(_TreeBucket(i32(sel), len(sel), kmax, threads))
+        if (sizes > bucket_edges[-1]).any():
+            raise ValueError(f"tree larger than {bucket_edges[-1]} compartments")
+        self.reset()
+
+    @property
+    def total(self) -> int:
+        return self.spec.total
+
+    def _plan_args(self):
+        return (self.spec.n_trees, _lib.ptr(self.node_off), _lib.ptr(self.parent),
+                _lib.ptr(self.ch_ptr), _lib.ptr(self.ch_idx), _lib.ptr(self.sec_off),
+                _lib.ptr(self.sec_head), _lib.ptr(self.sec_tail), _lib.ptr(self.lvl_off),
+                _lib.ptr(self.lvl_ptr))
+
+    def reset(self, v_init: float | None = None):
+        v0 = self.spec.v_init if v_init is None else v_init
+        m0, h0, n0 = hh_steady_state(v0)
+        n = self.total
+        mk = lambda x: torch.full((n,), float(x), dtype=self.dtype, device=self.device)
+        self.v, self.m, self.h, self.n = mk(v0), mk(m0), mk(h0), mk(n0)
+        self.t_step = 0
+
+    def state(self) -> dict:
+        out = {}
+        for name in ("v", "m", "h", "n"):
+            a = np.empty(self.total)
+            a[self.perm] = getattr(self, name).double().cpu().numpy()
+            out[name] = a
+        return out
+
+    def unpermute(self, rec: torch.Tensor) -> np.ndarray:
+        r = rec.double().cpu().numpy()
+        out = np.empty_like(r)
+        out[:, self.perm] = r
+        return out
+
+    def run(self, n_steps: int, wave=None, record_every: int = 0,
+            path: str = "persistent", steps_per_launch: int | None = None, stream=None):
+        lib = _lib.load()
+        f64 = self.dtype == torch.float64
+        wave_t = None
+        if wave is not None:
+            wave_t = torch.as_tensor(np.asarray(wave), dtype=self.dtype, device=self.device)
+            if wave_t.numel() < self.t_step + n_steps:
+                raise ValueError("waveform shorter than the run")
+        rec = None
+        rec_base = 0
+        if record_every > 0:
+            n_rec = (self.t_step + n_steps) // record_every - self.t_step // record_every
+            rec_base = self.t_step // record_every
+            rec = torch.empty((n_rec, self.total), dtype=self.dtype, device=self.device)
+        strm = stream if stream is not None else _lib.stream_handle(self.device)
+        if path == "persistent":
+            fn = lib.dn_hh_tree_run_f64 if f64 else lib.dn_hh_tree_run_f32
+            chunk = steps_per_launch or n_steps
+            done = 0
+            while done < n_steps:
+                k = min(chunk, n_steps - done)
+                rec_ptr = None
+                if rec is not None:
+                    rec_ptr = rec.data_ptr() - rec_base * self.total * rec.element_size()
+                for b in self.buckets:
+                    st = fn(
+                        *self._plan_args(), _lib.ptr(b.order), b.n, b.kmax, b.threads,
+                        _lib.ptr(self.v), _lib.ptr(self.m), _lib.ptr(self.h),
+                        _lib.ptr(self.n), _lib.ptr(self.area), _lib.ptr(self.cm_dt),
+                        _lib.ptr(self.g), _lib.ptr(self.drive), _lib.ptr(self.sna),
+                        _lib.ptr(self.sk), _lib.ptr(wave_t), self.t_step, k, self.dt,
+                        self.hh_arr.ctypes.data, rec_ptr, max(record_every, 1),
+                        self.total, strm,
+                    )
+                    _lib.check(st, "persistent tree run")
+                self.t_step += k
+                done += k
+        elif path == "plain":
+            fn = lib.dn_hh_tree_step_f64 if f64 else lib.dn_hh_tree_step_f32
+            mainv = torch.empty_like(self.v)
+            rhs = torch.empty_like(self.v)
+            w_host = None if wave is None else np.asarray(wave, dtype=np.float64)
+            for s in range(n_steps):
+                w = 0.0 if w_host is None else float(w_host[self.t_step])
+                st = fn(
+                    *self._plan_args(), _lib.ptr(self.node_tree), self.kmax, self.total,
+                    _lib.ptr(self.v), _lib.ptr(self.m), _lib.ptr(self.h), _lib.ptr(self.n),
+                    _lib.ptr(self.area), _lib.ptr(self.cm_dt), _lib.ptr(self.g),
+                    _lib.ptr(self.drive), _lib.ptr(self.sna), _lib.ptr(self.sk), w,
+                    self.dt, self.hh_arr.ctypes.data, _lib.ptr(mainv), _lib.ptr(rhs), strm,
+                )
+                _lib.check(st, "plain tree step")
+                self.t_step += 1
+                if rec is not None and self.t_step % record_every == 0:
+                    rec[self.t_step // record_every - 1 - rec_base].copy_(self.v)
+        else:
+            raise ValueError(f"unknown path {path!r}")
+        return rec

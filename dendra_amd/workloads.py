@@ -1,0 +1,170 @@
+"""Synthetic workloads of the benchmark configurations (seeded, counter RNG).
+
+* ``single_cable``: one unbranched squid-HH cable, 101 compartments of
+  10 um (the reference's Unmyelinated(L=1000 um, dx=10) discretisation,
+  models/core.py:10926-10945), d 2 um, Ra 100 ohm cm, cm 1 uF/cm^2,
+  6.3 degC, v0 -65 mV, with a batch of intracellular 1 ms pulses at
+  compartment 0, amplitudes uniform in [0.05, 1] nA.
+* ``uneven_population``: cables with compartment counts uniform in
+  [kmin, kmax], 10 um compartments, lognormal diameters (median 8 um,
+  sigma 0.4), each driven by a point electrode in a 300 ohm cm medium at a
+  perpendicular distance uniform in 0.1-3 mm above a uniform position
+  along the cable, one monophasic 0.1 ms pulse of amplitude uniform in
+  +-2 mA at 0.5 ms; plus seeded random binary trees grown by a branching
+  process (sections of 5-20 compartments of 10 um, a uniformly chosen leaf
+  section splits into two children of 0.8x its diameter until the size
+  target, uniform in [tmin, tmax], is reached).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import rng
+from .model import CableSpec, TreeSpec, point_source_lead, rect_wave
+
+# RNG stream ids
+S_K, S_DIAM, S_DIST, S_POS, S_AMP, S_TK, S_TREE, S_TDIAM = range(1, 9)
+
+
+def single_cable(batch: int = 16, seed: int = 1, n_comp: int = 101,
+                 dx: float = 10.0) -> CableSpec:
+    off = np.arange(batch + 1, dtype=np.int64) * n_comp
+    total = batch * n_comp
+    amps_na = 0.05 + 0.95 * rng.uniform_n(seed, S_AMP, batch)
+    iamp = np.zeros(total)
+    iamp[off[:-1]] = amps_na * 1e-6  # nA -> mA
+    return CableSpec(
+        off=off, diam=np.full(total, 2.0), dx=np.full(total, dx),
+        rhoa=np.full(total, 100.0), cm=np.full(total, 1.0), iamp=iamp, v_init=-65.0,
+    )
+
+
+def single_cable_wave(n_steps: int = 800, dt: float = 0.025, dtype=np.float64):
+    """1 ms intracellular pulse from t = 0 on the start-of-step grid."""
+    return rect_wave(n_steps, dt, delay=0.0, pw=1.0, dtype=dtype)
+
+
+def extracellular_wave(n_steps: int, dt: float, dtype=np.float32, delay: float = 0.5,
+                       pw: float = 0.1):
+    return rect_wave(n_steps, dt, delay=delay, pw=pw, dtype=dtype)
+
+
+def uneven_cables(n: int, seed: int = 2, kmin: int = 21, kmax: int = 801,
+                  dx: float = 10.0, rhoa: float = 100.0) -> CableSpec:
+    K = rng.integers_n(seed, S_K, n, kmin, kmax)
+    diam_c = np.exp(np.log(8.0) + 0.4 * rng.normal_n(seed, S_DIAM, n))
+    dist_um = 1e3 * (0.1 + 2.9 * rng.uniform_n(seed, S_DIST, n))
+    pos_frac = rng.uniform_n(seed, S_POS, n)
+    amp = -2.0 + 4.0 * rng.uniform_n(seed, S_AMP, n)
+    off = np.zeros(n + 1, dtype=np.int64)
+    off[1:] = np.cumsum(K)
+    total = int(off[-1])
+    cell = np.repeat(np.arange(n), K)
+    j = np.arange(total) - off[cell]
+    x = j * dx
+    ex = pos_frac[cell] * (K[cell] - 1) * dx
+    lead = point_source_lead(x, 0.0, 0.0, ex, 0.0, dist_um[cell])
+    return CableSpec(
+        off=off, diam=diam_c[cell], dx=np.full(total, dx), rhoa=np.full(total, rhoa),
+        cm=np.ones(total), lead=lead, ve_amp=amp, v_init=-65.0,
+    )
+
+
+def random_binary_tree(target: int, key: int, seed: int):
+    """One tree: parent (parent < child), per-node diameter, x/y/z (um)."""
+    u = lambda n, stream: rng.uniform_n(seed, stream, n, start=key * 1_000_003)
+    draws = u(4 * target + 16, S_TREE)
+    di = 0
+
+    def nxt():
+        nonlocal di
+        val = draws[di % len(draws)]
+        di += 1
+        return val
+
+    d0 = float(np.exp(np.log(8.0) + 0.4 * rng.normal_n(seed + key, S_TDIAM, 1)[0]))
+    parent, diam, pos = [], [], []
+    # section: (first node, last node, diameter, end position, direction)
+    leaves = []
+
+    def grow(par_node, d, start, direction):
+        n = 5 + int(nxt() * 16)
+        n = min(n, target - len(parent))
+        p = par_node
+        cur = np.array(start, dtype=np.float64)
+        for _ in range(n):
+            cur = cur + 10.0 * direction
+            parent.append(p)
+            diam.append(d)
+            pos.append(cur.copy())
+            p = len(parent) - 1
+        return p, cur
+
+    def rand_dir():
+        z = 2.0 * nxt() - 1.0
+        phi = 2.0 * np.pi * nxt()
+        s = np.sqrt(max(0.0, 1.0 - z * z))
+        return np.array([s * np.cos(phi), s * np.sin(phi), z])
+
+    last, end = grow(-1, d0, (0.0, 0.0, 0.0), rand_dir())
+    leaves.append((last, d0, end))
+    while len(parent) < target:
+        k = int(nxt() * len(leaves))
+        node, d, end = leaves.pop(k)
+        for _ in range(2):
+            if len(parent) >= target:
+                break
+            last, e2 = grow(node, 0.8 * d, end, rand_dir())
+            leaves.append((last, 0.8 * d, e2))
+    return (np.asarray(parent, dtype=np.int64), np.asarray(diam),
+            np.asarray(pos, dtype=np.float64))
+
+
+def random_trees(n: int, seed: int = 3, tmin: int = 200, tmax: int = 4000,
+                 ra: float = 100.0, dx: float = 10.0) -> TreeSpec:
+    sizes = rng.integers_n(seed, S_TK, n, tmin, tmax)
+    dist_um = 1e3 * (0.1 + 2.9 * rng.uniform_n(seed, S_DIST, n))
+    amp = -2.0 + 4.0 * rng.uniform_n(seed, S_AMP, n)
+    parents, diams, leads = [], [], []
+    for t in range(n):
+        par, d, pos = random_binary_tree(int(sizes[t]), t, seed)
+        parents.append(par)
+        diams.append(d)
+        leads.append(point_source_lead(pos[:, 0], pos[:, 1], pos[:, 2],
+                                       0.0, 0.0, dist_um[t]))
+    off = np.zeros(n + 1, dtype=np.int64)
+    off[1:] = np.cumsum(sizes)
+    total = int(off[-1])
+    return TreeSpec(
+        off=off, parent=np.concatenate(parents), diam=np.concatenate(diams),
+        length=np.full(total, dx), ra=np.full(total, ra), cm=np.ones(total),
+        lead=np.concatenate(leads), ve_amp=amp, v_init=-65.0,
+    )
+
+
+@dataclass
+class UnevenPopulation:
+    cables: CableSpec
+    trees: TreeSpec
+    dt: float
+    n_steps: int
+    wave: np.ndarray
+
+    @property
+    def compartments(self) -> int:
+        return self.cables.total + self.trees.total
+
+    @property
+    def compartment_steps(self) -> int:
+        return self.compartments * self.n_steps
+
+
+def uneven_population(n_cables: int = 8192, n_trees: int = 1024, n_steps: int = 4000,
+                      dt: float = 0.005, seed: int = 7) -> UnevenPopulation:
+    cables = uneven_cables(n_cables, seed=seed)
+    trees = random_trees(n_trees, seed=seed + 1)
+    return UnevenPopulation(cables, trees, dt, n_steps,
+                            extracellular_wave(n_steps, dt, np.float32))

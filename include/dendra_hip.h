@@ -1,0 +1,168 @@
+/*
+ * dendra_hip.h -- C ABI of libdnhip.so, the gfx950 cable hot path.
+ *
+ * Conventions for every entry point:
+ *  - pointers are device pointers (hipMalloc / torch CUDA tensors) unless
+ *    marked "host"; sizes are element counts;
+ *  - inputs are never modified; outputs are caller-allocated;
+ *  - work is enqueued asynchronously on `stream` (0 = null stream); no entry
+ *    point synchronises the device or allocates device memory;
+ *  - return 0 on success, DN_EINVAL (-1) for invalid arguments, DN_ELAUNCH
+ *    (-2) when a kernel launch fails, DN_ERANGE (-3) when a size exceeds what
+ *    the kernel supports without the optional caller scratch.
+ * Citations name the reference interface each entry point replaces
+ * (paths relative to the wmglab-duke/dendra repository root).
+ */
+#ifndef DENDRA_HIP_H
+#define DENDRA_HIP_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DN_OK 0
+#define DN_EINVAL -1
+#define DN_ELAUNCH -2
+#define DN_ERANGE -3
+#define DN_EUNSUPPORTED -4
+
+/* ---- batched tridiagonal solve ------------------------------------------
+ * Replaces thomas_solve_cuda_t / ThomasSolve._solve
+ * (dendra/models/integrators/triton/t_kernel_thomas.py:83-130,218-232) and
+ * pcr_solve_cuda_t (triton/pcr_kernel_thomas.py:99-195,224-226).
+ * a, c: (B, K-1); b, d, x: (B, K), row-major.  method: 0 auto, 1 Thomas
+ * (lane per system), 2 wavefront hybrid PCR.  The adjoint solve of
+ * ThomasSolve.backward (t_kernel_thomas.py:147-189) is the same call with a
+ * and c exchanged.  `scratch` holds dn_tridiag_scratch_elems() values (may
+ * be NULL when that is 0). */
+int64_t dn_tridiag_scratch_elems(int64_t B, int64_t K, int method, int elem_bytes);
+int dn_tridiag_solve_f32(const float* a, const float* b, const float* c,
+                         const float* d, float* x, int64_t B, int64_t K, int method,
+                         float* scratch, int64_t scratch_elems, hipStream_t stream);
+int dn_tridiag_solve_f64(const double* a, const double* b, const double* c,
+                         const double* d, double* x, int64_t B, int64_t K, int method,
+                         double* scratch, int64_t scratch_elems, hipStream_t stream);
+
+/* ---- tree (DHS / Hines) solve ---------------------------------------------
+ * Replaces dhs_solve_cuda / _single_dhs_kernel(_packed)
+ * (dendra/models/integrators/triton/dhs_kernel.py:14-212,493-651), bound by
+ * _dhs._select_solver (integrators/tree.py:812-839).  d_mem, a_geom, b, x:
+ * (B, K) in solver order; parent (K), order (K), layer_ptr (L+1) shared by
+ * the batch, int64, as built by build_dhs_layers (tree.py:615-673). */
+int64_t dn_tree_dhs_scratch_elems(int64_t B, int64_t K, int elem_bytes);
+int dn_tree_dhs_f32(const float* d_mem, const float* a_geom, const float* b,
+                    const int64_t* parent, const int64_t* order,
+                    const int64_t* layer_ptr, float* x, int64_t B, int64_t K, int64_t L,
+                    float* scratch, int64_t scratch_elems, hipStream_t stream);
+int dn_tree_dhs_f64(const double* d_mem, const double* a_geom, const double* b,
+                    const int64_t* parent, const int64_t* order,
+                    const int64_t* layer_ptr, double* x, int64_t B, int64_t K, int64_t L,
+                    double* scratch, int64_t scratch_elems, hipStream_t stream);
+
+/* ---- fused HH + backward Euler, unbranched cables -------------------------
+ * Replaces the per-step transition of _bwd_euler_ub
+ * (dendra/models/integrators/implicit.py:561-664) in Integrator._step order
+ * (integrators/core.py:769-834) with HH gating (mod/HH.py:24-116,
+ * mechanisms/_cnexp.py:96-132), driven from Population.run
+ * (models/core.py:4555-4614).  Ragged batch in one flat compartment axis,
+ * cells in plan order with offsets `off` (n_cells+1).  Per compartment:
+ * gL = g_j/C_j, gRp = g_{j-1}/C_j (0 at cable ends), scale = area/C,
+ * drive = activating-function coefficient scaled by wave[step] (nullable),
+ * sna/sk gNa/gK scales (nullable).  hh (host): {gnabar, gkbar, gl, ena, ek,
+ * el, q10}.  dt in ms.  rec (nullable): V after every rec_every-th absolute
+ * step, row (step0+s+1)/rec_every-1 of a [rows][total] array. */
+int dn_hh_cable_rmax(void);
+int dn_hh_cable_run_f32(int n_tasks, const int32_t* task_cell0,
+                        const int32_t* task_ncell, const int32_t* task_nr,
+                        const int32_t* task_p, const int64_t* off, float* v, float* m,
+                        float* h, float* n, const float* gL, const float* gRp,
+                        const float* scale, const float* drive, const float* sna,
+                        const float* sk, const float* wave, int64_t step0,
+                        int64_t n_steps, float dt, const float* hh, float* rec,
+                        int64_t rec_every, int64_t total, int waves_per_block,
+                        hipStream_t stream);
+int dn_hh_cable_run_f64(int n_tasks, const int32_t* task_cell0,
+                        const int32_t* task_ncell, const int32_t* task_nr,
+                        const int32_t* task_p, const int64_t* off, double* v, double* m,
+                        double* h, double* n, const double* gL, const double* gRp,
+                        const double* scale, const double* drive, const double* sna,
+                        const double* sk, const double* wave, int64_t step0,
+                        int64_t n_steps, double dt, const double* hh, double* rec,
+                        int64_t rec_every, int64_t total, int waves_per_block,
+                        hipStream_t stream);
+/* One plain step (gating+assembly launch, solve launch); mainv/rhs are
+ * caller scratch of `total` values; w is this step's waveform value. */
+int dn_hh_cable_step_f32(int64_t n_cells, const int64_t* off, int64_t total, int max_nr,
+                         float* v, float* m, float* h, float* n, const float* gL,
+                         const float* gRp, const float* scale, const float* drive,
+                         const float* sna, const float* sk, float w, float dt,
+                         const float* hh, float* mainv, float* rhs, hipStream_t stream);
+int dn_hh_cable_step_f64(int64_t n_cells, const int64_t* off, int64_t total, int max_nr,
+                         double* v, double* m, double* h, double* n, const double* gL,
+                         const double* gRp, const double* scale, const double* drive,
+                         const double* sna, const double* sk, double w, double dt,
+                         const double* hh, double* mainv, double* rhs,
+                         hipStream_t stream);
+
+/* ---- fused HH + backward Euler, branched trees ----------------------------
+ * Replaces the per-step transition of _dhs (integrators/tree.py:1202-1357)
+ * and the packed multi-tree path _dhs_multi / dhs_solve_multi_cuda
+ * (tree.py:2097-2334, triton/dhs_kernel_multi.py:16-135,831-888).  Trees in
+ * section-chain numbering: parent/ch_ptr/ch_idx/sec_head/sec_tail local to
+ * each tree; lvl_ptr holds per-tree section offsets per level.  Per node:
+ * area, cmdt = C/dt_s, g = child-indexed axial conductance (0 at the root),
+ * drive (mA per unit waveform, nullable).  The persistent run launches the
+ * `n_launch` trees listed in `order` with one block of `threads` threads per
+ * tree and requires kmax >= each tree's size, kmax <= 8 * threads. */
+int dn_hh_tree_slots(void);
+int dn_hh_tree_run_f32(int n_trees, const int64_t* node_off, const int32_t* parent,
+                       const int32_t* ch_ptr, const int32_t* ch_idx,
+                       const int32_t* sec_off, const int32_t* sec_head,
+                       const int32_t* sec_tail, const int32_t* lvl_off,
+                       const int32_t* lvl_ptr, const int32_t* order, int n_launch,
+                       int kmax, int threads, float* v, float* m, float* h, float* n,
+                       const float* area, const float* cmdt, const float* g,
+                       const float* drive, const float* sna, const float* sk,
+                       const float* wave, int64_t step0, int64_t n_steps, float dt,
+                       const float* hh, float* rec, int64_t rec_every, int64_t total,
+                       hipStream_t stream);
+int dn_hh_tree_run_f64(int n_trees, const int64_t* node_off, const int32_t* parent,
+                       const int32_t* ch_ptr, const int32_t* ch_idx,
+                       const int32_t* sec_off, const int32_t* sec_head,
+                       const int32_t* sec_tail, const int32_t* lvl_off,
+                       const int32_t* lvl_ptr, const int32_t* order, int n_launch,
+                       int kmax, int threads, double* v, double* m, double* h, double* n,
+                       const double* area, const double* cmdt, const double* g,
+                       const double* drive, const double* sna, const double* sk,
+                       const double* wave, int64_t step0, int64_t n_steps, double dt,
+                       const double* hh, double* rec, int64_t rec_every, int64_t total,
+                       hipStream_t stream);
+int dn_hh_tree_step_f32(int n_trees, const int64_t* node_off, const int32_t* parent,
+                        const int32_t* ch_ptr, const int32_t* ch_idx,
+                        const int32_t* sec_off, const int32_t* sec_head,
+                        const int32_t* sec_tail, const int32_t* lvl_off,
+                        const int32_t* lvl_ptr, const int32_t* node_tree, int kmax,
+                        int64_t total, float* v, float* m, float* h, float* n,
+                        const float* area, const float* cmdt, const float* g,
+                        const float* drive, const float* sna, const float* sk, float w,
+                        float dt, const float* hh, float* mainv, float* rhs,
+                        hipStream_t stream);
+int dn_hh_tree_step_f64(int n_trees, const int64_t* node_off, const int32_t* parent,
+                        const int32_t* ch_ptr, const int32_t* ch_idx,
+                        const int32_t* sec_off, const int32_t* sec_head,
+                        const int32_t* sec_tail, const int32_t* lvl_off,
+                        const int32_t* lvl_ptr, const int32_t* node_tree, int kmax,
+                        int64_t total, double* v, double* m, double* h, double* n,
+                        const double* area, const double* cmdt, const double* g,
+                        const double* drive, const double* sna, const double* sk,
+                        double w, double dt, const double* hh, double* mainv,
+                        double* rhs, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DENDRA_HIP_H */
