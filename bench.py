@@ -93,7 +93,7 @@ def main():
             with torch.cuda.stream(side):
                 if record:
                     e = torch.cuda.Event(enable_timing=True); e.record(side); ev["t0"].append(e)
-                tre.run(work.n_steps, wave=wave, stream=side.cuda_stream)
+                tre.run(work.n_steps, wave=wave, stream=side)
                 if record:
                     e = torch.cuda.Event(enable_timing=True); e.record(side); ev["t1"].append(e)
         if cab is not None:

@@ -32,7 +32,7 @@ f32 = C.c_float
 f64 = C.c_double
 
 _CABLE_RUN = [i32, p, p, p, p, p, p, p, p, p, p, p, p, p, p, p, p, i64, i64]
-_TREE_PLAN = [i32, p, p, p, p, p, p, p, p, p]
+_TREE_PLAN = [i32, p, p, p, p, p, p, p, p, p, p, p, p, p, i32, i32]
 
 _SIGNATURES = {
     "dn_tridiag_scratch_elems": (i64, [i64, i64, i32, i32]),
@@ -56,13 +56,15 @@ _SIGNATURES = {
     "dn_hh_tree_run_f32": (
         i32,
         _TREE_PLAN
-        + [p, i32, i32, i32, p, p, p, p, p, p, p, p, p, p, p, i64, i64, f32, p, p]
+        + [p, i32, p, i32, i32, i32, i32, p, p, p, p, p, p, p, p, p, p, p, i64, i64, f32]
+        + [p, p]
         + [i64, i64, p],
     ),
     "dn_hh_tree_run_f64": (
         i32,
         _TREE_PLAN
-        + [p, i32, i32, i32, p, p, p, p, p, p, p, p, p, p, p, i64, i64, f64, p, p]
+        + [p, i32, p, i32, i32, i32, i32, p, p, p, p, p, p, p, p, p, p, p, i64, i64, f64]
+        + [p, p]
         + [i64, i64, p],
     ),
     "dn_hh_tree_step_f32": (

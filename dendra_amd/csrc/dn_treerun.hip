@@ -26,6 +26,7 @@
 namespace dn {
 
 constexpr int kTreeSlots = 8;  // nodes per thread in the persistent kernel
+constexpr int kSecMax = 16;    // nodes per section (the host splits longer chains)
 
 struct TreePlan {
   int n_trees;
@@ -33,11 +34,16 @@ struct TreePlan {
   const int32_t* parent;     // local parent (solver order), -1 at the root
   const int32_t* ch_ptr;     // per node, into ch_idx (global), total + 1
   const int32_t* ch_idx;     // local child indices
-  const int32_t* sec_off;    // n_trees + 1 -> sec_head/sec_tail
-  const int32_t* sec_head;   // local node index
-  const int32_t* sec_tail;
+  const int32_t* sec_off;    // n_trees + 1 -> per-section arrays
+  const int32_t* sec_head;   // local node index of the section head
+  const int32_t* sec_len;    // nodes in the section, 1..kSecMax
+  const int32_t* sec_child;  // local index of the first child section
+  const int32_t* sec_nchild; // number of child sections (contiguous)
+  const int32_t* sec_ptail;  // local parent node of the head (-1 at the root)
+  const int32_t* sec_parent; // local parent section (-1 at the root)
   const int32_t* lvl_off;    // n_trees + 1 -> lvl_ptr
   const int32_t* lvl_ptr;    // per tree: n_levels + 1 local section offsets
+  int smax, lmax;            // largest section and level count of any tree
 };
 
 template <typename T> struct TreeRunArgs {
@@ -52,60 +58,147 @@ template <typename T> struct TreeRunArgs {
   int64_t rec_every, total;
 };
 
-// LDS arrays of one tree.
+// LDS image of one tree: four node arrays and the section table.
 template <typename T> struct TreeLds {
-  T* vs;  // v_old, then accumulated diagonal (before the node's own edge)
-  T* dv;  // main, then the inverse pivot
-  T* bv;  // rhs, then the eliminated rhs, then dv
-  T* gs;  // child-indexed axial conductance
+  T* vs;       // v_old, then accumulated diagonal (before the node's own edge)
+  T* dv;       // main, then the inverse pivot
+  T* bv;       // rhs, then the eliminated rhs, then dv
+  T* gs;       // child-indexed axial conductance
+  int32_t* s0; // head | len << 16
+  int32_t* s1; // first child section | n children << 16
+  int32_t* s2; // parent node of the head, -1 at the root
+  int32_t* lp; // level pointers
+  int n_sec, n_lvl;
 };
 
-// Elimination + back-substitution of one tree held in LDS.  Requires
-// vs = v_old (unused here), dv = main, bv = f, gs = g on entry; leaves the
-// voltage increment in bv.  All threads of the block must call it.
 template <typename T>
-__device__ void tree_chain_solve(const TreePlan& pl, int tr, TreeLds<T> L) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int32_t* lp = pl.lvl_ptr + pl.lvl_off[tr];
-  const int n_lvl = pl.lvl_off[tr + 1] - pl.lvl_off[tr] - 1;
+__device__ __forceinline__ TreeLds<T> tree_lds_carve(unsigned char* raw, int kmax,
+                                                     int smax, int lmax) {
+  TreeLds<T> L;
+  T* base = reinterpret_cast<T*>(raw);
+  L.vs = base; L.dv = base + kmax; L.bv = base + 2 * kmax; L.gs = base + 3 * kmax;
+  int32_t* ib = reinterpret_cast<int32_t*>(base + 4 * kmax);
+  L.s0 = ib; L.s1 = ib + smax; L.s2 = ib + 2 * smax; L.lp = ib + 3 * smax;
+  (void)lmax;
+  return L;
+}
+
+__host__ __device__ constexpr size_t tree_lds_bytes(int kmax, int smax, int lmax,
+                                                    size_t elem) {
+  return 4 * (size_t)kmax * elem + (3 * (size_t)smax + (size_t)lmax + 1) * 4;
+}
+
+// Stage the section table and level pointers of tree `tr` (block-wide).
+template <typename T>
+__device__ void tree_stage_topology(const TreePlan& pl, int tr, TreeLds<T>& L) {
   const int32_t so = pl.sec_off[tr];
-  const int64_t no = pl.node_off[tr];
-  const int32_t* par = pl.parent + no;
-  const int32_t* cptr = pl.ch_ptr + no;
-  for (int l = n_lvl - 1; l >= 0; --l) {
-    for (int s = lp[l] + tid; s < lp[l + 1]; s += nt) {
-      const int hd = pl.sec_head[so + s], tl = pl.sec_tail[so + s];
-      T D = L.dv[tl], B = L.bv[tl];
-      for (int32_t q = cptr[tl]; q < cptr[tl + 1]; ++q) {
-        const int c = pl.ch_idx[q];
-        const T fac = L.gs[c] * L.dv[c];
-        D += fac * L.vs[c];
-        B += fac * L.bv[c];
+  L.n_sec = pl.sec_off[tr + 1] - so;
+  const int32_t lo = pl.lvl_off[tr];
+  L.n_lvl = pl.lvl_off[tr + 1] - lo - 1;
+  for (int s = threadIdx.x; s < L.n_sec; s += blockDim.x) {
+    L.s0[s] = pl.sec_head[so + s] | (pl.sec_len[so + s] << 16);
+    L.s1[s] = pl.sec_child[so + s] | (pl.sec_nchild[so + s] << 16);
+    L.s2[s] = pl.sec_ptail[so + s];
+  }
+  for (int l = threadIdx.x; l <= L.n_lvl; l += blockDim.x) L.lp[l] = pl.lvl_ptr[lo + l];
+}
+
+// Add the axial edge currents -g_k (v_k - v_p) + sum_children g_c (v_c - v_k)
+// to bv, one section per thread (all sections in parallel).  vs holds v_old.
+template <typename T>
+__device__ void tree_axial(TreeLds<T>& L) {
+  for (int s = threadIdx.x; s < L.n_sec; s += blockDim.x) {
+    const int w0 = L.s0[s], w1 = L.s1[s];
+    const int hd = w0 & 0xFFFF, len = w0 >> 16;
+    const int cf = w1 & 0xFFFF, nc = w1 >> 16;
+    const int pt = L.s2[s];
+    T vp = pt >= 0 ? L.vs[pt] : T(0);
+    T vr[kSecMax], gr[kSecMax];
+#pragma unroll
+    for (int r = 0; r < kSecMax; ++r)
+      if (r < len) { vr[r] = L.vs[hd + r]; gr[r] = L.gs[hd + r]; }
+    T F[kSecMax];
+#pragma unroll
+    for (int r = 0; r < kSecMax; ++r) {
+      if (r < len) {
+        F[r] = gr[r] * (vr[r] - vp);
+        vp = vr[r];
       }
-      for (int k = tl; k >= hd; --k) {
-        const T gk = L.gs[k];
-        const T inv = frcp(D + gk);
-        L.dv[k] = inv;
-        L.vs[k] = D;
-        L.bv[k] = B;
-        if (k > hd) {
-          const T fac = gk * inv;
-          D = L.dv[k - 1] + fac * D;
-          B = L.bv[k - 1] + fac * B;
+    }
+    T vt = vp;  // tail voltage
+    T tail_in = T(0);
+    for (int c = cf; c < cf + nc; ++c) {
+      const int hc = L.s0[c] & 0xFFFF;
+      tail_in += L.gs[hc] * (L.vs[hc] - vt);
+    }
+#pragma unroll
+    for (int r = 0; r < kSecMax; ++r) {
+      if (r < len) {
+        T inflow = (r + 1 < len) ? F[r + 1 < kSecMax ? r + 1 : r] : tail_in;
+        L.bv[hd + r] += inflow - F[r];
+      }
+    }
+  }
+}
+
+// Elimination (deepest level first) and back-substitution (root level
+// first) over section chains held in registers.  Entry: dv = main, bv = f,
+// gs = g.  Exit: bv = dv (the voltage increment).  Block-wide.
+template <typename T>
+__device__ void tree_chain_solve(TreeLds<T>& L) {
+  for (int l = L.n_lvl - 1; l >= 0; --l) {
+    for (int s = L.lp[l] + threadIdx.x; s < L.lp[l + 1]; s += blockDim.x) {
+      const int w0 = L.s0[s], w1 = L.s1[s];
+      const int hd = w0 & 0xFFFF, len = w0 >> 16;
+      const int cf = w1 & 0xFFFF, nc = w1 >> 16;
+      T Dr[kSecMax], Br[kSecMax], gr[kSecMax];
+#pragma unroll
+      for (int r = 0; r < kSecMax; ++r)
+        if (r < len) { Dr[r] = L.dv[hd + r]; Br[r] = L.bv[hd + r]; gr[r] = L.gs[hd + r]; }
+      T pullD = T(0), pullB = T(0);
+      for (int c = cf; c < cf + nc; ++c) {
+        const int hc = L.s0[c] & 0xFFFF;
+        const T fac = L.gs[hc] * L.dv[hc];
+        pullD += fac * L.vs[hc];
+        pullB += fac * L.bv[hc];
+      }
+      T D = T(0), B = T(0), fac = T(0);
+#pragma unroll
+      for (int r = kSecMax - 1; r >= 0; --r) {
+        if (r < len) {
+          if (r == len - 1) {
+            D = Dr[r] + pullD;
+            B = Br[r] + pullB;
+          } else {
+            D = Dr[r] + fac * D;
+            B = Br[r] + fac * B;
+          }
+          const T inv = frcp(D + gr[r]);
+          L.dv[hd + r] = inv;
+          L.vs[hd + r] = D;
+          L.bv[hd + r] = B;
+          fac = gr[r] * inv;
         }
       }
     }
     __syncthreads();
   }
-  for (int l = 0; l < n_lvl; ++l) {
-    for (int s = lp[l] + tid; s < lp[l + 1]; s += nt) {
-      const int hd = pl.sec_head[so + s], tl = pl.sec_tail[so + s];
-      const int p = par[hd];
-      T x = (L.bv[hd] + L.gs[hd] * (p >= 0 ? L.bv[p] : T(0))) * L.dv[hd];
-      L.bv[hd] = x;
-      for (int k = hd + 1; k <= tl; ++k) {
-        x = (L.bv[k] + L.gs[k] * x) * L.dv[k];
-        L.bv[k] = x;
+  for (int l = 0; l < L.n_lvl; ++l) {
+    for (int s = L.lp[l] + threadIdx.x; s < L.lp[l + 1]; s += blockDim.x) {
+      const int w0 = L.s0[s];
+      const int hd = w0 & 0xFFFF, len = w0 >> 16;
+      const int pt = L.s2[s];
+      T ir[kSecMax], Br[kSecMax], gr[kSecMax];
+#pragma unroll
+      for (int r = 0; r < kSecMax; ++r)
+        if (r < len) { ir[r] = L.dv[hd + r]; Br[r] = L.bv[hd + r]; gr[r] = L.gs[hd + r]; }
+      T x = pt >= 0 ? L.bv[pt] : T(0);
+#pragma unroll
+      for (int r = 0; r < kSecMax; ++r) {
+        if (r < len) {
+          x = (Br[r] + gr[r] * x) * ir[r];
+          L.bv[hd + r] = x;
+        }
       }
     }
     __syncthreads();
@@ -117,17 +210,14 @@ __device__ void tree_chain_solve(const TreePlan& pl, int tr, TreeLds<T> L) {
 template <typename T, bool SCALES>
 __global__ __launch_bounds__(512) void hh_tree_run_kernel(TreeRunArgs<T> A,
                                                            const int32_t* order,
-                                                           int kmax) {
+                                                           int kmax, int smax, int lmax) {
   extern __shared__ unsigned char dn_smem_raw[];
   const int tr = order ? order[blockIdx.x] : (int)blockIdx.x;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int64_t no = A.pl.node_off[tr];
   const int K = (int)(A.pl.node_off[tr + 1] - no);
-  TreeLds<T> L;
-  T* base = reinterpret_cast<T*>(dn_smem_raw);
-  L.vs = base; L.dv = base + kmax; L.bv = base + 2 * kmax; L.gs = base + 3 * kmax;
-  const int32_t* par = A.pl.parent + no;
-  const int32_t* cptr = A.pl.ch_ptr + no;
+  TreeLds<T> L = tree_lds_carve<T>(dn_smem_raw, kmax, smax, lmax);
+  tree_stage_topology<T>(A.pl, tr, L);
   T v[kTreeSlots], m[kTreeSlots], h[kTreeSlots], n[kTreeSlots];
   T ar[kTreeSlots], cd[kTreeSlots], dr[kTreeSlots], sna[kTreeSlots], sk[kTreeSlots];
 #pragma unroll
@@ -161,21 +251,9 @@ __global__ __launch_bounds__(512) void hh_tree_run_kernel(TreeRunArgs<T> A,
       }
     }
     __syncthreads();
-#pragma unroll
-    for (int s = 0; s < kTreeSlots; ++s) {
-      const int i = tid + s * nt;
-      if (i < K) {
-        const int p = par[i];
-        T ax = p >= 0 ? -L.gs[i] * (v[s] - L.vs[p]) : T(0);
-        for (int32_t q = cptr[i]; q < cptr[i + 1]; ++q) {
-          const int c = A.pl.ch_idx[q];
-          ax += L.gs[c] * (L.vs[c] - v[s]);
-        }
-        L.bv[i] += ax;
-      }
-    }
+    tree_axial<T>(L);
     __syncthreads();
-    tree_chain_solve<T>(A.pl, tr, L);
+    tree_chain_solve<T>(L);
 #pragma unroll
     for (int s = 0; s < kTreeSlots; ++s) {
       const int i = tid + s * nt;
@@ -202,6 +280,204 @@ __global__ __launch_bounds__(512) void hh_tree_run_kernel(TreeRunArgs<T> A,
       A.n[idx] = n[s];
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent run, one thread per section, several trees per block.
+// Block b runs the trees order[grp_off[b] .. grp_off[b+1]) whose section
+// counts sum to at most blockDim.x; thread s owns block-local section s.  The
+// thread keeps the V, m, h, n of its section's nodes (<= kSecMax, contiguous)
+// in registers for the whole run, re-reads their membrane coefficients from
+// L2 each step, and performs their gating, their share of the axial assembly,
+// their chain elimination and their back-substitution.  LDS carries:
+//   gs      axial conductance of every node of the block's trees
+//   vh/vt   head and tail voltage (axial currents across section joins)
+//   cD/cB   the head's Schur contribution (g/d) old_d and (g/d) b to the parent
+//   xt      the tail's solution, needed by the child sections' heads.
+// The level loops run to the deepest tree of the block; a thread acts when
+// the loop reaches its own section's level.
+// ---------------------------------------------------------------------------
+// Four waves per SIMD (two 512-thread blocks per CU) beat three waves without
+// spills: the level phases are latency-bound, so residency matters more than
+// the dozen registers the bound spills.
+template <typename T, bool SCALES>
+__global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
+                                                          const int32_t* order,
+                                                          const int32_t* grp_off,
+                                                          int node_cap) {
+  extern __shared__ unsigned char dn_smem_raw[];
+  const int nt = blockDim.x;
+  T* gs = reinterpret_cast<T*>(dn_smem_raw);
+  T *vh = gs + node_cap, *vt = vh + nt, *cD = vh + 2 * nt, *cB = vh + 3 * nt,
+    *xt = vh + 4 * nt, *gh = vh + 5 * nt;
+  const int s = threadIdx.x;
+  const int g0 = grp_off[blockIdx.x], g1 = grp_off[blockIdx.x + 1];
+  // locate this thread's tree; every thread walks the group (a few trees)
+  int sec_base = 0, node_base = 0, my_tree = -1, tree_sec_base = 0, tree_node_base = 0;
+  int nlvl_max = 0;
+  for (int k = g0; k < g1; ++k) {
+    const int t = order[k];
+    const int ns = A.pl.sec_off[t + 1] - A.pl.sec_off[t];
+    const int nn = (int)(A.pl.node_off[t + 1] - A.pl.node_off[t]);
+    const int nl = A.pl.lvl_off[t + 1] - A.pl.lvl_off[t] - 1;
+    nlvl_max = nl > nlvl_max ? nl : nlvl_max;
+    if (s >= sec_base && s < sec_base + ns) {
+      my_tree = t;
+      tree_sec_base = sec_base;
+      tree_node_base = node_base;
+    }
+    sec_base += ns;
+    node_base += nn;
+  }
+  if (sec_base > nt || node_base > node_cap) return;  // launch contract violated
+  // stage the group's axial conductances into the block's node table
+  node_base = 0;
+  for (int k = g0; k < g1; ++k) {
+    const int t = order[k];
+    const int64_t to = A.pl.node_off[t];
+    const int nn = (int)(A.pl.node_off[t + 1] - to);
+    for (int i = s; i < nn; i += nt) gs[node_base + i] = A.g[to + i];
+    node_base += nn;
+  }
+  const bool own = my_tree >= 0;
+  int hd = 0, len = 0, cf = 0, nc = 0, psec = -1, my_lvl = -1;
+  int64_t no = 0;
+  if (own) {
+    const int ls = s - tree_sec_base;
+    const int32_t so = A.pl.sec_off[my_tree];
+    no = A.pl.node_off[my_tree];
+    hd = A.pl.sec_head[so + ls];
+    len = A.pl.sec_len[so + ls];
+    cf = A.pl.sec_child[so + ls] + tree_sec_base;
+    nc = A.pl.sec_nchild[so + ls];
+    const int ps = A.pl.sec_parent[so + ls];
+    psec = ps >= 0 ? ps + tree_sec_base : -1;
+    const int32_t* lp = A.pl.lvl_ptr + A.pl.lvl_off[my_tree];
+    const int nl = A.pl.lvl_off[my_tree + 1] - A.pl.lvl_off[my_tree] - 1;
+    for (int l = 0; l < nl; ++l)
+      if (ls >= lp[l] && ls < lp[l + 1]) my_lvl = l;
+  }
+  const int gb = tree_node_base + hd;  // block-local index of the section head
+  const T* arp = A.area + no + hd;
+  const T* cdp = A.cmdt + no + hd;
+  const T* drp = A.drive ? A.drive + no + hd : nullptr;
+  T v[kSecMax], m[kSecMax], h[kSecMax], n[kSecMax];
+#pragma unroll
+  for (int r = 0; r < kSecMax; ++r) {
+    const bool in = r < len;
+    const int64_t idx = no + hd + r;
+    v[r] = in ? A.v[idx] : T(0);
+    m[r] = in ? A.m[idx] : T(0);
+    h[r] = in ? A.h[idx] : T(0);
+    n[r] = in ? A.n[idx] : T(0);
+  }
+  __syncthreads();
+  if (own) gh[s] = gs[gb];
+  const T dt = A.dt;
+  for (int64_t st = 0; st < A.n_steps; ++st) {
+    const T w = A.wave ? A.wave[A.step0 + st] : T(0);
+    T vtail = v[0];
+#pragma unroll
+    for (int r = 1; r < kSecMax; ++r)
+      if (r < len) vtail = v[r];
+    if (own) { vh[s] = v[0]; vt[s] = vtail; }
+    T M[kSecMax], F[kSecMax];
+#pragma unroll
+    for (int r = 0; r < kSecMax; ++r) {
+      if (r < len) {
+        const T sna = SCALES && A.sna ? A.sna[no + hd + r] : T(1);
+        const T sk = SCALES && A.sk ? A.sk[no + hd + r] : T(1);
+        T G, J;
+        hh_advance<T>(v[r], dt, A.hh, sna, sk, m[r], h[r], n[r], G, J);
+        const T ar = arp[r];
+        M[r] = cdp[r] + G * ar;
+        F[r] = (J - G * v[r]) * ar + (drp ? drp[r] * w : T(0));
+      }
+    }
+    __syncthreads();
+    // axial currents: -g_k (v_k - v_parent) + sum_children g_c (v_c - v_k)
+    if (own) {
+      T vprev = psec >= 0 ? vt[psec] : T(0);
+      T tail_in = T(0);
+      for (int c = cf; c < cf + nc; ++c) tail_in += gh[c] * (vh[c] - vtail);
+#pragma unroll
+      for (int r = 0; r < kSecMax; ++r) {
+        if (r < len) {
+          const T fl = gs[gb + r] * (v[r] - vprev);
+          F[r] -= fl;
+          if (r > 0) F[r - 1 >= 0 ? r - 1 : 0] += fl;
+          vprev = v[r];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kSecMax; ++r)
+        if (r == len - 1) F[r] += tail_in;
+    }
+    // elimination, deepest level first
+    for (int l = nlvl_max - 1; l >= 0; --l) {
+      if (my_lvl == l) {
+        T pD = T(0), pB = T(0);
+        for (int c = cf; c < cf + nc; ++c) { pD += cD[c]; pB += cB[c]; }
+        T D = T(0), B = T(0), fac = T(0);
+#pragma unroll
+        for (int r = kSecMax - 1; r >= 0; --r) {
+          if (r < len) {
+            if (r == len - 1) {
+              D = M[r] + pD;
+              B = F[r] + pB;
+            } else {
+              D = M[r] + fac * D;
+              B = F[r] + fac * B;
+            }
+            const T gr = gs[gb + r];
+            const T inv = frcp(D + gr);
+            M[r] = inv;
+            F[r] = B;
+            fac = gr * inv;
+          }
+        }
+        cD[s] = fac * D;
+        cB[s] = fac * B;
+      }
+      __syncthreads();
+    }
+    // back-substitution, root level first
+    for (int l = 0; l < nlvl_max; ++l) {
+      if (my_lvl == l) {
+        T x = psec >= 0 ? xt[psec] : T(0);
+#pragma unroll
+        for (int r = 0; r < kSecMax; ++r) {
+          if (r < len) {
+            x = (F[r] + gs[gb + r] * x) * M[r];
+            v[r] += x;
+          }
+        }
+        xt[s] = x;
+      }
+      __syncthreads();
+    }
+    if (A.rec && ((A.step0 + st + 1) % A.rec_every) == 0) {
+      const int64_t row = (A.step0 + st + 1) / A.rec_every - 1;
+#pragma unroll
+      for (int r = 0; r < kSecMax; ++r)
+        if (r < len) A.rec[row * A.total + no + hd + r] = v[r];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kSecMax; ++r) {
+    if (r < len) {
+      const int64_t idx = no + hd + r;
+      A.v[idx] = v[r];
+      A.m[idx] = m[r];
+      A.h[idx] = h[r];
+      A.n[idx] = n[r];
+    }
+  }
+}
+
+__host__ __device__ constexpr size_t tree_sec_lds_bytes(int threads, int node_cap,
+                                                        size_t elem) {
+  return ((size_t)node_cap + 6 * (size_t)threads) * elem;
 }
 
 // ---------------------------------------------------------------------------
@@ -236,27 +512,28 @@ __global__ __launch_bounds__(256) void hh_tree_assemble_kernel(
 template <typename T>
 __global__ __launch_bounds__(256) void hh_tree_solve_kernel(
     TreePlan pl, const T* __restrict__ g, const T* __restrict__ mainv,
-    const T* __restrict__ rhs, T* __restrict__ v, int kmax) {
+    const T* __restrict__ rhs, T* __restrict__ v, int kmax, int smax, int lmax) {
   extern __shared__ unsigned char dn_smem_raw[];
   const int tr = blockIdx.x;
   const int64_t no = pl.node_off[tr];
   const int K = (int)(pl.node_off[tr + 1] - no);
-  TreeLds<T> L;
-  T* base = reinterpret_cast<T*>(dn_smem_raw);
-  L.vs = base; L.dv = base + kmax; L.bv = base + 2 * kmax; L.gs = base + 3 * kmax;
+  TreeLds<T> L = tree_lds_carve<T>(dn_smem_raw, kmax, smax, lmax);
+  tree_stage_topology<T>(pl, tr, L);
   for (int i = threadIdx.x; i < K; i += blockDim.x) {
     L.dv[i] = mainv[no + i];
     L.bv[i] = rhs[no + i];
     L.gs[i] = g[no + i];
   }
   __syncthreads();
-  tree_chain_solve<T>(pl, tr, L);
+  tree_chain_solve<T>(L);
   for (int i = threadIdx.x; i < K; i += blockDim.x) v[no + i] += L.bv[i];
 }
 
 static bool plan_ok(const TreePlan& p) {
   return p.n_trees >= 1 && p.node_off && p.parent && p.ch_ptr && p.ch_idx &&
-         p.sec_off && p.sec_head && p.sec_tail && p.lvl_off && p.lvl_ptr;
+         p.sec_off && p.sec_head && p.sec_len && p.sec_child && p.sec_nchild &&
+         p.sec_ptail && p.sec_parent && p.lvl_off && p.lvl_ptr && p.smax >= 1 &&
+         p.lmax >= 1;
 }
 
 }  // namespace dn
@@ -264,15 +541,21 @@ static bool plan_ok(const TreePlan& p) {
 #define DN_TREE_PLAN_ARGS                                                            \
   int n_trees, const int64_t *node_off, const int32_t *parent, const int32_t *ch_ptr, \
       const int32_t *ch_idx, const int32_t *sec_off, const int32_t *sec_head,         \
-      const int32_t *sec_tail, const int32_t *lvl_off, const int32_t *lvl_ptr
+      const int32_t *sec_len, const int32_t *sec_child, const int32_t *sec_nchild,    \
+      const int32_t *sec_ptail, const int32_t *sec_parent, const int32_t *lvl_off,    \
+      const int32_t *lvl_ptr, int smax, int lmax
 
 #define DN_TREE_PLAN_FILL(P)                                                          \
   P.n_trees = n_trees; P.node_off = node_off; P.parent = parent; P.ch_ptr = ch_ptr;   \
   P.ch_idx = ch_idx; P.sec_off = sec_off; P.sec_head = sec_head;                      \
-  P.sec_tail = sec_tail; P.lvl_off = lvl_off; P.lvl_ptr = lvl_ptr;
+  P.sec_len = sec_len; P.sec_child = sec_child; P.sec_nchild = sec_nchild;            \
+  P.sec_ptail = sec_ptail; P.sec_parent = sec_parent; P.lvl_off = lvl_off;          \
+  P.lvl_ptr = lvl_ptr;                                                                \
+  P.smax = smax; P.lmax = lmax;
 
 template <typename T>
-static int tree_run_impl(dn::TreePlan P, const int32_t* order, int kmax, int threads,
+static int tree_run_impl(dn::TreePlan P, const int32_t* order, const int32_t* grp_off,
+                         int n_groups, int layout, int kmax, int threads,
                          T* v, T* m, T* h, T* n, const T* area, const T* cmdt,
                          const T* g, const T* drive, const T* sna, const T* sk,
                          const T* wave, int64_t step0, int64_t n_steps, T dt,
@@ -281,9 +564,12 @@ static int tree_run_impl(dn::TreePlan P, const int32_t* order, int kmax, int thr
   if (!dn::plan_ok(P) || !v || !m || !h || !n || !area || !cmdt || !g || !hh)
     return DN_EINVAL;
   if (threads < 64 || threads > 512 || (threads & 63)) return DN_EINVAL;
-  if (kmax < 1 || kmax > dn::kTreeSlots * threads) return DN_ERANGE;
+  if (layout == 0 && (kmax < 1 || kmax > dn::kTreeSlots * threads)) return DN_ERANGE;
+  if (layout == 1 && (!grp_off || kmax < 1)) return DN_EINVAL;
+  if (layout != 0 && layout != 1) return DN_EINVAL;
   if (rec && rec_every < 1) return DN_EINVAL;
-  size_t lds = (size_t)4 * kmax * sizeof(T);
+  size_t lds = layout == 0 ? dn::tree_lds_bytes(kmax, P.smax, P.lmax, sizeof(T))
+                           : dn::tree_sec_lds_bytes(threads, kmax, sizeof(T));
   if (lds > 160 * 1024) return DN_ERANGE;
   if (n_steps == 0) return DN_OK;
   dn::TreeRunArgs<T> A;
@@ -292,19 +578,30 @@ static int tree_run_impl(dn::TreePlan P, const int32_t* order, int kmax, int thr
   A.n_steps = n_steps; A.dt = dt;
   A.hh = {hh[0], hh[1], hh[2], hh[3], hh[4], hh[5], hh[6]};
   A.rec = rec; A.rec_every = rec_every; A.total = total;
-  if (sna || sk)
+  if (layout == 1) {
+    if (sna || sk)
+      hipLaunchKernelGGL((dn::hh_tree_sec_kernel<T, true>), dim3(n_groups),
+                         dim3(threads), lds, stream, A, order, grp_off, kmax);
+    else
+      hipLaunchKernelGGL((dn::hh_tree_sec_kernel<T, false>), dim3(n_groups),
+                         dim3(threads), lds, stream, A, order, grp_off, kmax);
+  } else if (sna || sk)
     hipLaunchKernelGGL((dn::hh_tree_run_kernel<T, true>), dim3(P.n_trees), dim3(threads),
-                       lds, stream, A, order, kmax);
+                       lds, stream, A, order, kmax, P.smax, P.lmax);
   else
     hipLaunchKernelGGL((dn::hh_tree_run_kernel<T, false>), dim3(P.n_trees),
-                       dim3(threads), lds, stream, A, order, kmax);
+                       dim3(threads), lds, stream, A, order, kmax, P.smax, P.lmax);
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
 
 // Persistent multi-step run of a planned ragged tree batch (trees listed in
-// `order`, or all trees when order is null).  kmax >= every tree's size in the
-// launch and kmax <= 8 * threads.
+// `order`, or all trees when order is null).  layout 0: one tree per block,
+// node slots, requires kmax >= every tree's size and kmax <= 8 * threads.
+// layout 1: one thread per section, block b runs the trees
+// order[grp_off[b]..grp_off[b+1]) whose section counts sum to <= threads and
+// whose node counts sum to <= kmax (the LDS node capacity).
 DN_EXPORT int dn_hh_tree_run_f32(DN_TREE_PLAN_ARGS, const int32_t* order, int n_launch,
+                                 const int32_t* grp_off, int n_groups, int layout,
                                  int kmax, int threads, float* v, float* m, float* h,
                                  float* n, const float* area, const float* cmdt,
                                  const float* g, const float* drive, const float* sna,
@@ -314,12 +611,13 @@ DN_EXPORT int dn_hh_tree_run_f32(DN_TREE_PLAN_ARGS, const int32_t* order, int n_
   dn::TreePlan P;
   DN_TREE_PLAN_FILL(P)
   if (order) P.n_trees = n_launch;
-  return tree_run_impl<float>(P, order, kmax, threads, v, m, h, n, area, cmdt, g, drive,
+  return tree_run_impl<float>(P, order, grp_off, n_groups, layout, kmax, threads, v, m, h, n, area, cmdt, g, drive,
                               sna, sk, wave, step0, n_steps, dt, hh, rec, rec_every,
                               total, stream);
 }
 
 DN_EXPORT int dn_hh_tree_run_f64(DN_TREE_PLAN_ARGS, const int32_t* order, int n_launch,
+                                 const int32_t* grp_off, int n_groups, int layout,
                                  int kmax, int threads, double* v, double* m, double* h,
                                  double* n, const double* area, const double* cmdt,
                                  const double* g, const double* drive, const double* sna,
@@ -330,7 +628,7 @@ DN_EXPORT int dn_hh_tree_run_f64(DN_TREE_PLAN_ARGS, const int32_t* order, int n_
   dn::TreePlan P;
   DN_TREE_PLAN_FILL(P)
   if (order) P.n_trees = n_launch;
-  return tree_run_impl<double>(P, order, kmax, threads, v, m, h, n, area, cmdt, g,
+  return tree_run_impl<double>(P, order, grp_off, n_groups, layout, kmax, threads, v, m, h, n, area, cmdt, g,
                                drive, sna, sk, wave, step0, n_steps, dt, hh, rec,
                                rec_every, total, stream);
 }
@@ -345,7 +643,7 @@ static int tree_step_impl(dn::TreePlan P, const int32_t* node_tree, int kmax,
       !hh || !mainv || !rhs)
     return DN_EINVAL;
   if (total < 1 || kmax < 1) return DN_EINVAL;
-  size_t lds = (size_t)4 * kmax * sizeof(T);
+  size_t lds = dn::tree_lds_bytes(kmax, P.smax, P.lmax, sizeof(T));
   if (lds > 160 * 1024) return DN_ERANGE;
   dn::HHParams<T> p = {hh[0], hh[1], hh[2], hh[3], hh[4], hh[5], hh[6]};
   int blocks = (int)((total + 255) / 256);
@@ -353,7 +651,7 @@ static int tree_step_impl(dn::TreePlan P, const int32_t* node_tree, int kmax,
                      total, node_tree, P.node_off, P.parent, P.ch_ptr, P.ch_idx, v, m, h,
                      n, area, cmdt, g, drive, sna, sk, w, dt, p, mainv, rhs);
   hipLaunchKernelGGL(dn::hh_tree_solve_kernel<T>, dim3(P.n_trees), dim3(256), lds,
-                     stream, P, g, mainv, rhs, v, kmax);
+                     stream, P, g, mainv, rhs, v, kmax, P.smax, P.lmax);
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
 

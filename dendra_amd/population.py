@@ -126,7 +126,9 @@ class CablePopulation:
             n_rec = (self.t_step + n_steps) // record_every - self.t_step // record_every
             rec_base = self.t_step // record_every
             rec = torch.empty((n_rec, self.total), dtype=self.dtype, device=self.device)
-        strm = stream if stream is not None else _lib.stream_handle(self.device)
+        # `stream`: a torch.cuda.Stream (None = the current stream)
+        strm = (stream if stream is not None
+                else torch.cuda.current_stream(self.device)).cuda_stream
         if path == "persistent":
             fn = lib.dn_hh_cable_run_f64 if f64 else lib.dn_hh_cable_run_f32
             chunk = steps_per_launch or n_steps
@@ -181,13 +183,17 @@ class _TreeBucket:
     n: int
     kmax: int
     threads: int
+    layout: int
+    grp_off: torch.Tensor | None
+    n_groups: int
 
 
 class TreePopulation:
     """Ragged batch of HH trees (backward Euler in current units, DHS)."""
 
     def __init__(self, spec: TreeSpec, dt: float, hh: HH | None = None,
-                 dtype=torch.float32, device="cuda", bucket_edges=(1024, 2048, 4096)):
+                 dtype=torch.float32, device="cuda", bucket_edges=(1024, 2048, 4096),
+                 block: int = 512):
         lib = _lib.load()
         self.spec, self.dt, self.hh = spec, float(dt), hh or HH()
         self.dtype, self.device = dtype, torch.device(device)
@@ -195,8 +201,9 @@ class TreePopulation:
         slots = lib.dn_hh_tree_slots()
         ws = tree_workspace(spec, dt)
         perm, parent, ch_ptr, ch_idx = [], [], [0], []
-        sec_off, sec_head, sec_tail, lvl_off, lvl_ptr = [0], [], [], [0], []
-        node_tree = []
+        sec = {k: [] for k in ("head", "len", "child", "nchild", "ptail", "parent")}
+        sec_off, lvl_off, lvl_ptr, node_tree = [0], [0], [], []
+        n_sec, n_lvl = [], []
         for t in range(spec.n_trees):
             o, e = int(spec.off[t]), int(spec.off[t + 1])
             sp = section_plan(spec.parent[o:e])
@@ -204,12 +211,14 @@ class TreePopulation:
             parent.append(sp.parent)
             ch_ptr.extend((sp.ch_ptr[1:] + len(ch_idx)).tolist())
             ch_idx.extend(sp.ch_idx.tolist())
-            sec_head.append(sp.sec_head)
-            sec_tail.append(sp.sec_tail)
+            for k in sec:
+                sec[k].append(getattr(sp, "sec_" + k))
             sec_off.append(sec_off[-1] + len(sp.sec_head))
             lvl_ptr.append(sp.lvl_ptr)
             lvl_off.append(lvl_off[-1] + len(sp.lvl_ptr))
             node_tree.append(np.full(e - o, t))
+            n_sec.append(len(sp.sec_head))
+            n_lvl.append(sp.n_levels)
         self.perm = np.concatenate(perm)
         P = self.perm
         i32 = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.int32, device=dev)
@@ -218,12 +227,14 @@ class TreePopulation:
         self.ch_ptr = i32(np.asarray(ch_ptr))
         self.ch_idx = i32(np.asarray(ch_idx) if ch_idx else np.zeros(1))
         self.sec_off = i32(sec_off)
-        self.sec_head = i32(np.concatenate(sec_head))
-        self.sec_tail = i32(np.concatenate(sec_tail))
+        self.sec = {k: i32(np.concatenate(v)) for k, v in sec.items()}
         self.lvl_off = i32(lvl_off)
         self.lvl_ptr = i32(np.concatenate(lvl_ptr))
         self.node_tree = i32(np.concatenate(node_tree))
-        self.n_levels = np.diff(np.asarray(lvl_off)) - 1
+        self.n_sec = np.asarray(n_sec)
+        self.n_levels = np.asarray(n_lvl)
+        self.smax = int(self.n_sec.max())
+        self.lmax = int(self.n_levels.max())
         dt_ = dtype
         self.area = _dev(ws.area[P], dt_, dev)
         self.cm_dt = _dev(ws.cm_dt[P], dt_, dev)
@@ -234,19 +245,44 @@ class TreePopulation:
         self.hh_arr = self.hh.array().astype(_DT[dtype])
         sizes = spec.sizes
         self.kmax = int(sizes.max())
-        # launch buckets by size, largest trees first inside each bucket
+        # Launch buckets.  Trees of at most 512 sections run one thread per
+        # section (layout 1), packed first-fit-decreasing into blocks of
+        # `block` threads (trees above `block` sections get 512-thread
+        # blocks of their own bucket); larger trees run eight node slots per
+        # thread (layout 0), bucketed by size.
         self.buckets = []
-        lo = 0
-        for hi in bucket_edges:
-            sel = np.flatnonzero((sizes > lo) & (sizes <= hi))
-            lo = hi
+        by_sec = self.n_sec <= 512
+        for lo_s, hi_s, threads in ((0, block, block), (block, 512, 512)):
+            sel = np.flatnonzero(by_sec & (self.n_sec > lo_s) & (self.n_sec <= hi_s))
             if len(sel) == 0:
                 continue
-            sel = sel[np.argsort(-sizes[sel], kind="stable")]
-            kmax = int(sizes[sel].max())
-            threads = int(min(512, max(64, -(-kmax // (slots * 64)) * 64)))
-            self.buckets.append(_TreeBucket(i32(sel), len(sel), kmax, threads))
-        if (sizes > bucket_edges[-1]).any():
+            sel = sel[np.argsort(-self.n_sec[sel], kind="stable")]
+            groups, loads, nodes = [], [], []
+            for t in sel:
+                for gi in range(len(groups)):
+                    if loads[gi] + self.n_sec[t] <= threads:
+                        groups[gi].append(t); loads[gi] += self.n_sec[t]
+                        nodes[gi] += int(sizes[t])
+                        break
+                else:
+                    groups.append([t]); loads.append(int(self.n_sec[t]))
+                    nodes.append(int(sizes[t]))
+            order = np.concatenate([np.asarray(g_) for g_ in groups])
+            goff = np.zeros(len(groups) + 1, dtype=np.int64)
+            goff[1:] = np.cumsum([len(g_) for g_ in groups])
+            self.buckets.append(_TreeBucket(i32(order), len(order), int(max(nodes)),
+                                            threads, 1, i32(goff), len(groups)))
+        lo = 0
+        for hi in bucket_edges:
+            sel = np.flatnonzero(~by_sec & (sizes > lo) & (sizes <= hi))
+            lo = hi
+            if len(sel):
+                sel = sel[np.argsort(-sizes[sel], kind="stable")]
+                kmax = int(sizes[sel].max())
+                threads = int(min(512, max(64, -(-kmax // (slots * 64)) * 64)))
+                self.buckets.append(_TreeBucket(i32(sel), len(sel), kmax, threads, 0,
+                                                None, 0))
+        if (~by_sec & (sizes > bucket_edges[-1])).any():
             raise ValueError(f"tree larger than {bucket_edges[-1]} compartments")
         self.reset()
 
@@ -257,8 +293,11 @@ class TreePopulation:
     def _plan_args(self):
         return (self.spec.n_trees, _lib.ptr(self.node_off), _lib.ptr(self.parent),
                 _lib.ptr(self.ch_ptr), _lib.ptr(self.ch_idx), _lib.ptr(self.sec_off),
-                _lib.ptr(self.sec_head), _lib.ptr(self.sec_tail), _lib.ptr(self.lvl_off),
-                _lib.ptr(self.lvl_ptr))
+                _lib.ptr(self.sec["head"]), _lib.ptr(self.sec["len"]),
+                _lib.ptr(self.sec["child"]), _lib.ptr(self.sec["nchild"]),
+                _lib.ptr(self.sec["ptail"]), _lib.ptr(self.sec["parent"]),
+                _lib.ptr(self.lvl_off),
+                _lib.ptr(self.lvl_ptr), self.smax, self.lmax)
 
     def reset(self, v_init: float | None = None):
         v0 = self.spec.v_init if v_init is None else v_init
@@ -267,6 +306,12 @@ class TreePopulation:
         mk = lambda x: torch.full((n,), float(x), dtype=self.dtype, device=self.device)
         self.v, self.m, self.h, self.n = mk(v0), mk(m0), mk(h0), mk(n0)
         self.t_step = 0
+
+    def _bucket_streams(self):
+        """One default-priority stream per launch bucket, so buckets overlap."""
+        if getattr(self, "_streams", None) is None:
+            self._streams = [torch.cuda.Stream(self.device) for _ in self.buckets]
+        return self._streams
 
     def state(self) -> dict:
         out = {}
@@ -297,7 +342,9 @@ class TreePopulation:
             n_rec = (self.t_step + n_steps) // record_every - self.t_step // record_every
             rec_base = self.t_step // record_every
             rec = torch.empty((n_rec, self.total), dtype=self.dtype, device=self.device)
-        strm = stream if stream is not None else _lib.stream_handle(self.device)
+        # `stream`: a torch.cuda.Stream (None = the current stream)
+        strm = (stream if stream is not None
+                else torch.cuda.current_stream(self.device)).cuda_stream
         if path == "persistent":
             fn = lib.dn_hh_tree_run_f64 if f64 else lib.dn_hh_tree_run_f32
             chunk = steps_per_launch or n_steps
@@ -307,17 +354,27 @@ class TreePopulation:
                 rec_ptr = None
                 if rec is not None:
                     rec_ptr = rec.data_ptr() - rec_base * self.total * rec.element_size()
-                for b in self.buckets:
+                caller = stream if stream is not None else torch.cuda.current_stream(
+                    self.device)
+                streams = self._bucket_streams()
+                start = torch.cuda.Event()
+                start.record(caller)
+                for b, bs in zip(self.buckets, streams):
+                    bs.wait_event(start)
                     st = fn(
-                        *self._plan_args(), _lib.ptr(b.order), b.n, b.kmax, b.threads,
+                        *self._plan_args(), _lib.ptr(b.order), b.n, _lib.ptr(b.grp_off),
+                        b.n_groups, b.layout, b.kmax, b.threads,
                         _lib.ptr(self.v), _lib.ptr(self.m), _lib.ptr(self.h),
                         _lib.ptr(self.n), _lib.ptr(self.area), _lib.ptr(self.cm_dt),
                         _lib.ptr(self.g), _lib.ptr(self.drive), _lib.ptr(self.sna),
                         _lib.ptr(self.sk), _lib.ptr(wave_t), self.t_step, k, self.dt,
                         self.hh_arr.ctypes.data, rec_ptr, max(record_every, 1),
-                        self.total, strm,
+                        self.total, bs.cuda_stream,
                     )
                     _lib.check(st, "persistent tree run")
+                    done_ev = torch.cuda.Event()
+                    done_ev.record(bs)
+                    caller.wait_event(done_ev)
                 self.t_step += k
                 done += k
         elif path == "plain":

@@ -77,75 +77,103 @@ def build_dhs_layers(depth, k_threads: int):
     return np.asarray(order, dtype=np.int32), np.asarray(layer_ptr, dtype=np.int32)
 
 
+SEC_MAX = 16  # longest section the fused tree kernels hold in registers
+
+
 @dataclass
 class SectionPlan:
     """Section-chain numbering of one tree.
 
-    perm[new] = old node index; inverse inv[old] = new.  Sections are
-    contiguous head..tail in the new numbering and listed level by level.
+    perm[new] = old node index; inv[old] = new.  A section is a chain of at
+    most ``SEC_MAX`` nodes in which every node but the last has exactly one
+    child; sections are contiguous head..tail in the new numbering, listed
+    level by level (breadth-first over the section tree), and the child
+    sections of a section are contiguous.
     """
 
     perm: np.ndarray
     inv: np.ndarray
-    parent: np.ndarray  # new numbering, -1 at the root
-    ch_ptr: np.ndarray  # CSR over new numbering
+    parent: np.ndarray      # new numbering, -1 at the root
+    ch_ptr: np.ndarray      # CSR over new numbering
     ch_idx: np.ndarray
     sec_head: np.ndarray
-    sec_tail: np.ndarray
-    lvl_ptr: np.ndarray  # section offsets per level
+    sec_len: np.ndarray
+    sec_child: np.ndarray   # first child section
+    sec_nchild: np.ndarray
+    sec_ptail: np.ndarray   # parent node of the head (new numbering), -1 at root
+    sec_parent: np.ndarray  # parent section, -1 at the root
+    lvl_ptr: np.ndarray     # section offsets per level
 
     @property
     def n_levels(self) -> int:
         return len(self.lvl_ptr) - 1
 
+    @property
+    def sec_tail(self) -> np.ndarray:
+        return self.sec_head + self.sec_len - 1
 
-def section_plan(parent_idx) -> SectionPlan:
+
+def section_plan(parent_idx, sec_max: int = SEC_MAX) -> SectionPlan:
     parent, children, _ = build_morphology(parent_idx)
     K = len(parent)
     root = int(np.flatnonzero(parent == -1)[0])
     nchild = np.array([len(c) for c in children])
-    # heads: root, or a node whose parent does not have exactly one child
-    sections = []  # (level, head_old, [nodes...])
-    q = deque([(root, 0)])
+    sections = []   # (level, [old nodes], parent section)
+    q = deque([(root, 0, -1)])
     while q:
-        head, lvl = q.popleft()
+        head, lvl, psec = q.popleft()
         nodes = [head]
         cur = head
-        while nchild[cur] == 1:
+        while nchild[cur] == 1 and len(nodes) < sec_max:
             cur = children[cur][0]
             nodes.append(cur)
-        sections.append((lvl, nodes))
-        for c in children[cur]:
-            q.append((c, lvl + 1))
-    # BFS already visits sections level by level
-    perm = np.concatenate([np.asarray(nodes, dtype=np.int64) for _, nodes in sections])
+        me = len(sections)
+        sections.append((lvl, nodes, psec))
+        if nchild[cur] == 1:  # split chain: the continuation is the only child
+            q.append((children[cur][0], lvl + 1, me))
+        else:
+            for c in children[cur]:
+                q.append((c, lvl + 1, me))
+    perm = np.concatenate([np.asarray(nodes, dtype=np.int64) for _, nodes, _ in sections])
     inv = np.empty(K, dtype=np.int64)
     inv[perm] = np.arange(K)
     new_parent = np.where(parent[perm] >= 0, inv[np.maximum(parent[perm], 0)], -1)
-    sec_head, sec_tail, levels = [], [], []
+    n_sec = len(sections)
+    sec_head = np.zeros(n_sec, dtype=np.int64)
+    sec_len = np.zeros(n_sec, dtype=np.int64)
+    sec_child = np.zeros(n_sec, dtype=np.int64)
+    sec_nchild = np.zeros(n_sec, dtype=np.int64)
+    sec_ptail = np.full(n_sec, -1, dtype=np.int64)
+    sec_parent = np.full(n_sec, -1, dtype=np.int64)
+    levels = np.zeros(n_sec, dtype=np.int64)
     pos = 0
-    for lvl, nodes in sections:
-        sec_head.append(pos)
-        sec_tail.append(pos + len(nodes) - 1)
-        levels.append(lvl)
+    for s_i, (lvl, nodes, psec) in enumerate(sections):
+        sec_head[s_i] = pos
+        sec_len[s_i] = len(nodes)
+        levels[s_i] = lvl
         pos += len(nodes)
-    n_lvl = max(levels) + 1
-    lvl_ptr = np.searchsorted(np.asarray(levels), np.arange(n_lvl + 1), side="left")
-    ch_lists = [[] for _ in range(K)]
-    for new_i in range(K):
-        p = new_parent[new_i]
-        if p >= 0:
-            ch_lists[p].append(new_i)
+    for s_i, (_, _, psec) in enumerate(sections):
+        if psec >= 0:
+            if sec_nchild[psec] == 0:
+                sec_child[psec] = s_i
+            sec_nchild[psec] += 1
+            sec_ptail[s_i] = sec_head[psec] + sec_len[psec] - 1
+            sec_parent[s_i] = psec
+    for s_i in range(n_sec):  # childless sections point past themselves
+        if sec_nchild[s_i] == 0:
+            sec_child[s_i] = 0
+    assert (np.diff(levels) >= 0).all()
+    n_lvl = int(levels.max()) + 1
+    lvl_ptr = np.searchsorted(levels, np.arange(n_lvl + 1), side="left")
+    order = np.argsort(new_parent, kind="stable")
+    order = order[new_parent[order] >= 0]
+    counts = np.bincount(new_parent[order], minlength=K)
     ch_ptr = np.zeros(K + 1, dtype=np.int64)
-    ch_ptr[1:] = np.cumsum([len(c) for c in ch_lists])
-    ch_idx = np.asarray([c for lst in ch_lists for c in sorted(lst)], dtype=np.int64)
+    ch_ptr[1:] = np.cumsum(counts)
     return SectionPlan(
-        perm=perm,
-        inv=inv,
-        parent=new_parent.astype(np.int64),
-        ch_ptr=ch_ptr,
-        ch_idx=ch_idx,
-        sec_head=np.asarray(sec_head, dtype=np.int64),
-        sec_tail=np.asarray(sec_tail, dtype=np.int64),
+        perm=perm, inv=inv, parent=new_parent.astype(np.int64), ch_ptr=ch_ptr,
+        ch_idx=order.astype(np.int64), sec_head=sec_head, sec_len=sec_len,
+        sec_child=sec_child, sec_nchild=sec_nchild, sec_ptail=sec_ptail,
+        sec_parent=sec_parent,
         lvl_ptr=lvl_ptr.astype(np.int64),
     )

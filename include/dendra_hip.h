@@ -111,18 +111,28 @@ int dn_hh_cable_step_f64(int64_t n_cells, const int64_t* off, int64_t total, int
  * Replaces the per-step transition of _dhs (integrators/tree.py:1202-1357)
  * and the packed multi-tree path _dhs_multi / dhs_solve_multi_cuda
  * (tree.py:2097-2334, triton/dhs_kernel_multi.py:16-135,831-888).  Trees in
- * section-chain numbering: parent/ch_ptr/ch_idx/sec_head/sec_tail local to
- * each tree; lvl_ptr holds per-tree section offsets per level.  Per node:
+ * section-chain numbering (sections of <= 16 nodes, contiguous, breadth-first
+ * over the section tree): parent/ch_ptr/ch_idx and the section arrays
+ * (head, len, first child section, child count, parent node of the head) are
+ * local to each tree; lvl_ptr holds per-tree section offsets per level;
+ * smax / lmax bound every tree's section and level counts.  Per node:
  * area, cmdt = C/dt_s, g = child-indexed axial conductance (0 at the root),
  * drive (mA per unit waveform, nullable).  The persistent run launches the
  * `n_launch` trees listed in `order` with one block of `threads` threads per
- * tree and requires kmax >= each tree's size, kmax <= 8 * threads. */
+ * tree; layout 1 (one thread per section, state in registers) takes kmax =
+ * the launch's largest section count <= threads <= 512, layout 0 (eight node
+ * slots per thread) requires
+ * kmax >= each tree's size and kmax <= 8 * threads. */
 int dn_hh_tree_slots(void);
 int dn_hh_tree_run_f32(int n_trees, const int64_t* node_off, const int32_t* parent,
                        const int32_t* ch_ptr, const int32_t* ch_idx,
                        const int32_t* sec_off, const int32_t* sec_head,
-                       const int32_t* sec_tail, const int32_t* lvl_off,
-                       const int32_t* lvl_ptr, const int32_t* order, int n_launch,
+                       const int32_t* sec_len, const int32_t* sec_child,
+                       const int32_t* sec_nchild, const int32_t* sec_ptail,
+                       const int32_t* sec_parent, const int32_t* lvl_off,
+                       const int32_t* lvl_ptr, int smax, int lmax,
+                       const int32_t* order, int n_launch, const int32_t* grp_off,
+                       int n_groups, int layout,
                        int kmax, int threads, float* v, float* m, float* h, float* n,
                        const float* area, const float* cmdt, const float* g,
                        const float* drive, const float* sna, const float* sk,
@@ -132,8 +142,12 @@ int dn_hh_tree_run_f32(int n_trees, const int64_t* node_off, const int32_t* pare
 int dn_hh_tree_run_f64(int n_trees, const int64_t* node_off, const int32_t* parent,
                        const int32_t* ch_ptr, const int32_t* ch_idx,
                        const int32_t* sec_off, const int32_t* sec_head,
-                       const int32_t* sec_tail, const int32_t* lvl_off,
-                       const int32_t* lvl_ptr, const int32_t* order, int n_launch,
+                       const int32_t* sec_len, const int32_t* sec_child,
+                       const int32_t* sec_nchild, const int32_t* sec_ptail,
+                       const int32_t* sec_parent, const int32_t* lvl_off,
+                       const int32_t* lvl_ptr, int smax, int lmax,
+                       const int32_t* order, int n_launch, const int32_t* grp_off,
+                       int n_groups, int layout,
                        int kmax, int threads, double* v, double* m, double* h, double* n,
                        const double* area, const double* cmdt, const double* g,
                        const double* drive, const double* sna, const double* sk,
@@ -143,8 +157,11 @@ int dn_hh_tree_run_f64(int n_trees, const int64_t* node_off, const int32_t* pare
 int dn_hh_tree_step_f32(int n_trees, const int64_t* node_off, const int32_t* parent,
                         const int32_t* ch_ptr, const int32_t* ch_idx,
                         const int32_t* sec_off, const int32_t* sec_head,
-                        const int32_t* sec_tail, const int32_t* lvl_off,
-                        const int32_t* lvl_ptr, const int32_t* node_tree, int kmax,
+                        const int32_t* sec_len, const int32_t* sec_child,
+                        const int32_t* sec_nchild, const int32_t* sec_ptail,
+                        const int32_t* sec_parent, const int32_t* lvl_off,
+                        const int32_t* lvl_ptr, int smax, int lmax,
+                        const int32_t* node_tree, int kmax,
                         int64_t total, float* v, float* m, float* h, float* n,
                         const float* area, const float* cmdt, const float* g,
                         const float* drive, const float* sna, const float* sk, float w,
@@ -153,8 +170,11 @@ int dn_hh_tree_step_f32(int n_trees, const int64_t* node_off, const int32_t* par
 int dn_hh_tree_step_f64(int n_trees, const int64_t* node_off, const int32_t* parent,
                         const int32_t* ch_ptr, const int32_t* ch_idx,
                         const int32_t* sec_off, const int32_t* sec_head,
-                        const int32_t* sec_tail, const int32_t* lvl_off,
-                        const int32_t* lvl_ptr, const int32_t* node_tree, int kmax,
+                        const int32_t* sec_len, const int32_t* sec_child,
+                        const int32_t* sec_nchild, const int32_t* sec_ptail,
+                        const int32_t* sec_parent, const int32_t* lvl_off,
+                        const int32_t* lvl_ptr, int smax, int lmax,
+                        const int32_t* node_tree, int kmax,
                         int64_t total, double* v, double* m, double* h, double* n,
                         const double* area, const double* cmdt, const double* g,
                         const double* drive, const double* sna, const double* sk,

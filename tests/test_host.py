@@ -1,0 +1,219 @@
+"""Host-side logic: topology preprocessing, plans, workspaces, RNG, C ABI.
+
+CPU only.  The library checks load libdnhip.so and compare its exported
+symbols with include/dendra_hip.h; no kernel is called.
+"""
+
+import ctypes
+import re
+from pathlib import Path
+
+import networkx as nx
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_header_symbols_are_exported_and_bound():
+    from dendra_amd import _lib
+
+    hdr = (ROOT / "include" / "dendra_hip.h").read_text()
+    declared = set(re.findall(r"\b(dn_[a-z0-9_]+)\s*\(", hdr))
+    assert declared, "no declarations parsed"
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    for name in sorted(declared):
+        assert hasattr(lib, name), f"{name} declared but not exported"
+    assert declared == set(_lib.exported_symbols()), "ctypes binding out of sync"
+
+
+def test_library_load_fails_loudly_when_missing(monkeypatch, tmp_path):
+    from dendra_amd import _lib
+
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "missing.so")
+    with pytest.raises(RuntimeError, match="not built"):
+        _lib.load()
+
+
+def test_build_dhs_layers_matches_reference_semantics():
+    from dendra_amd.topology import build_dhs_layers, build_morphology
+
+    parent = [-1, 0, 0, 1, 1, 2, 5]
+    _, children, depth = build_morphology(parent)
+    assert depth.tolist() == [0, 1, 1, 2, 2, 2, 3]
+    order, layer_ptr = build_dhs_layers(depth, 2)
+    # deepest first, chunks of at most 2 nodes, ascending node order per depth
+    assert order.tolist() == [6, 3, 4, 5, 1, 2, 0]
+    assert layer_ptr.tolist() == [0, 1, 3, 4, 6, 7]
+    pos = {n: i for i, n in enumerate(order.tolist())}
+    for i, p in enumerate(parent):
+        if p >= 0:
+            assert pos[i] < pos[p], "children eliminated before parents"
+
+
+@pytest.mark.parametrize("bad,msg", [
+    ([], "at least one"), ([-1, -1], "exactly one root"), ([-1, 5], "must be in"),
+    ([-1, 1], "own parent"), ([2, 2, -1, -1], "exactly one root"),
+])
+def test_build_morphology_rejects_invalid(bad, msg):
+    from dendra_amd.topology import build_morphology
+
+    with pytest.raises(ValueError, match=msg):
+        build_morphology(bad)
+
+
+def test_build_morphology_rejects_cycles():
+    from dendra_amd.topology import build_morphology
+
+    with pytest.raises(ValueError, match="cycle|unreachable"):
+        build_morphology([-1, 2, 1])
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_section_plan_invariants(seed):
+    from dendra_amd.topology import SEC_MAX, section_plan
+    from dendra_amd.workloads import random_binary_tree
+
+    parent, _, _ = random_binary_tree(1500, seed, 17)
+    sp = section_plan(parent)
+    K = len(parent)
+    assert sorted(sp.perm.tolist()) == list(range(K))
+    # topology preserved under renumbering
+    for new_i, old_i in enumerate(sp.perm):
+        p_old = parent[old_i]
+        p_new = sp.parent[new_i]
+        assert (p_old < 0 and p_new < 0) or sp.perm[p_new] == p_old
+    G = nx.DiGraph([(int(sp.parent[i]), i) for i in range(K) if sp.parent[i] >= 0])
+    assert nx.is_arborescence(G)
+    # sections: contiguous chains, <= SEC_MAX, internal nodes have one child
+    nch = np.bincount(sp.parent[sp.parent >= 0], minlength=K)
+    covered = np.zeros(K, dtype=int)
+    for s, (h, ln) in enumerate(zip(sp.sec_head, sp.sec_len)):
+        assert 1 <= ln <= SEC_MAX
+        covered[h:h + ln] += 1
+        for k in range(h + 1, h + ln):
+            assert sp.parent[k] == k - 1
+        for k in range(h, h + ln - 1):
+            assert nch[k] == 1
+        t = h + ln - 1
+        kids = np.flatnonzero(sp.parent == t)
+        heads = sp.sec_head[sp.sec_child[s]:sp.sec_child[s] + sp.sec_nchild[s]]
+        assert sorted(kids.tolist()) == sorted(heads.tolist())
+        assert sp.sec_ptail[s] == sp.parent[h]
+    assert (covered == 1).all()
+    # level order: a section's parent section lies on the previous level
+    lvl = np.repeat(np.arange(sp.n_levels), np.diff(sp.lvl_ptr))
+    for s in range(len(sp.sec_head)):
+        for c in range(sp.sec_child[s], sp.sec_child[s] + sp.sec_nchild[s]):
+            assert lvl[c] == lvl[s] + 1
+    # CSR children
+    for i in range(K):
+        kids = sp.ch_idx[sp.ch_ptr[i]:sp.ch_ptr[i + 1]]
+        assert (sp.parent[kids] == i).all() and len(kids) == nch[i]
+
+
+def test_long_chain_is_split():
+    from dendra_amd.topology import SEC_MAX, section_plan
+
+    K = 5 * SEC_MAX + 3
+    sp = section_plan([-1] + list(range(K - 1)))
+    assert sp.n_levels == 6 and sp.sec_len.tolist() == [SEC_MAX] * 5 + [3]
+
+
+def test_cable_plan_covers_every_cell_once():
+    from dendra_amd.model import plan_cables
+
+    sizes = np.array([1, 2, 3, 21, 52, 53, 64, 65, 104, 208, 209, 416, 417, 801, 832, 7])
+    pl = plan_cables(sizes, rmax=13)
+    seen = np.zeros(len(sizes), dtype=int)
+    for c0, nc, nr, p in zip(pl.task_cell0, pl.task_ncell, pl.task_nr, pl.task_p):
+        assert 1 <= nc <= 64 // p and 1 <= nr <= 13
+        for k in range(nc):
+            cell = pl.order[c0 + k]
+            seen[cell] += 1
+            assert sizes[cell] <= p * nr and pl.cell_p[c0 + k] == p
+    assert (seen == 1).all()
+    assert (np.diff(pl.off) == sizes[pl.order]).all()
+    with pytest.raises(ValueError):
+        plan_cables(np.array([833]), rmax=13)
+
+
+def test_cable_workspace_matches_dense_operator():
+    """gL/gRp/scale/drive against the cable equation built independently."""
+    from dendra_amd.model import CableSpec, cable_workspace
+
+    K = 7
+    rs = np.random.default_rng(0)
+    d = 1 + rs.random(K)
+    dx = 5 + 10 * rs.random(K)
+    ra = 70 + 60 * rs.random(K)
+    cm = 0.9 + 0.2 * rs.random(K)
+    lead = rs.standard_normal(K) * 100
+    spec = CableSpec(np.array([0, K]), d, dx, ra, cm, lead=lead, ve_amp=np.array([0.7]))
+    ws = cable_workspace(spec)
+    area = np.pi * (d * 1e-4) * (dx * 1e-4)
+    C = 1e-6 * cm * area
+    r = d * 1e-4 / 2
+    half = ra * (dx * 1e-4 / 2) / (np.pi * r * r)
+    g = 1 / (half[:-1] + half[1:])  # series half-compartment resistances
+    A = np.zeros((K, K))
+    for j in range(K - 1):
+        A[j, j] -= g[j]; A[j, j + 1] += g[j]
+        A[j + 1, j + 1] -= g[j]; A[j + 1, j] += g[j]
+    A = A / C[:, None]
+    np.testing.assert_allclose(ws.gL[:-1], np.diag(A, 1), rtol=1e-13)
+    np.testing.assert_allclose(ws.gRp[1:], np.diag(A, -1), rtol=1e-13)
+    assert ws.gL[-1] == 0 and ws.gRp[0] == 0
+    np.testing.assert_allclose(ws.scale, area / C, rtol=1e-13)
+    np.testing.assert_allclose(ws.drive, A @ (0.7 * lead), rtol=1e-10, atol=1e-9)
+
+
+def test_tree_workspace_drive_is_edge_current_divergence():
+    from dendra_amd.model import TreeSpec, tree_workspace
+
+    parent = np.array([-1, 0, 0, 1, 1, 2, 5])
+    K = len(parent)
+    rs = np.random.default_rng(1)
+    spec = TreeSpec(np.array([0, K]), parent, 1 + rs.random(K), 10 + rs.random(K),
+                    np.full(K, 100.0), np.ones(K), lead=rs.standard_normal(K),
+                    ve_amp=np.array([2.0]))
+    ws = tree_workspace(spec, 0.01)
+    L = np.zeros((K, K))
+    for c, p in enumerate(parent):
+        if p >= 0:
+            g = ws.g[c]
+            L[c, c] -= g; L[c, p] += g; L[p, p] -= g; L[p, c] += g
+    np.testing.assert_allclose(ws.drive, L @ (2.0 * spec.lead), rtol=1e-12, atol=1e-15)
+    assert ws.g[0] == 0
+    np.testing.assert_allclose(ws.cm_dt, 1e-6 * np.pi * spec.diam * spec.length * 1e-8 / 1e-5)
+
+
+def test_counter_rng_is_deterministic_and_uniform():
+    from dendra_amd import rng
+
+    a = rng.uniform_n(5, 1, 100000)
+    b = rng.uniform(5, 1, np.arange(100000))
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a, rng.uniform_n(5, 2, 100000))
+    assert 0.0 <= a.min() and a.max() < 1.0
+    assert abs(a.mean() - 0.5) < 0.005 and abs(a.var() - 1 / 12) < 0.002
+    z = rng.normal_n(3, 4, 200000)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+    k = rng.integers_n(1, 2, 50000, 21, 801)
+    assert k.min() == 21 and k.max() == 801
+
+
+def test_rect_wave_is_half_open():
+    from dendra_amd.model import rect_wave
+
+    w = rect_wave(200, 0.005, delay=0.5, pw=0.1, dtype=np.float32)
+    on = np.flatnonzero(w)
+    assert on[0] == 100 and len(on) == 20
+
+
+def test_hh_steady_state_matches_oracle(orc):
+    from dendra_amd.model import hh_steady_state
+
+    for v in (-80.0, -65.0, -55.0, -40.0, -40.000001, 0.0, 30.0):
+        np.testing.assert_allclose(hh_steady_state(v), orc.hh_steady(v), rtol=1e-14)
