@@ -33,7 +33,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CS = 48.0
+BYTES_PER_CS = 48.0       # float32 cable compartment-step
+BYTES_PER_CS_TREE = 52.0  # float32 tree compartment-step (+ int32 parent index)
 
 
 def _parse():
@@ -50,6 +51,9 @@ def _parse():
     ap.add_argument("--cpu-steps", type=int, default=800)
     ap.add_argument("--waves-per-block", type=int, default=4)
     ap.add_argument("--only", choices=["all", "cables", "trees"], default="all")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="MiB copied once by a plain device copy before the timed region "
+                         "(reference kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -104,6 +108,13 @@ def main():
                 e = torch.cuda.Event(enable_timing=True); e.record(main_stream); ev["c1"].append(e)
         main_stream.wait_stream(side)
 
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         one_run()
     torch.cuda.synchronize()
@@ -145,7 +156,7 @@ def main():
         ach = c_bytes / c_avg / 1e9
         out["roofline"] = {
             "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": ach / HBM_PEAK_GBS, "traffic": None,
+            "frac": ach / HBM_PEAK_GBS, "traffic": _measured_traffic("hh_cable_run_kernel"),
             "kernel": "hh_cable_run_kernel<float>",
             "kernel_ms": c_avg * 1e3,
             "algorithmic_bytes_per_launch": c_bytes,
@@ -155,6 +166,12 @@ def main():
         t_avg = float(np.mean(t_ms)) * 1e-3
         out["tree_kernel_ms"] = t_avg * 1e3
         out["tree_cs_per_s"] = work.trees.total * work.n_steps / t_avg
+        t_bytes = BYTES_PER_CS_TREE * work.trees.total * work.n_steps
+        out["tree_roofline"] = {"achieved": t_bytes / t_avg / 1e9, "unit": "GB/s",
+                                "frac": t_bytes / t_avg / 1e9 / HBM_PEAK_GBS,
+                                "kernel": "hh_tree_sec_kernel<float>"}
+        if c_ms and t_avg > c_avg:
+            out["roofline_note"] = "tree kernel is the longer of the two concurrent kernels"
     if args.plain > 0:
         torch.cuda.synchronize()
         n_pl = work.n_steps
@@ -177,6 +194,16 @@ def main():
         out["cpu_baseline"] = cpu_baseline(work, hh, args.cpu_sample, args.cpu_steps)
     print(json.dumps(out))
     return 0
+
+
+def _measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy), or None."""
+    path = ROOT / "profiles" / "hh_population_uneven_traffic.json"
+    try:
+        return float(json.loads(path.read_text())["traffic_bytes_per_launch"][kernel])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(work, hh, frac, n_steps):
