@@ -78,6 +78,18 @@ _SIGNATURES = {
 }
 
 
+u32 = C.c_uint32
+u64 = C.c_uint64
+_SIGNATURES["dn_net_run_f32"] = (
+    i32,
+    [i32, i32] + [p] * 12 + [f32, p, p, p, p, i32, p, f32, u64, u32, f64]
+    + [p] * 6 + [i32] + [p] * 6 + [p, p, i64, p, i64, i64, i32, p],
+)
+_SIGNATURES["dn_net_deliver_tape_f32"] = (
+    i32, [i32, i32] + [p] * 6 + [i32] + [p] * 8 + [i64, p],
+)
+
+
 def _bind(lib) -> None:
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name, None)

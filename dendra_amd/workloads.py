@@ -168,3 +168,57 @@ def uneven_population(n_cables: int = 8192, n_trees: int = 1024, n_steps: int = 
     trees = random_trees(n_trees, seed=seed + 1)
     return UnevenPopulation(cables, trees, dt, n_steps,
                             extracellular_wave(n_steps, dt, np.float32))
+
+
+S_PRE, S_W, S_DELAY = 21, 22, 23
+
+
+def sparse_network(n_point: int = 20000, n_bs: int = 1000, fan_in: int = 200,
+                   dt: float = 0.025, seed: int = 11, w_bg: float = 0.01,
+                   delay_ms=(1.0, 5.0)):
+    """Recurrent HH network of the network-delivery benchmark.
+
+    Point cells: one HH compartment, 20 um x 20 um.  Ball-and-stick cells: an
+    HH soma (20 um x 20 um) and a passive 100-compartment dendrite of 1 um x
+    500 um (5 um compartments; gNa = gK = 0).  Ra 100 ohm cm, cm 1 uF/cm^2.
+    Each cell gets `fan_in` sources uniform over all cells, exp2syn weights
+    lognormal (median 1 nS, sigma 0.5) and delays that are integer multiples of
+    dt uniform in [1, 5] ms, plus one Poisson background source of weight
+    w_bg uS and a one-step delay.  Connections are numbered target-major.
+    """
+    from .model import CableSpec
+    from .network import Connectivity, NetworkSpec, delay_steps
+
+    n_cells = n_point + n_bs
+    sizes = np.concatenate([np.ones(n_point, np.int64), np.full(n_bs, 101, np.int64)])
+    off = np.zeros(n_cells + 1, dtype=np.int64)
+    off[1:] = np.cumsum(sizes)
+    total = int(off[-1])
+    diam = np.full(total, 20.0)
+    dx = np.full(total, 20.0)
+    sna = np.ones(total)
+    sk = np.ones(total)
+    for c in range(n_point, n_cells):
+        o = off[c]
+        diam[o + 1:o + 101] = 1.0
+        dx[o + 1:o + 101] = 5.0
+        sna[o + 1:o + 101] = 0.0
+        sk[o + 1:o + 101] = 0.0
+    cells = CableSpec(off, diam, dx, np.full(total, 100.0), np.ones(total), sna=sna,
+                      sk=sk, v_init=-65.0)
+    n_rec = n_cells * fan_in
+    pre_rec = rng.integers_n(seed, S_PRE, n_rec, 0, n_cells - 1).reshape(n_cells, fan_in)
+    w_rec = np.exp(np.log(1e-3) + 0.5 * rng.normal_n(seed, S_W, n_rec)).reshape(n_cells,
+                                                                               fan_in)
+    lo, hi = (int(round(x / dt)) for x in delay_ms)
+    d_rec = delay_steps(rng.integers_n(seed, S_DELAY, n_rec, lo, hi) * dt, dt)
+    d_rec = d_rec.reshape(n_cells, fan_in)
+    pre = np.concatenate([pre_rec, n_cells + np.arange(n_cells)[:, None]], axis=1)
+    w = np.concatenate([w_rec, np.full((n_cells, 1), w_bg)], axis=1)
+    dly = np.concatenate([d_rec, np.ones((n_cells, 1), np.int64)], axis=1)
+    post = np.repeat(np.arange(n_cells), fan_in + 1)
+    conn = Connectivity(pre.ravel().astype(np.int32), post.astype(np.int32),
+                        dly.ravel().astype(np.int32), w.ravel().astype(np.float32),
+                        n_src=2 * n_cells, n_tgt=n_cells)
+    return NetworkSpec(cells=cells, n_point=n_point, soma=off[:-1].copy(), conn=conn,
+                       seed=seed)

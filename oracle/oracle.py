@@ -222,3 +222,62 @@ def ring_replay(n_src, n_syn, pre, post, delay, w, depth, spikes, ring=None, cur
 
 def cpu_threads() -> int:
     return len(os.sched_getaffinity(0))
+
+
+def ring_replay_gates(n_syn, post, delay, w, depth, gates, ring=None, cur=0):
+    lib = load()
+    lib.orc_ring_replay_gates_f32.argtypes = [C.c_int64, C.c_int64, ip, ip, fp, C.c_int64,
+                                              C.c_int64, fp, fp, fp, C.POINTER(C.c_int64)]
+    lib.orc_ring_replay_gates_f32.restype = C.c_int
+    gates = _c(gates, np.float32)
+    n_steps = gates.shape[0]
+    delivered = np.zeros((n_steps, n_syn), dtype=np.float32)
+    ring = np.zeros((depth, n_syn), np.float32) if ring is None else _c(ring, np.float32)
+    c = C.c_int64(cur)
+    st = lib.orc_ring_replay_gates_f32(n_syn, len(post), _c(post, np.int32),
+                                       _c(delay, np.int32), _c(w, np.float32), depth,
+                                       n_steps, gates, delivered, ring, C.byref(c))
+    if st != 0:
+        raise ValueError("invalid ring replay arguments")
+    return delivered, ring, c.value
+
+
+def uniform(seed, stream, i):
+    lib = load()
+    lib.orc_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+    lib.orc_uniform.restype = C.c_double
+    return lib.orc_uniform(seed, stream, i)
+
+
+def network_run(spec, hh_arr, dt, n_steps, record_spikes=True):
+    """Binary64 network simulation; returns (state dict, spikes [n_steps][n_src])."""
+    lib = load()
+    i64, f64 = C.c_int64, C.c_double
+    lib.orc_network_run.argtypes = [i64, lp, lp, dp, dp, dp, dp, vp, vp, dp, dp, f64, i64,
+                                    C.c_uint64, C.c_uint32, f64, i64, ip, ip, ip, dp, i64,
+                                    f64, i64, dp, dp, dp, dp, dp, dp, bp, vp]
+    lib.orc_network_run.restype = C.c_int
+    cells, conn = spec.cells, spec.conn
+    total, n_cells = cells.total, cells.n_cells
+    m0, h0, n0 = hh_steady(cells.v_init)
+    V = np.full(total, float(cells.v_init))
+    M, H, N = np.full(total, m0), np.full(total, h0), np.full(total, n0)
+    A, B = np.zeros(n_cells), np.zeros(n_cells)
+    latch = np.zeros(n_cells, dtype=np.uint8)
+    sna, sna_p = _opt(cells.sna)
+    sk, sk_p = _opt(cells.sk)
+    ratio = min(max(spec.tau1 / spec.tau2, 1e-9), 0.9999)
+    factor = exp2syn_factor(spec.tau1, spec.tau2)
+    syn = np.array([factor, ratio * spec.tau2, spec.tau2, spec.e_syn])
+    spikes = np.zeros((n_steps, conn.n_src), dtype=np.uint8) if record_spikes else None
+    st = lib.orc_network_run(
+        n_cells, _c(cells.off, np.int64), _c(spec.soma, np.int64), _c(cells.diam),
+        _c(cells.dx), _c(cells.rhoa), _c(cells.cm), sna_p, sk_p, _c(hh_arr), syn,
+        float(spec.theta), conn.n_src, spec.seed, spec.bg_stream,
+        spec.bg_rate_hz * dt * 1e-3, len(conn.pre), _c(conn.pre, np.int32),
+        _c(conn.post, np.int32), _c(conn.delay, np.int32), _c(conn.w), conn.depth,
+        float(dt), int(n_steps), V, M, H, N, A, B, latch,
+        None if spikes is None else spikes.ctypes.data_as(vp))
+    if st != 0:
+        raise RuntimeError(f"oracle network run failed ({st})")
+    return {"v": V, "m": M, "h": H, "n": N, "A": A, "B": B}, spikes
