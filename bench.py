@@ -51,6 +51,8 @@ def _parse():
     ap.add_argument("--cpu-steps", type=int, default=800)
     ap.add_argument("--waves-per-block", type=int, default=4)
     ap.add_argument("--only", choices=["all", "cables", "trees"], default="all")
+    ap.add_argument("--network-steps", type=int, default=4000,
+                    help="time steps of the network workload line (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB copied once by a plain device copy before the timed region "
                          "(reference kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -192,8 +194,54 @@ def main():
                         "speedup_persistent_over_plain": plain_wall / wall}
     if args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(work, hh, args.cpu_sample, args.cpu_steps)
+    if args.network_steps > 0:
+        out["network"] = network_bench(args.network_steps, hh, dev, args.cpu_sample > 0)
     print(json.dumps(out))
     return 0
+
+
+def network_bench(n_steps, hh, dev, with_cpu):
+    """Secondary line: the event-driven network (20,000 point cells + 1,000
+    ball-and-stick cells, 200 sources per cell, exp2syn, Poisson background),
+    dt 0.025 ms, steps replayed as HIP graphs of 100 steps."""
+    import torch
+
+    from dendra_amd.network import Network
+    from dendra_amd.workloads import sparse_network
+
+    spec = sparse_network()
+    net = Network(spec, 0.025, hh, dev)
+    net.run(200, graph_steps=100)
+    torch.cuda.synchronize()
+    net.reset()
+    t = time.perf_counter()
+    events, _ = net.run(n_steps, record_events=True, graph_steps=100,
+                        ev_cap=n_steps * 4096)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    out_deg = np.bincount(spec.conn.pre, minlength=spec.conn.n_src)
+    deliveries = int(out_deg[events[:, 1]].sum())
+    res = {
+        "workload": "20000 point + 1000 ball-and-stick HH cells, 4.2M connections, "
+                    "exp2syn, 5 Hz Poisson background, dt 0.025 ms",
+        "time_steps": n_steps, "wall_s": wall, "steps_per_s": n_steps / wall,
+        "spikes": int(len(events)), "deliveries": deliveries,
+        "deliveries_per_s": deliveries / wall,
+        "compartment_steps_per_s": spec.cells.total * n_steps / wall,
+    }
+    if with_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle
+
+        cpu_steps = 100
+        t = time.perf_counter()
+        _, sp = oracle.network_run(spec, hh.array(), 0.025, cpu_steps)
+        dt_cpu = time.perf_counter() - t
+        res["cpu_baseline"] = {"steps_per_s": cpu_steps / dt_cpu, "cores": 1,
+                               "kind": "port",
+                               "sample": f"first {cpu_steps} steps of the same network, "
+                                         f"float64 C oracle, 1 thread, {dt_cpu:.1f}s"}
+    return res
 
 
 def _measured_traffic(kernel):
