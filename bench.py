@@ -53,6 +53,8 @@ def _parse():
     ap.add_argument("--only", choices=["all", "cables", "trees"], default="all")
     ap.add_argument("--network-steps", type=int, default=4000,
                     help="time steps of the network workload line (0 = skip)")
+    ap.add_argument("--fit", type=int, default=1,
+                    help="also time the adjoint workload (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB copied once by a plain device copy before the timed region "
                          "(reference kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -196,6 +198,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(work, hh, args.cpu_sample, args.cpu_steps)
     if args.network_steps > 0:
         out["network"] = network_bench(args.network_steps, hh, dev, args.cpu_sample > 0)
+    if args.fit:
+        out["adjoint"] = fit_bench(dev, args.cpu_sample > 0)
     print(json.dumps(out))
     return 0
 
@@ -241,6 +245,84 @@ def network_bench(n_steps, hh, dev, with_cpu):
                                "kind": "port",
                                "sample": f"first {cpu_steps} steps of the same network, "
                                          f"float64 C oracle, 1 thread, {dt_cpu:.1f}s"}
+    return res
+
+
+def fit_bench(dev, with_cpu, reps=3):
+    """Secondary line: loss and hand-written adjoint of 512 HH cables x 201
+    compartments, 10,000 steps of 0.01 ms in float32, gradients w.r.t. the
+    per-compartment gNa/gK scales and the 201 x 64 Ve knot table, state
+    checkpoints every 500 steps.  Algorithmic bytes: 56 per compartment-step
+    forward, 144 per compartment-step forward + adjoint."""
+    import torch
+
+    from dendra_amd.fit import CableFit
+    from dendra_amd.model import HH
+    from dendra_amd.workloads import fit_cables
+
+    w = fit_cables()
+    fit = CableFit(w.spec, w.dt, w.n_steps, w.sites, w.targets, knot_times=w.knot_times,
+                   margin_site=w.margin_site, margin_sign=w.margin_sign,
+                   margin_weight=w.margin_weight, theta=w.theta, tau=w.tau,
+                   hh=HH(celsius=6.3), dtype=torch.float32, device=dev)
+    p = [torch.tensor(np.asarray(x).reshape(-1), dtype=torch.float32, device=dev)
+         for x in (w.sna, w.sk)] + [torch.tensor(w.ve, dtype=torch.float32, device=dev)]
+    fit.forward(*p)
+    fit.backward()
+    torch.cuda.synchronize()
+    tf, tb = [], []
+    for _ in range(reps):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record()
+        loss = fit.forward(*p)
+        e[1].record()
+        fit.backward()
+        e[2].record()
+        torch.cuda.synchronize()
+        tf.append(e[0].elapsed_time(e[1]))
+        tb.append(e[1].elapsed_time(e[2]))
+    tf, tb = float(np.median(tf)), float(np.median(tb))
+    cs = w.spec.total * w.n_steps
+    both = tf + tb
+    res = {
+        "workload": "512 HH cables x 201 compartments, 10000 steps of 0.01 ms, per-"
+                    "compartment gNa/gK scales, 201x64 Ve knot table, MSE at 8 sites + "
+                    "sigmoid margin, checkpoints every 500 steps",
+        "dtype": "f32", "loss": loss.item(),
+        "forward": {"ms": tf, "compartment_steps_per_s": cs / (tf * 1e-3)},
+        "forward_adjoint": {"ms": both, "compartment_steps_per_s": cs / (both * 1e-3)},
+        "roofline": {"bound": "hbm", "bytes_per_cs": 144,
+                     "achieved": 144 * cs / (both * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s",
+                     "frac": 144 * cs / (both * 1e-3) / 1e9 / HBM_PEAK_GBS},
+    }
+    if with_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import fit_ref
+
+        sw = fit_cables(n_cells=64, n_steps=100, n_knots=8)
+        geom = fit_ref.geometry(sw.spec.diam, sw.spec.dx, sw.spec.rhoa, sw.spec.cm,
+                                sw.spec.off)
+        kk, ka = fit_ref.knot_table(sw.knot_times, sw.dt, sw.n_steps)
+        B, K = geom["gL"].shape
+        mw = torch.zeros((B, K), dtype=torch.float64)
+        mw[:, sw.sites[-1]] = torch.as_tensor(sw.margin_weight * sw.margin_sign
+                                              / (B * sw.n_steps))
+        a = torch.as_tensor(sw.sna).clone().requires_grad_()
+        b = torch.as_tensor(sw.sk).clone().requires_grad_()
+        v = torch.as_tensor(sw.ve).clone().requires_grad_()
+        t = time.perf_counter()
+        lc = fit_ref.fit_loss(geom, a, b, v, kk, ka, HH(celsius=6.3).array(), sw.dt,
+                              sw.n_steps, sw.sites, sw.targets.astype(np.float64),
+                              1.0 / (B * 8 * sw.n_steps), mw, sw.theta, sw.tau)
+        torch.autograd.grad(lc, [a, b, v])
+        dt_cpu = time.perf_counter() - t
+        scs = B * K * sw.n_steps
+        res["cpu_baseline"] = {
+            "value": scs / dt_cpu, "unit": "compartment-steps/s (forward + adjoint)",
+            "cores": torch.get_num_threads(), "kind": "port",
+            "sample": f"64 cells x 201 compartments x 100 steps, float64 eager PyTorch "
+                      f"autograd through a Thomas sweep, {dt_cpu:.1f}s"}
     return res
 
 

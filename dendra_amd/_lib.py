@@ -89,6 +89,19 @@ _SIGNATURES["dn_net_deliver_tape_f32"] = (
     i32, [i32, i32] + [p] * 6 + [i32] + [p] * 8 + [i64, p],
 )
 
+# n_tasks, rcap, task_nr, task_p, gl, grp, scale, sna, sk, act, n_knots, knot_k,
+# knot_a, site, mw, target, tstride, mse_scale, theta, tau, step0, n_steps, dt, hh
+def _fit_common(f):
+    return [i32, i32, p, p, p, p, p, p, p, p, i32, p, p, p, p, p, i64, f, f, f, i64, i64,
+            f, p]
+
+
+for _sfx, _f in (("f32", f32), ("f64", f64)):
+    _SIGNATURES[f"dn_fit_forward_{_sfx}"] = (i32, _fit_common(_f) + [p, p, p, i64, p, p, p])
+    _SIGNATURES[f"dn_fit_adjoint_{_sfx}"] = (i32, _fit_common(_f) + [p, p, p, p, p, p])
+    _SIGNATURES[f"dn_fit_csr_knots_{_sfx}"] = (
+        i32, [i64, i32, p, p, p, p, i64, i64, p, i64, i64, p])
+
 
 def _bind(lib) -> None:
     for name, (res, args) in _SIGNATURES.items():

@@ -14,6 +14,12 @@
   process (sections of 5-20 compartments of 10 um, a uniformly chosen leaf
   section splits into two children of 0.8x its diameter until the size
   target, uniform in [tmin, tmax], is reached).
+* ``fit_cables``: cables of equal compartment count for trace fitting
+  with per-compartment gNa/gK scales (lognormal, sigma 0.2) and a shared
+  Ve table of (compartments x knots): a point electrode 0.3 mm above the
+  middle compartment delivering cathodic triangular pulses (one knot wide
+  on each side) of amplitude uniform in [0.02, 0.12] mA at every fourth
+  knot; targets are seeded spike-shaped traces at 8 recording sites.
 """
 
 from __future__ import annotations
@@ -27,6 +33,7 @@ from .model import CableSpec, TreeSpec, point_source_lead, rect_wave
 
 # RNG stream ids
 S_K, S_DIAM, S_DIST, S_POS, S_AMP, S_TK, S_TREE, S_TDIAM = range(1, 9)
+S_SNA, S_SK, S_PULSE, S_TGT, S_SIGN = range(40, 45)
 
 
 def single_cable(batch: int = 16, seed: int = 1, n_comp: int = 101,
@@ -222,3 +229,53 @@ def sparse_network(n_point: int = 20000, n_bs: int = 1000, fan_in: int = 200,
                         n_src=2 * n_cells, n_tgt=n_cells)
     return NetworkSpec(cells=cells, n_point=n_point, soma=off[:-1].copy(), conn=conn,
                        seed=seed)
+
+
+@dataclass
+class FitWorkload:
+    spec: CableSpec
+    dt: float
+    n_steps: int
+    sna: np.ndarray          # (n_cells, K)
+    sk: np.ndarray           # (n_cells, K)
+    ve: np.ndarray           # (K, n_knots), mV
+    knot_times: np.ndarray   # (n_knots,), ms
+    sites: np.ndarray        # (8,)
+    targets: np.ndarray      # (n_steps, n_cells, 8), float32
+    margin_site: int
+    margin_sign: np.ndarray  # (n_cells,) +-1
+    margin_weight: float
+    theta: float
+    tau: float
+
+
+def fit_cables(n_cells: int = 512, n_comp: int = 201, n_steps: int = 10000,
+               dt: float = 0.01, n_knots: int = 64, seed: int = 5,
+               n_sites: int = 8, ve_gain: float = 1.0) -> FitWorkload:
+    dx = 10.0
+    off = np.arange(n_cells + 1, dtype=np.int64) * n_comp
+    total = n_cells * n_comp
+    diam = np.exp(np.log(4.0) + 0.2 * rng.normal_n(seed, S_DIAM, n_cells))
+    spec = CableSpec(off=off, diam=np.repeat(diam, n_comp), dx=np.full(total, dx),
+                     rhoa=np.full(total, 100.0), cm=np.ones(total), v_init=-65.0)
+    sna = np.exp(0.2 * rng.normal_n(seed, S_SNA, total)).reshape(n_cells, n_comp)
+    sk = np.exp(0.2 * rng.normal_n(seed, S_SK, total)).reshape(n_cells, n_comp)
+    x = np.arange(n_comp) * dx
+    lead = point_source_lead(x, 0.0, 0.0, x[n_comp // 2], 0.0, 300.0)
+    knot_times = np.linspace(0.0, n_steps * dt, n_knots)
+    cur = np.zeros(n_knots)
+    pulse = np.arange(2, n_knots - 1, 4)
+    cur[pulse] = -(0.02 + 0.10 * rng.uniform_n(seed, S_PULSE, len(pulse)))
+    ve = ve_gain * lead[:, None] * cur[None, :]
+    sites = np.round(np.linspace(0, n_comp - 1, n_sites)).astype(np.int64)
+    # targets: resting trace with spike-shaped bumps 1 ms after a random
+    # subset of the pulses, per cell and site
+    t = (np.arange(n_steps) + 1) * dt
+    hit = rng.uniform_n(seed, S_TGT, n_cells * len(pulse)).reshape(n_cells, len(pulse)) < 0.5
+    targets = np.full((n_steps, n_cells, n_sites), -65.0, dtype=np.float32)
+    for i, kp in enumerate(pulse):
+        bump = (105.0 * np.exp(-((t - knot_times[kp] - 1.0) / 0.6) ** 2)).astype(np.float32)
+        targets += bump[:, None, None] * hit[None, :, i, None]
+    sign = np.where(rng.uniform_n(seed, S_SIGN, n_cells) < 0.5, -1.0, 1.0)
+    return FitWorkload(spec, dt, n_steps, sna, sk, ve, knot_times, sites, targets,
+                       n_sites - 1, sign, 10.0, 0.0, 1.0)

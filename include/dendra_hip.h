@@ -228,6 +228,74 @@ int dn_net_run_f32(int n_cells, int n_point, const int64_t* off, const int32_t* 
                    int64_t ev_cap, float* delivered, int64_t rec_steps, int64_t n_steps,
                    int graph_steps, hipStream_t stream);
 
+/* ---- trace-fitting loss and its hand-written adjoint ---------------------
+ * Replace autograd through Population.run (dendra/models/core.py:4555-4614)
+ * for HH cables driven by a piecewise-linear Ve knot table (waveform
+ * `arbitrary`, dendra/models/stim/waveform/implementations.py:748-796):
+ * per step ThomasSolve.backward's transposed solve
+ * (dendra/models/integrators/triton/t_kernel_thomas.py:147-189) plus the
+ * gating vector-Jacobian products autograd would record
+ * (mechanisms/_cnexp.py:96-132, mod/HH.py:24-56,109-116), with checkpointed
+ * recomputation as Population.longrun_checkpointed (core.py:7802).
+ * Slot layout: task t owns 64*rcap slots, slot t*64*rcap + r*64 + lane is
+ * row r of that lane (rcap 4 or 8; task_p in {4,8,16,32,64}); per-slot
+ * arrays are [slots], state/adjoint [4][slots] (v, m, h, n), knots
+ * [n_knots][slots], trajectory [n_steps+1][4][slots].  act holds the
+ * activating function of each Ve knot (dn_fit_csr_knots_*).  knot_k/knot_a
+ * (length >= step0+n_steps) give the segment and weight of step n; target
+ * is [steps][tstride], site[] its column for recorded rows (-1 otherwise),
+ * mw[] the margin weight.  hh (host) = {gnabar, gkbar, gl, ena, ek, el, q10}.
+ * loss (optional, [n_tasks][2]) receives per-wavefront (mse, margin) sums. */
+int dn_fit_forward_f32(int n_tasks, int rcap, const int32_t* task_nr, const int32_t* task_p,
+                       const float* gl, const float* grp, const float* scale,
+                       const float* sna, const float* sk, const float* act, int n_knots,
+                       const int32_t* knot_k, const float* knot_a, const int32_t* site,
+                       const float* mw, const float* target, int64_t tstride,
+                       float mse_scale, float theta, float tau, int64_t step0,
+                       int64_t n_steps, float dt, const float* hh, const float* x0,
+                       float* xout, float* ckpt, int64_t ckpt_every, float* traj,
+                       float* loss, hipStream_t stream);
+int dn_fit_forward_f64(int n_tasks, int rcap, const int32_t* task_nr, const int32_t* task_p,
+                       const double* gl, const double* grp, const double* scale,
+                       const double* sna, const double* sk, const double* act,
+                       int n_knots, const int32_t* knot_k, const double* knot_a,
+                       const int32_t* site, const double* mw, const double* target,
+                       int64_t tstride, double mse_scale, double theta, double tau,
+                       int64_t step0, int64_t n_steps, double dt, const double* hh,
+                       const double* x0, double* xout, double* ckpt, int64_t ckpt_every,
+                       double* traj, double* loss, hipStream_t stream);
+/* Reverse sweep over steps [step0, step0+n_steps) from the trajectory the
+ * forward wrote for that segment; adj carries (dL/dv, dL/dm, dL/dh, dL/dn)
+ * in and out; g_sna, g_sk, g_act are accumulated (+=). */
+int dn_fit_adjoint_f32(int n_tasks, int rcap, const int32_t* task_nr, const int32_t* task_p,
+                       const float* gl, const float* grp, const float* scale,
+                       const float* sna, const float* sk, const float* act, int n_knots,
+                       const int32_t* knot_k, const float* knot_a, const int32_t* site,
+                       const float* mw, const float* target, int64_t tstride,
+                       float mse_scale, float theta, float tau, int64_t step0,
+                       int64_t n_steps, float dt, const float* hh, const float* traj,
+                       float* adj, float* g_sna, float* g_sk, float* g_act,
+                       hipStream_t stream);
+int dn_fit_adjoint_f64(int n_tasks, int rcap, const int32_t* task_nr, const int32_t* task_p,
+                       const double* gl, const double* grp, const double* scale,
+                       const double* sna, const double* sk, const double* act,
+                       int n_knots, const int32_t* knot_k, const double* knot_a,
+                       const int32_t* site, const double* mw, const double* target,
+                       int64_t tstride, double mse_scale, double theta, double tau,
+                       int64_t step0, int64_t n_steps, double dt, const double* hh,
+                       const double* traj, double* adj, double* g_sna, double* g_sk,
+                       double* g_act, hipStream_t stream);
+/* out[i*out_row + k*out_k] = sum_{e in [ptr[i], ptr[i+1])} w[e] *
+ * in[idx[e]*in_row + k*in_k] for k < n_k: the activating-function operator
+ * (implicit.py:618-638) applied to every Ve knot, and its transpose for the
+ * knot gradients, in a fixed summation order (deterministic). */
+int dn_fit_csr_knots_f32(int64_t n_out, int n_k, const int64_t* ptr, const int32_t* idx,
+                         const float* w, const float* in, int64_t in_row, int64_t in_k,
+                         float* out, int64_t out_row, int64_t out_k, hipStream_t stream);
+int dn_fit_csr_knots_f64(int64_t n_out, int n_k, const int64_t* ptr, const int32_t* idx,
+                         const double* w, const double* in, int64_t in_row, int64_t in_k,
+                         double* out, int64_t out_row, int64_t out_k, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
