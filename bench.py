@@ -53,6 +53,10 @@ def _parse():
     ap.add_argument("--only", choices=["all", "cables", "trees"], default="all")
     ap.add_argument("--network-steps", type=int, default=4000,
                     help="time steps of the network workload line (0 = skip)")
+    ap.add_argument("--surrogate-train", type=int, default=2_000_000,
+                    help="fibers in the surrogate training set (0 = skip the surrogate)")
+    ap.add_argument("--surrogate-test", type=int, default=1_000_000)
+    ap.add_argument("--surrogate-epochs", type=int, default=40)
     ap.add_argument("--fit", type=int, default=1,
                     help="also time the adjoint workload (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -200,6 +204,9 @@ def main():
         out["network"] = network_bench(args.network_steps, hh, dev, args.cpu_sample > 0)
     if args.fit:
         out["adjoint"] = fit_bench(dev, args.cpu_sample > 0)
+    if args.surrogate_train > 0:
+        out["surrogate"] = surrogate_bench(args.surrogate_train, args.surrogate_test,
+                                           args.surrogate_epochs, args.cpu_sample > 0)
     print(json.dumps(out))
     return 0
 
@@ -323,6 +330,103 @@ def fit_bench(dev, with_cpu, reps=3):
             "cores": torch.get_num_threads(), "kind": "port",
             "sample": f"64 cells x 201 compartments x 100 steps, float64 eager PyTorch "
                       f"autograd through a Thomas sweep, {dt_cpu:.1f}s"}
+    return res
+
+
+def _mlp_flops_per_sample(train):
+    """GEMM flops per sample per member: forward 2*(128*512 + 2*512*512 + 512);
+    training adds the two data products and the three weight products."""
+    fwd = 2 * (128 * 512 + 2 * 512 * 512 + 512)
+    if not train:
+        return fwd
+    return fwd + 2 * (2 * 512 * 512 + 512) + 2 * (128 * 512 + 2 * 512 * 512 + 512)
+
+
+def surrogate_bench(n_train, n_test, epochs, with_cpu):
+    """Surrogate line: GPU bisection labels for n_train seeded fibers (101
+    compartments, d ~ U[2, 16] um, point electrode), a 4-member BF16 MLP
+    ensemble trained on them, then thresholds of n_test held-out fibers by the
+    ensemble with bisection fallback (spread > 3 % or out of range).  Error is
+    measured against bisection on the first 50,000 held-out fibers."""
+    import torch
+
+    from dendra_amd.surrogate import ThresholdSurrogate
+    from dendra_amd.thresholder import GPUThresholder, bisection_judge
+    from dendra_amd.workloads import threshold_fibers
+
+    tr = threshold_fibers(n_train, seed=9)
+    gt = GPUThresholder(tr.spec, tr.profile)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    ytr, _ = gt.calculate_thresholds()
+    torch.cuda.synchronize()
+    t_label = time.perf_counter() - t
+    trials = float(gt.trials.mean())
+    del gt
+    sur = ThresholdSurrogate(n_members=4, seed=0)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    steps = sur.fit(tr.profile, tr.diam.astype(np.float32), ytr, epochs=epochs, batch=8192)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t
+    del tr
+    te = threshold_fibers(n_test, seed=9, start=10_000_000)
+    judge = bisection_judge(te.spec, te.profile)
+    d = te.diam.astype(np.float32)
+    sur.thresholds(te.profile[:8192], d[:8192], judge)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    thr, fb = sur.thresholds(te.profile, d, judge)
+    torch.cuda.synchronize()
+    t_inf = time.perf_counter() - t
+    # the network alone (features, 4 members, ensemble) on the same inputs
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    sur.predict(te.profile, d)
+    torch.cuda.synchronize()
+    t_net = time.perf_counter() - t
+    ns = min(50_000, n_test)
+    idx = np.arange(ns)
+    ref = judge(idx)
+    err = np.abs(thr[:ns] - ref) / ref
+    bis_rate = n_train / t_label
+    sur_rate = n_test / t_inf
+    train_flops = steps * 8192 * 4 * _mlp_flops_per_sample(True)
+    inf_flops = n_test * 4 * _mlp_flops_per_sample(False)
+    res = {
+        "workload": f"{n_train} training fibers (bisection labels, rtol 0.005, FP64), "
+                    f"{n_test} held-out fibers; 101 compartments, d ~ U[2,16] um, point "
+                    "electrode 0.1-1.5 mm, 0.1 ms cathodic pulse",
+        "thresholds_per_s": sur_rate, "unit": "thresholds/s",
+        "bisection_thresholds_per_s": bis_rate, "speedup_vs_bisection": sur_rate / bis_rate,
+        "bisection_trials_per_threshold": trials,
+        "median_error": float(np.median(err)), "p90_error": float(np.quantile(err, 0.9)),
+        "error_sample": ns, "fallback_fraction": float(fb.mean()),
+        "train_s": t_train, "train_steps": steps, "label_s": t_label,
+        "network_thresholds_per_s": n_test / t_net,
+        "roofline": {"bound": "mfma", "unit": "TFLOP/s", "peak": 2500.0,
+                     "achieved_train": train_flops / t_train / 1e12,
+                     "frac_train": train_flops / t_train / 1e12 / 2500.0,
+                     "achieved_inference": inf_flops / t_net / 1e12,
+                     "frac_inference": inf_flops / t_net / 1e12 / 2500.0},
+    }
+    if with_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle
+
+        from dendra_amd.model import HH, rect_wave
+
+        cs = threshold_fibers(256, seed=9, start=10_000_000)
+        wave = rect_wave(1000, 0.005, 0.5, 0.1, amp=-1.0, dtype=np.float64)
+        t = time.perf_counter()
+        oracle.threshold(cs.spec, HH(celsius=6.3).array(),
+                         cs.profile.astype(np.float64).reshape(-1), wave, 0.005,
+                         threads=oracle.cpu_threads())
+        dt_cpu = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": 256 / dt_cpu, "unit": "thresholds/s",
+                               "cores": oracle.cpu_threads(), "kind": "port",
+                               "sample": f"256 held-out fibers, float64 C oracle bisection, "
+                                         f"OpenMP, {dt_cpu:.1f}s"}
     return res
 
 

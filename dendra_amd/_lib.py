@@ -96,6 +96,22 @@ def _fit_common(f):
             f, p]
 
 
+_SIGNATURES.update({
+    "dn_mlp_gemm": (i32, [i32, i32, i32, i32, i32, i32, p, i64, i64, p, i64, i64, p, i64,
+                          p, i64, i64, p, i64, i64, p, i64, i64, p, i64, i64, p]),
+    "dn_mlp_out": (i32, [i32, i32, i32, p, i64, p, i64, p, p, i64, p, p, i64, p]),
+    "dn_mlp_out_bwd": (i32, [i32, i32, i32, p, i64, p, i64, p, i64, p, p, i64, p]),
+    "dn_mlp_row_dot": (i32, [i32, i32, i32, i32, p, i64, p, i64, p, i64, p]),
+    "dn_mlp_splitk_reduce": (i32, [i32, i64, i32, p, p, p]),
+    "dn_mlp_adam": (i32, [i64, i32, i32, p, p, p, p, f32, f32, f32, f32, p, p, p]),
+    "dn_mlp_featurize": (i32, [i64, i32, p, p, f32, f32, p, i32, p, p, p, f32, p, p]),
+    "dn_mlp_gather": (i32, [i32, i32, i32, p, p, p, p, p, p, p]),
+    "dn_mlp_ensemble": (i32, [i64, i32, p, i64, p, f32, f32, f32, p, p, p]),
+})
+
+_SIGNATURES["dn_threshold_f64"] = (
+    i32, [i32, i32] + [p] * 10 + [i64, f64, p, p, f64, f64, f64, i32, i32] + [p] * 5)
+
 for _sfx, _f in (("f32", f32), ("f64", f64)):
     _SIGNATURES[f"dn_fit_forward_{_sfx}"] = (i32, _fit_common(_f) + [p, p, p, i64, p, p, p])
     _SIGNATURES[f"dn_fit_adjoint_{_sfx}"] = (i32, _fit_common(_f) + [p, p, p, p, p, p])

@@ -33,7 +33,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .model import HH, CableSpec, cable_workspace, hh_steady_state, plan_cables
+from .model import HH, CableSpec, cable_workspace, hh_steady_state, plan_cables, slot_map
 
 _NP = {torch.float32: np.float32, torch.float64: np.float64}
 
@@ -94,13 +94,7 @@ class CableFit:
         self.n_tasks, self.slots = nt, nt * 64 * R
 
         # slot of every compartment (spec order)
-        comp_slot = np.empty(spec.total, dtype=np.int64)
-        for t in range(nt):
-            P, nr = int(plan.task_p[t]), int(plan.task_nr[t])
-            for sub in range(int(plan.task_ncell[t])):
-                cell = int(plan.order[plan.task_cell0[t] + sub])
-                j = np.arange(sizes[cell])
-                comp_slot[spec.off[cell] + j] = t * 64 * R + (j % nr) * 64 + sub * P + j // nr
+        comp_slot = slot_map(plan, sizes, spec.off, R)
         self.comp_slot = comp_slot
 
         ws = cable_workspace(spec)

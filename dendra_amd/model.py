@@ -308,3 +308,28 @@ def plan_cables(sizes: np.ndarray, rmax: int = 13, max_lanes: int = 64) -> Cable
     off[1:] = np.cumsum(sizes[order])
     return CablePlan(order, off, task_cell0[t_order], task_ncell[t_order],
                      task_nr[t_order], task_p[t_order], lanes[order])
+
+
+def slot_map(plan: CablePlan, sizes: np.ndarray, off: np.ndarray, rcap: int) -> np.ndarray:
+    """Slot of every compartment (spec order) in the row-slot layout of the
+    fitting and threshold kernels: task t owns 64*rcap slots and row r of
+    lane `sub*P + pl` is slot t*64*rcap + r*64 + sub*P + pl, where
+    compartment j of the task's sub-th cell sits on lane pl = j // nr, row
+    r = j % nr."""
+    sizes = np.asarray(sizes, dtype=np.int64)
+    nt = len(plan.task_p)
+    ncell = np.asarray(plan.task_ncell, dtype=np.int64)
+    k = np.arange(int(ncell.sum())) - np.repeat(np.cumsum(ncell) - ncell, ncell)
+    pos = np.repeat(np.asarray(plan.task_cell0, dtype=np.int64), ncell) + k
+    cell = np.asarray(plan.order, dtype=np.int64)[pos]
+    # per cell (spec order): its task and its index inside the task
+    task_c = np.empty(len(cell), dtype=np.int64)
+    sub_c = np.empty(len(cell), dtype=np.int64)
+    task_c[cell] = np.repeat(np.arange(nt), ncell)
+    sub_c[cell] = k
+    P_c = np.asarray(plan.task_p, dtype=np.int64)[task_c]
+    nr_c = np.asarray(plan.task_nr, dtype=np.int64)[task_c]
+    c_of = np.repeat(np.arange(len(sizes)), sizes)
+    j = np.arange(int(sizes.sum())) - np.asarray(off, dtype=np.int64)[c_of]
+    nr, P = nr_c[c_of], P_c[c_of]
+    return task_c[c_of] * 64 * rcap + (j % nr) * 64 + sub_c[c_of] * P + j // nr

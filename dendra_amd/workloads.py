@@ -20,6 +20,11 @@
   middle compartment delivering cathodic triangular pulses (one knot wide
   on each side) of amplitude uniform in [0.02, 0.12] mA at every fourth
   knot; targets are seeded spike-shaped traces at 8 recording sites.
+* ``threshold_fibers``: HH fibers of 101 compartments of 10 um, diameters
+  uniform in [dmin, dmax] um, each under a point electrode in a 300 ohm cm
+  medium at a position uniform in x over the middle half of the fiber and
+  in perpendicular distance over [zmin, zmax] mm; the Ve profile is the
+  float32 lead field for 1 mA.
 """
 
 from __future__ import annotations
@@ -34,6 +39,7 @@ from .model import CableSpec, TreeSpec, point_source_lead, rect_wave
 # RNG stream ids
 S_K, S_DIAM, S_DIST, S_POS, S_AMP, S_TK, S_TREE, S_TDIAM = range(1, 9)
 S_SNA, S_SK, S_PULSE, S_TGT, S_SIGN = range(40, 45)
+S_FD, S_FX, S_FZ = range(50, 53)
 
 
 def single_cable(batch: int = 16, seed: int = 1, n_comp: int = 101,
@@ -279,3 +285,30 @@ def fit_cables(n_cells: int = 512, n_comp: int = 201, n_steps: int = 10000,
     sign = np.where(rng.uniform_n(seed, S_SIGN, n_cells) < 0.5, -1.0, 1.0)
     return FitWorkload(spec, dt, n_steps, sna, sk, ve, knot_times, sites, targets,
                        n_sites - 1, sign, 10.0, 0.0, 1.0)
+
+
+@dataclass
+class FiberSet:
+    spec: CableSpec
+    profile: np.ndarray   # (n, K) float32, mV per mA
+    diam: np.ndarray      # (n,) um
+
+
+def threshold_fibers(n: int, seed: int = 9, n_comp: int = 101, dx: float = 10.0,
+                     dmin: float = 2.0, dmax: float = 16.0, zmin: float = 0.1,
+                     zmax: float = 1.5, start: int = 0) -> FiberSet:
+    """Fibers start..start+n-1 of the seeded stream (disjoint ranges give
+    disjoint training / held-out sets)."""
+    u = lambda stream: rng.uniform_n(seed, stream, n, start=start)
+    diam = dmin + (dmax - dmin) * u(S_FD)
+    L = (n_comp - 1) * dx
+    ex = L * (0.25 + 0.5 * u(S_FX))
+    ez = 1e3 * (zmin + (zmax - zmin) * u(S_FZ))
+    x = np.arange(n_comp) * dx
+    prof = point_source_lead(x[None, :], 0.0, 0.0, ex[:, None], 0.0, ez[:, None])
+    prof = prof.astype(np.float32)
+    off = np.arange(n + 1, dtype=np.int64) * n_comp
+    total = n * n_comp
+    spec = CableSpec(off=off, diam=np.repeat(diam, n_comp), dx=np.full(total, dx),
+                     rhoa=np.full(total, 100.0), cm=np.ones(total), v_init=-65.0)
+    return FiberSet(spec, prof, diam)

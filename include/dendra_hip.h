@@ -296,6 +296,75 @@ int dn_fit_csr_knots_f64(int64_t n_out, int n_k, const int64_t* ptr, const int32
                          const double* w, const double* in, int64_t in_row, int64_t in_k,
                          double* out, int64_t out_row, int64_t out_k, hipStream_t stream);
 
+/* ---- activation thresholds by bisection ----------------------------------
+ * Replaces Thresholder.calculate_thresholds with _fix_bounds (block_possible
+ * False) and arithmetic calc_stimamp (dendra/models/instruments/
+ * thresholder.py:835-993) under the Active callback (models/callbacks.py:
+ * 1946-1990) for unbranched HH cables with Ve_j(t) = lead_j * amp * wave[t].
+ * Slot layout with rcap 4 (slot task*256 + r*64 + lane); dr holds the
+ * activating function of the unit lead field, chk flags checked rows.
+ * ub/lb (plan order): initial bounds in, final bounds out; status 0
+ * converged, 1 bound fixing failed, 2 tolerance not met; trials = trials run.
+ * hh (host) = {gnabar, gkbar, gl, ena, ek, el, q10}; state0 (host) = rest
+ * {v, m, h, n}.  Each wavefront runs all trials of its fibers in one launch. */
+int dn_threshold_f64(int n_tasks, int rcap, const int32_t* task_nr, const int32_t* task_p,
+                     const int32_t* task_cell0, const int32_t* task_ncell,
+                     const double* gl, const double* grp, const double* scale,
+                     const double* dr, const int32_t* chk, const double* wave,
+                     int64_t n_steps, double dt, const double* hh, const double* state0,
+                     double vthr, double rtol, double fix_up, int max_fix, int max_bisect,
+                     double* ub, double* lb, int32_t* status, int32_t* trials,
+                     hipStream_t stream);
+
+/* ---- threshold surrogate (BF16 MFMA multilayer perceptron) ---------------
+ * No counterpart in the reference: the learned part that predicts the
+ * thresholds dn_threshold_f64 computes, used with bisection as fallback.
+ * BF16 operands are passed as void* (2-byte elements); strides are in
+ * elements; z indexes ensemble members.  dn_mlp_gemm: C = A B^T for nz
+ * products, A [M][K] and B [N][K] BF16 rows, float32 accumulation on
+ * v_mfma_f32_32x32x16_bf16; epi 0: Z = C + bias, H = gelu(Z) (BF16), HT =
+ * H^T; epi 1: H = C * gelu'(Zp), HT = H^T; epi 2: split-K float32 partials
+ * Z[z*splits + s].  M, N multiples of 128, K a multiple of 32*splits. */
+int dn_mlp_gemm(int epi, int nz, int M, int N, int K, int splits, const void* A, int64_t lda,
+                int64_t sa, const void* B, int64_t ldb, int64_t sb, const float* bias,
+                int64_t sbias, float* Z, int64_t ldz, int64_t sz, void* H, int64_t ldh,
+                int64_t sh, void* HT, int64_t ldht, int64_t sht, const float* Zp,
+                int64_t ldzp, int64_t szp, hipStream_t stream);
+/* Output unit y = H3 . w4 + b4; with targets t also dy = 2 (y - t) / Bn. */
+int dn_mlp_out(int nz, int Bn, int Hd, const void* H3, int64_t sh, const void* w4, int64_t sw,
+               const float* b4, const float* t, int64_t st, float* y, float* dy, int64_t sy,
+               hipStream_t stream);
+/* dZ3 = dy w4^T * gelu'(Z3) in [b][k] and [k][b] BF16 layouts. */
+int dn_mlp_out_bwd(int nz, int Bn, int Hd, const float* dy, int64_t sy, const void* w4,
+                   int64_t sw, const float* Z3, int64_t sz, void* dZ, void* dZT, int64_t sd,
+                   hipStream_t stream);
+/* out[z][r] = sum_c X[z][r][c] * (w ? w[z][c] : 1) (bias and output-weight
+ * gradients), X BF16 when x_bf16 else float32. */
+int dn_mlp_row_dot(int nz, int rows, int cols, int x_bf16, const void* X, int64_t sx,
+                   const float* w, int64_t sw, float* out, int64_t so, hipStream_t stream);
+/* out[z][i] = sum_s P[z][s][i] over split-K partials. */
+int dn_mlp_splitk_reduce(int nz, int64_t n, int splits, const float* P, float* out,
+                         hipStream_t stream);
+/* One Adam step on float32 masters with bias-corrected step lr_t; refreshes
+ * the BF16 copy pb and the transposed BF16 copy pbt (either may be NULL). */
+int dn_mlp_adam(int64_t n, int rows, int cols, float* p, const float* g, float* m, float* v,
+                float lr_t, float b1, float b2, float eps, void* pb, void* pbt,
+                hipStream_t stream);
+/* Features x[0..P-1] = profile / max|profile|, x[P] = (diam - dc) / ds, zero
+ * padding to ldx; lmag = log max|profile|; with lo/hi (P+2 bounds, the last
+ * for lmag) flag = 1 where a value is outside [lo - margin, hi + margin]. */
+int dn_mlp_featurize(int64_t n, int P, const float* prof, const float* diam, float dc, float ds,
+                     void* X, int ldx, float* lmag, const float* lo, const float* hi,
+                     float margin, int32_t* flag, hipStream_t stream);
+/* Minibatch gather X0[z][b] = X[idx[z][b]] in both layouts, t0 = t[idx]. */
+int dn_mlp_gather(int nz, int Bn, int F, const int32_t* idx, const void* X, const float* t,
+                  void* X0, void* X0T, float* t0, hipStream_t stream);
+/* thr = median over members of exp(pred * ystd + ymean - lmag); flag = 1
+ * where (max - min) > spread * median (flags already set stay set). */
+int dn_mlp_ensemble(int64_t n, int nz, const float* pred, int64_t sp, const float* lmag,
+                    float ymean, float ystd, float spread, float* thr, int32_t* flag,
+                    hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -508,3 +508,125 @@ ORC_API int orc_tree_run(int64_t n_trees, const int64_t* off, const int64_t* par
   }
   return bad ? -1 : 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Activation thresholds by bisection                                         */
+/* ------------------------------------------------------------------------- */
+
+/* One trial: the cable at rest, Ve_j(t) = lead_j * amp * wave[s] applied for
+ * n_steps backward-Euler steps (as orc_cable_run).  Returns 1 as soon as a
+ * checked compartment has v >= vthr after a step (the Active callback,
+ * models/callbacks.py:1946-1990; _threshold_callbacks.py:50-53), else 0. */
+static int cable_trial(int64_t K, const double* gL, const double* gR,
+                       const double* scale, const double* lower, const double* upper,
+                       const double* diag, const orc_hh_t* p, const double* lead,
+                       double amp, const double* wave, int64_t n_steps, double dt,
+                       double v_init, const int64_t* chk, int n_chk, double vthr,
+                       double* V, double* M, double* H, double* N, double* f,
+                       double* mainv, double* rhs, double* dv, double* ve) {
+  double dt_s = dt * 1.0e-3, m0, h0, n0;
+  orc_hh_steady(v_init, &m0, &h0, &n0);
+  for (int64_t j = 0; j < K; ++j) { V[j] = v_init; M[j] = m0; H[j] = h0; N[j] = n0; }
+  for (int64_t s = 0; s < n_steps; ++s) {
+    for (int64_t j = 0; j < K; ++j) {
+      double G, J, v = V[j];
+      hh_advance(v, dt, p, 1.0, 1.0, &M[j], &H[j], &N[j], &G, &J);
+      f[j] = (J - G * v) * scale[j];
+      mainv[j] = 1.0 - dt_s * (diag[j] - G * scale[j]);
+      ve[j] = lead[j] * amp * wave[s];
+    }
+    for (int64_t j = 0; j < K; ++j) {
+      double a = 0.0;
+      if (j + 1 < K) a += gL[j] * (V[j + 1] - V[j]) + gL[j] * (ve[j + 1] - ve[j]);
+      if (j > 0) a -= gR[j - 1] * (V[j] - V[j - 1]) + gR[j - 1] * (ve[j] - ve[j - 1]);
+      rhs[j] = dt_s * (f[j] + a);
+    }
+    if (K == 1) dv[0] = rhs[0] / mainv[0];
+    else orc_thomas(lower, mainv, upper, rhs, dv, 1, K);
+    int hit = 0;
+    for (int64_t j = 0; j < K; ++j) V[j] += dv[j];
+    for (int c = 0; c < n_chk; ++c) hit |= V[chk[c]] >= vthr;
+    if (hit) return 1;
+  }
+  return 0;
+}
+
+/*
+ * Thresholder.calculate_thresholds (instruments/thresholder.py:873-979) per
+ * fiber: _fix_bounds without block (:835-871) multiplies ub by fix_up while
+ * the trial at ub is inactive (at most max_fix times; status 1 if it never
+ * activates), then arithmetic bisection (:981-993) while (ub - lb) / ub >=
+ * rtol (:476-478), at most max_bisect trials (status 2 if not met).
+ * ub/lb: initial bounds in, final bounds out.  chk: checked compartments of
+ * every fiber (local indices).  trials: simulations run per fiber.
+ */
+ORC_API int orc_threshold(int64_t n_fibers, const int64_t* off, const double* diam,
+                          const double* dx, const double* rhoa, const double* cm,
+                          const double* hhp, const double* lead, const double* wave,
+                          int64_t n_steps, double dt, double v_init, const int64_t* chk,
+                          int n_chk, double vthr, double rtol, double fix_up, int max_fix,
+                          int max_bisect, double* ub, double* lb, int32_t* status,
+                          int32_t* trials, int n_threads) {
+  orc_hh_t p = {hhp[0], hhp[1], hhp[2], hhp[3], hhp[4], hhp[5], hhp[6]};
+  double dt_s = dt * 1.0e-3;
+  int bad = 0;
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+#endif
+  for (int64_t c = 0; c < n_fibers; ++c) {
+    int64_t o = off[c], K = off[c + 1] - off[c];
+    double* w = (double*)malloc(sizeof(double) * (size_t)K * 16);
+    if (!w) { bad |= 1; continue; }
+    double *scale = w, *gL = w + K, *gR = w + 2 * K, *diag = w + 3 * K,
+           *lower = w + 4 * K, *upper = w + 5 * K, *V = w + 6 * K, *M = w + 7 * K,
+           *H = w + 8 * K, *N = w + 9 * K, *f = w + 10 * K, *mainv = w + 11 * K,
+           *rhs = w + 12 * K, *dv = w + 13 * K, *ve = w + 14 * K, *cinv = w + 15 * K;
+    for (int64_t j = 0; j < K; ++j) {
+      double area = cyl_area(diam[o + j], dx[o + j]);
+      cinv[j] = 1.0 / (1.0e-6 * cm[o + j] * area);
+      scale[j] = area * cinv[j];
+      gL[j] = gR[j] = 0.0;
+    }
+    for (int64_t j = 0; j + 1 < K; ++j) {
+      double g = 2.0 / (cyl_resistance(diam[o + j], dx[o + j], rhoa[o + j]) +
+                        cyl_resistance(diam[o + j + 1], dx[o + j + 1], rhoa[o + j + 1]));
+      gL[j] = g * cinv[j];
+      gR[j] = g * cinv[j + 1];
+      lower[j] = -dt_s * gR[j];
+      upper[j] = -dt_s * gL[j];
+    }
+    for (int64_t j = 0; j < K; ++j)
+      diag[j] = (j + 1 < K ? -gL[j] : 0.0) + (j > 0 ? -gR[j - 1] : 0.0);
+    double u = ub[c], l = lb[c];
+    int st = 0, ntr = 0, tries = 0, active = 0;
+#define TRIAL(amp)                                                                 \
+  (++ntr, cable_trial(K, gL, gR, scale, lower, upper, diag, &p, lead + o, (amp), wave, \
+                      n_steps, dt, v_init, chk, n_chk, vthr, V, M, H, N, f, mainv, rhs, \
+                      dv, ve))
+    for (;;) {
+      active = TRIAL(u);
+      if (active) break;
+      if (tries >= max_fix) { st = 1; break; }
+      u *= fix_up;
+      ++tries;
+    }
+    if (st == 0) {
+      tries = 0;
+      while ((u - l) / u >= rtol) {
+        if (tries >= max_bisect) { st = 2; break; }
+        double amp = (u + l) / 2.0;
+        if (TRIAL(amp)) u = amp;
+        else l = amp;
+        ++tries;
+      }
+    }
+#undef TRIAL
+    ub[c] = u;
+    lb[c] = l;
+    status[c] = st;
+    trials[c] = ntr;
+    free(w);
+  }
+  return bad ? -1 : 0;
+}

@@ -79,6 +79,9 @@ def load():
         lib.orc_cable_run.restype = i32
         lib.orc_tree_run.argtypes = [i64, lp, lp] + run_args[2:]
         lib.orc_tree_run.restype = i32
+        lib.orc_threshold.argtypes = [i64, lp, dp, dp, dp, dp, dp, dp, dp, i64, f64, f64, lp,
+                                      i32, f64, f64, f64, i32, i32, dp, dp, ip, ip, i32]
+        lib.orc_threshold.restype = i32
         lib.orc_delay_steps.argtypes = [dp, f64, i64, lp]
         lib.orc_ring_replay_f32.argtypes = [i64, i64, i64, ip, ip, ip, fp, i64, i64, bp,
                                             fp, fp, C.POINTER(i64)]
@@ -196,6 +199,33 @@ def tree_run(spec, hh_arr, dt, n_steps, ve_wave=None, i_wave=None, record_every=
     return _run(load().orc_tree_run, spec.n_trees, spec.off,
                 (_c(spec.parent, np.int64),), arrays, spec, hh_arr, dt, n_steps,
                 ve_wave, i_wave, record_every, threads, state)
+
+
+def threshold(spec, hh_arr, lead, wave, dt, node_check=(5, -5), vthr=0.0, ub0=0.1,
+              rtol=0.005, fix_up=2.0, max_fix=10, max_bisect=25, threads=0):
+    """Bisection thresholds (mA) of every cable of `spec` for Ve = lead*amp*wave.
+
+    Returns (ub, lb, status, trials); node_check entries are resolved per
+    fiber like the reference's negative node indices."""
+    sizes = np.diff(spec.off)
+    if len(set(sizes.tolist())) != 1:
+        raise ValueError("oracle thresholds need equal compartment counts")
+    K = int(sizes[0])
+    chk = np.array([k % K for k in node_check], dtype=np.int64)
+    n = spec.n_cells
+    ub = np.full(n, float(ub0)) if np.ndim(ub0) == 0 else np.array(ub0, dtype=np.float64)
+    lb = np.zeros(n)
+    status = np.zeros(n, np.int32)
+    trials = np.zeros(n, np.int32)
+    wave = _c(wave)
+    st = load().orc_threshold(n, _c(spec.off, np.int64), _c(spec.diam), _c(spec.dx),
+                              _c(spec.rhoa), _c(spec.cm), _c(hh_arr), _c(lead), wave,
+                              len(wave), float(dt), float(spec.v_init), chk, len(chk),
+                              float(vthr), float(rtol), float(fix_up), int(max_fix),
+                              int(max_bisect), ub, lb, status, trials, int(threads))
+    if st != 0:
+        raise RuntimeError("oracle threshold failed")
+    return ub, lb, status, trials
 
 
 def delay_steps(delay_ms, dt):

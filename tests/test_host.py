@@ -217,3 +217,24 @@ def test_hh_steady_state_matches_oracle(orc):
 
     for v in (-80.0, -65.0, -55.0, -40.0, -40.000001, 0.0, 30.0):
         np.testing.assert_allclose(hh_steady_state(v), orc.hh_steady(v), rtol=1e-14)
+
+
+def test_slot_map_matches_definition():
+    """Row-slot layout of the fitting/threshold kernels on ragged plans."""
+    from dendra_amd.model import plan_cables, slot_map
+
+    rs = np.random.default_rng(5)
+    sizes = rs.integers(1, 250, 200)
+    off = np.zeros(201, dtype=np.int64)
+    off[1:] = np.cumsum(sizes)
+    for R in (4, 8):
+        plan = plan_cables(sizes, R, 64)
+        got = slot_map(plan, sizes, off, R)
+        assert len(np.unique(got)) == len(got) and got.max() < len(plan.task_p) * 64 * R
+        for t in range(len(plan.task_p)):
+            P, nr = int(plan.task_p[t]), int(plan.task_nr[t])
+            for sub in range(int(plan.task_ncell[t])):
+                cell = int(plan.order[plan.task_cell0[t] + sub])
+                j = np.arange(sizes[cell])
+                want = t * 64 * R + (j % nr) * 64 + sub * P + j // nr
+                assert np.array_equal(got[off[cell] + j], want)
