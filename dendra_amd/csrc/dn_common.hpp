@@ -38,21 +38,28 @@ template <typename T> struct HHParams {
 };
 
 // x / (exp(x/10) - 1) with the local even Bernoulli series near x = 0.
+// The series is evaluated only when some lane of the wavefront needs it; the
+// per-lane selection is unchanged, so results do not depend on the skip.
 __device__ __forceinline__ float vtrap10(float x) {
-  float q = x * 0.1f;
-  float q2 = q * q;
+  const float q = x * 0.1f;
+  const bool near0 = fabsf(q) <= 1.0f;
+  const float direct = x * frcp(fexp(q) - 1.0f);
+  if (__ballot(near0) == 0ull) return direct;
+  const float q2 = q * q;
   float e = fmaf(q2, -691.0f / 1307674368000.0f, 1.0f / 47900160.0f);
   e = fmaf(e, q2, -1.0f / 1209600.0f);
   e = fmaf(e, q2, 1.0f / 30240.0f);
   e = fmaf(e, q2, -1.0f / 720.0f);
   e = fmaf(e, q2, 1.0f / 12.0f);
-  float series = (10.0f - x * 0.5f) + x * (q * e);
-  float direct = x * frcp(fexp(q) - 1.0f);
-  return fabsf(q) <= 1.0f ? series : direct;
+  const float series = (10.0f - x * 0.5f) + x * (q * e);
+  return near0 ? series : direct;
 }
 __device__ __forceinline__ double vtrap10(double x) {
-  double q = x / 10.0;
-  double q2 = q * q;
+  const double q = x / 10.0;
+  const bool near0 = fabs(q) <= 0.5;
+  const double direct = x / (exp(q) - 1.0);
+  if (__ballot(near0) == 0ull) return direct;
+  const double q2 = q * q;
   double e = q2 * (43867.0 / 5109094217170944000.0) - 3617.0 / 10670622842880000.0;
   e = e * q2 + 1.0 / 74724249600.0;
   e = e * q2 - 691.0 / 1307674368000.0;
@@ -61,9 +68,8 @@ __device__ __forceinline__ double vtrap10(double x) {
   e = e * q2 + 1.0 / 30240.0;
   e = e * q2 - 1.0 / 720.0;
   e = e * q2 + 1.0 / 12.0;
-  double series = (10.0 - x / 2.0) + x * (q * e);
-  double direct = x / (exp(q) - 1.0);
-  return fabs(q) <= 0.5 ? series : direct;
+  const double series = (10.0 - x / 2.0) + x * (q * e);
+  return near0 ? series : direct;
 }
 
 // phi1(z) = expm1(z)/z on |z| <= 0.5 (Taylor, degree 9 / 16).
@@ -99,14 +105,19 @@ __device__ __forceinline__ double phi1(double z) {
   return v;
 }
 
-// Exact affine step of x' = xa - xb x over dt (ms).
+// Exact affine step of x' = xa - xb x over dt (ms): the Taylor form for
+// |z| <= 0.5, the exponential relaxation beyond.  The relaxation is
+// evaluated only when some lane of the wavefront needs it (the per-lane
+// selection is unchanged).
 template <typename T>
 __device__ __forceinline__ T cnexp(T x, T xa, T xb, T dt) {
-  T z = -xb * dt;
-  T inc = x + dt * (xa - xb * x) * phi1(z);
-  T eq = xa * frcp(xb);
-  T rel = eq + (x - eq) * fexp(z);
-  return fabs_(z) <= T(0.5) ? inc : rel;
+  const T z = -xb * dt;
+  const T inc = x + dt * (xa - xb * x) * phi1(z);
+  const bool far = fabs_(z) > T(0.5);
+  if (__ballot(far) == 0ull) return inc;
+  const T eq = xa * frcp(xb);
+  const T rel = eq + (x - eq) * fexp(z);
+  return far ? rel : inc;
 }
 
 // Gate advance at v_old, then the affine current terms from the new gates.

@@ -416,6 +416,12 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
     // elimination, deepest level first
     for (int l = nlvl_max - 1; l >= 0; --l) {
       if (my_lvl == l) {
+        // All conductances of the chain are read before the dependent sweep
+        // (reads past the section end stay inside the block's LDS and are
+        // unused), so the sweep waits on LDS once instead of once per node.
+        T gq[kSecMax];
+#pragma unroll
+        for (int r = 0; r < kSecMax; ++r) gq[r] = gs[gb + r];
         T pD = T(0), pB = T(0);
         for (int c = cf; c < cf + nc; ++c) { pD += cD[c]; pB += cB[c]; }
         T D = T(0), B = T(0), fac = T(0);
@@ -429,11 +435,12 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
               D = M[r] + fac * D;
               B = F[r] + fac * B;
             }
-            const T gr = gs[gb + r];
+            const T gr = gq[r];
             const T inv = frcp(D + gr);
-            M[r] = inv;
-            F[r] = B;
             fac = gr * inv;
+            // back-substitution is then x_r = B_r inv_r + fac_r x_parent
+            M[r] = fac;
+            F[r] = B * inv;
           }
         }
         cD[s] = fac * D;
@@ -448,7 +455,7 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
 #pragma unroll
         for (int r = 0; r < kSecMax; ++r) {
           if (r < len) {
-            x = (F[r] + gs[gb + r] * x) * M[r];
+            x = F[r] + M[r] * x;
             v[r] += x;
           }
         }
