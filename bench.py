@@ -50,7 +50,8 @@ def _parse():
                     help="fraction of cells in the CPU oracle sample (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=800)
     ap.add_argument("--waves-per-block", type=int, default=4)
-    ap.add_argument("--only", choices=["all", "cables", "trees"], default="all")
+    ap.add_argument("--only", choices=["all", "cables", "trees", "none"], default="all",
+                    help="parts of the uneven population to run (none: only the other lines)")
     ap.add_argument("--network-steps", type=int, default=4000,
                     help="time steps of the network workload line (0 = skip)")
     ap.add_argument("--surrogate-train", type=int, default=2_000_000,
@@ -69,23 +70,45 @@ def main():
     args = _parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(cells are independent: see DESIGN.md for how the population shards)",
+              "(cells are independent; a multi-GPU run would partition them over ranks)",
               file=sys.stderr)
         return 2
     import torch
 
     from dendra_amd import _lib
     from dendra_amd.model import HH
-    from dendra_amd.population import CablePopulation, TreePopulation
-    from dendra_amd.workloads import uneven_population
 
     _lib.load()
     dev = torch.device("cuda:0")
+    hh = HH(celsius=6.3)
+    if args.only == "none":
+        out = {"metric": "compartment-steps/s (batched forward, uneven HH population)",
+               "value": None, "unit": "compartment-steps/s", "n_gpus": 1,
+               "note": "uneven population skipped (--only none)"}
+    else:
+        out = population_bench(args, dev, hh)
+    if args.network_steps > 0:
+        out["network"] = network_bench(args.network_steps, hh, dev, args.cpu_sample > 0)
+    if args.fit:
+        out["adjoint"] = fit_bench(dev, args.cpu_sample > 0)
+    if args.surrogate_train > 0:
+        out["surrogate"] = surrogate_bench(args.surrogate_train, args.surrogate_test,
+                                           args.surrogate_epochs, args.cpu_sample > 0)
+    print(json.dumps(out))
+    return 0
+
+
+def population_bench(args, dev, hh):
+    """Primary line: the uneven population (cables and trees on two streams)."""
+    import torch
+
+    from dendra_amd.population import CablePopulation, TreePopulation
+    from dendra_amd.workloads import uneven_population
+
     t0 = time.time()
     work = uneven_population(args.cables, args.trees, args.time_steps)
     print(f"# workload built in {time.time() - t0:.1f}s: {work.cables.total} cable + "
           f"{work.trees.total} tree compartments, {work.n_steps} steps", file=sys.stderr)
-    hh = HH(celsius=6.3)
     use_c = args.only in ("all", "cables")
     use_t = args.only in ("all", "trees")
     cab = CablePopulation(work.cables, work.dt, hh, torch.float32, dev) if use_c else None
@@ -200,15 +223,7 @@ def main():
                         "speedup_persistent_over_plain": plain_wall / wall}
     if args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(work, hh, args.cpu_sample, args.cpu_steps)
-    if args.network_steps > 0:
-        out["network"] = network_bench(args.network_steps, hh, dev, args.cpu_sample > 0)
-    if args.fit:
-        out["adjoint"] = fit_bench(dev, args.cpu_sample > 0)
-    if args.surrogate_train > 0:
-        out["surrogate"] = surrogate_bench(args.surrogate_train, args.surrogate_test,
-                                           args.surrogate_epochs, args.cpu_sample > 0)
-    print(json.dumps(out))
-    return 0
+    return out
 
 
 def network_bench(n_steps, hh, dev, with_cpu):
