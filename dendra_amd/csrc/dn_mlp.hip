@@ -28,9 +28,10 @@ typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kTM = 128, kTN = 128, kTK = 32;
-constexpr int kAS = kTK + 8;    // LDS row pitch of the staged operand tiles
+constexpr int kTM = 128, kTN = 128, kTK = 64;
+constexpr int kAS = kTK + 8;    // LDS row pitch (144 B: 16-byte reads of 16 rows hit distinct banks)
 constexpr int kTS = kTM + 8;    // LDS row pitch of the transposed output tile
+constexpr int kStage = 2 * kTM * kAS;  // one stage: A and B tiles
 
 enum { EPI_GELU = 0, EPI_DGELU = 1, EPI_F32 = 2 };
 
@@ -57,15 +58,17 @@ __device__ __forceinline__ float gelu_d(float z) {
 
 template <int EPI>
 __global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs G) {
-  __shared__ __align__(16) bf16 smem[kTN * kTS];  // >= 2 * kTM * kAS
-  bf16* sA = smem;
-  bf16* sB = smem + kTM * kAS;
+  // two stages of (A, B) tiles; the epilogue reuses the space for the
+  // transposed output tile
+  __shared__ __align__(16) bf16 smem[2 * kStage];
+  static_assert(2 * kStage >= kTN * kTS, "transpose tile must fit");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
   const int z = blockIdx.z / G.splits, split = blockIdx.z % G.splits;
   const int kspan = G.K / G.splits;
   const int kbeg = split * kspan;
+  const int ntile = kspan / kTK;
   const bf16* A = G.A + z * G.sa;
   const bf16* B = G.B + z * G.sb;
   f32x16 acc[2][2];
@@ -75,26 +78,57 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs G) {
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
-  for (int k0 = kbeg; k0 < kbeg + kspan; k0 += kTK) {
-#pragma unroll
-    for (int c2 = 0; c2 < 2; ++c2) {
-      const int c = tid + c2 * 256;
-      const int row = c >> 2, kc = (c & 3) * 8;
-      *reinterpret_cast<uint4*>(&sA[row * kAS + kc]) =
-          *reinterpret_cast<const uint4*>(&A[(int64_t)(m0 + row) * G.lda + k0 + kc]);
-      *reinterpret_cast<uint4*>(&sB[row * kAS + kc]) =
-          *reinterpret_cast<const uint4*>(&B[(int64_t)(n0 + row) * G.ldb + k0 + kc]);
+  // each thread moves 4 16-byte chunks of A and 4 of B per tile:
+  // chunk c = tid + 256 q -> row c / 8, k offset (c % 8) * 8; the chunks are
+  // named registers (no array) so that they never leave the register file
+  const int crow = tid >> 3, ckc = (tid & 7) * 8;
+  const bf16* gA = A + (int64_t)(m0 + crow) * G.lda + ckc;
+  const bf16* gB = B + (int64_t)(n0 + crow) * G.ldb + ckc;
+  const int64_t sa32 = 32 * G.lda, sb32 = 32 * G.ldb;
+  uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+#define DN_GEMM_FETCH(K0)                                                             \
+  {                                                                                   \
+    const int k_ = (K0);                                                              \
+    ra0 = *reinterpret_cast<const uint4*>(gA + k_);                                   \
+    ra1 = *reinterpret_cast<const uint4*>(gA + sa32 + k_);                            \
+    ra2 = *reinterpret_cast<const uint4*>(gA + 2 * sa32 + k_);                        \
+    ra3 = *reinterpret_cast<const uint4*>(gA + 3 * sa32 + k_);                        \
+    rb0 = *reinterpret_cast<const uint4*>(gB + k_);                                   \
+    rb1 = *reinterpret_cast<const uint4*>(gB + sb32 + k_);                            \
+    rb2 = *reinterpret_cast<const uint4*>(gB + 2 * sb32 + k_);                        \
+    rb3 = *reinterpret_cast<const uint4*>(gB + 3 * sb32 + k_);                        \
+  }
+#define DN_GEMM_STASH(STAGE)                                                          \
+  {                                                                                   \
+    bf16* sA_ = smem + (STAGE) * kStage + crow * kAS + ckc;                           \
+    bf16* sB_ = sA_ + kTM * kAS;                                                      \
+    *reinterpret_cast<uint4*>(sA_) = ra0;                                             \
+    *reinterpret_cast<uint4*>(sA_ + 32 * kAS) = ra1;                                  \
+    *reinterpret_cast<uint4*>(sA_ + 64 * kAS) = ra2;                                  \
+    *reinterpret_cast<uint4*>(sA_ + 96 * kAS) = ra3;                                  \
+    *reinterpret_cast<uint4*>(sB_) = rb0;                                             \
+    *reinterpret_cast<uint4*>(sB_ + 32 * kAS) = rb1;                                  \
+    *reinterpret_cast<uint4*>(sB_ + 64 * kAS) = rb2;                                  \
+    *reinterpret_cast<uint4*>(sB_ + 96 * kAS) = rb3;                                  \
+  }
+  DN_GEMM_FETCH(kbeg)
+  DN_GEMM_STASH(0)
+  __syncthreads();
+  for (int t = 0; t < ntile; ++t) {
+    if (t + 1 < ntile) {  // in flight during the products
+      DN_GEMM_FETCH(kbeg + (t + 1) * kTK)
     }
-    __syncthreads();
+    const bf16* sA = smem + (t & 1) * kStage;
+    const bf16* sB = sA + kTM * kAS;
 #pragma unroll
     for (int ks = 0; ks < kTK; ks += 16) {
       bf16x8 af[2], bfr[2];
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        af[t] = *reinterpret_cast<const bf16x8*>(
-            &sA[(wm * 64 + t * 32 + (lane & 31)) * kAS + ks + 8 * (lane >> 5)]);
-        bfr[t] = *reinterpret_cast<const bf16x8*>(
-            &sB[(wn * 64 + t * 32 + (lane & 31)) * kAS + ks + 8 * (lane >> 5)]);
+      for (int u = 0; u < 2; ++u) {
+        af[u] = *reinterpret_cast<const bf16x8*>(
+            &sA[(wm * 64 + u * 32 + (lane & 31)) * kAS + ks + 8 * (lane >> 5)]);
+        bfr[u] = *reinterpret_cast<const bf16x8*>(
+            &sB[(wn * 64 + u * 32 + (lane & 31)) * kAS + ks + 8 * (lane >> 5)]);
       }
 #pragma unroll
       for (int a = 0; a < 2; ++a)
@@ -103,8 +137,14 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs G) {
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[a], bfr[b], acc[a][b], 0, 0,
                                                               0);
     }
+    // the other stage was last read before the previous barrier
+    if (t + 1 < ntile) {
+      DN_GEMM_STASH((t + 1) & 1)
+    }
     __syncthreads();
   }
+#undef DN_GEMM_FETCH
+#undef DN_GEMM_STASH
   // epilogue: element (i, j) of the block tile held by (a, b, r, lane)
   const bool want_t = (EPI != EPI_F32) && G.HT;
 #pragma unroll
@@ -209,8 +249,19 @@ __global__ __launch_bounds__(256) void row_dot_kernel(int rows, int cols, const 
   if (wave >= rows * nz) return;
   const int z = wave / rows, row = wave % rows;
   const TX* x = X + z * sx + (int64_t)row * cols;
+  const float* wz = w ? w + z * swz : nullptr;
   float s = 0.0f;
-  for (int c = lane; c < cols; c += 64) s += (float)x[c] * (w ? w[z * swz + c] : 1.0f);
+  if (sizeof(TX) == 2 && (cols % 8) == 0 &&
+      (reinterpret_cast<uintptr_t>(x) % 16) == 0) {
+    // 8 BF16 values (16 bytes) per lane and load
+    for (int c = lane * 8; c < cols; c += 512) {
+      const bf16x8 xv = *reinterpret_cast<const bf16x8*>(x + c);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) s += (float)xv[e] * (wz ? wz[c + e] : 1.0f);
+    }
+  } else {
+    for (int c = lane; c < cols; c += 64) s += (float)x[c] * (wz ? wz[c] : 1.0f);
+  }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
   if (lane == 0) out[z * so + row] = s;

@@ -312,24 +312,80 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
     *xt = vh + 4 * nt, *gh = vh + 5 * nt;
   const int s = threadIdx.x;
   const int g0 = grp_off[blockIdx.x], g1 = grp_off[blockIdx.x + 1];
-  // locate this thread's tree; every thread walks the group (a few trees)
-  int sec_base = 0, node_base = 0, my_tree = -1, tree_sec_base = 0, tree_node_base = 0;
-  int nlvl_max = 0;
+  // Sections of the group's trees, numbered tree by tree in breadth-first
+  // order ("BFS slot"), are redistributed over the threads in level-major
+  // order (level, tree, section): the wavefronts then hold one or two levels
+  // each, so a level's elimination and back-substitution run on few
+  // wavefronts.  locate() resolves a BFS slot to its tree and section.
+  int sec_base = 0, node_base = 0, nlvl_max = 0;
   for (int k = g0; k < g1; ++k) {
     const int t = order[k];
-    const int ns = A.pl.sec_off[t + 1] - A.pl.sec_off[t];
-    const int nn = (int)(A.pl.node_off[t + 1] - A.pl.node_off[t]);
+    sec_base += A.pl.sec_off[t + 1] - A.pl.sec_off[t];
+    node_base += (int)(A.pl.node_off[t + 1] - A.pl.node_off[t]);
     const int nl = A.pl.lvl_off[t + 1] - A.pl.lvl_off[t] - 1;
     nlvl_max = nl > nlvl_max ? nl : nlvl_max;
-    if (s >= sec_base && s < sec_base + ns) {
-      my_tree = t;
-      tree_sec_base = sec_base;
-      tree_node_base = node_base;
-    }
-    sec_base += ns;
-    node_base += nn;
   }
   if (sec_base > nt || node_base > node_cap) return;  // launch contract violated
+  struct Loc { int tree, k, sec_base, node_base, ls, lvl; };
+  auto locate = [&](int slot) {
+    Loc L = {-1, -1, 0, 0, 0, -1};
+    int sb = 0, nb = 0;
+    for (int k = g0; k < g1; ++k) {
+      const int t = order[k];
+      const int ns = A.pl.sec_off[t + 1] - A.pl.sec_off[t];
+      if (slot >= sb && slot < sb + ns) {
+        L.tree = t; L.k = k; L.sec_base = sb; L.node_base = nb; L.ls = slot - sb;
+        const int32_t* lp = A.pl.lvl_ptr + A.pl.lvl_off[t];
+        const int nl = A.pl.lvl_off[t + 1] - A.pl.lvl_off[t] - 1;
+        for (int l = 0; l < nl; ++l)
+          if (L.ls >= lp[l] && L.ls < lp[l + 1]) L.lvl = l;
+      }
+      sb += ns;
+      nb += (int)(A.pl.node_off[t + 1] - A.pl.node_off[t]);
+    }
+    return L;
+  };
+  // level-major rank of this thread's BFS slot
+  int32_t* new_of = reinterpret_cast<int32_t*>(vh);   // BFS slot -> thread
+  int32_t* who = reinterpret_cast<int32_t*>(vt);      // thread -> BFS slot
+  who[s] = -1;
+  __syncthreads();
+  {
+    const Loc L = locate(s);
+    if (L.tree >= 0) {
+      int rank = 0;
+      for (int k = g0; k < g1; ++k) {
+        const int t = order[k];
+        const int32_t* lp = A.pl.lvl_ptr + A.pl.lvl_off[t];
+        const int nl = A.pl.lvl_off[t + 1] - A.pl.lvl_off[t] - 1;
+        rank += lp[L.lvl < nl ? L.lvl : nl];                       // levels above
+        if (k < L.k && L.lvl < nl) rank += lp[L.lvl + 1] - lp[L.lvl];  // earlier trees
+      }
+      rank += L.ls - A.pl.lvl_ptr[A.pl.lvl_off[L.tree] + L.lvl];
+      new_of[s] = rank;
+      who[rank] = s;
+    }
+  }
+  __syncthreads();
+  const int bfs = who[s];
+  const Loc L = bfs >= 0 ? locate(bfs) : Loc{-1, -1, 0, 0, 0, -1};
+  const int my_tree = L.tree, tree_node_base = L.node_base;
+  const bool own = my_tree >= 0;
+  int hd = 0, len = 0, cf = 0, nc = 0, psec = -1, my_lvl = -1;
+  int64_t no = 0;
+  if (own) {
+    const int32_t so = A.pl.sec_off[my_tree];
+    no = A.pl.node_off[my_tree];
+    hd = A.pl.sec_head[so + L.ls];
+    len = A.pl.sec_len[so + L.ls];
+    nc = A.pl.sec_nchild[so + L.ls];
+    // children are consecutive at the next level, so they stay consecutive
+    cf = nc > 0 ? new_of[A.pl.sec_child[so + L.ls] + L.sec_base] : 0;
+    const int ps = A.pl.sec_parent[so + L.ls];
+    psec = ps >= 0 ? new_of[ps + L.sec_base] : -1;
+    my_lvl = L.lvl;
+  }
+  __syncthreads();
   // stage the group's axial conductances into the block's node table
   node_base = 0;
   for (int k = g0; k < g1; ++k) {
@@ -338,24 +394,6 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
     const int nn = (int)(A.pl.node_off[t + 1] - to);
     for (int i = s; i < nn; i += nt) gs[node_base + i] = A.g[to + i];
     node_base += nn;
-  }
-  const bool own = my_tree >= 0;
-  int hd = 0, len = 0, cf = 0, nc = 0, psec = -1, my_lvl = -1;
-  int64_t no = 0;
-  if (own) {
-    const int ls = s - tree_sec_base;
-    const int32_t so = A.pl.sec_off[my_tree];
-    no = A.pl.node_off[my_tree];
-    hd = A.pl.sec_head[so + ls];
-    len = A.pl.sec_len[so + ls];
-    cf = A.pl.sec_child[so + ls] + tree_sec_base;
-    nc = A.pl.sec_nchild[so + ls];
-    const int ps = A.pl.sec_parent[so + ls];
-    psec = ps >= 0 ? ps + tree_sec_base : -1;
-    const int32_t* lp = A.pl.lvl_ptr + A.pl.lvl_off[my_tree];
-    const int nl = A.pl.lvl_off[my_tree + 1] - A.pl.lvl_off[my_tree] - 1;
-    for (int l = 0; l < nl; ++l)
-      if (ls >= lp[l] && ls < lp[l + 1]) my_lvl = l;
   }
   const int gb = tree_node_base + hd;  // block-local index of the section head
   const T* arp = A.area + no + hd;

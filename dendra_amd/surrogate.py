@@ -194,6 +194,9 @@ class ThresholdSurrogate:
     def train_step(self, idx, B, lr, b1=0.9, b2=0.999, eps=1e-8, splits=8):
         """One Adam step of every member on its minibatch idx[z] (int32 [nz, B])."""
         nz = self.nz
+        # weight-gradient products split the batch in multiples of the 64-wide K tile
+        while splits > 1 and B % (64 * splits):
+            splits //= 2
         bf, bft, P, G = self.bf, self.bft, self.params, self.grads
         buf = self._buf
         X0 = buf("x0", (nz, B, F_PAD), _BF)
