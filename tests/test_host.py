@@ -238,3 +238,64 @@ def test_slot_map_matches_definition():
                 j = np.arange(sizes[cell])
                 want = t * 64 * R + (j % nr) * 64 + sub * P + j // nr
                 assert np.array_equal(got[off[cell] + j], want)
+
+
+def test_entry_points_reject_invalid_arguments_before_any_launch():
+    """Argument validation happens on the host before any HIP call, so the
+    error contract (DN_EINVAL = -1, DN_ERANGE = -3) is checkable without a GPU."""
+    from dendra_amd import _lib
+
+    lib = _lib.load()
+    hh = np.array([0.12, 0.036, 3e-4, 50.0, -77.0, -54.3, 1.0])
+    hp = hh.ctypes.data
+    fake = 1 << 20  # never dereferenced: validation fails first
+    # fit forward: rcap must be 4 or 8; null task arrays
+    args = [1, 5, fake, fake] + [fake] * 6 + [4, fake, fake, fake, fake, fake, 8, 1.0, 0.0,
+                                              1.0, 0, 10, 0.01, hp, fake, None, None, 1,
+                                              None, None, None]
+    assert lib.dn_fit_forward_f32(*args) == -1
+    args[1] = 4
+    args[2] = None
+    assert lib.dn_fit_forward_f32(*args) == -1
+    # tau must be positive
+    args[2] = fake
+    args[19] = 0.0
+    assert lib.dn_fit_forward_f64(*args) == -1
+    # adjoint needs a trajectory
+    adj = [1, 4, fake, fake] + [fake] * 6 + [4, fake, fake, fake, fake, fake, 8, 1.0, 0.0,
+                                             1.0, 0, 10, 0.01, hp, None, fake, fake, fake,
+                                             fake, None]
+    assert lib.dn_fit_adjoint_f32(*adj) == -1
+    # knot CSR: null pointer table
+    assert lib.dn_fit_csr_knots_f32(4, 2, None, fake, fake, fake, 1, 1, fake, 1, 1, None) == -1
+    # threshold: rcap 4 only, rtol > 0, fix_up > 1
+    st0 = np.zeros(4).ctypes.data
+    thr = [1, 4] + [fake] * 10 + [100, 0.005, hp, st0, 0.0, 0.005, 2.0, 10, 25] + \
+        [fake] * 4 + [None]
+    thr[1] = 8
+    assert lib.dn_threshold_f64(*thr) == -1
+    thr[1] = 4
+    thr[17] = 0.0
+    assert lib.dn_threshold_f64(*thr) == -1
+    thr[17] = 0.005
+    thr[18] = 1.0
+    assert lib.dn_threshold_f64(*thr) == -1
+    # MLP products: M, N multiples of 128, K of 64 * splits
+    g = [0, 1, 100, 128, 128, 1, fake, 128, 0, fake, 128, 0, fake, 128, None, 0, 0, fake,
+         128, 0, None, 0, 0, None, 0, 0, None]
+    assert lib.dn_mlp_gemm(*g) == -1
+    g[2], g[4] = 128, 96
+    assert lib.dn_mlp_gemm(*g) == -1
+    g[4], g[5] = 128, 3
+    assert lib.dn_mlp_gemm(2, *g[1:5], 3, *g[6:]) == -1
+    # epilogue 0 needs a bias
+    g[5] = 1
+    g[12] = None
+    assert lib.dn_mlp_gemm(*g) == -1
+    assert lib.dn_mlp_out_bwd(1, 100, 512, fake, 100, fake, 512, fake, 1, fake, fake, 1,
+                              None) == -1
+    assert lib.dn_mlp_gather(1, 64, 129, fake, fake, fake, fake, fake, fake, None) == -1
+    assert lib.dn_mlp_adam(10, 3, 3, fake, fake, fake, fake, 1e-3, 0.9, 0.999, 1e-8, None,
+                           None, None) == -1
+    assert lib.dn_mlp_ensemble(10, 9, fake, 10, fake, 0.0, 1.0, 0.03, fake, fake,
+                               None) == -1
