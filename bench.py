@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import argparse
 import json
+from dataclasses import asdict
 import os
 import sys
 import time
@@ -50,6 +51,8 @@ def _parse():
                     help="fraction of cells in the CPU oracle sample (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=800)
     ap.add_argument("--waves-per-block", type=int, default=4)
+    ap.add_argument("--tune", type=int, default=1,
+                    help="autotune the launch parameters before the timed region (0 = defaults)")
     ap.add_argument("--only", choices=["all", "cables", "trees", "none"], default="all",
                     help="parts of the uneven population to run (none: only the other lines)")
     ap.add_argument("--network-steps", type=int, default=4000,
@@ -111,9 +114,24 @@ def population_bench(args, dev, hh):
           f"{work.trees.total} tree compartments, {work.n_steps} steps", file=sys.stderr)
     use_c = args.only in ("all", "cables")
     use_t = args.only in ("all", "trees")
-    cab = CablePopulation(work.cables, work.dt, hh, torch.float32, dev) if use_c else None
-    tre = TreePopulation(work.trees, work.dt, hh, torch.float32, dev) if use_t else None
     wave = work.wave
+    tuned = {}
+    t_tune = time.time()
+    if args.tune:
+        from dendra_amd.autotune import tune_cables, tune_trees
+
+        cab = tre = None
+        if use_c:
+            ct, cab = tune_cables(work.cables, work.dt, hh, torch.float32, dev, wave=wave)
+            args.waves_per_block = ct.waves_per_block
+            tuned["cables"] = asdict(ct)
+        if use_t:
+            tre = TreePopulation(work.trees, work.dt, hh, torch.float32, dev)
+            tuned["trees"] = asdict(tune_trees(tre, wave=wave))
+        tuned["tune_s"] = time.time() - t_tune
+    else:
+        cab = CablePopulation(work.cables, work.dt, hh, torch.float32, dev) if use_c else None
+        tre = TreePopulation(work.trees, work.dt, hh, torch.float32, dev) if use_t else None
     main_stream = torch.cuda.current_stream(dev)
     side = torch.cuda.Stream(dev)
     ev = {k: [] for k in ("c0", "c1", "t0", "t1")}
@@ -179,6 +197,7 @@ def population_bench(args, dev, hh):
             "cables": args.cables if use_c else 0, "trees": args.trees if use_t else 0,
             "compartments": comps, "time_steps": work.n_steps,
             "compartment_steps_per_run": cs_per_run, "path": "persistent",
+            "tuned": tuned,
         },
     }
     if c_ms:

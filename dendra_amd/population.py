@@ -245,12 +245,39 @@ class TreePopulation:
         self.hh_arr = self.hh.array().astype(_DT[dtype])
         sizes = spec.sizes
         self.kmax = int(sizes.max())
-        # Launch buckets.  Trees of at most 512 sections run one thread per
-        # section (layout 1), packed first-fit-decreasing into blocks of
-        # `block` threads (trees above `block` sections get 512-thread
-        # blocks of their own bucket); larger trees run eight node slots per
-        # thread (layout 0), bucketed by size.
+        self.bucket_edges = tuple(bucket_edges)
+        self.slots = slots
+        self.set_block(block)
+        self.reset()
+
+    @property
+    def total(self) -> int:
+        return self.spec.total
+
+    def _plan_args(self):
+        return (self.spec.n_trees, _lib.ptr(self.node_off), _lib.ptr(self.parent),
+                _lib.ptr(self.ch_ptr), _lib.ptr(self.ch_idx), _lib.ptr(self.sec_off),
+                _lib.ptr(self.sec["head"]), _lib.ptr(self.sec["len"]),
+                _lib.ptr(self.sec["child"]), _lib.ptr(self.sec["nchild"]),
+                _lib.ptr(self.sec["ptail"]), _lib.ptr(self.sec["parent"]),
+                _lib.ptr(self.lvl_off),
+                _lib.ptr(self.lvl_ptr), self.smax, self.lmax)
+
+    def set_block(self, block: int):
+        """(Re)pack the trees into launch buckets for workgroups of `block`
+        threads.  Trees of at most 512 sections run one thread per section
+        (layout 1), packed first-fit-decreasing into blocks of `block`
+        threads (trees above `block` sections get 512-thread blocks of their
+        own bucket); larger trees run eight node slots per thread (layout 0),
+        bucketed by size."""
+        if block not in (64, 128, 256, 512):
+            raise ValueError("block must be 64, 128, 256 or 512")
+        dev = self.device
+        i32 = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.int32, device=dev)
+        sizes = self.spec.sizes
+        self.block = block
         self.buckets = []
+        self._streams = None
         by_sec = self.n_sec <= 512
         for lo_s, hi_s, threads in ((0, block, block), (block, 512, 512)):
             sel = np.flatnonzero(by_sec & (self.n_sec > lo_s) & (self.n_sec <= hi_s))
@@ -273,31 +300,17 @@ class TreePopulation:
             self.buckets.append(_TreeBucket(i32(order), len(order), int(max(nodes)),
                                             threads, 1, i32(goff), len(groups)))
         lo = 0
-        for hi in bucket_edges:
+        for hi in self.bucket_edges:
             sel = np.flatnonzero(~by_sec & (sizes > lo) & (sizes <= hi))
             lo = hi
             if len(sel):
                 sel = sel[np.argsort(-sizes[sel], kind="stable")]
                 kmax = int(sizes[sel].max())
-                threads = int(min(512, max(64, -(-kmax // (slots * 64)) * 64)))
+                threads = int(min(512, max(64, -(-kmax // (self.slots * 64)) * 64)))
                 self.buckets.append(_TreeBucket(i32(sel), len(sel), kmax, threads, 0,
                                                 None, 0))
-        if (~by_sec & (sizes > bucket_edges[-1])).any():
-            raise ValueError(f"tree larger than {bucket_edges[-1]} compartments")
-        self.reset()
-
-    @property
-    def total(self) -> int:
-        return self.spec.total
-
-    def _plan_args(self):
-        return (self.spec.n_trees, _lib.ptr(self.node_off), _lib.ptr(self.parent),
-                _lib.ptr(self.ch_ptr), _lib.ptr(self.ch_idx), _lib.ptr(self.sec_off),
-                _lib.ptr(self.sec["head"]), _lib.ptr(self.sec["len"]),
-                _lib.ptr(self.sec["child"]), _lib.ptr(self.sec["nchild"]),
-                _lib.ptr(self.sec["ptail"]), _lib.ptr(self.sec["parent"]),
-                _lib.ptr(self.lvl_off),
-                _lib.ptr(self.lvl_ptr), self.smax, self.lmax)
+        if (~by_sec & (sizes > self.bucket_edges[-1])).any():
+            raise ValueError(f"tree larger than {self.bucket_edges[-1]} compartments")
 
     def reset(self, v_init: float | None = None):
         v0 = self.spec.v_init if v_init is None else v_init
