@@ -200,6 +200,12 @@ def population_bench(args, dev, hh):
             "tuned": tuned,
         },
     }
+    job_bytes = (BYTES_PER_CS * (work.cables.total if use_c else 0)
+                 + BYTES_PER_CS_TREE * (work.trees.total if use_t else 0)) * work.n_steps
+    out["job_roofline"] = {"bound": "hbm", "achieved": job_bytes / wall / 1e9,
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": job_bytes / wall / 1e9 / HBM_PEAK_GBS,
+                           "algorithmic_bytes_per_run": job_bytes}
     if c_ms:
         c_avg = float(np.mean(c_ms)) * 1e-3
         c_bytes = BYTES_PER_CS * work.cables.total * work.n_steps
