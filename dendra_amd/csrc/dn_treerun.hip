@@ -288,12 +288,23 @@ __global__ __launch_bounds__(512) void hh_tree_run_kernel(TreeRunArgs<T> A,
 // counts sum to at most blockDim.x; thread s owns block-local section s.  The
 // thread keeps the V, m, h, n of its section's nodes (<= kSecMax, contiguous)
 // in registers for the whole run, re-reads their membrane coefficients from
-// L2 each step, and performs their gating, their share of the axial assembly,
-// their chain elimination and their back-substitution.  LDS carries:
+// L2 each step, and performs their gating and their share of the axial
+// assembly.  The solve is a two-level contraction of each tree:
+//   1. compress: every section eliminates its nodes but the tail, head
+//      first, all sections at once (the chain becomes one edge from its tail
+//      to the parent section's tail);
+//   2. the tree of section tails (the section tree's shape, one node per
+//      section) is eliminated level by level, deepest first, and
+//      back-substituted root level first: one reciprocal per level;
+//   3. the interior nodes of every section are back-substituted at once.
+// The dependent path per step is one section chain plus two passes over the
+// section levels, instead of every section chain on a root-to-leaf path.
+// LDS carries:
 //   gs      axial conductance of every node of the block's trees
 //   vh/vt   head and tail voltage (axial currents across section joins)
-//   cD/cB   the head's Schur contribution (g/d) old_d and (g/d) b to the parent
-//   xt      the tail's solution, needed by the child sections' heads.
+//   cD/cB   a section's contribution to its parent tail's self term and
+//           right-hand side (compressed chain + Schur term of its tail)
+//   xt      the tails' solutions.
 // The level loops run to the deepest tree of the block; a thread acts when
 // the loop reaches its own section's level.
 // ---------------------------------------------------------------------------
@@ -451,55 +462,72 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
       for (int r = 0; r < kSecMax; ++r)
         if (r == len - 1) F[r] += tail_in;
     }
-    // elimination, deepest level first
+    // Chain compression, every section at once.  The matrix is
+    // diag_i = M_i + (edges of i), off-diagonal -g on each edge, with g_r the
+    // edge from node r to its parent (node r-1, or the parent section's tail
+    // for the head).  Eliminating the section's nodes head first, all but
+    // its tail, leaves
+    //   * one equivalent edge e from the tail to the parent section's tail,
+    //   * (dM, dF) added to the parent tail's self term and right-hand side,
+    //   * per eliminated node r: x_r = a_r + b_r x_ptail + c_r x_{r+1},
+    //     kept in F[r], M[r] and gq[r+1] (each is dead once node r is done).
+    // Every term of a self term or an edge stays a sum of positive products.
+    // All conductances are read before the dependent sweep (reads past the
+    // section end stay inside the block's LDS and are unused).
+    T gq[kSecMax];
+#pragma unroll
+    for (int r = 0; r < kSecMax; ++r) gq[r] = gs[gb + r];
+    T e = gq[0], dM = T(0), dF = T(0), Mc = T(0), Fc = T(0);
+    if (len > 0) { Mc = M[0]; Fc = F[0]; }
+#pragma unroll
+    for (int r = 0; r + 1 < kSecMax; ++r) {
+      if (r + 1 < len) {
+        const T gn = gq[r + 1];
+        const T inv = frcp(Mc + e + gn);
+        const T be = e * inv, ce = gn * inv;
+        dM += be * Mc;
+        dF += be * Fc;
+        F[r] = Fc * inv;
+        M[r] = be;
+        gq[r + 1] = ce;
+        Mc = M[r + 1] + ce * Mc;
+        Fc = F[r + 1] + ce * Fc;
+        e *= ce;
+      }
+    }
+    // The sections' tails form a tree with the section tree's shape: node s
+    // has self term Mc + sum over child sections c of (dM_c + fac_c D_c),
+    // right-hand side Fc + sum of (dF_c + fac_c B_c), edge e to its parent.
+    // Elimination, deepest level first; one node per section per level.
+    T fac = T(0), Binv = T(0);
     for (int l = nlvl_max - 1; l >= 0; --l) {
       if (my_lvl == l) {
-        // All conductances of the chain are read before the dependent sweep
-        // (reads past the section end stay inside the block's LDS and are
-        // unused), so the sweep waits on LDS once instead of once per node.
-        T gq[kSecMax];
-#pragma unroll
-        for (int r = 0; r < kSecMax; ++r) gq[r] = gs[gb + r];
-        T pD = T(0), pB = T(0);
-        for (int c = cf; c < cf + nc; ++c) { pD += cD[c]; pB += cB[c]; }
-        T D = T(0), B = T(0), fac = T(0);
-#pragma unroll
-        for (int r = kSecMax - 1; r >= 0; --r) {
-          if (r < len) {
-            if (r == len - 1) {
-              D = M[r] + pD;
-              B = F[r] + pB;
-            } else {
-              D = M[r] + fac * D;
-              B = F[r] + fac * B;
-            }
-            const T gr = gq[r];
-            const T inv = frcp(D + gr);
-            fac = gr * inv;
-            // back-substitution is then x_r = B_r inv_r + fac_r x_parent
-            M[r] = fac;
-            F[r] = B * inv;
-          }
-        }
-        cD[s] = fac * D;
-        cB[s] = fac * B;
+        T D = Mc, B = Fc;
+        for (int c = cf; c < cf + nc; ++c) { D += cD[c]; B += cB[c]; }
+        const T inv = frcp(D + e);
+        fac = e * inv;
+        Binv = B * inv;
+        cD[s] = dM + fac * D;
+        cB[s] = dF + fac * B;
       }
       __syncthreads();
     }
-    // back-substitution, root level first
+    // tails, root level first
     for (int l = 0; l < nlvl_max; ++l) {
-      if (my_lvl == l) {
-        T x = psec >= 0 ? xt[psec] : T(0);
-#pragma unroll
-        for (int r = 0; r < kSecMax; ++r) {
-          if (r < len) {
-            x = F[r] + M[r] * x;
-            v[r] += x;
-          }
-        }
-        xt[s] = x;
-      }
+      if (my_lvl == l) xt[s] = Binv + fac * (psec >= 0 ? xt[psec] : T(0));
       __syncthreads();
+    }
+    // interior nodes of every section at once, tail first
+    if (own) {
+      const T xp = psec >= 0 ? xt[psec] : T(0);
+      T x = xt[s];
+#pragma unroll
+      for (int r = kSecMax - 1; r >= 0; --r) {
+        if (r < len) {
+          if (r < len - 1) x = F[r] + M[r] * xp + gq[r + 1 < kSecMax ? r + 1 : r] * x;
+          v[r] += x;
+        }
+      }
     }
     if (A.rec && ((A.step0 + st + 1) % A.rec_every) == 0) {
       const int64_t row = (A.step0 + st + 1) / A.rec_every - 1;
