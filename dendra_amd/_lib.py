@@ -117,6 +117,11 @@ for _sfx, _f in (("f32", f32), ("f64", f64)):
     _SIGNATURES[f"dn_fit_adjoint_{_sfx}"] = (i32, _fit_common(_f) + [p, p, p, p, p, p])
     _SIGNATURES[f"dn_fit_csr_knots_{_sfx}"] = (
         i32, [i64, i32, p, p, p, p, i64, i64, p, i64, i64, p])
+    _SIGNATURES[f"dn_tree_contract_{_sfx}"] = (
+        i32, [p, p, p, p, i64, i64, p, i32] + [p] * 5 + [i32] + [p] * 8 + [p, i64, p])
+
+_SIGNATURES["dn_tree_contract_work_elems"] = (i64, [i64, i64])
+_SIGNATURES["dn_tree_contract_max_sections"] = (i32, [i32])
 
 
 def _bind(lib) -> None:

@@ -62,6 +62,42 @@ int dn_tree_dhs_f64(const double* d_mem, const double* a_geom, const double* b,
                     const int64_t* layer_ptr, double* x, int64_t B, int64_t K, int64_t L,
                     double* scratch, int64_t scratch_elems, hipStream_t stream);
 
+/* ---- tree solve by contraction --------------------------------------------
+ * Replaces the tree-contraction solve(d, g, rhs, plan)
+ * (dendra/models/integrators/triton/tree_contraction.py:748-770; the
+ * integrator factory dn.tree_contraction, integrators/__init__.py:73-110,
+ * binds it for _dhs._step, integrators/tree.py:186-189,841-867).  d, g, rhs,
+ * x: (B, K) in the caller's node order; g is the edge conductance to the
+ * parent (to ground at the root).  The plan tables (section chains and the
+ * rake/compress rounds over them) are int32 device arrays built by
+ * dendra_amd.contraction.TreeContractionPlan: perm (K), sec_* (n_sec),
+ * elim_ptr (n_rounds + 1) over elim_sec/elim_par/elim_child, recv_ptr
+ * (n_rounds + 1) over recv_sec, src_ptr (n_recv + 1) over src.  work holds
+ * dn_tree_contract_work_elems(B, K) elements of the solve's dtype.  Returns
+ * DN_ERANGE when n_sec exceeds dn_tree_contract_max_sections. */
+int64_t dn_tree_contract_work_elems(int64_t B, int64_t K);
+int dn_tree_contract_max_sections(int elem_bytes);
+int dn_tree_contract_f32(const float* d, const float* g, const float* rhs, float* x,
+                         int64_t B, int64_t K, const int32_t* perm, int n_sec,
+                         const int32_t* sec_head, const int32_t* sec_len,
+                         const int32_t* sec_child, const int32_t* sec_nchild,
+                         const int32_t* sec_parent, int n_rounds, const int32_t* elim_ptr,
+                         const int32_t* elim_sec, const int32_t* elim_par,
+                         const int32_t* elim_child, const int32_t* recv_ptr,
+                         const int32_t* recv_sec, const int32_t* src_ptr,
+                         const int32_t* src, float* work, int64_t work_elems,
+                         hipStream_t stream);
+int dn_tree_contract_f64(const double* d, const double* g, const double* rhs, double* x,
+                         int64_t B, int64_t K, const int32_t* perm, int n_sec,
+                         const int32_t* sec_head, const int32_t* sec_len,
+                         const int32_t* sec_child, const int32_t* sec_nchild,
+                         const int32_t* sec_parent, int n_rounds, const int32_t* elim_ptr,
+                         const int32_t* elim_sec, const int32_t* elim_par,
+                         const int32_t* elim_child, const int32_t* recv_ptr,
+                         const int32_t* recv_sec, const int32_t* src_ptr,
+                         const int32_t* src, double* work, int64_t work_elems,
+                         hipStream_t stream);
+
 /* ---- fused HH + backward Euler, unbranched cables -------------------------
  * Replaces the per-step transition of _bwd_euler_ub
  * (dendra/models/integrators/implicit.py:561-664) in Integrator._step order
