@@ -146,6 +146,7 @@ class TreeContractionPlan:
             "sec_parent": sp.sec_parent,
         }
         self._devices: dict = {}
+        self._caps = None
 
     @property
     def n_rounds(self) -> int:
@@ -157,6 +158,20 @@ class TreeContractionPlan:
             raise ValueError("CUDA tree contraction requires a CUDA device")
         if device.index is None:
             device = torch.device("cuda", torch.cuda.current_device())
+        self.section_cap(4)
+        return self.tables_on(device)
+
+    def section_cap(self, elem_bytes: int) -> int:
+        """Most sections one system may have at this element size (the
+        kernel's LDS bound); read from the library once per plan."""
+        if self._caps is None:
+            lib = _lib.load()
+            self._caps = {e: int(lib.dn_tree_contract_max_sections(e)) for e in (4, 8)}
+        return self._caps[elem_bytes]
+
+    def tables_on(self, device):
+        """The plan's int32 tables (and the int64 parent) on any device."""
+        device = torch.device(device)
         if device not in self._devices:
             i32 = lambda a: torch.as_tensor(np.asarray(a, dtype=np.int32), device=device)
             sc = self.schedule
@@ -165,6 +180,7 @@ class TreeContractionPlan:
                 v = getattr(sc, k)
                 t[k] = i32(v if len(v) else np.zeros(1))
             par = torch.as_tensor(self.parent, dtype=torch.int64, device=device)
+            t["parent_i64"] = par
             t["is_root"] = par < 0
             t["parent_safe"] = torch.where(t["is_root"], torch.zeros_like(par), par)
             self._devices[device] = t
@@ -176,9 +192,22 @@ def prepare(plan: TreeContractionPlan, device):
     return plan.prepare(device)
 
 
-def _contract_raw(d, g, rhs, plan: TreeContractionPlan):
+TABLE_NAMES = ("perm", "sec_head", "sec_len", "sec_child", "sec_nchild", "sec_parent",
+               "ptr", "sec", "par", "child", "rptr", "recv", "sptr", "src")
+
+
+def plan_tables(plan: TreeContractionPlan, device) -> tuple:
+    """The plan's device tables in the entry point's argument order."""
+    t = plan.prepare(device)
+    return tuple(t[k] for k in TABLE_NAMES)
+
+
+def contract_tables(d, g, rhs, tables, n_sec: int, n_rounds: int):
+    """One launch of the contraction kernel on explicit plan tables (the
+    opaque operator of dendra_amd.functional calls this)."""
     lib = _lib.load()
-    t = plan.prepare(d.device)
+    (perm, sec_head, sec_len, sec_child, sec_nchild, sec_parent,
+     ptr, sec, par, child, rptr, recv, sptr, src) = (v.contiguous() for v in tables)
     d, g, rhs = (v.contiguous() for v in (d, g, rhs))
     B, K = map(int, d.shape)
     x = torch.empty_like(rhs, memory_format=torch.contiguous_format)
@@ -187,13 +216,17 @@ def _contract_raw(d, g, rhs, plan: TreeContractionPlan):
     f64 = rhs.dtype == torch.float64
     fn = lib.dn_tree_contract_f64 if f64 else lib.dn_tree_contract_f32
     p = _lib.ptr
-    st = fn(p(d), p(g), p(rhs), p(x), B, K, p(t["perm"]), plan.n_sections,
-            p(t["sec_head"]), p(t["sec_len"]), p(t["sec_child"]), p(t["sec_nchild"]),
-            p(t["sec_parent"]), plan.n_rounds, p(t["ptr"]), p(t["sec"]), p(t["par"]),
-            p(t["child"]), p(t["rptr"]), p(t["recv"]), p(t["sptr"]), p(t["src"]),
-            p(work), n_work, _lib.stream_handle(rhs.device))
+    st = fn(p(d), p(g), p(rhs), p(x), B, K, p(perm), int(n_sec), p(sec_head), p(sec_len),
+            p(sec_child), p(sec_nchild), p(sec_parent), int(n_rounds), p(ptr), p(sec),
+            p(par), p(child), p(rptr), p(recv), p(sptr), p(src), p(work), n_work,
+            _lib.stream_handle(rhs.device))
     _lib.check(st, "tree contraction solve")
     return x
+
+
+def _contract_raw(d, g, rhs, plan: TreeContractionPlan):
+    return contract_tables(d, g, rhs, plan_tables(plan, d.device), plan.n_sections,
+                           plan.n_rounds)
 
 
 def _gradients(y, x, plan, device, need):
@@ -224,8 +257,9 @@ class _TreeContraction(torch.autograd.Function):
         return gd, gg, gr, None
 
 
-def solve(d, g, rhs, plan: TreeContractionPlan):
-    """Solve a scalar Hines right-hand side in float32 or float64 on the GPU."""
+def _check_inputs(d, g, rhs, plan: TreeContractionPlan):
+    """The reference's argument checks (tree_contraction.py:750-766), plus the
+    LDS bound on one system's section count."""
     if d.ndim != 2 or d.shape[1] != plan.size:
         raise ValueError("tree contraction inputs must have shape (B, K) matching the plan")
     if d.dtype not in (torch.float32, torch.float64):
@@ -236,12 +270,17 @@ def solve(d, g, rhs, plan: TreeContractionPlan):
                 or value.layout != torch.strided):
             raise ValueError("tree contraction inputs must be same-shaped strided CUDA "
                              "tensors with matching dtypes")
-    if d.shape[0] < 1:
-        return torch.empty_like(rhs, memory_format=torch.contiguous_format)
-    cap = int(_lib.load().dn_tree_contract_max_sections(d.element_size()))
+    cap = plan.section_cap(d.element_size())
     if plan.n_sections > cap:
         raise ValueError(f"tree contraction plan has {plan.n_sections} sections; at most "
                          f"{cap} fit one workgroup in {d.dtype}")
+
+
+def solve(d, g, rhs, plan: TreeContractionPlan):
+    """Solve a scalar Hines right-hand side in float32 or float64 on the GPU."""
+    _check_inputs(d, g, rhs, plan)
+    if d.shape[0] < 1:
+        return torch.empty_like(rhs, memory_format=torch.contiguous_format)
     return _TreeContraction.apply(d, g, rhs, plan)
 
 
