@@ -1,0 +1,247 @@
+"""3x3 block solvers of the double-layer extracellular (ExtCell) model.
+
+Drop-ins for the reference's block solvers, with the same shapes, argument
+checks, non-mutating inputs, fresh outputs and autograd contracts:
+
+* ``thomas_solve_cuda_bt(lower, main, upper, rhs)`` / ``thomas_bt3_solve``
+  (dendra/models/integrators/triton/bt_kernel.py:192-298): main (B,K,3,3),
+  diagonal coupling blocks lower/upper (B,K-1,3), rhs (B,K,3).  Backward: the
+  transposed system (upper and lower exchanged, main blocks transposed) and
+  ``grad_main = -(g x^T)``, ``grad_lower = -(g[1:] x[:-1])``,
+  ``grad_upper = -(g[:-1] x[1:])`` (bt_kernel.py:266-292).
+* ``block_spd_solve(lower, main, upper, rhs)`` and the low-level
+  ``solve_bt_spd_fwd`` / ``solve_bt_spd_with_chol`` (triton/bt_spd_kernel.py:
+  575-700): block Cholesky; the backward reuses the factors and returns the
+  symmetric part of ``-(g x^T)`` for main (integrators/_cuda_functional.py:
+  306-333).  A block still indefinite after two diagonal shifts raises.
+* ``dhs_bt_solve(main, edge, rhs, parent_idx, order, layer_ptr, threads)``
+  (triton/dhs_kernel_bt.py): tree with 3x3 node blocks and per-component edge
+  conductances (``A_ii = main_i + diag(e_i) + sum_c diag(e_c)``,
+  ``A_ip = -diag(e_i)``); backward: the same solve with transposed main
+  blocks, ``grad_main = -(g x^T)``, ``grad_edge = -(g_i - g_p)(x_i - x_p)``
+  per component (``-g x`` at the root) (_cuda_functional.py:365-374).
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from .solvers import validate_threads
+
+_FLOAT = (torch.float32, torch.float64)
+_INDEX = (torch.int32, torch.int64)
+KIND_THOMAS, KIND_SPD, KIND_TREE = 0, 1, 2
+
+
+def _float_group(named, what="dendra_amd block solvers"):
+    first = named[0][1]
+    for name, t in named:
+        if t.device != first.device:
+            raise ValueError(f"{name} must be on device {first.device}; got {t.device}")
+        if t.dtype != first.dtype:
+            raise TypeError(f"{name} must have dtype {first.dtype}; got {t.dtype}")
+    if first.device.type != "cuda":
+        raise ValueError(f"{what} require GPU tensors")
+    if first.dtype not in _FLOAT:
+        raise TypeError(f"{what} support only float32 and float64")
+
+
+def validate_block_tridiagonal(lower, main, upper, rhs):
+    """Contract of the reference's validate_block_tridiagonal
+    (triton/_contracts.py:162-196)."""
+    named = (("lower", lower), ("main", main), ("upper", upper), ("rhs", rhs))
+    for name, t in named:
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"{name} must be a torch.Tensor")
+    if rhs.ndim != 3 or rhs.shape[-1] != 3:
+        raise ValueError(f"rhs must have shape (B, K, 3), got {tuple(rhs.shape)}")
+    batch, size, _ = map(int, rhs.shape)
+    if batch < 1 or size < 1:
+        raise ValueError("B and K must both be >= 1")
+    band = (batch, size - 1, 3)
+    if tuple(lower.shape) != band:
+        raise ValueError(f"lower must have shape {band}, got {tuple(lower.shape)}")
+    if tuple(upper.shape) != band:
+        raise ValueError(f"upper must have shape {band}, got {tuple(upper.shape)}")
+    if tuple(main.shape) != (batch, size, 3, 3):
+        raise ValueError(f"main must have shape {(batch, size, 3, 3)}, "
+                         f"got {tuple(main.shape)}")
+    _float_group(named)
+    return batch, size
+
+
+def validate_tree_block(main, edge, rhs, parent_idx, order, layer_ptr, threads):
+    """Contract of the reference's validate_tree_block
+    (triton/_contracts.py:232-268)."""
+    data = (("main", main), ("edge", edge), ("rhs", rhs))
+    topo = (("parent_idx", parent_idx), ("order", order), ("layer_ptr", layer_ptr))
+    for name, t in data + topo:
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"{name} must be a torch.Tensor")
+    if rhs.ndim != 3 or rhs.shape[-1] != 3:
+        raise ValueError(f"rhs must have shape (B, K, 3), got {tuple(rhs.shape)}")
+    batch, size, _ = map(int, rhs.shape)
+    if batch < 1 or size < 1:
+        raise ValueError("B and K must both be >= 1")
+    if tuple(edge.shape) != (batch, size, 3):
+        raise ValueError(f"edge must have shape {(batch, size, 3)}, got {tuple(edge.shape)}")
+    if tuple(main.shape) != (batch, size, 3, 3):
+        raise ValueError(f"main must have shape {(batch, size, 3, 3)}, "
+                         f"got {tuple(main.shape)}")
+    _float_group(data)
+    for name, t in topo:
+        if t.ndim != 1:
+            raise ValueError(f"{name} must be 1-D")
+        if t.dtype not in _INDEX:
+            raise TypeError(f"{name} must have dtype int32 or int64")
+        if t.device != rhs.device:
+            raise ValueError(f"{name} must be on device {rhs.device}; got {t.device}")
+    if parent_idx.numel() != size or order.numel() != size:
+        raise ValueError("parent_idx and order must each contain K entries")
+    if layer_ptr.numel() < 2:
+        raise ValueError("layer_ptr must contain at least two entries")
+    return batch, size, validate_threads(threads)
+
+
+def _work(kind, B, K, like):
+    n = int(_lib.load().dn_block_work_elems(kind, B, K))
+    return torch.empty(max(n, 1), dtype=like.dtype, device=like.device), n
+
+
+def _fn(name, like):
+    return getattr(_lib.load(), f"{name}_{'f64' if like.dtype == torch.float64 else 'f32'}")
+
+
+def _bt3_raw(lower, main, upper, rhs):
+    lower, main, upper, rhs = (t.contiguous() for t in (lower, main, upper, rhs))
+    B, K = map(int, rhs.shape[:2])
+    x = torch.empty_like(rhs, memory_format=torch.contiguous_format)
+    work, n = _work(KIND_THOMAS, B, K, rhs)
+    p = _lib.ptr
+    st = _fn("dn_bt3_thomas", rhs)(p(lower), p(main), p(upper), p(rhs), p(x), B, K,
+                                   p(work), n, _lib.stream_handle(rhs.device))
+    _lib.check(st, "block Thomas solve")
+    return x
+
+
+def solve_bt_spd_fwd(lower, main, upper, rhs):
+    """Factor and solve; returns (x, chol).  Raises when a block stays
+    indefinite after the diagonal shifts (bt_spd_kernel.py:571-581)."""
+    lower, main, upper, rhs = (t.contiguous() for t in (lower, main, upper, rhs))
+    B, K = map(int, rhs.shape[:2])
+    x = torch.empty_like(rhs, memory_format=torch.contiguous_format)
+    chol = torch.empty(B, K, 9, dtype=rhs.dtype, device=rhs.device)
+    info = torch.empty(B, dtype=torch.int32, device=rhs.device)
+    work, n = _work(KIND_SPD, B, K, rhs)
+    p = _lib.ptr
+    st = _fn("dn_bt3_spd", rhs)(p(lower), p(main), p(upper), p(rhs), p(x), p(chol), p(info),
+                                B, K, p(work), n, 1, _lib.stream_handle(rhs.device))
+    _lib.check(st, "block SPD solve")
+    if int(info.max()) > 0:
+        raise RuntimeError("solve_bt_spd_cuda: Cholesky failed after all diagonal bumps")
+    return x, chol
+
+
+def solve_bt_spd_with_chol(lower, upper, chol, rhs):
+    """Solve with the factors of a previous solve_bt_spd_fwd."""
+    lower, upper, chol, rhs = (t.contiguous() for t in (lower, upper, chol, rhs))
+    B, K = map(int, rhs.shape[:2])
+    if tuple(chol.shape) != (B, K, 9):
+        raise ValueError(f"chol must have shape {(B, K, 9)}, got {tuple(chol.shape)}")
+    x = torch.empty_like(rhs, memory_format=torch.contiguous_format)
+    work, n = _work(KIND_SPD, B, K, rhs)
+    p = _lib.ptr
+    st = _fn("dn_bt3_spd", rhs)(p(lower), None, p(upper), p(rhs), p(x), p(chol), None, B, K,
+                                p(work), n, 0, _lib.stream_handle(rhs.device))
+    _lib.check(st, "block SPD solve with factors")
+    return x
+
+
+def _tree_raw(main, edge, rhs, parent, order):
+    main, edge, rhs = (t.contiguous() for t in (main, edge, rhs))
+    parent, order = (t.to(torch.int64).contiguous() for t in (parent, order))
+    B, K = map(int, rhs.shape[:2])
+    x = torch.empty_like(rhs, memory_format=torch.contiguous_format)
+    work, n = _work(KIND_TREE, B, K, rhs)
+    p = _lib.ptr
+    st = _fn("dn_bt3_tree", rhs)(p(main), p(edge), p(rhs), p(parent), p(order), p(x), B, K,
+                                 p(work), n, _lib.stream_handle(rhs.device))
+    _lib.check(st, "block tree solve")
+    return x
+
+
+def _sym(m):
+    return 0.5 * (m + m.transpose(-1, -2))
+
+
+class _BT3Solve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, lower, main, upper, rhs):
+        x = _bt3_raw(lower, main, upper, rhs)
+        ctx.save_for_backward(lower, main, upper, x)
+        return x
+
+    @staticmethod
+    def backward(ctx, grad_x):
+        lower, main, upper, x = ctx.saved_tensors
+        g = _bt3_raw(upper, main.transpose(-1, -2), lower, grad_x)
+        return (-(g[:, 1:] * x[:, :-1]), -(g.unsqueeze(-1) * x.unsqueeze(-2)),
+                -(g[:, :-1] * x[:, 1:]), g)
+
+
+class _SPDSolve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, lower, main, upper, rhs):
+        x, chol = solve_bt_spd_fwd(lower, main, upper, rhs)
+        ctx.save_for_backward(lower, upper, x, chol)
+        return x
+
+    @staticmethod
+    def backward(ctx, grad_x):
+        lower, upper, x, chol = ctx.saved_tensors
+        g = solve_bt_spd_with_chol(lower, upper, chol, grad_x)
+        return (-(g[:, 1:] * x[:, :-1]), -_sym(g.unsqueeze(-1) * x.unsqueeze(-2)),
+                -(g[:, :-1] * x[:, 1:]), g)
+
+
+class _TreeBT3Solve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, main, edge, rhs, parent, order):
+        x = _tree_raw(main, edge, rhs, parent, order)
+        ctx.save_for_backward(main, edge, x, parent, order)
+        return x
+
+    @staticmethod
+    def backward(ctx, grad_x):
+        main, edge, x, parent, order = ctx.saved_tensors
+        g = _tree_raw(main.transpose(-1, -2), edge, grad_x, parent, order)
+        par = parent.to(torch.long)
+        root = (par < 0).reshape(1, -1, 1)
+        safe = par.clamp_min(0)
+        ge = torch.where(root, -(g * x),
+                         -((g - g.index_select(1, safe)) * (x - x.index_select(1, safe))))
+        return -(g.unsqueeze(-1) * x.unsqueeze(-2)), ge, g, None, None
+
+
+def thomas_solve_cuda_bt(lower, main, upper, rhs):
+    """Block Thomas solve of a 3x3-block tridiagonal system."""
+    validate_block_tridiagonal(lower, main, upper, rhs)
+    return _BT3Solve.apply(lower, main, upper, rhs)
+
+
+thomas_bt3_solve = thomas_solve_cuda_bt
+
+
+def block_spd_solve(lower, main, upper, rhs):
+    """Block Cholesky solve of a symmetric positive definite 3x3-block
+    tridiagonal system (main blocks are symmetrised)."""
+    validate_block_tridiagonal(lower, main, upper, rhs)
+    return _SPDSolve.apply(lower, main, upper, rhs)
+
+
+def dhs_bt_solve(main, edge, rhs, parent_idx, order, layer_ptr, threads: int = 16):
+    """Block tree (Hines) solve with 3x3 node blocks over the caller's DHS
+    order (children before parents, as build_dhs_layers produces)."""
+    validate_tree_block(main, edge, rhs, parent_idx, order, layer_ptr, threads)
+    return _TreeBT3Solve.apply(main, edge, rhs, parent_idx, order)

@@ -1,0 +1,396 @@
+// dn_block.hip -- 3x3 block solvers of the double-layer extracellular
+// (ExtCell) model.
+//
+// Drop-ins for the reference's block solvers:
+//   * block Thomas  thomas_bt3_solve (dendra/models/integrators/triton/
+//     bt_kernel.py:192-240): tridiagonal with 3x3 main blocks and diagonal
+//     coupling blocks, lower (B,K-1,3) at (k,k-1), upper (B,K-1,3) at (k,k+1);
+//   * block SPD     solve_bt_spd_fwd_consume_impl / ..._solve_with_chol_impl
+//     (triton/bt_spd_kernel.py:575-700): the same system by block Cholesky,
+//     returning the factors for reuse by the adjoint;
+//   * block tree    _dhs_bt3_adjoint (triton/dhs_kernel_bt.py): Hines matrix
+//     with 3x3 node blocks and per-component edge conductances,
+//       A_ii = main_i + diag(e_i) + sum_children diag(e_c),
+//       A_ip = A_pi = -diag(e_i),  (the root's e is diagonal only).
+//
+// One lane per system (a wavefront solves 64 systems); the elimination
+// coefficients live in a caller workspace laid out [row][component][system],
+// so every workspace access of a wavefront is one coalesced transaction.
+#include "dn_common.hpp"
+
+namespace dn {
+
+// inverse of a row-major 3x3 by its adjugate
+template <typename T>
+__device__ __forceinline__ void inv3(const T* a, T* r) {
+  const T c0 = a[4] * a[8] - a[5] * a[7];
+  const T c1 = a[5] * a[6] - a[3] * a[8];
+  const T c2 = a[3] * a[7] - a[4] * a[6];
+  const T id = T(1) / (a[0] * c0 + a[1] * c1 + a[2] * c2);
+  r[0] = c0 * id;
+  r[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+  r[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+  r[3] = c1 * id;
+  r[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+  r[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+  r[6] = c2 * id;
+  r[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+  r[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+}
+
+template <typename T>
+__device__ __forceinline__ void mv3(const T* m, const T* v, T* r) {
+  r[0] = m[0] * v[0] + m[1] * v[1] + m[2] * v[2];
+  r[1] = m[3] * v[0] + m[4] * v[1] + m[5] * v[2];
+  r[2] = m[6] * v[0] + m[7] * v[1] + m[8] * v[2];
+}
+
+// ---------------------------------------------------------------------------
+// block Thomas: M'_0 = M_0, M'_k = M_k - diag(l) M'^-1_{k-1} diag(u),
+// d'_k = d_k - diag(l) M'^-1_{k-1} d'_{k-1};  x_k = M'^-1_k (d'_k - diag(u_k) x_{k+1})
+// work: [K][12][B] = inverse (9) and d' (3) per row
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void bt3_thomas_kernel(const T* __restrict__ lower,
+                                                         const T* __restrict__ main,
+                                                         const T* __restrict__ upper,
+                                                         const T* __restrict__ rhs,
+                                                         T* __restrict__ x,
+                                                         T* __restrict__ work, int64_t B,
+                                                         int64_t K) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B) return;
+  const T* L = lower + s * (K - 1) * 3;
+  const T* M = main + s * K * 9;
+  const T* U = upper + s * (K - 1) * 3;
+  const T* D = rhs + s * K * 3;
+  T* X = x + s * K * 3;
+  T* W = work + s;
+  auto w = [&](int64_t k, int c) -> T& { return W[(k * 12 + c) * B]; };
+  T inv[9], d[3], a[9];
+  for (int c = 0; c < 9; ++c) a[c] = M[c];
+  inv3(a, inv);
+  for (int c = 0; c < 3; ++c) d[c] = D[c];
+  for (int c = 0; c < 9; ++c) w(0, c) = inv[c];
+  for (int c = 0; c < 3; ++c) w(0, 9 + c) = d[c];
+  for (int64_t k = 1; k < K; ++k) {
+    const T l[3] = {L[(k - 1) * 3], L[(k - 1) * 3 + 1], L[(k - 1) * 3 + 2]};
+    const T u[3] = {U[(k - 1) * 3], U[(k - 1) * 3 + 1], U[(k - 1) * 3 + 2]};
+    T wd[3];
+    mv3(inv, d, wd);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) a[r * 3 + c] = M[k * 9 + r * 3 + c] - l[r] * inv[r * 3 + c] * u[c];
+    for (int r = 0; r < 3; ++r) d[r] = D[k * 3 + r] - l[r] * wd[r];
+    inv3(a, inv);
+    for (int c = 0; c < 9; ++c) w(k, c) = inv[c];
+    for (int c = 0; c < 3; ++c) w(k, 9 + c) = d[c];
+  }
+  T xn[3];
+  mv3(inv, d, xn);
+  for (int c = 0; c < 3; ++c) X[(K - 1) * 3 + c] = xn[c];
+  for (int64_t k = K - 2; k >= 0; --k) {
+    T t[3], iv[9];
+    for (int c = 0; c < 9; ++c) iv[c] = w(k, c);
+    for (int c = 0; c < 3; ++c) t[c] = w(k, 9 + c) - U[k * 3 + c] * xn[c];
+    mv3(iv, t, xn);
+    for (int c = 0; c < 3; ++c) X[k * 3 + c] = xn[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// block SPD: the same recurrence with every M'_k factored by Cholesky.
+// A block that is not positive definite after symmetrisation is shifted
+// along its diagonal: first by its Gershgorin deficit plus a margin
+// (max(eps_abs, eps_rel * sum |a_rr|)), then by 10 more margins; info
+// records a block that fails all three.  chol: [B][K][9] row-major lower
+// factor; work: [K][3][B] = d'.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T mid(T a, T b) {
+  const T big = std::numeric_limits<T>::max() / T(2);
+  return (fabs_(a) <= big && fabs_(b) <= big) ? T(0.5) * (a + b) : T(0.5) * a + T(0.5) * b;
+}
+
+template <typename T>
+__device__ __forceinline__ bool chol3(const T* a, T* l) {
+  const bool c0 = a[0] > T(0);
+  const T l00 = sqrt(c0 ? a[0] : T(1));
+  const T l10 = a[3] / l00, l20 = a[6] / l00;
+  const T s11 = a[4] - l10 * l10;
+  const bool c1 = s11 > T(0);
+  const T l11 = sqrt(c1 ? s11 : T(1));
+  const T l21 = (a[7] - l10 * l20) / l11;
+  const T s22 = a[8] - l20 * l20 - l21 * l21;
+  const bool c2 = s22 > T(0);
+  const T l22 = sqrt(c2 ? s22 : T(1));
+  l[0] = l00; l[1] = T(0); l[2] = T(0);
+  l[3] = l10; l[4] = l11; l[5] = T(0);
+  l[6] = l20; l[7] = l21; l[8] = l22;
+  return c0 && c1 && c2;
+}
+
+template <typename T>
+__device__ __forceinline__ bool chol3_shifted(T* a, T* l, T eps_rel, T eps_abs) {
+  const T s01 = mid(a[1], a[3]), s02 = mid(a[2], a[6]), s12 = mid(a[5], a[7]);
+  a[1] = a[3] = s01;
+  a[2] = a[6] = s02;
+  a[5] = a[7] = s12;
+  if (chol3(a, l)) return true;
+  const T tr = fabs_(a[0]) + fabs_(a[4]) + fabs_(a[8]);
+  const T g0 = a[0] - fabs_(a[1]) - fabs_(a[2]);
+  const T g1 = a[4] - fabs_(a[3]) - fabs_(a[5]);
+  const T g2 = a[8] - fabs_(a[6]) - fabs_(a[7]);
+  T gmin = g0 < g1 ? g0 : g1;
+  gmin = gmin < g2 ? gmin : g2;
+  const T margin = eps_abs > eps_rel * tr ? eps_abs : eps_rel * tr;
+  T tau = gmin > T(0) ? margin : margin - gmin;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    T b[9];
+    for (int c = 0; c < 9; ++c) b[c] = a[c];
+    b[0] += tau; b[4] += tau; b[8] += tau;
+    if (chol3(b, l)) return true;
+    tau += T(10) * margin;
+  }
+  return false;
+}
+
+// solve L L^T x = b
+template <typename T>
+__device__ __forceinline__ void chol_solve3(const T* l, const T* b, T* x) {
+  const T y0 = b[0] / l[0];
+  const T y1 = (b[1] - l[3] * y0) / l[4];
+  const T y2 = (b[2] - l[6] * y0 - l[7] * y1) / l[8];
+  x[2] = y2 / l[8];
+  x[1] = (y1 - l[7] * x[2]) / l[4];
+  x[0] = (y0 - l[3] * x[1] - l[6] * x[2]) / l[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void bt3_spd_kernel(
+    const T* __restrict__ lower, const T* __restrict__ main, const T* __restrict__ upper,
+    const T* __restrict__ rhs, T* __restrict__ x, T* __restrict__ chol,
+    int32_t* __restrict__ info, T* __restrict__ work, int64_t B, int64_t K, T eps_rel,
+    T eps_abs, int factor) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B) return;
+  const T* L = lower + s * (K - 1) * 3;
+  const T* U = upper + s * (K - 1) * 3;
+  const T* D = rhs + s * K * 3;
+  T* X = x + s * K * 3;
+  T* C = chol + s * K * 9;
+  T* W = work + s;
+  auto w = [&](int64_t k, int c) -> T& { return W[(k * 3 + c) * B]; };
+  T lk[9], d[3];
+  int bad = 0;
+  if (factor) {
+    const T* M = main + s * K * 9;
+    T a[9];
+    for (int c = 0; c < 9; ++c) a[c] = M[c];
+    bad += !chol3_shifted(a, lk, eps_rel, eps_abs);
+    for (int c = 0; c < 9; ++c) C[c] = lk[c];
+    for (int c = 0; c < 3; ++c) d[c] = D[c];
+    for (int c = 0; c < 3; ++c) w(0, c) = d[c];
+    for (int64_t k = 1; k < K; ++k) {
+      const T l[3] = {L[(k - 1) * 3], L[(k - 1) * 3 + 1], L[(k - 1) * 3 + 2]};
+      const T u[3] = {U[(k - 1) * 3], U[(k - 1) * 3 + 1], U[(k - 1) * 3 + 2]};
+      // columns of M'^-1_{k-1} diag(u), and M'^-1_{k-1} d'_{k-1}
+      T col[3][3], wd[3];
+      for (int j = 0; j < 3; ++j) {
+        const T e[3] = {j == 0 ? u[0] : T(0), j == 1 ? u[1] : T(0), j == 2 ? u[2] : T(0)};
+        chol_solve3(lk, e, col[j]);
+      }
+      chol_solve3(lk, d, wd);
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) a[r * 3 + c] = M[k * 9 + r * 3 + c] - l[r] * col[c][r];
+      for (int r = 0; r < 3; ++r) d[r] = D[k * 3 + r] - l[r] * wd[r];
+      bad += !chol3_shifted(a, lk, eps_rel, eps_abs);
+      for (int c = 0; c < 9; ++c) C[k * 9 + c] = lk[c];
+      for (int c = 0; c < 3; ++c) w(k, c) = d[c];
+    }
+  } else {  // reuse the factors: forward sweep of the right-hand side only
+    for (int c = 0; c < 9; ++c) lk[c] = C[c];
+    for (int c = 0; c < 3; ++c) d[c] = D[c];
+    for (int c = 0; c < 3; ++c) w(0, c) = d[c];
+    for (int64_t k = 1; k < K; ++k) {
+      T wd[3];
+      chol_solve3(lk, d, wd);
+      for (int r = 0; r < 3; ++r) d[r] = D[k * 3 + r] - L[(k - 1) * 3 + r] * wd[r];
+      for (int c = 0; c < 9; ++c) lk[c] = C[k * 9 + c];
+      for (int c = 0; c < 3; ++c) w(k, c) = d[c];
+    }
+  }
+  T xn[3];
+  chol_solve3(lk, d, xn);
+  for (int c = 0; c < 3; ++c) X[(K - 1) * 3 + c] = xn[c];
+  for (int64_t k = K - 2; k >= 0; --k) {
+    T t[3];
+    for (int c = 0; c < 9; ++c) lk[c] = C[k * 9 + c];
+    for (int c = 0; c < 3; ++c) t[c] = w(k, c) - U[k * 3 + c] * xn[c];
+    chol_solve3(lk, t, xn);
+    for (int c = 0; c < 3; ++c) X[k * 3 + c] = xn[c];
+  }
+  if (info) info[s] = bad;
+}
+
+// ---------------------------------------------------------------------------
+// block tree (Hines) solve over the caller's elimination order (children
+// before parents).  work: [K][12][B] = accumulated diagonal block, then its
+// inverse (9), and the eliminated right-hand side (3).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void bt3_tree_kernel(const T* __restrict__ main,
+                                                       const T* __restrict__ edge,
+                                                       const T* __restrict__ rhs,
+                                                       const int64_t* __restrict__ parent,
+                                                       const int64_t* __restrict__ order,
+                                                       T* __restrict__ x,
+                                                       T* __restrict__ work, int64_t B,
+                                                       int64_t K) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B) return;
+  const T* M = main + s * K * 9;
+  const T* E = edge + s * K * 3;
+  const T* R = rhs + s * K * 3;
+  T* X = x + s * K * 3;
+  T* W = work + s;
+  auto w = [&](int64_t k, int c) -> T& { return W[(k * 12 + c) * B]; };
+  for (int64_t i = 0; i < K; ++i) {
+    for (int c = 0; c < 9; ++c) w(i, c) = M[i * 9 + c];
+    for (int c = 0; c < 3; ++c) {
+      w(i, c * 4) += E[i * 3 + c];
+      w(i, 9 + c) = R[i * 3 + c];
+    }
+  }
+  // out-of-range parents count as roots and out-of-range order entries are
+  // skipped, so a malformed plan cannot address outside the system
+  auto par = [&](int64_t i) -> int64_t {
+    const int64_t p = parent[i];
+    return (p >= 0 && p < K) ? p : -1;
+  };
+  for (int64_t i = 0; i < K; ++i) {
+    const int64_t p = par(i);
+    if (p >= 0)
+      for (int c = 0; c < 3; ++c) w(p, c * 4) += E[i * 3 + c];
+  }
+  for (int64_t n = 0; n < K; ++n) {
+    const int64_t i = order[n];
+    if (i < 0 || i >= K) continue;
+    T a[9], iv[9];
+    for (int c = 0; c < 9; ++c) a[c] = w(i, c);
+    inv3(a, iv);
+    for (int c = 0; c < 9; ++c) w(i, c) = iv[c];
+    const int64_t p = par(i);
+    if (p >= 0) {
+      const T e[3] = {E[i * 3], E[i * 3 + 1], E[i * 3 + 2]};
+      T b[3], y[3];
+      for (int c = 0; c < 3; ++c) b[c] = w(i, 9 + c);
+      mv3(iv, b, y);
+      for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) w(p, r * 3 + c) -= e[r] * iv[r * 3 + c] * e[c];
+        w(p, 9 + r) += e[r] * y[r];
+      }
+    }
+  }
+  for (int64_t n = K - 1; n >= 0; --n) {
+    const int64_t i = order[n];
+    if (i < 0 || i >= K) continue;
+    const int64_t p = par(i);
+    T b[3], iv[9], xi[3];
+    for (int c = 0; c < 3; ++c)
+      b[c] = w(i, 9 + c) + (p >= 0 ? E[i * 3 + c] * X[p * 3 + c] : T(0));
+    for (int c = 0; c < 9; ++c) iv[c] = w(i, c);
+    mv3(iv, b, xi);
+    for (int c = 0; c < 3; ++c) X[i * 3 + c] = xi[c];
+  }
+}
+
+}  // namespace dn
+
+template <typename T>
+static unsigned blocks_for(int64_t B) {
+  return (unsigned)((B + 255) / 256);
+}
+
+DN_EXPORT int64_t dn_block_work_elems(int kind, int64_t B, int64_t K) {
+  if (B < 0 || K < 0) return -1;
+  return (kind == 1 ? 3 : 12) * B * K;  // 0: Thomas, 1: SPD, 2: tree
+}
+
+template <typename T>
+static int bt3_thomas_impl(const T* lower, const T* main, const T* upper, const T* rhs,
+                           T* x, int64_t B, int64_t K, T* work, int64_t work_elems,
+                           hipStream_t stream) {
+  if (!main || !rhs || !x || !work || B < 1 || K < 1) return DN_EINVAL;
+  if (K > 1 && (!lower || !upper)) return DN_EINVAL;
+  if (work_elems < 12 * B * K || B > (int64_t)INT32_MAX * 256) return DN_EINVAL;
+  hipLaunchKernelGGL((dn::bt3_thomas_kernel<T>), dim3(blocks_for<T>(B)), dim3(256), 0,
+                     stream, lower, main, upper, rhs, x, work, B, K);
+  return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+}
+
+template <typename T>
+static int bt3_spd_impl(const T* lower, const T* main, const T* upper, const T* rhs, T* x,
+                        T* chol, int32_t* info, int64_t B, int64_t K, T* work,
+                        int64_t work_elems, int factor, T eps_rel, T eps_abs,
+                        hipStream_t stream) {
+  if (!rhs || !x || !chol || !work || B < 1 || K < 1) return DN_EINVAL;
+  if (factor && !main) return DN_EINVAL;
+  if (K > 1 && (!lower || !upper)) return DN_EINVAL;
+  if (work_elems < 3 * B * K || B > (int64_t)INT32_MAX * 256) return DN_EINVAL;
+  hipLaunchKernelGGL((dn::bt3_spd_kernel<T>), dim3(blocks_for<T>(B)), dim3(256), 0, stream,
+                     lower, main, upper, rhs, x, chol, info, work, B, K, eps_rel, eps_abs,
+                     factor);
+  return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+}
+
+template <typename T>
+static int bt3_tree_impl(const T* main, const T* edge, const T* rhs, const int64_t* parent,
+                         const int64_t* order, T* x, int64_t B, int64_t K, T* work,
+                         int64_t work_elems, hipStream_t stream) {
+  if (!main || !edge || !rhs || !parent || !order || !x || !work || B < 1 || K < 1)
+    return DN_EINVAL;
+  if (work_elems < 12 * B * K || B > (int64_t)INT32_MAX * 256) return DN_EINVAL;
+  hipLaunchKernelGGL((dn::bt3_tree_kernel<T>), dim3(blocks_for<T>(B)), dim3(256), 0, stream,
+                     main, edge, rhs, parent, order, x, work, B, K);
+  return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+}
+
+DN_EXPORT int dn_bt3_thomas_f32(const float* lower, const float* main, const float* upper,
+                                const float* rhs, float* x, int64_t B, int64_t K,
+                                float* work, int64_t work_elems, hipStream_t stream) {
+  return bt3_thomas_impl<float>(lower, main, upper, rhs, x, B, K, work, work_elems, stream);
+}
+DN_EXPORT int dn_bt3_thomas_f64(const double* lower, const double* main,
+                                const double* upper, const double* rhs, double* x,
+                                int64_t B, int64_t K, double* work, int64_t work_elems,
+                                hipStream_t stream) {
+  return bt3_thomas_impl<double>(lower, main, upper, rhs, x, B, K, work, work_elems, stream);
+}
+DN_EXPORT int dn_bt3_spd_f32(const float* lower, const float* main, const float* upper,
+                             const float* rhs, float* x, float* chol, int32_t* info,
+                             int64_t B, int64_t K, float* work, int64_t work_elems,
+                             int factor, hipStream_t stream) {
+  return bt3_spd_impl<float>(lower, main, upper, rhs, x, chol, info, B, K, work,
+                             work_elems, factor, 1e-6f, 1e-18f, stream);
+}
+DN_EXPORT int dn_bt3_spd_f64(const double* lower, const double* main, const double* upper,
+                             const double* rhs, double* x, double* chol, int32_t* info,
+                             int64_t B, int64_t K, double* work, int64_t work_elems,
+                             int factor, hipStream_t stream) {
+  return bt3_spd_impl<double>(lower, main, upper, rhs, x, chol, info, B, K, work,
+                              work_elems, factor, 1e-12, 1e-30, stream);
+}
+DN_EXPORT int dn_bt3_tree_f32(const float* main, const float* edge, const float* rhs,
+                              const int64_t* parent, const int64_t* order, float* x,
+                              int64_t B, int64_t K, float* work, int64_t work_elems,
+                              hipStream_t stream) {
+  return bt3_tree_impl<float>(main, edge, rhs, parent, order, x, B, K, work, work_elems,
+                              stream);
+}
+DN_EXPORT int dn_bt3_tree_f64(const double* main, const double* edge, const double* rhs,
+                              const int64_t* parent, const int64_t* order, double* x,
+                              int64_t B, int64_t K, double* work, int64_t work_elems,
+                              hipStream_t stream) {
+  return bt3_tree_impl<double>(main, edge, rhs, parent, order, x, B, K, work, work_elems,
+                               stream);
+}

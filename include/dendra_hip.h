@@ -98,6 +98,44 @@ int dn_tree_contract_f64(const double* d, const double* g, const double* rhs, do
                          const int32_t* src, double* work, int64_t work_elems,
                          hipStream_t stream);
 
+/* ---- 3x3 block solvers (double-layer extracellular model) -----------------
+ * Replace the block solvers that _bwd_euler_bt (integrators/implicit.py:
+ * 669-1136) and the block tree integrator (integrators/tree_bt.py) select:
+ *   dn_bt3_thomas_*  thomas_bt3_solve / thomas_solve_cuda_bt
+ *                    (triton/bt_kernel.py:192-298): main (B,K,3,3), lower and
+ *                    upper (B,K-1,3) diagonal coupling blocks, rhs/x (B,K,3);
+ *   dn_bt3_spd_*     solve_bt_spd_fwd_consume_impl (factor = 1: factors main,
+ *                    writes chol (B,K,9) and per-system info = blocks that
+ *                    stayed indefinite after the diagonal shifts) and
+ *                    solve_bt_spd_solve_with_chol_impl (factor = 0: reuses
+ *                    chol; main may be null) (triton/bt_spd_kernel.py:575-700);
+ *   dn_bt3_tree_*    _dhs_bt3_adjoint (triton/dhs_kernel_bt.py): main
+ *                    (B,K,3,3), edge (B,K,3) per-component conductance to the
+ *                    parent, parent (K), order (K, children before parents).
+ * Inputs are not modified; work holds dn_block_work_elems(kind, B, K)
+ * elements (kind 0 Thomas, 1 SPD, 2 tree). */
+int64_t dn_block_work_elems(int kind, int64_t B, int64_t K);
+int dn_bt3_thomas_f32(const float* lower, const float* main, const float* upper,
+                      const float* rhs, float* x, int64_t B, int64_t K, float* work,
+                      int64_t work_elems, hipStream_t stream);
+int dn_bt3_thomas_f64(const double* lower, const double* main, const double* upper,
+                      const double* rhs, double* x, int64_t B, int64_t K, double* work,
+                      int64_t work_elems, hipStream_t stream);
+int dn_bt3_spd_f32(const float* lower, const float* main, const float* upper,
+                   const float* rhs, float* x, float* chol, int32_t* info, int64_t B,
+                   int64_t K, float* work, int64_t work_elems, int factor,
+                   hipStream_t stream);
+int dn_bt3_spd_f64(const double* lower, const double* main, const double* upper,
+                   const double* rhs, double* x, double* chol, int32_t* info, int64_t B,
+                   int64_t K, double* work, int64_t work_elems, int factor,
+                   hipStream_t stream);
+int dn_bt3_tree_f32(const float* main, const float* edge, const float* rhs,
+                    const int64_t* parent, const int64_t* order, float* x, int64_t B,
+                    int64_t K, float* work, int64_t work_elems, hipStream_t stream);
+int dn_bt3_tree_f64(const double* main, const double* edge, const double* rhs,
+                    const int64_t* parent, const int64_t* order, double* x, int64_t B,
+                    int64_t K, double* work, int64_t work_elems, hipStream_t stream);
+
 /* ---- fused HH + backward Euler, unbranched cables -------------------------
  * Replaces the per-step transition of _bwd_euler_ub
  * (dendra/models/integrators/implicit.py:561-664) in Integrator._step order
