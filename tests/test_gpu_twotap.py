@@ -1,0 +1,117 @@
+"""GPU two-tap delay transition (dendra_amd.delays) against the ordered
+oracle, bitwise in float32 and float64, and its gradients against autograd of
+the reference's index_add form."""
+
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from test_twotap_ref import torch_transition
+
+sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
+import twotap_ref  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def case(rng, dtype, depth=201, width=300, n=5000):
+    q = rng.standard_normal((depth, width)).astype(dtype)
+    w = rng.lognormal(np.log(1e-3), 0.5, n).astype(dtype)
+    g = (rng.uniform(0, 1, n) < 0.3).astype(dtype) * rng.uniform(0.5, 1.5, n).astype(dtype)
+    delays = rng.uniform(0.5, 5.0, n)            # ms
+    post = rng.integers(0, width, n)
+    return q, w, g, delays, post
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_two_tap_transition_is_bitwise_the_ordered_enqueue(dtype):
+    from dendra_amd.delays import dense_interpolated_delay_transition
+
+    rng = np.random.default_rng(7)
+    q, w, g, delays, post = case(rng, dtype)
+    dt = 0.025
+    lower, frac = twotap_ref.two_tap_split(delays, dt, dtype)
+    dev = lambda a, t=None: torch.as_tensor(a, device="cuda", dtype=t)
+    tq = dev(q)
+    slot = dev(np.array([195]), torch.int64)
+    step = dev(np.array([0]), torch.int64)
+    want = q
+    s = 195
+    for _ in range(8):  # consecutive steps, the slot wrapping past depth
+        tr = dense_interpolated_delay_transition(
+            tq, slot, step, dev(w), dev(g), dev(delays.astype(dtype)), dt,
+            post_idx=dev(post, torch.int64), depth=q.shape[0], width=q.shape[1])
+        want = twotap_ref.enqueue(want, s, w, g, frac, lower, post)
+        got = tr.queue.cpu().numpy()
+        assert got.dtype == want.dtype
+        assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+        s = (s + 1) % q.shape[0]
+        assert int(tr.slot) == s and tr.event_queue is None
+        tq, slot, step = tr.queue, tr.slot, tr.global_step
+    assert int(step) == 8
+
+
+def test_two_tap_gradients_match_autograd():
+    from dendra_amd.delays import dense_interpolated_delay_transition, delay_interpolation
+
+    rng = np.random.default_rng(8)
+    q, w, g, delays, post = case(rng, np.float64, depth=31, width=20, n=200)
+    delays = rng.uniform(0.3, 0.7, 200)  # dt 0.025: 12..28 steps, all inside the ring
+    dt = 0.025
+    cot = torch.as_tensor(rng.standard_normal(q.shape))
+    slot = 29
+
+    def run(dev, fn):
+        args = [torch.as_tensor(v, device=dev).requires_grad_() for v in (q, w, g, delays)]
+        out = fn(*args)
+        return torch.autograd.grad((out * cot.to(dev)).sum(), args)
+
+    def ref(tq, tw, tg, td):
+        lam = (td / dt).clamp_min(1.0)
+        lower = lam.floor()
+        return torch_transition(tq, slot, tw, tg, lam - lower, lower.long(),
+                                torch.as_tensor(post))
+
+    def gpu(tq, tw, tg, td):
+        return dense_interpolated_delay_transition(
+            tq, torch.tensor([slot], device="cuda"), torch.tensor([0], device="cuda"),
+            tw, tg, td, dt, post_idx=torch.as_tensor(post, device="cuda"), depth=31,
+            width=20).queue
+
+    exp = run("cpu", ref)
+    got = run("cuda", gpu)
+    for a, b in zip(got, exp):
+        torch.testing.assert_close(a.cpu(), b, rtol=1e-12, atol=1e-14)
+    offsets, fractions = delay_interpolation(torch.as_tensor(delays), torch.tensor(dt),
+                                             dtype=torch.float64)
+    assert offsets.dtype == torch.int64 and (offsets >= 1).all()
+
+
+def test_two_tap_no_connections_and_validation():
+    from dendra_amd.delays import dense_interpolated_delay_transition
+
+    q = torch.randn(5, 3, device="cuda")
+    e = torch.empty(0, device="cuda")
+    tr = dense_interpolated_delay_transition(
+        q, torch.tensor([2], device="cuda"), torch.tensor([0], device="cuda"), e, e, e, 0.1,
+        post_idx=torch.empty(0, dtype=torch.int64, device="cuda"), depth=5, width=3)
+    want = q.clone()
+    want[2] = 0
+    assert torch.equal(tr.queue, want)
+    with pytest.raises(ValueError, match="queue must have shape"):
+        dense_interpolated_delay_transition(
+            q, torch.tensor([0], device="cuda"), torch.tensor([0], device="cuda"), e, e, e,
+            0.1, post_idx=torch.empty(0, dtype=torch.int64, device="cuda"), depth=4, width=3)
+    with pytest.raises(ValueError, match=r"post_idx entries must lie in \[0, width\)"):
+        one = torch.ones(1, device="cuda")
+        dense_interpolated_delay_transition(
+            q, torch.tensor([0], device="cuda"), torch.tensor([0], device="cuda"), one, one,
+            one, 0.1, post_idx=torch.tensor([3], device="cuda"), depth=5, width=3)
+    with pytest.raises(NotImplementedError, match="two taps"):
+        dense_interpolated_delay_transition(
+            q, torch.tensor([0], device="cuda"), torch.tensor([0], device="cuda"), e, e, e,
+            0.1, post_idx=torch.empty(0, dtype=torch.int64, device="cuda"), depth=5, width=3,
+            taps=3)
