@@ -136,34 +136,40 @@ int dn_bt3_tree_f64(const double* main, const double* edge, const double* rhs,
                     const int64_t* parent, const int64_t* order, double* x, int64_t B,
                     int64_t K, double* work, int64_t work_elems, hipStream_t stream);
 
-/* ---- NetCon delivery with fractional (two-tap) delays --------------------
+/* ---- NetCon delivery with fractional (interpolated) delays ----------------
  * Replaces the ordered two-tap enqueue (dendra/models/networks/
  * _two_tap_enqueue_cpu.cpp:26-152, dendra_native::ordered_two_tap_enqueue)
- * behind dense_interpolated_delay_transition (networks/_event_delivery.py:
- * 226-310).  q (depth, width) is updated in place: row *slot (device int64,
- * reduced modulo depth) is cleared, then every connection adds
- * w g (1 - f) at row slot + off and w g f at row slot + off + 1 (modulo depth)
- * of its target column, all lower taps before all upper taps, each cell in
- * ascending connection order (bitwise the reference's sequence).  in_ptr
- * (width + 1) / in_edge (n_edge) list each target's connections in ascending
- * order; w, g, f, off (n_edge) are per connection.  dn_two_tap_vjp_* writes
- * the per-connection cotangents dw, dg, df for a ring cotangent G. */
-int dn_two_tap_enqueue_f32(float* q, int64_t depth, int64_t width, const int64_t* slot,
-                           const int32_t* in_ptr, const int32_t* in_edge, const float* w,
-                           const float* g, const float* f, const int32_t* off,
-                           hipStream_t stream);
-int dn_two_tap_enqueue_f64(double* q, int64_t depth, int64_t width, const int64_t* slot,
-                           const int32_t* in_ptr, const int32_t* in_edge, const double* w,
-                           const double* g, const double* f, const int32_t* off,
-                           hipStream_t stream);
-int dn_two_tap_vjp_f32(const float* G, int64_t depth, int64_t width, const int64_t* slot,
-                       int64_t n_edge, const int32_t* post, const float* w, const float* g,
-                       const float* f, const int32_t* off, float* dw, float* dg, float* df,
-                       hipStream_t stream);
-int dn_two_tap_vjp_f64(const double* G, int64_t depth, int64_t width, const int64_t* slot,
-                       int64_t n_edge, const int32_t* post, const double* w, const double* g,
-                       const double* f, const int32_t* off, double* dw, double* dg,
-                       double* df, hipStream_t stream);
+ * and the three-tap scatter of dense_interpolated_delay_transition
+ * (networks/_event_delivery.py:226-310).  q (depth, width) is updated in
+ * place: row *slot (device int64, reduced modulo depth) is cleared; then
+ *   taps 2: every connection adds w g (1 - f) at row slot + base and w g f at
+ *           row slot + base + 1, all lower taps before all upper taps;
+ *   taps 3: every connection adds w g f[3e + j] at row slot + base + j,
+ *           j = 0..2, connection-major;
+ * rows modulo depth, each cell in ascending connection order (bitwise the
+ * reference's sequence).  in_ptr (width + 1) / in_edge (n_edge) list each
+ * target's connections in ascending order; w, g, base (n_edge) and f
+ * (n_edge, or 3 n_edge) are per connection; base is the first tap's row
+ * offset modulo depth.  dn_interp_delay_vjp_* writes the per-connection
+ * cotangents dw, dg, df for a ring cotangent G. */
+int dn_interp_delay_enqueue_f32(float* q, int64_t depth, int64_t width, const int64_t* slot,
+                                int taps, const int32_t* in_ptr, const int32_t* in_edge,
+                                const float* w, const float* g, const float* f,
+                                const int32_t* base, hipStream_t stream);
+int dn_interp_delay_enqueue_f64(double* q, int64_t depth, int64_t width,
+                                const int64_t* slot, int taps, const int32_t* in_ptr,
+                                const int32_t* in_edge, const double* w, const double* g,
+                                const double* f, const int32_t* base, hipStream_t stream);
+int dn_interp_delay_vjp_f32(const float* G, int64_t depth, int64_t width,
+                            const int64_t* slot, int taps, int64_t n_edge,
+                            const int32_t* post, const float* w, const float* g,
+                            const float* f, const int32_t* base, float* dw, float* dg,
+                            float* df, hipStream_t stream);
+int dn_interp_delay_vjp_f64(const double* G, int64_t depth, int64_t width,
+                            const int64_t* slot, int taps, int64_t n_edge,
+                            const int32_t* post, const double* w, const double* g,
+                            const double* f, const int32_t* base, double* dw, double* dg,
+                            double* df, hipStream_t stream);
 
 /* ---- fused HH + backward Euler, unbranched cables -------------------------
  * Replaces the per-step transition of _bwd_euler_ub

@@ -43,7 +43,8 @@ def test_two_tap_transition_is_bitwise_the_ordered_enqueue(dtype):
     for _ in range(8):  # consecutive steps, the slot wrapping past depth
         tr = dense_interpolated_delay_transition(
             tq, slot, step, dev(w), dev(g), dev(delays.astype(dtype)), dt,
-            post_idx=dev(post, torch.int64), depth=q.shape[0], width=q.shape[1])
+            post_idx=dev(post, torch.int64), depth=q.shape[0], width=q.shape[1], taps=2,
+            sigma=0.0)
         want = twotap_ref.enqueue(want, s, w, g, frac, lower, post)
         got = tr.queue.cpu().numpy()
         assert got.dtype == want.dtype
@@ -79,14 +80,14 @@ def test_two_tap_gradients_match_autograd():
         return dense_interpolated_delay_transition(
             tq, torch.tensor([slot], device="cuda"), torch.tensor([0], device="cuda"),
             tw, tg, td, dt, post_idx=torch.as_tensor(post, device="cuda"), depth=31,
-            width=20).queue
+            width=20, taps=2, sigma=0.0).queue
 
     exp = run("cpu", ref)
     got = run("cuda", gpu)
     for a, b in zip(got, exp):
         torch.testing.assert_close(a.cpu(), b, rtol=1e-12, atol=1e-14)
     offsets, fractions = delay_interpolation(torch.as_tensor(delays), torch.tensor(dt),
-                                             dtype=torch.float64)
+                                             dtype=torch.float64, taps=2, sigma=0.0)
     assert offsets.dtype == torch.int64 and (offsets >= 1).all()
 
 
@@ -97,21 +98,103 @@ def test_two_tap_no_connections_and_validation():
     e = torch.empty(0, device="cuda")
     tr = dense_interpolated_delay_transition(
         q, torch.tensor([2], device="cuda"), torch.tensor([0], device="cuda"), e, e, e, 0.1,
-        post_idx=torch.empty(0, dtype=torch.int64, device="cuda"), depth=5, width=3)
+        post_idx=torch.empty(0, dtype=torch.int64, device="cuda"), depth=5, width=3, taps=2,
+        sigma=0.0)
     want = q.clone()
     want[2] = 0
     assert torch.equal(tr.queue, want)
     with pytest.raises(ValueError, match="queue must have shape"):
         dense_interpolated_delay_transition(
             q, torch.tensor([0], device="cuda"), torch.tensor([0], device="cuda"), e, e, e,
-            0.1, post_idx=torch.empty(0, dtype=torch.int64, device="cuda"), depth=4, width=3)
+            0.1, post_idx=torch.empty(0, dtype=torch.int64, device="cuda"), depth=4, width=3,
+            taps=2, sigma=0.0)
     with pytest.raises(ValueError, match=r"post_idx entries must lie in \[0, width\)"):
         one = torch.ones(1, device="cuda")
         dense_interpolated_delay_transition(
             q, torch.tensor([0], device="cuda"), torch.tensor([0], device="cuda"), one, one,
-            one, 0.1, post_idx=torch.tensor([3], device="cuda"), depth=5, width=3)
-    with pytest.raises(NotImplementedError, match="two taps"):
+            one, 0.1, post_idx=torch.tensor([3], device="cuda"), depth=5, width=3, taps=2,
+            sigma=0.0)
+    with pytest.raises(ValueError, match="taps must be 2 or 3"):
         dense_interpolated_delay_transition(
             q, torch.tensor([0], device="cuda"), torch.tensor([0], device="cuda"), e, e, e,
             0.1, post_idx=torch.empty(0, dtype=torch.int64, device="cuda"), depth=5, width=3,
-            taps=3)
+            taps=4, sigma=0.0)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_three_tap_transition_is_bitwise_the_flattened_scatter(dtype):
+    from dendra_amd.delays import delay_interpolation, dense_interpolated_delay_transition
+
+    rng = np.random.default_rng(9)
+    q, w, g, delays, post = case(rng, dtype)
+    dt, sigma = 0.025, 0.7
+    offsets, fractions = twotap_ref.three_tap_split(delays, dt, sigma, dtype)
+    dev = lambda a, t=None: torch.as_tensor(a, device="cuda", dtype=t)
+    tr = dense_interpolated_delay_transition(
+        dev(q), dev(np.array([200]), torch.int64), dev(np.array([0]), torch.int64), dev(w),
+        dev(g), dev(delays.astype(dtype)), dt, post_idx=dev(post, torch.int64),
+        depth=q.shape[0], width=q.shape[1], taps=3, sigma=sigma,
+        delay_taps=(dev(offsets, torch.int64), dev(fractions)))
+    want = twotap_ref.enqueue3(q, 200, w, g, fractions, offsets, post)
+    assert np.array_equal(tr.queue.cpu().numpy().view(np.uint8), want.view(np.uint8))
+    off_t, frac_t = delay_interpolation(dev(delays), torch.tensor(dt, dtype=torch.float64),
+                                        dtype=torch.float64, taps=3, sigma=sigma)
+    assert torch.equal(off_t.cpu(), torch.as_tensor(offsets))
+    np.testing.assert_allclose(frac_t.cpu().numpy(), fractions.astype(np.float64),
+                               rtol=1e-6 if dtype == np.float32 else 1e-12)
+
+
+def test_three_tap_gradients_match_autograd():
+    from dendra_amd.delays import dense_interpolated_delay_transition
+
+    rng = np.random.default_rng(10)
+    q, w, g, _, post = case(rng, np.float64, depth=31, width=20, n=200)
+    delays = rng.uniform(0.3, 0.7, 200)
+    dt, sigma, slot = 0.025, 0.8, 29
+    cot = torch.as_tensor(rng.standard_normal(q.shape))
+
+    def run(dev, fn):
+        args = [torch.as_tensor(v, device=dev).requires_grad_() for v in (q, w, g, delays)]
+        return torch.autograd.grad((fn(*args) * cot.to(dev)).sum(), args)
+
+    def ref(tq, tw, tg, td):
+        lam = (td / dt).clamp_min(1.0)
+        lower = lam.floor().long()
+        offs = torch.stack([lower - 1, lower, lower + 1], -1)
+        fr = torch.softmax(-0.5 * ((lam[:, None] - offs.double()) / (sigma + 1e-6)) ** 2, -1)
+        out = tq.clone()
+        out[slot] = 0
+        rows = (slot + offs) % 31
+        dest = rows * 20 + torch.as_tensor(post)[:, None]
+        vals = (tw * tg)[:, None] * fr
+        return out.reshape(-1).index_add(0, dest.reshape(-1), vals.reshape(-1)).reshape(31, 20)
+
+    def gpu(tq, tw, tg, td):
+        return dense_interpolated_delay_transition(
+            tq, torch.tensor([slot], device="cuda"), torch.tensor([0], device="cuda"), tw, tg,
+            td, dt, post_idx=torch.as_tensor(post, device="cuda"), depth=31, width=20, taps=3,
+            sigma=sigma).queue
+
+    for a, b in zip(run("cuda", gpu), run("cpu", ref)):
+        torch.testing.assert_close(a.cpu(), b, rtol=1e-11, atol=1e-13)
+
+
+def test_delay_plan_follows_each_post_idx_tensor():
+    """A freed post_idx whose storage the allocator hands to a new tensor of
+    the same size must not lend its plan to it."""
+    from dendra_amd.delays import dense_interpolated_delay_transition
+
+    rng = np.random.default_rng(11)
+    q, w, g, delays, _ = case(rng, np.float32, depth=31, width=40, n=300)
+    delays = rng.uniform(0.3, 0.7, 300).astype(np.float32)
+    lower, frac = twotap_ref.two_tap_split(delays, 0.025, np.float32)
+    dev = lambda a, t=None: torch.as_tensor(a, device="cuda", dtype=t)
+    for seed in range(3):
+        post = np.random.default_rng(100 + seed).integers(0, 40, 300)
+        p_t = dev(post, torch.int64)
+        tr = dense_interpolated_delay_transition(
+            dev(q), dev(np.array([3]), torch.int64), dev(np.array([0]), torch.int64), dev(w),
+            dev(g), dev(delays), 0.025, post_idx=p_t, depth=31, width=40, taps=2, sigma=0.0)
+        want = twotap_ref.enqueue(q, 3, w, g, frac, lower, post)
+        assert np.array_equal(tr.queue.cpu().numpy().view(np.uint8), want.view(np.uint8))
+        del p_t, tr

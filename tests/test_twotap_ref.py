@@ -65,3 +65,22 @@ def test_delay_split_clamps_to_one_step():
     lower, frac = twotap_ref.two_tap_split([0.1, 1.0, 2.5, 7.25], 0.5, np.float64)
     np.testing.assert_array_equal(lower, [1, 2, 5, 14])
     np.testing.assert_allclose(frac, [0.0, 0.0, 0.0, 0.5])
+
+
+def test_three_tap_split_and_scatter_order():
+    offsets, fr = twotap_ref.three_tap_split([0.1, 1.3], 1.0, 0.5, np.float64)
+    np.testing.assert_array_equal(offsets, [[0, 1, 2], [0, 1, 2]])
+    np.testing.assert_allclose(fr.sum(1), 1.0)
+    assert fr[0, 1] > fr[0, 0] and fr[1, 1] > fr[1, 2] > fr[1, 0]
+    rng = np.random.default_rng(3)
+    q = rng.standard_normal((6, 4)).astype(np.float32)
+    w, g = rng.standard_normal(30).astype(np.float32), rng.uniform(0, 1, 30).astype(np.float32)
+    offs, fr = twotap_ref.three_tap_split(rng.uniform(0, 8, 30), 1.0, 0.6, np.float32)
+    post = rng.integers(0, 4, 30)
+    want = twotap_ref.enqueue3(q, 2, w, g, fr, offs, post)
+    out = torch.as_tensor(q).clone()
+    out[2] = 0
+    dest = ((2 + torch.as_tensor(offs)) % 6) * 4 + torch.as_tensor(post)[:, None]
+    vals = (torch.as_tensor(w) * torch.as_tensor(g))[:, None] * torch.as_tensor(fr)
+    got = out.reshape(-1).scatter_add(0, dest.reshape(-1), vals.reshape(-1)).reshape(6, 4)
+    assert np.array_equal(got.numpy().view(np.uint8), want.view(np.uint8))
