@@ -206,22 +206,26 @@ __device__ __forceinline__ void cable_run_wave(const CableRunArgs<T>& A, int tas
   }
 }
 
-template <typename T, bool SCALES>
+// RMAX (rows per lane held in registers) is a compile-time bound of the
+// launch: a launch whose tasks all have few rows uses a smaller instance,
+// with fewer VGPRs and less LDS per wavefront, so more wavefronts share a CU.
+template <typename T, bool SCALES, int RMAX>
 __global__ __launch_bounds__(256) void hh_cable_run_kernel(CableRunArgs<T> A) {
   extern __shared__ unsigned char dn_smem_raw[];
   const int wv = threadIdx.x >> 6;
   const int task = blockIdx.x * (blockDim.x >> 6) + wv;
   if (task >= A.n_tasks) return;
-  RowCoef<T>* cf = reinterpret_cast<RowCoef<T>*>(dn_smem_raw) + wv * kCableRMax * 64;
+  if (A.task_nr[task] > RMAX) return;  // launch contract violated: leave the cells
+  RowCoef<T>* cf = reinterpret_cast<RowCoef<T>*>(dn_smem_raw) + wv * RMAX * 64;
   RowScale<T>* cs = reinterpret_cast<RowScale<T>*>(
-      reinterpret_cast<RowCoef<T>*>(dn_smem_raw) + (blockDim.x >> 6) * kCableRMax * 64) +
-      wv * kCableRMax * 64;
+      reinterpret_cast<RowCoef<T>*>(dn_smem_raw) + (blockDim.x >> 6) * RMAX * 64) +
+      wv * RMAX * 64;
   switch (A.task_p[task]) {
-    case 4: cable_run_wave<T, 4, kCableRMax, SCALES>(A, task, cf, cs); break;
-    case 8: cable_run_wave<T, 8, kCableRMax, SCALES>(A, task, cf, cs); break;
-    case 16: cable_run_wave<T, 16, kCableRMax, SCALES>(A, task, cf, cs); break;
-    case 32: cable_run_wave<T, 32, kCableRMax, SCALES>(A, task, cf, cs); break;
-    default: cable_run_wave<T, 64, kCableRMax, SCALES>(A, task, cf, cs); break;
+    case 4: cable_run_wave<T, 4, RMAX, SCALES>(A, task, cf, cs); break;
+    case 8: cable_run_wave<T, 8, RMAX, SCALES>(A, task, cf, cs); break;
+    case 16: cable_run_wave<T, 16, RMAX, SCALES>(A, task, cf, cs); break;
+    case 32: cable_run_wave<T, 32, RMAX, SCALES>(A, task, cf, cs); break;
+    default: cable_run_wave<T, 64, RMAX, SCALES>(A, task, cf, cs); break;
   }
 }
 
@@ -284,25 +288,37 @@ __global__ __launch_bounds__(64) void hh_cable_solve_kernel(
   for (int64_t j = lane; j < K; j += 64) v[o + j] += sd[j];
 }
 
+template <typename T, int RMAX>
+static void cable_run_dispatch(const CableRunArgs<T>& A, int blocks, int waves_per_block,
+                               bool scales, hipStream_t stream) {
+  const size_t lds = (size_t)waves_per_block * RMAX * 64 *
+                     (sizeof(RowCoef<T>) + (scales ? sizeof(RowScale<T>) : 0));
+  if (scales)
+    hipLaunchKernelGGL((hh_cable_run_kernel<T, true, RMAX>), dim3(blocks),
+                       dim3(64 * waves_per_block), lds, stream, A);
+  else
+    hipLaunchKernelGGL((hh_cable_run_kernel<T, false, RMAX>), dim3(blocks),
+                       dim3(64 * waves_per_block), lds, stream, A);
+}
+
 template <typename T>
-static int cable_run_launch(const CableRunArgs<T>& A, int waves_per_block,
+static int cable_run_launch(const CableRunArgs<T>& A, int waves_per_block, int max_rows,
                             hipStream_t stream) {
   if (A.n_tasks < 0 || !A.task_cell0 || !A.task_ncell || !A.task_nr || !A.task_p ||
       !A.off || !A.v || !A.m || !A.h || !A.n || !A.gL || !A.gRp || !A.scale)
     return DN_EINVAL;
   if (A.rec && A.rec_every < 1) return DN_EINVAL;
   if (waves_per_block < 1 || waves_per_block > 4) return DN_EINVAL;
+  if (max_rows < 1 || max_rows > kCableRMax) return DN_ERANGE;
   if (A.n_tasks == 0 || A.n_steps == 0) return DN_OK;
   int blocks = (A.n_tasks + waves_per_block - 1) / waves_per_block;
   const bool scales = A.sna || A.sk;
-  size_t lds = (size_t)waves_per_block * kCableRMax * 64 *
-               (sizeof(RowCoef<T>) + (scales ? sizeof(RowScale<T>) : 0));
-  if (scales)
-    hipLaunchKernelGGL((hh_cable_run_kernel<T, true>), dim3(blocks),
-                       dim3(64 * waves_per_block), lds, stream, A);
+  if (max_rows <= 8)
+    cable_run_dispatch<T, 8>(A, blocks, waves_per_block, scales, stream);
+  else if (max_rows <= 10)
+    cable_run_dispatch<T, 10>(A, blocks, waves_per_block, scales, stream);
   else
-    hipLaunchKernelGGL((hh_cable_run_kernel<T, false>), dim3(blocks),
-                       dim3(64 * waves_per_block), lds, stream, A);
+    cable_run_dispatch<T, kCableRMax>(A, blocks, waves_per_block, scales, stream);
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
 
@@ -335,7 +351,7 @@ static int cable_step_launch(int64_t n_cells, const int64_t* off, int64_t total,
       T *h, T *n, const T *gL, const T *gRp, const T *scale, const T *drive,         \
       const T *sna, const T *sk, const T *wave, int64_t step0, int64_t n_steps,      \
       T dt, const T *hh, T *rec, int64_t rec_every, int64_t total,                   \
-      int waves_per_block, hipStream_t stream
+      int waves_per_block, int max_rows, hipStream_t stream
 
 #define DN_CABLE_RUN_BODY(T)                                                         \
   dn::CableRunArgs<T> A;                                                            \
@@ -347,7 +363,7 @@ static int cable_step_launch(int64_t n_cells, const int64_t* off, int64_t total,
   if (!hh) return DN_EINVAL;                                                        \
   A.hh = {hh[0], hh[1], hh[2], hh[3], hh[4], hh[5], hh[6]};                         \
   A.rec = rec; A.rec_every = rec_every; A.total = total;                            \
-  return dn::cable_run_launch<T>(A, waves_per_block, stream);
+  return dn::cable_run_launch<T>(A, waves_per_block, max_rows, stream);
 
 // Persistent multi-step run of a planned ragged cable batch.  hh holds
 // {gnabar, gkbar, gl, ena, ek, el, q10}.  Every task row of the plan must

@@ -33,6 +33,11 @@ from .topology import section_plan
 _DT = {torch.float32: np.float32, torch.float64: np.float64}
 
 
+# upper row bounds of the cable launch classes (kernel instances for 8, 10
+# and 13 rows per lane exist; 10 and 13 hold 4 and 3 wavefronts per SIMD)
+ROW_CLASSES = (10, 13)
+
+
 def _dev(a, dtype, device):
     if a is None:
         return None
@@ -67,10 +72,23 @@ class CablePopulation:
         self.sk = _dev(None if spec.sk is None else spec.sk[P], dt_, dev)
         self.off = torch.as_tensor(self.plan.off, dtype=torch.int64, device=dev)
         i32 = lambda a: torch.as_tensor(a, dtype=torch.int32, device=dev)
-        self.task_cell0 = i32(self.plan.task_cell0)
-        self.task_ncell = i32(self.plan.task_ncell)
-        self.task_nr = i32(self.plan.task_nr)
-        self.task_p = i32(self.plan.task_p)
+        # Tasks are grouped into row classes, each one launch of a kernel
+        # instance sized for its rows (fewer rows: fewer VGPRs and less LDS
+        # per wavefront, so more wavefronts per CU); cost order is kept
+        # within a class.  The launches run concurrently on two streams.
+        nr = self.plan.task_nr
+        cls = np.searchsorted(np.asarray(ROW_CLASSES), nr)
+        cls = cls.max() - cls if len(nr) else cls  # the widest class first
+        t_order = np.argsort(cls, kind="stable")
+        self._launches = []
+        for c in np.unique(cls):
+            sel = np.flatnonzero(cls[t_order] == c)
+            self._launches.append((int(sel[0]), int(sel[-1]) + 1, int(nr[t_order][sel].max())))
+        self.task_cell0 = i32(self.plan.task_cell0[t_order])
+        self.task_ncell = i32(self.plan.task_ncell[t_order])
+        self.task_nr = i32(nr[t_order])
+        self.task_p = i32(self.plan.task_p[t_order])
+        self._side = None
         self.max_nr_plain = int(-(-int(sizes.max()) // 64)) if len(sizes) else 1
         self.hh_arr = self.hh.array().astype(_DT[dtype])
         self.reset()
@@ -131,6 +149,9 @@ class CablePopulation:
                 else torch.cuda.current_stream(self.device)).cuda_stream
         if path == "persistent":
             fn = lib.dn_hh_cable_run_f64 if f64 else lib.dn_hh_cable_run_f32
+            caller = stream if stream is not None else torch.cuda.current_stream(self.device)
+            if len(self._launches) > 1 and self._side is None:
+                self._side = torch.cuda.Stream(self.device)
             chunk = steps_per_launch or n_steps
             done = 0
             while done < n_steps:
@@ -139,18 +160,24 @@ class CablePopulation:
                 if rec is not None:
                     # the kernel indexes rows by absolute step; shift the base
                     rec_ptr = rec.data_ptr() - rec_base * self.total * rec.element_size()
-                st = fn(
-                    len(self.plan.task_p), _lib.ptr(self.task_cell0),
-                    _lib.ptr(self.task_ncell), _lib.ptr(self.task_nr),
-                    _lib.ptr(self.task_p), _lib.ptr(self.off),
-                    _lib.ptr(self.v), _lib.ptr(self.m), _lib.ptr(self.h), _lib.ptr(self.n),
-                    _lib.ptr(self.gL), _lib.ptr(self.gRp), _lib.ptr(self.scale),
-                    _lib.ptr(self.drive), _lib.ptr(self.sna), _lib.ptr(self.sk),
-                    _lib.ptr(wave_t), self.t_step, k, self.dt,
-                    self.hh_arr.ctypes.data, rec_ptr, max(record_every, 1), self.total,
-                    waves_per_block, strm,
-                )
-                _lib.check(st, "persistent cable run")
+                if len(self._launches) > 1:
+                    self._side.wait_stream(caller)
+                for i, (t0, t1, rows) in enumerate(self._launches):
+                    s_ = caller if i == 0 else self._side
+                    at = lambda t: t.data_ptr() + t0 * t.element_size()
+                    st = fn(
+                        t1 - t0, at(self.task_cell0), at(self.task_ncell),
+                        at(self.task_nr), at(self.task_p), _lib.ptr(self.off),
+                        _lib.ptr(self.v), _lib.ptr(self.m), _lib.ptr(self.h),
+                        _lib.ptr(self.n), _lib.ptr(self.gL), _lib.ptr(self.gRp),
+                        _lib.ptr(self.scale), _lib.ptr(self.drive), _lib.ptr(self.sna),
+                        _lib.ptr(self.sk), _lib.ptr(wave_t), self.t_step, k, self.dt,
+                        self.hh_arr.ctypes.data, rec_ptr, max(record_every, 1), self.total,
+                        waves_per_block, rows, s_.cuda_stream,
+                    )
+                    _lib.check(st, "persistent cable run")
+                if len(self._launches) > 1:
+                    caller.wait_stream(self._side)
                 self.t_step += k
                 done += k
         elif path == "plain":
@@ -369,11 +396,13 @@ class TreePopulation:
                     rec_ptr = rec.data_ptr() - rec_base * self.total * rec.element_size()
                 caller = stream if stream is not None else torch.cuda.current_stream(
                     self.device)
-                streams = self._bucket_streams()
+                # a single bucket runs on the caller's stream (no extra queue)
+                streams = [caller] if len(self.buckets) == 1 else self._bucket_streams()
                 start = torch.cuda.Event()
                 start.record(caller)
                 for b, bs in zip(self.buckets, streams):
-                    bs.wait_event(start)
+                    if bs is not caller:
+                        bs.wait_event(start)
                     st = fn(
                         *self._plan_args(), _lib.ptr(b.order), b.n, _lib.ptr(b.grp_off),
                         b.n_groups, b.layout, b.kmax, b.threads,
@@ -385,9 +414,10 @@ class TreePopulation:
                         self.total, bs.cuda_stream,
                     )
                     _lib.check(st, "persistent tree run")
-                    done_ev = torch.cuda.Event()
-                    done_ev.record(bs)
-                    caller.wait_event(done_ev)
+                    if bs is not caller:
+                        done_ev = torch.cuda.Event()
+                        done_ev.record(bs)
+                        caller.wait_event(done_ev)
                 self.t_step += k
                 done += k
         elif path == "plain":

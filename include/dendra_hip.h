@@ -182,7 +182,10 @@ int dn_interp_delay_vjp_f64(const double* G, int64_t depth, int64_t width,
  * drive = activating-function coefficient scaled by wave[step] (nullable),
  * sna/sk gNa/gK scales (nullable).  hh (host): {gnabar, gkbar, gl, ena, ek,
  * el, q10}.  dt in ms.  rec (nullable): V after every rec_every-th absolute
- * step, row (step0+s+1)/rec_every-1 of a [rows][total] array. */
+ * step, row (step0+s+1)/rec_every-1 of a [rows][total] array.  max_rows
+ * (1..dn_hh_cable_rmax()) bounds task_nr over the launch and selects the
+ * kernel instance (8, 10 or 13 rows per lane); a task with more rows than
+ * the instance holds is left unadvanced. */
 int dn_hh_cable_rmax(void);
 int dn_hh_cable_run_f32(int n_tasks, const int32_t* task_cell0,
                         const int32_t* task_ncell, const int32_t* task_nr,
@@ -192,7 +195,7 @@ int dn_hh_cable_run_f32(int n_tasks, const int32_t* task_cell0,
                         const float* sk, const float* wave, int64_t step0,
                         int64_t n_steps, float dt, const float* hh, float* rec,
                         int64_t rec_every, int64_t total, int waves_per_block,
-                        hipStream_t stream);
+                        int max_rows, hipStream_t stream);
 int dn_hh_cable_run_f64(int n_tasks, const int32_t* task_cell0,
                         const int32_t* task_ncell, const int32_t* task_nr,
                         const int32_t* task_p, const int64_t* off, double* v, double* m,
@@ -201,7 +204,7 @@ int dn_hh_cable_run_f64(int n_tasks, const int32_t* task_cell0,
                         const double* sk, const double* wave, int64_t step0,
                         int64_t n_steps, double dt, const double* hh, double* rec,
                         int64_t rec_every, int64_t total, int waves_per_block,
-                        hipStream_t stream);
+                        int max_rows, hipStream_t stream);
 /* One plain step (gating+assembly launch, solve launch); mainv/rhs are
  * caller scratch of `total` values; w is this step's waveform value. */
 int dn_hh_cable_step_f32(int64_t n_cells, const int64_t* off, int64_t total, int max_nr,
