@@ -63,6 +63,8 @@ def _parse():
     ap.add_argument("--surrogate-epochs", type=int, default=40)
     ap.add_argument("--fit", type=int, default=1,
                     help="also time the adjoint workload (0 = skip)")
+    ap.add_argument("--solvers", type=int, default=1,
+                    help="also time the drop-in solver and delay operators (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB copied once by a plain device copy before the timed region "
                          "(reference kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -94,6 +96,8 @@ def main():
         out["network"] = network_bench(args.network_steps, hh, dev, args.cpu_sample > 0)
     if args.fit:
         out["adjoint"] = fit_bench(dev, args.cpu_sample > 0)
+    if args.solvers:
+        out["operators"] = operator_bench(dev)
     if args.surrogate_train > 0:
         out["surrogate"] = surrogate_bench(args.surrogate_train, args.surrogate_test,
                                            args.surrogate_epochs, args.cpu_sample > 0)
@@ -467,6 +471,128 @@ def surrogate_bench(n_train, n_test, epochs, with_cpu):
                                "cores": oracle.cpu_threads(), "kind": "port",
                                "sample": f"256 held-out fibers, float64 C oracle bisection, "
                                          f"OpenMP, {dt_cpu:.1f}s"}
+    return res
+
+
+def _time_op(fn, reps=20, warmup=3):
+    """Average milliseconds of fn() on torch's current stream (HIP events)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / reps
+
+
+def _rate(ms, nbytes, **extra):
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"ms": round(ms, 4), "bytes": int(nbytes), "GB_s": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 3), **extra}
+
+
+def operator_bench(dev):
+    """Secondary lines: the drop-in operators beside the cable path, float32,
+    inputs resident; algorithmic bytes per call over the average call time.
+
+    * tree solves of 2048 systems on a 2000-node random binary tree and on a
+      2000-node path: the DHS drop-in (one lane group per system, layer by
+      layer) and the tree contraction (O(log depth) rounds), 16 B per node
+      (read d, g, rhs; write x);
+    * 3x3-block solves, 4096 systems x 201 blocks: block Thomas and block
+      Cholesky, 84 B per block row (lower 3, main 9, upper 3, rhs 3, x 3);
+      block tree over a 2000-node binary tree x 512 systems, 72 B per node;
+    * the dense interpolated-delay transition at the network's size (4.2M
+      connections, 21,000 targets, ring depth 201): 16 B per connection
+      (w, g, tap base, CSR edge) + 4 B per stored fraction (1 or 3) + 8 B
+      per tap (ring cell read and write) + 4 B per target (cleared row) + the
+      copy of the ring that the pure transition returns (8 B per cell).
+    """
+    import torch
+
+    from dendra_amd import blocks, contraction, delays, solvers
+    from dendra_amd.topology import build_dhs_layers, build_morphology
+    from dendra_amd.workloads import random_binary_tree
+
+    gen = torch.Generator(device=dev).manual_seed(5)
+    u = lambda shape, lo, hi: torch.rand(shape, device=dev, generator=gen) * (hi - lo) + lo
+    res = {}
+
+    B, K = 2048, 2000
+    trees = {"binary_tree": np.asarray(random_binary_tree(K, 0, 11)[0]),
+             "path": np.arange(-1, K - 1)}
+    for name, par in trees.items():
+        K = len(par)
+        _, _, depth = build_morphology(par)
+        order, layers = build_dhs_layers(depth, 16)
+        topo = [torch.as_tensor(v, dtype=torch.int64, device=dev) for v in (par, order, layers)]
+        plan = contraction.TreeContractionPlan(par, max_segment_interior_nodes=16)
+        contraction.prepare(plan, dev)
+        d, g, rhs = u((B, K), 0.01, 2.0), u((B, K), 0.1, 5.0), u((B, K), -1.0, 1.0)
+        with torch.no_grad():
+            ms_dhs = _time_op(lambda: solvers.dhs_solve(d, g, rhs, *topo, threads=16))
+            ms_con = _time_op(lambda: contraction.solve(d, g, rhs, plan))
+            diff = (solvers.dhs_solve(d, g, rhs, *topo) - contraction.solve(d, g, rhs, plan))
+        nbytes = 16.0 * B * K
+        res[f"tree_solve_{name}"] = {
+            "systems": B, "nodes": K, "dhs_layers": len(layers) - 1,
+            "dhs": _rate(ms_dhs, nbytes), "contraction": _rate(ms_con, nbytes),
+            "contraction_speedup": round(ms_dhs / ms_con, 2),
+            "max_abs_diff": float(diff.abs().max()),
+        }
+
+    B, K = 4096, 201
+    low, up = u((B, K - 1, 3), -1.0, 1.0), u((B, K - 1, 3), -1.0, 1.0)
+    m = u((B, K, 3, 3), -0.5, 0.5)
+    eye = torch.eye(3, device=dev)
+    main = m + 4.0 * eye
+    spd_main = m @ m.transpose(-1, -2) + 4.0 * eye
+    rhs = u((B, K, 3), -1.0, 1.0)
+    with torch.no_grad():
+        ms_bt = _time_op(lambda: blocks.thomas_solve_cuda_bt(low, main, up, rhs))
+        ms_spd = _time_op(lambda: blocks.block_spd_solve(low, spd_main, low, rhs))
+    res["block_tridiagonal"] = {"systems": B, "blocks": K,
+                                "thomas": _rate(ms_bt, 84.0 * B * K),
+                                "cholesky": _rate(ms_spd, 84.0 * B * K)}
+    B = 512
+    par = np.asarray(random_binary_tree(2000, 1, 11)[0])
+    K = len(par)
+    _, _, depth = build_morphology(par)
+    order, layers = build_dhs_layers(depth, 16)
+    topo = [torch.as_tensor(v, dtype=torch.int64, device=dev) for v in (par, order, layers)]
+    tmain = u((B, K, 3, 3), -0.3, 0.3) + 2.0 * eye
+    tedge, trhs = u((B, K, 3), 0.1, 2.0), u((B, K, 3), -1.0, 1.0)
+    with torch.no_grad():
+        ms_bt_tree = _time_op(lambda: blocks.dhs_bt_solve(tmain, tedge, trhs, *topo))
+    res["block_tree"] = {"systems": B, "nodes": K, **_rate(ms_bt_tree, 72.0 * B * K)}
+
+    n, width, depth, dt = 4_200_000, 21_000, 201, 0.025
+    post = torch.randint(0, width, (n,), device=dev, generator=gen)
+    w = u((n,), 1e-4, 2e-3)
+    gates = (u((n,), 0.0, 1.0) < 0.3).float()
+    dly = u((n,), 1.0, 5.0)
+    q = u((depth, width), 0.0, 1e-3)
+    slot = torch.zeros(1, dtype=torch.int64, device=dev)
+    step = torch.zeros(1, dtype=torch.int64, device=dev)
+    for taps, sigma in ((2, 0.0), (3, 0.35)):
+        dtaps = delays.delay_interpolation(dly, dt, dtype=torch.float32, taps=taps,
+                                           sigma=sigma)
+        call = lambda: delays.dense_interpolated_delay_transition(
+            q, slot, step, w, gates, dly, dt, post_idx=post, depth=depth, width=width,
+            taps=taps, sigma=sigma, delay_taps=dtaps)
+        with torch.no_grad():
+            ms = _time_op(call)
+        # w, g, base, edge 16 B + fractions (1 or 3 floats) + 8 B per tap, the
+        # cleared row, and the copy of the ring the pure transition returns
+        nbytes = n * (16.0 + 4.0 * (1 if taps == 2 else 3) + 8.0 * taps) \
+            + 4.0 * width + 8.0 * depth * width
+        res[f"delay_transition_{taps}tap"] = {
+            "connections": n, "targets": width, "depth": depth, **_rate(ms, nbytes)}
     return res
 
 

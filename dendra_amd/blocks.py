@@ -24,7 +24,9 @@ checks, non-mutating inputs, fresh outputs and autograd contracts:
 
 from __future__ import annotations
 
+import numpy as np
 import torch
+from torch.utils.weak import WeakIdKeyDictionary
 
 from . import _lib
 from .solvers import validate_threads
@@ -158,15 +160,42 @@ def solve_bt_spd_with_chol(lower, upper, chol, rhs):
     return x
 
 
-def _tree_raw(main, edge, rhs, parent, order):
+# Children CSR per parent tensor object (weakly held, so reused storage never
+# inherits another tensor's plan), checked against the tensor's version.
+_children = WeakIdKeyDictionary()
+
+
+def _children_csr(parent):
+    stamp = (parent._version, parent.numel(), parent.device)
+    hit = _children.get(parent)
+    if hit is not None and hit[0] == stamp:
+        return hit[1]
+    par = parent.detach().to("cpu", torch.int64).numpy().reshape(-1)
+    K = par.size
+    par = np.where((par >= 0) & (par < K), par, -1)
+    kids = np.flatnonzero(par >= 0)
+    order = kids[np.argsort(par[kids], kind="stable")].astype(np.int32)
+    ptr = np.zeros(K + 1, dtype=np.int32)
+    np.cumsum(np.bincount(par[kids], minlength=K), out=ptr[1:])
+    dev = parent.device
+    csr = (torch.as_tensor(ptr, device=dev),
+           torch.as_tensor(order if order.size else np.zeros(1, np.int32), device=dev))
+    _children[parent] = (stamp, csr)
+    return csr
+
+
+def _tree_raw(main, edge, rhs, parent, order, layer_ptr):
     main, edge, rhs = (t.contiguous() for t in (main, edge, rhs))
-    parent, order = (t.to(torch.int64).contiguous() for t in (parent, order))
+    child_ptr, child_idx = _children_csr(parent)
+    parent, order, layer_ptr = (t.to(torch.int64).contiguous()
+                                for t in (parent, order, layer_ptr))
     B, K = map(int, rhs.shape[:2])
     x = torch.empty_like(rhs, memory_format=torch.contiguous_format)
     work, n = _work(KIND_TREE, B, K, rhs)
     p = _lib.ptr
-    st = _fn("dn_bt3_tree", rhs)(p(main), p(edge), p(rhs), p(parent), p(order), p(x), B, K,
-                                 p(work), n, _lib.stream_handle(rhs.device))
+    st = _fn("dn_bt3_tree", rhs)(p(main), p(edge), p(rhs), p(parent), p(order), p(layer_ptr),
+                                 layer_ptr.numel() - 1, p(child_ptr), p(child_idx), p(x), B,
+                                 K, p(work), n, _lib.stream_handle(rhs.device))
     _lib.check(st, "block tree solve")
     return x
 
@@ -207,21 +236,21 @@ class _SPDSolve(torch.autograd.Function):
 
 class _TreeBT3Solve(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, main, edge, rhs, parent, order):
-        x = _tree_raw(main, edge, rhs, parent, order)
-        ctx.save_for_backward(main, edge, x, parent, order)
+    def forward(ctx, main, edge, rhs, parent, order, layer_ptr):
+        x = _tree_raw(main, edge, rhs, parent, order, layer_ptr)
+        ctx.save_for_backward(main, edge, x, parent, order, layer_ptr)
         return x
 
     @staticmethod
     def backward(ctx, grad_x):
-        main, edge, x, parent, order = ctx.saved_tensors
-        g = _tree_raw(main.transpose(-1, -2), edge, grad_x, parent, order)
+        main, edge, x, parent, order, layer_ptr = ctx.saved_tensors
+        g = _tree_raw(main.transpose(-1, -2), edge, grad_x, parent, order, layer_ptr)
         par = parent.to(torch.long)
         root = (par < 0).reshape(1, -1, 1)
         safe = par.clamp_min(0)
         ge = torch.where(root, -(g * x),
                          -((g - g.index_select(1, safe)) * (x - x.index_select(1, safe))))
-        return -(g.unsqueeze(-1) * x.unsqueeze(-2)), ge, g, None, None
+        return -(g.unsqueeze(-1) * x.unsqueeze(-2)), ge, g, None, None, None
 
 
 def thomas_solve_cuda_bt(lower, main, upper, rhs):
@@ -244,4 +273,4 @@ def dhs_bt_solve(main, edge, rhs, parent_idx, order, layer_ptr, threads: int = 1
     """Block tree (Hines) solve with 3x3 node blocks over the caller's DHS
     order (children before parents, as build_dhs_layers produces)."""
     validate_tree_block(main, edge, rhs, parent_idx, order, layer_ptr, threads)
-    return _TreeBT3Solve.apply(main, edge, rhs, parent_idx, order)
+    return _TreeBT3Solve.apply(main, edge, rhs, parent_idx, order, layer_ptr)

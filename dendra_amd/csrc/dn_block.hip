@@ -13,9 +13,11 @@
 //       A_ii = main_i + diag(e_i) + sum_children diag(e_c),
 //       A_ip = A_pi = -diag(e_i),  (the root's e is diagonal only).
 //
-// One lane per system (a wavefront solves 64 systems); the elimination
-// coefficients live in a caller workspace laid out [row][component][system],
-// so every workspace access of a wavefront is one coalesced transaction.
+// Block Thomas and block Cholesky: one lane per system (a wavefront solves
+// 64 systems); the elimination coefficients live in a caller workspace laid
+// out [row][component][system], so every workspace access of a wavefront is
+// one coalesced transaction.  Block tree: one wavefront per system over the
+// DHS layers (or one lane per system for path-like trees).
 #include "dn_common.hpp"
 
 namespace dn {
@@ -304,6 +306,82 @@ __global__ __launch_bounds__(256) void bt3_tree_kernel(const T* __restrict__ mai
   }
 }
 
+// Layer-parallel block tree solve: one wavefront per system, the nodes of
+// each DHS layer chunk (layer_ptr over `order`, all of one depth) in
+// parallel.  Each node pulls its children's eliminated terms through the
+// children CSR (ascending child index), so no two lanes write one node:
+//   A_i = main_i + diag(e_i) + sum_c [diag(e_c) - diag(e_c) Ainv_c diag(e_c)]
+//   b_i = rhs_i + sum_c e_c * (Ainv_c b_c)
+// then top-down x_i = Ainv_i (b_i + e_i * x_p).  work: [B][K][12] = Ainv (9)
+// and the eliminated right-hand side (3) per node.
+template <typename T>
+__global__ __launch_bounds__(64) void bt3_tree_layer_kernel(
+    const T* __restrict__ main, const T* __restrict__ edge, const T* __restrict__ rhs,
+    const int64_t* __restrict__ parent, const int64_t* __restrict__ order,
+    const int64_t* __restrict__ layer_ptr, int n_layers, const int32_t* __restrict__ child_ptr,
+    const int32_t* __restrict__ child_idx, T* __restrict__ x, T* __restrict__ work,
+    int64_t K) {
+  const int64_t s = blockIdx.x;
+  const T* M = main + s * K * 9;
+  const T* E = edge + s * K * 3;
+  const T* R = rhs + s * K * 3;
+  T* X = x + s * K * 3;
+  T* W = work + s * K * 12;
+  auto span = [&](int l, int64_t& lo, int64_t& hi) {
+    lo = layer_ptr[l];
+    hi = layer_ptr[l + 1];
+    lo = lo < 0 ? 0 : (lo > K ? K : lo);
+    hi = hi < lo ? lo : (hi > K ? K : hi);
+  };
+  for (int l = 0; l < n_layers; ++l) {
+    int64_t lo, hi;
+    span(l, lo, hi);
+    for (int64_t n = lo + threadIdx.x; n < hi; n += 64) {
+      const int64_t i = order[n];
+      if (i < 0 || i >= K) continue;
+      T a[9], b[3];
+      for (int c = 0; c < 9; ++c) a[c] = M[i * 9 + c];
+      for (int c = 0; c < 3; ++c) {
+        a[c * 4] += E[i * 3 + c];
+        b[c] = R[i * 3 + c];
+      }
+      for (int j = child_ptr[i]; j < child_ptr[i + 1]; ++j) {
+        const int64_t ch = child_idx[j];
+        const T* Wc = W + ch * 12;
+        const T e[3] = {E[ch * 3], E[ch * 3 + 1], E[ch * 3 + 2]};
+        T iv[9], y[3];
+        for (int c = 0; c < 9; ++c) iv[c] = Wc[c];
+        mv3(iv, Wc + 9, y);
+        for (int r = 0; r < 3; ++r) {
+          a[r * 4] += e[r];
+          for (int c = 0; c < 3; ++c) a[r * 3 + c] -= e[r] * iv[r * 3 + c] * e[c];
+          b[r] += e[r] * y[r];
+        }
+      }
+      T iv[9];
+      inv3(a, iv);
+      for (int c = 0; c < 9; ++c) W[i * 12 + c] = iv[c];
+      for (int c = 0; c < 3; ++c) W[i * 12 + 9 + c] = b[c];
+    }
+    __syncthreads();
+  }
+  for (int l = n_layers - 1; l >= 0; --l) {
+    int64_t lo, hi;
+    span(l, lo, hi);
+    for (int64_t n = lo + threadIdx.x; n < hi; n += 64) {
+      const int64_t i = order[n];
+      if (i < 0 || i >= K) continue;
+      const int64_t p = parent[i];
+      T b[3], xi[3];
+      for (int c = 0; c < 3; ++c)
+        b[c] = W[i * 12 + 9 + c] + ((p >= 0 && p < K) ? E[i * 3 + c] * X[p * 3 + c] : T(0));
+      mv3(W + i * 12, b, xi);
+      for (int c = 0; c < 3; ++c) X[i * 3 + c] = xi[c];
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace dn
 
 template <typename T>
@@ -345,13 +423,25 @@ static int bt3_spd_impl(const T* lower, const T* main, const T* upper, const T* 
 
 template <typename T>
 static int bt3_tree_impl(const T* main, const T* edge, const T* rhs, const int64_t* parent,
-                         const int64_t* order, T* x, int64_t B, int64_t K, T* work,
-                         int64_t work_elems, hipStream_t stream) {
+                         const int64_t* order, const int64_t* layer_ptr, int64_t n_layers,
+                         const int32_t* child_ptr, const int32_t* child_idx, T* x, int64_t B,
+                         int64_t K, T* work, int64_t work_elems, hipStream_t stream) {
   if (!main || !edge || !rhs || !parent || !order || !x || !work || B < 1 || K < 1)
     return DN_EINVAL;
-  if (work_elems < 12 * B * K || B > (int64_t)INT32_MAX * 256) return DN_EINVAL;
-  hipLaunchKernelGGL((dn::bt3_tree_kernel<T>), dim3(blocks_for<T>(B)), dim3(256), 0, stream,
-                     main, edge, rhs, parent, order, x, work, B, K);
+  if (work_elems < 12 * B * K || B > (int64_t)INT32_MAX * 256 || n_layers < 0 ||
+      n_layers > INT32_MAX)
+    return DN_EINVAL;
+  // Layer-parallel form when the layers average at least two nodes and the
+  // children CSR is given; else one lane per system walks `order`.
+  if (layer_ptr && child_ptr && child_idx && n_layers >= 1 && 2 * n_layers <= K &&
+      B <= INT32_MAX) {
+    hipLaunchKernelGGL((dn::bt3_tree_layer_kernel<T>), dim3((unsigned)B), dim3(64), 0, stream,
+                       main, edge, rhs, parent, order, layer_ptr, (int)n_layers, child_ptr,
+                       child_idx, x, work, K);
+  } else {
+    hipLaunchKernelGGL((dn::bt3_tree_kernel<T>), dim3(blocks_for<T>(B)), dim3(256), 0,
+                       stream, main, edge, rhs, parent, order, x, work, B, K);
+  }
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
 
@@ -381,16 +471,20 @@ DN_EXPORT int dn_bt3_spd_f64(const double* lower, const double* main, const doub
                               work_elems, factor, 1e-12, 1e-30, stream);
 }
 DN_EXPORT int dn_bt3_tree_f32(const float* main, const float* edge, const float* rhs,
-                              const int64_t* parent, const int64_t* order, float* x,
+                              const int64_t* parent, const int64_t* order,
+                              const int64_t* layer_ptr, int64_t n_layers,
+                              const int32_t* child_ptr, const int32_t* child_idx, float* x,
                               int64_t B, int64_t K, float* work, int64_t work_elems,
                               hipStream_t stream) {
-  return bt3_tree_impl<float>(main, edge, rhs, parent, order, x, B, K, work, work_elems,
-                              stream);
+  return bt3_tree_impl<float>(main, edge, rhs, parent, order, layer_ptr, n_layers, child_ptr,
+                            child_idx, x, B, K, work, work_elems, stream);
 }
 DN_EXPORT int dn_bt3_tree_f64(const double* main, const double* edge, const double* rhs,
-                              const int64_t* parent, const int64_t* order, double* x,
+                              const int64_t* parent, const int64_t* order,
+                              const int64_t* layer_ptr, int64_t n_layers,
+                              const int32_t* child_ptr, const int32_t* child_idx, double* x,
                               int64_t B, int64_t K, double* work, int64_t work_elems,
                               hipStream_t stream) {
-  return bt3_tree_impl<double>(main, edge, rhs, parent, order, x, B, K, work, work_elems,
-                               stream);
+  return bt3_tree_impl<double>(main, edge, rhs, parent, order, layer_ptr, n_layers, child_ptr,
+                            child_idx, x, B, K, work, work_elems, stream);
 }

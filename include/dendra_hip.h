@@ -111,7 +111,13 @@ int dn_tree_contract_f64(const double* d, const double* g, const double* rhs, do
  *                    chol; main may be null) (triton/bt_spd_kernel.py:575-700);
  *   dn_bt3_tree_*    _dhs_bt3_adjoint (triton/dhs_kernel_bt.py): main
  *                    (B,K,3,3), edge (B,K,3) per-component conductance to the
- *                    parent, parent (K), order (K, children before parents).
+ *                    parent, parent (K), order (K, children before parents),
+ *                    layer_ptr (n_layers + 1) chunking `order` into runs of
+ *                    one depth, and the children CSR child_ptr (K + 1) /
+ *                    child_idx; with the layers and CSR given (and layers
+ *                    averaging two or more nodes) one wavefront solves a
+ *                    system layer by layer, else (layer_ptr null) one lane
+ *                    walks `order`.
  * Inputs are not modified; work holds dn_block_work_elems(kind, B, K)
  * elements (kind 0 Thomas, 1 SPD, 2 tree). */
 int64_t dn_block_work_elems(int kind, int64_t B, int64_t K);
@@ -130,11 +136,15 @@ int dn_bt3_spd_f64(const double* lower, const double* main, const double* upper,
                    int64_t K, double* work, int64_t work_elems, int factor,
                    hipStream_t stream);
 int dn_bt3_tree_f32(const float* main, const float* edge, const float* rhs,
-                    const int64_t* parent, const int64_t* order, float* x, int64_t B,
-                    int64_t K, float* work, int64_t work_elems, hipStream_t stream);
+                    const int64_t* parent, const int64_t* order, const int64_t* layer_ptr,
+                    int64_t n_layers, const int32_t* child_ptr, const int32_t* child_idx,
+                    float* x, int64_t B, int64_t K, float* work, int64_t work_elems,
+                    hipStream_t stream);
 int dn_bt3_tree_f64(const double* main, const double* edge, const double* rhs,
-                    const int64_t* parent, const int64_t* order, double* x, int64_t B,
-                    int64_t K, double* work, int64_t work_elems, hipStream_t stream);
+                    const int64_t* parent, const int64_t* order, const int64_t* layer_ptr,
+                    int64_t n_layers, const int32_t* child_ptr, const int32_t* child_idx,
+                    double* x, int64_t B, int64_t K, double* work, int64_t work_elems,
+                    hipStream_t stream);
 
 /* ---- NetCon delivery with fractional (interpolated) delays ----------------
  * Replaces the ordered two-tap enqueue (dendra/models/networks/

@@ -136,13 +136,22 @@ def test_block_spd_shifts_an_indefinite_block_and_rejects_nan():
         solve_bt_spd_fwd(empty.cuda(), bad.cuda(), empty.cuda(), rhs.cuda())
 
 
-@pytest.mark.parametrize("K", [1, 2, 9, 400])
-def test_block_tree_matches_dense_with_gradients(K):
+@pytest.mark.parametrize("K", [1, 2, 9, 40, 400])
+@pytest.mark.parametrize("kind", ["near", "binary", "path"])
+def test_block_tree_matches_dense_with_gradients(K, kind):
+    """Layered trees (binary; layers average two or more nodes) take the
+    wavefront-per-system layer kernel, paths and single nodes the
+    lane-per-system kernel; both against dense float64 solves."""
     from dendra_amd.blocks import dhs_bt_solve
     from dendra_amd.topology import build_dhs_layers, build_morphology
 
     rng = np.random.default_rng(200 + K)
-    parents = random_parents(K, rng)
+    if kind == "near":
+        parents = random_parents(K, rng)
+    elif kind == "binary":
+        parents = [-1] + [(i - 1) // 2 for i in range(1, K)]
+    else:
+        parents = list(range(-1, K - 1))
     _, _, depth = build_morphology(parents)
     order, layers = build_dhs_layers(depth, 16)
     main = torch.as_tensor(rng.uniform(-0.3, 0.3, (3, K, 3, 3)) + np.eye(3) * 2.0)
