@@ -593,6 +593,27 @@ def operator_bench(dev):
             + 4.0 * width + 8.0 * depth * width
         res[f"delay_transition_{taps}tap"] = {
             "connections": n, "targets": width, "depth": depth, **_rate(ms, nbytes)}
+
+    from dendra_amd import history
+
+    n_src = 21_000
+    src = torch.randint(0, n_src, (n,), device=dev, generator=gen)
+    hist = u((depth, n_src), 0.0, 1.0)
+    for taps, sigma in ((2, 0.0), (3, 0.35)):
+        dtaps = delays.delay_interpolation(dly, dt, dtype=torch.float32, taps=taps,
+                                           sigma=sigma)
+        call = lambda: history.source_history_project(
+            hist, slot, w, post_idx=post, depth=depth, width=width, source_pos=src,
+            delay_taps=dtaps, taps=taps)
+        with torch.no_grad():
+            ms = _time_op(call)
+        # per connection: CSR edge, source column, w 4 B each, tap offset and
+        # gate read 4 B each per tap, tap weights (1 or 3 floats); per target
+        # the payload
+        nbytes = n * (12.0 + 8.0 * taps + 4.0 * (1 if taps == 2 else 3)) + 4.0 * width
+        res[f"history_projection_{taps}tap"] = {
+            "connections": n, "sources": n_src, "targets": width, "depth": depth,
+            **_rate(ms, nbytes)}
     return res
 
 

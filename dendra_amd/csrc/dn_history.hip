@@ -1,0 +1,306 @@
+// dn_history.hip -- source-history transport of NetCon training.
+//
+// GPU form of the reference's pure source-history kernels
+// (dendra/models/networks/_source_history.py:83-186), used by NetCon's
+// source-history training backend (networks/netcon.py:3660-3725):
+//
+//   project: today's payload from a history ring hist[depth][n_col] of
+//     per-source gates, floating (hist[row][source]) or packed
+//     ((hist[row][word] & mask) != 0 as 0/1).  Connection e with target t
+//     reads its source at rows (slot - off[e][j]) mod depth, j < taps, and
+//     contributes
+//       taps 1 (fixed delays)  v = w g0
+//       taps 2                 v = w (g0 (1 - a) + g1 a)
+//       taps 3                 v = w ((g0 f0 + g1 f1) + g2 f2)
+//     with every product and sum rounded separately, as the reference's
+//     elementwise torch ops round them.  out[t] = 0 + v_e1 + v_e2 + ... in
+//     ascending connection order (the sequential index_add), one lane per
+//     target walking the target-major CSR, so the payload is bitwise the
+//     reference's.
+//   project vjp: per connection dw, the tap-weight cotangents and, for a
+//     floating history, the history cotangent scattered with atomics.
+//   record: overwrite row `slot` with the current gates (floating), or with
+//     the packed words sum_s mask_s [gate_s != 0] (integer sums, exact in any
+//     order).
+#include "dn_common.hpp"
+
+namespace dn {
+
+__device__ __forceinline__ int hist_slot(const int64_t* slot, int64_t depth) {
+  const int64_t s = *slot % depth;
+  return (int)(s < 0 ? s + depth : s);
+}
+
+// row of tap j: (slot - off) mod depth with slot, off in [0, depth)
+__device__ __forceinline__ int64_t hist_row(int slot, int32_t off, int64_t depth) {
+  const int64_t r = (int64_t)slot - (int64_t)off;
+  return r < 0 ? r + depth : r;
+}
+
+template <typename T, bool PACKED>
+__device__ __forceinline__ T hist_gate(const void* hist, int64_t n_col, int64_t row,
+                                       int32_t col, const int64_t* mask, int64_t e) {
+  if (PACKED) {
+    const int64_t word = static_cast<const int64_t*>(hist)[row * n_col + col];
+    return (word & mask[e]) != 0 ? T(1) : T(0);
+  }
+  return static_cast<const T*>(hist)[row * n_col + col];
+}
+
+template <typename T, bool PACKED, int TAPS>
+__global__ __launch_bounds__(256) void history_project_kernel(
+    const void* __restrict__ hist, int64_t depth, int64_t n_col,
+    const int64_t* __restrict__ slot_p, const int32_t* __restrict__ in_ptr,
+    const int32_t* __restrict__ in_edge, const int32_t* __restrict__ col,
+    const int64_t* __restrict__ mask, const int32_t* __restrict__ off,
+    const T* __restrict__ w, const T* __restrict__ frac, T* __restrict__ out, int64_t width) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= width) return;
+  const int slot = hist_slot(slot_p, depth);
+  T acc = T(0);
+  const int e0 = in_ptr[t], e1 = in_ptr[t + 1];
+  for (int i = e0; i < e1; ++i) {
+    const int64_t e = in_edge[i];
+    const int32_t c = col[e];
+    T g[TAPS];
+#pragma unroll
+    for (int j = 0; j < TAPS; ++j)
+      g[j] = hist_gate<T, PACKED>(hist, n_col, hist_row(slot, off[e * TAPS + j], depth), c,
+                                  mask, e);
+    T s;
+    if (TAPS == 1) {
+      s = g[0];
+    } else if (TAPS == 2) {
+      const T a = frac[e];
+      const T lo = g[0] * (T(1) - a);
+      const T hi = g[1] * a;
+      s = lo + hi;
+    } else {
+      const T p0 = g[0] * frac[e * 3];
+      const T p1 = g[1] * frac[e * 3 + 1];
+      const T p2 = g[2] * frac[e * 3 + 2];
+      s = (p0 + p1) + p2;
+    }
+    const T v = w[e] * s;
+    acc = acc + v;
+  }
+  out[t] = acc;
+}
+
+template <typename T, bool PACKED, int TAPS>
+__global__ __launch_bounds__(256) void history_project_vjp_kernel(
+    const T* __restrict__ G, const void* __restrict__ hist, int64_t depth, int64_t n_col,
+    const int64_t* __restrict__ slot_p, int64_t n_conn, const int32_t* __restrict__ post,
+    const int32_t* __restrict__ col, const int64_t* __restrict__ mask,
+    const int32_t* __restrict__ off, const T* __restrict__ w, const T* __restrict__ frac,
+    T* __restrict__ dw, T* __restrict__ dfrac, T* __restrict__ dhist) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_conn) return;
+  const int slot = hist_slot(slot_p, depth);
+  const T gt = G[post[e]];
+  const int32_t c = col[e];
+  int64_t rows[TAPS];
+  T g[TAPS], coef[TAPS];
+#pragma unroll
+  for (int j = 0; j < TAPS; ++j) {
+    rows[j] = hist_row(slot, off[e * TAPS + j], depth);
+    g[j] = hist_gate<T, PACKED>(hist, n_col, rows[j], c, mask, e);
+  }
+  T s;
+  if (TAPS == 1) {
+    coef[0] = T(1);
+    s = g[0];
+  } else if (TAPS == 2) {
+    const T a = frac[e];
+    coef[0] = T(1) - a;
+    coef[1] = a;
+    s = g[0] * coef[0] + g[1] * coef[1];
+  } else {
+    s = T(0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      coef[j] = frac[e * 3 + j];
+      s += g[j] * coef[j];
+    }
+  }
+  dw[e] = gt * s;
+  const T ds = gt * w[e];
+  if (TAPS == 2) dfrac[e] = ds * (g[1] - g[0]);
+  if (TAPS == 3)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dfrac[e * 3 + j] = ds * g[j];
+  if (!PACKED && dhist)
+#pragma unroll
+    for (int j = 0; j < TAPS; ++j) atomicAdd(dhist + rows[j] * n_col + c, ds * coef[j]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void history_record_kernel(T* __restrict__ hist,
+                                                             int64_t depth, int64_t n_col,
+                                                             const int64_t* __restrict__ slot_p,
+                                                             const T* __restrict__ gates) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_col) return;
+  hist[(int64_t)hist_slot(slot_p, depth) * n_col + s] = gates[s];
+}
+
+__global__ __launch_bounds__(256) void history_clear_row_kernel(
+    int64_t* __restrict__ hist, int64_t depth, int64_t n_words,
+    const int64_t* __restrict__ slot_p) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_words) return;
+  hist[(int64_t)hist_slot(slot_p, depth) * n_words + k] = 0;
+}
+
+__global__ __launch_bounds__(256) void history_pack_kernel(
+    int64_t* __restrict__ hist, int64_t depth, int64_t n_words,
+    const int64_t* __restrict__ slot_p, int64_t n_src, const uint8_t* __restrict__ active,
+    const int32_t* __restrict__ word, const int64_t* __restrict__ mask) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_src || !active[s]) return;
+  int64_t* row = hist + (int64_t)hist_slot(slot_p, depth) * n_words;
+  atomicAdd(reinterpret_cast<unsigned long long*>(row + word[s]),
+            static_cast<unsigned long long>(mask[s]));
+}
+
+}  // namespace dn
+
+static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+template <typename T, bool PACKED>
+static void project_launch(int taps, const void* hist, int64_t depth, int64_t n_col,
+                           const int64_t* slot, const int32_t* in_ptr, const int32_t* in_edge,
+                           const int32_t* col, const int64_t* mask, const int32_t* off,
+                           const T* w, const T* frac, T* out, int64_t width,
+                           hipStream_t stream) {
+  const dim3 g = grid_for(width), b(256);
+  if (taps == 1)
+    hipLaunchKernelGGL((dn::history_project_kernel<T, PACKED, 1>), g, b, 0, stream, hist,
+                       depth, n_col, slot, in_ptr, in_edge, col, mask, off, w, frac, out,
+                       width);
+  else if (taps == 2)
+    hipLaunchKernelGGL((dn::history_project_kernel<T, PACKED, 2>), g, b, 0, stream, hist,
+                       depth, n_col, slot, in_ptr, in_edge, col, mask, off, w, frac, out,
+                       width);
+  else
+    hipLaunchKernelGGL((dn::history_project_kernel<T, PACKED, 3>), g, b, 0, stream, hist,
+                       depth, n_col, slot, in_ptr, in_edge, col, mask, off, w, frac, out,
+                       width);
+}
+
+template <typename T>
+static int project_impl(const void* hist, int packed, int64_t depth, int64_t n_col,
+                        const int64_t* slot, int taps, const int32_t* in_ptr,
+                        const int32_t* in_edge, const int32_t* col, const int64_t* mask,
+                        const int32_t* off, const T* w, const T* frac, T* out, int64_t width,
+                        hipStream_t stream) {
+  if (!slot || !in_ptr || !out || depth < 1 || n_col < 0 || width < 0) return DN_EINVAL;
+  if (taps < 1 || taps > 3 || width > (int64_t)INT32_MAX * 256) return DN_EINVAL;
+  if (width == 0) return DN_OK;
+  if (!hist || !in_edge || !col || !off || !w || (taps > 1 && !frac) || (packed && !mask))
+    return DN_EINVAL;
+  if (packed)
+    project_launch<T, true>(taps, hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
+                            w, frac, out, width, stream);
+  else
+    project_launch<T, false>(taps, hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
+                             w, frac, out, width, stream);
+  return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+}
+
+template <typename T, bool PACKED>
+static void vjp_launch(int taps, const T* G, const void* hist, int64_t depth, int64_t n_col,
+                       const int64_t* slot, int64_t n_conn, const int32_t* post,
+                       const int32_t* col, const int64_t* mask, const int32_t* off, const T* w,
+                       const T* frac, T* dw, T* dfrac, T* dhist, hipStream_t stream) {
+  const dim3 g = grid_for(n_conn), b(256);
+  if (taps == 1)
+    hipLaunchKernelGGL((dn::history_project_vjp_kernel<T, PACKED, 1>), g, b, 0, stream, G,
+                       hist, depth, n_col, slot, n_conn, post, col, mask, off, w, frac, dw,
+                       dfrac, dhist);
+  else if (taps == 2)
+    hipLaunchKernelGGL((dn::history_project_vjp_kernel<T, PACKED, 2>), g, b, 0, stream, G,
+                       hist, depth, n_col, slot, n_conn, post, col, mask, off, w, frac, dw,
+                       dfrac, dhist);
+  else
+    hipLaunchKernelGGL((dn::history_project_vjp_kernel<T, PACKED, 3>), g, b, 0, stream, G,
+                       hist, depth, n_col, slot, n_conn, post, col, mask, off, w, frac, dw,
+                       dfrac, dhist);
+}
+
+template <typename T>
+static int vjp_impl(const T* G, const void* hist, int packed, int64_t depth, int64_t n_col,
+                    const int64_t* slot, int taps, int64_t n_conn, const int32_t* post,
+                    const int32_t* col, const int64_t* mask, const int32_t* off, const T* w,
+                    const T* frac, T* dw, T* dfrac, T* dhist, hipStream_t stream) {
+  if (!slot || depth < 1 || n_col < 0 || n_conn < 0 || taps < 1 || taps > 3)
+    return DN_EINVAL;
+  if (n_conn == 0) return DN_OK;
+  if (!G || !hist || !post || !col || !off || !w || !dw || (packed && !mask)) return DN_EINVAL;
+  if (taps > 1 && (!frac || !dfrac)) return DN_EINVAL;
+  if (n_conn > (int64_t)INT32_MAX * 256) return DN_EINVAL;
+  if (packed)
+    vjp_launch<T, true>(taps, G, hist, depth, n_col, slot, n_conn, post, col, mask, off, w,
+                        frac, dw, dfrac, nullptr, stream);
+  else
+    vjp_launch<T, false>(taps, G, hist, depth, n_col, slot, n_conn, post, col, mask, off, w,
+                         frac, dw, dfrac, dhist, stream);
+  return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+}
+
+template <typename T>
+static int record_impl(T* hist, int64_t depth, int64_t n_col, const int64_t* slot,
+                       const T* gates, hipStream_t stream) {
+  if (!slot || depth < 1 || n_col < 0 || n_col > (int64_t)INT32_MAX * 256) return DN_EINVAL;
+  if (n_col == 0) return DN_OK;  // an empty history may have no storage
+  if (!hist || !gates) return DN_EINVAL;
+  hipLaunchKernelGGL((dn::history_record_kernel<T>), grid_for(n_col), dim3(256), 0, stream,
+                     hist, depth, n_col, slot, gates);
+  return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+}
+
+#define DN_HISTORY_EXPORTS(T, SFX)                                                          \
+  DN_EXPORT int dn_history_project_##SFX(                                                    \
+      const void* hist, int packed, int64_t depth, int64_t n_col, const int64_t* slot,       \
+      int taps, const int32_t* in_ptr, const int32_t* in_edge, const int32_t* col,           \
+      const int64_t* mask, const int32_t* off, const T* w, const T* frac, T* out,            \
+      int64_t width, hipStream_t stream) {                                                   \
+    return project_impl<T>(hist, packed, depth, n_col, slot, taps, in_ptr, in_edge, col,     \
+                           mask, off, w, frac, out, width, stream);                          \
+  }                                                                                          \
+  DN_EXPORT int dn_history_project_vjp_##SFX(                                                \
+      const T* grad_out, const void* hist, int packed, int64_t depth, int64_t n_col,         \
+      const int64_t* slot, int taps, int64_t n_conn, const int32_t* post,                    \
+      const int32_t* col, const int64_t* mask, const int32_t* off, const T* w,               \
+      const T* frac, T* dw, T* dfrac, T* dhist, hipStream_t stream) {                        \
+    return vjp_impl<T>(grad_out, hist, packed, depth, n_col, slot, taps, n_conn, post, col,  \
+                       mask, off, w, frac, dw, dfrac, dhist, stream);                        \
+  }                                                                                          \
+  DN_EXPORT int dn_history_record_##SFX(T* hist, int64_t depth, int64_t n_col,              \
+                                        const int64_t* slot, const T* gates,                 \
+                                        hipStream_t stream) {                                \
+    return record_impl<T>(hist, depth, n_col, slot, gates, stream);                          \
+  }
+
+DN_HISTORY_EXPORTS(float, f32)
+DN_HISTORY_EXPORTS(double, f64)
+
+DN_EXPORT int dn_history_record_packed(int64_t* hist, int64_t depth, int64_t n_words,
+                                       const int64_t* slot, int64_t n_src,
+                                       const uint8_t* active, const int32_t* word,
+                                       const int64_t* mask, hipStream_t stream) {
+  if (!slot || depth < 1 || n_words < 0 || n_src < 0) return DN_EINVAL;
+  if (n_words > (int64_t)INT32_MAX * 256 || n_src > (int64_t)INT32_MAX * 256) return DN_EINVAL;
+  if (n_words == 0 && n_src == 0) return DN_OK;
+  if (!hist || n_words == 0) return DN_EINVAL;
+  if (n_words > 0)
+    hipLaunchKernelGGL(dn::history_clear_row_kernel, grid_for(n_words), dim3(256), 0, stream,
+                       hist, depth, n_words, slot);
+  if (n_src > 0) {
+    if (!active || !word || !mask) return DN_EINVAL;
+    hipLaunchKernelGGL(dn::history_pack_kernel, grid_for(n_src), dim3(256), 0, stream, hist,
+                       depth, n_words, slot, n_src, active, word, mask);
+  }
+  return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+}

@@ -181,6 +181,53 @@ int dn_interp_delay_vjp_f64(const double* G, int64_t depth, int64_t width,
                             const double* f, const int32_t* base, double* dw, double* dg,
                             double* df, hipStream_t stream);
 
+/* ---- source-history transport ---------------------------------------------
+ * Replaces source_history_project / source_history_record
+ * (dendra/models/networks/_source_history.py:83-186), the transport of the
+ * source-history training backend (networks/netcon.py:3660-3725).
+ *   dn_history_project_*: hist (depth, n_col) floating gates (packed = 0,
+ *     element type of w) or int64 words (packed = 1, gate = (word & mask[e])
+ *     != 0); connection e reads column col[e] at rows (slot - off[e][j]) mod
+ *     depth for j < taps (off int32 in [0, depth), taps 1 fixed, 2 linear
+ *     with upper weight frac[e], 3 Gaussian with frac[e][3]) and
+ *     out[t] = sum over the target-major CSR (in_ptr, in_edge) of
+ *     w[e] * mixture, accumulated from 0 in ascending connection order.
+ *   dn_history_project_vjp_*: per connection dw, dfrac (taps 2, 3) and, for
+ *     a floating history, dhist += (atomic; zero it first, may be null).
+ *   dn_history_record_*: row slot mod depth := gates (in place).
+ *   dn_history_record_packed: row slot mod depth := sum over sources with
+ *     active[s] != 0 of mask[s] at word[s] (in place, int64 sums).
+ * slot is a one-element device int64. */
+int dn_history_project_f32(const void* hist, int packed, int64_t depth, int64_t n_col,
+                           const int64_t* slot, int taps, const int32_t* in_ptr,
+                           const int32_t* in_edge, const int32_t* col, const int64_t* mask,
+                           const int32_t* off, const float* w, const float* frac,
+                           float* out, int64_t width, hipStream_t stream);
+int dn_history_project_f64(const void* hist, int packed, int64_t depth, int64_t n_col,
+                           const int64_t* slot, int taps, const int32_t* in_ptr,
+                           const int32_t* in_edge, const int32_t* col, const int64_t* mask,
+                           const int32_t* off, const double* w, const double* frac,
+                           double* out, int64_t width, hipStream_t stream);
+int dn_history_project_vjp_f32(const float* grad_out, const void* hist, int packed,
+                               int64_t depth, int64_t n_col, const int64_t* slot, int taps,
+                               int64_t n_conn, const int32_t* post, const int32_t* col,
+                               const int64_t* mask, const int32_t* off, const float* w,
+                               const float* frac, float* dw, float* dfrac, float* dhist,
+                               hipStream_t stream);
+int dn_history_project_vjp_f64(const double* grad_out, const void* hist, int packed,
+                               int64_t depth, int64_t n_col, const int64_t* slot, int taps,
+                               int64_t n_conn, const int32_t* post, const int32_t* col,
+                               const int64_t* mask, const int32_t* off, const double* w,
+                               const double* frac, double* dw, double* dfrac, double* dhist,
+                               hipStream_t stream);
+int dn_history_record_f32(float* hist, int64_t depth, int64_t n_col, const int64_t* slot,
+                          const float* gates, hipStream_t stream);
+int dn_history_record_f64(double* hist, int64_t depth, int64_t n_col, const int64_t* slot,
+                          const double* gates, hipStream_t stream);
+int dn_history_record_packed(int64_t* hist, int64_t depth, int64_t n_words,
+                             const int64_t* slot, int64_t n_src, const uint8_t* active,
+                             const int32_t* word, const int64_t* mask, hipStream_t stream);
+
 /* ---- fused HH + backward Euler, unbranched cables -------------------------
  * Replaces the per-step transition of _bwd_euler_ub
  * (dendra/models/integrators/implicit.py:561-664) in Integrator._step order

@@ -1,0 +1,259 @@
+"""GPU source-history transport (dendra_amd.history) against the oracle,
+bitwise, and against autograd of the reference's formula on the CPU; the
+reference's own cases from tests/networks/test_source_history_kernels.py."""
+
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from test_history_ref import (FIXED, POST, SOURCE_POS, VALUES, WEIGHTS, packing,
+                              torch_project)
+
+sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
+import history_ref  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a, dtype=None):
+    return torch.as_tensor(np.asarray(a), device="cuda", dtype=dtype)
+
+
+def case(rng, dtype, taps, packed, depth=201, n_src=700, n=20000, width=300):
+    w = rng.standard_normal(n).astype(dtype)
+    src = rng.integers(0, n_src, n)
+    post = rng.integers(0, width, n)
+    delays = rng.uniform(0.05, 4.9, n)  # ms, dt 0.025: 2..196 steps
+    if packed:
+        word, mask = packing(n_src)
+        history = np.zeros((depth, int(word.max()) + 1), dtype=np.int64)
+        bits = rng.uniform(0, 1, (depth, n_src)) < 0.3
+        for s in range(depth):
+            history = history_ref.record_packed(history, s, bits[s], word, mask)
+        meta = {"word_idx": word[src], "bit_mask": mask[src]}
+    else:
+        history = rng.uniform(-1, 1, (depth, n_src)).astype(dtype)
+        meta = {"source_pos": src}
+    return w, src, post, delays, history, meta
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("taps", [1, 2, 3])
+@pytest.mark.parametrize("packed", [False, True])
+def test_projection_is_bitwise_the_oracle(dtype, taps, packed):
+    from dendra_amd.history import source_history_project
+    from dendra_amd.delays import delay_interpolation
+
+    rng = np.random.default_rng(3 * taps + packed)
+    w, src, post, delays, history, meta = case(rng, dtype, taps, packed)
+    depth, width, slot, dt = history.shape[0], 300, 150, 0.025
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    kw = {k: dev(v) for k, v in meta.items()}
+    if taps == 1:
+        steps = np.maximum(np.rint(delays / dt), 1).astype(np.int64)
+        got = source_history_project(dev(history), dev([slot]), dev(w), post_idx=dev(post),
+                                     depth=depth, width=width, delay_steps=dev(steps), **kw)
+        offsets, frac = steps[:, None], None
+    else:
+        off_t, frac_t = delay_interpolation(dev(delays, tdt), dt, dtype=tdt, taps=taps,
+                                            sigma=0.5)
+        got = source_history_project(dev(history), dev([slot]), dev(w), post_idx=dev(post),
+                                     depth=depth, width=width, delay_taps=(off_t, frac_t),
+                                     taps=taps, **kw)
+        off = off_t.cpu().numpy()
+        offsets = np.stack([off, off + 1], -1) if taps == 2 else off
+        frac = frac_t.cpu().numpy()
+    col = meta["word_idx"] if packed else meta["source_pos"]
+    want = history_ref.project(history, slot, w, post, width, col=col, offsets=offsets,
+                               frac=frac, mask=meta.get("bit_mask"), packed=packed)
+    g = got.cpu().numpy()
+    assert g.dtype == want.dtype
+    assert np.array_equal(g.view(np.uint8), want.view(np.uint8))
+
+
+@pytest.mark.parametrize("packed", [False, True])
+@pytest.mark.parametrize("taps", [None, 2, 3])
+def test_reference_fixture_values_and_gradients(packed, taps):
+    """test_delivery_uses_current_weights_delays_and_retains_gradients
+    (test_source_history_kernels.py:334-405) on the GPU."""
+    from dendra_amd.history import source_history_project, source_history_record
+
+    word, mask = packing(3)
+    if packed:
+        history = torch.zeros((5, 1), dtype=torch.int64, device="cuda")
+        for s in range(5):
+            history = source_history_record(history, dev([s]), dev(VALUES[s]),
+                                            source_word_idx=dev(word),
+                                            source_bit_mask=dev(mask))
+        meta = {"word_idx": dev(word[SOURCE_POS]), "bit_mask": dev(mask[SOURCE_POS])}
+    else:
+        history = dev(VALUES).requires_grad_()
+        meta = {"source_pos": dev(SOURCE_POS)}
+    weights = dev(WEIGHTS).requires_grad_()
+    delays = dev([0.15, 0.23, 0.31, 0.42, 0.1], torch.float64).requires_grad_()
+    dt = torch.tensor(0.1, dtype=torch.float64)
+
+    def project(w):
+        return source_history_project(history, dev([4]), w, post_idx=dev(POST), depth=5,
+                                      width=2, delay_steps=dev(FIXED),
+                                      delays=delays if taps else None, dt=dt,
+                                      taps=taps or 2, sigma=0.4, **meta)
+
+    result = project(weights)
+    # the reference's per-edge construction, on the CPU
+    ch = torch.as_tensor(VALUES).requires_grad_(not packed)
+    cw = torch.as_tensor(WEIGHTS).requires_grad_()
+    cd = torch.tensor([0.15, 0.23, 0.31, 0.42, 0.1], dtype=torch.float64, requires_grad=True)
+    vals = []
+    for e, s in enumerate(SOURCE_POS.tolist()):
+        if taps is None:
+            gate = ch[(4 - FIXED[e]) % 5, s]
+        else:
+            dist = (cd[e] / dt).clamp_min(1)
+            base = dist.floor()
+            if taps == 2:
+                fr = dist - base
+                gate = ch[(4 - base.long()) % 5, s] * (1 - fr) + \
+                    ch[(3 - base.long()) % 5, s] * fr
+            else:
+                centers = torch.stack((base - 1, base, base + 1))
+                interp = torch.softmax(-0.5 * ((dist - centers) / 0.400001) ** 2, dim=0)
+                gate = (ch[(4 - centers.long()) % 5, s] * interp).sum()
+        vals.append(cw[e] * gate)
+    ref = torch.zeros(2, dtype=torch.float64).index_add(0, torch.as_tensor(POST),
+                                                        torch.stack(vals))
+    torch.testing.assert_close(result.cpu(), ref.detach())
+    gpu_t = (weights, *(() if packed else (history,)), *((delays,) if taps else ()))
+    cpu_t = (cw, *(() if packed else (ch,)), *((cd,) if taps else ()))
+    got = torch.autograd.grad(result.square().sum(), gpu_t)
+    exp = torch.autograd.grad(ref.square().sum(), cpu_t)
+    for a, b in zip(got, exp, strict=True):
+        torch.testing.assert_close(a.cpu(), b)
+    torch.testing.assert_close(project(weights * 3).cpu(), (result * 3).detach().cpu())
+
+
+@pytest.mark.parametrize("taps", [1, 2, 3])
+def test_projection_gradients_match_cpu_autograd(taps):
+    from dendra_amd.history import source_history_project
+    from dendra_amd.delays import delay_interpolation
+
+    rng = np.random.default_rng(40 + taps)
+    w, src, post, delays, history, _ = case(rng, np.float64, taps, False, depth=31,
+                                            n_src=40, n=600, width=25)
+    delays = rng.uniform(0.1, 0.6, 600)
+    dt, slot = 0.025, 17
+    cot = rng.standard_normal(25)
+    steps = np.maximum(np.rint(delays / dt), 1).astype(np.int64)
+
+    def run(device, fn):
+        args = [torch.as_tensor(v, device=device).requires_grad_() for v in (history, w, delays)]
+        out = fn(*args)
+        return out, torch.autograd.grad((out * torch.as_tensor(cot, device=device)).sum(),
+                                        args, allow_unused=True)
+
+    def gpu(h, tw, td):
+        kw = dict(delay_steps=dev(steps)) if taps == 1 else dict(delays=td, dt=dt, taps=taps,
+                                                                sigma=0.5)
+        return source_history_project(h, dev([slot]), tw, post_idx=dev(post), depth=31,
+                                      width=25, source_pos=dev(src), **kw)
+
+    def cpu(h, tw, td):
+        if taps == 1:
+            return torch_project(h, slot, tw, post, 25, src, steps[:, None])
+        off, fr = delay_interpolation(td, dt, dtype=torch.float64, taps=taps, sigma=0.5)
+        offsets = torch.stack([off, off + 1], -1) if taps == 2 else off
+        depth = h.shape[0]
+        rows = (slot - offsets).remainder(depth)
+        col = torch.as_tensor(src)
+        if taps == 2:
+            values = tw * (h[rows[:, 0], col] * (1.0 - fr) + h[rows[:, 1], col] * fr)
+        else:
+            g = torch.stack([h[rows[:, j], col] for j in range(3)], -1)
+            values = tw * (g * fr).sum(-1)
+        return torch.zeros(25, dtype=torch.float64).index_add(0, torch.as_tensor(post), values)
+
+    og, gg = run("cuda", gpu)
+    oc, gc = run("cpu", cpu)
+    if taps < 3:  # one and two taps: the same correctly rounded divisions
+        assert torch.equal(og.detach().cpu(), oc.detach())
+    else:         # Gaussian tap weights: exp on two devices
+        torch.testing.assert_close(og.detach().cpu(), oc.detach(), rtol=1e-13, atol=1e-14)
+    for a, b in zip(gg, gc):
+        if b is None:
+            assert a is None or not a.any()
+            continue
+        torch.testing.assert_close(a.cpu(), b, rtol=1e-12, atol=1e-13)
+
+
+def test_record_float_and_packed():
+    from dendra_amd.history import read_packed_history, source_history_record
+
+    rng = np.random.default_rng(5)
+    size, depth = 130, 4
+    word, mask = packing(size)
+    history = torch.full((depth, 3), 17, dtype=torch.int64, device="cuda")
+    gates = np.zeros(size)
+    active = [0, 61, 62, 63, 64, 125, 126, 129]
+    gates[active] = 1
+    out = source_history_record(history, dev([2]), dev(gates), source_word_idx=dev(word),
+                                source_bit_mask=dev(mask))
+    want = history_ref.record_packed(np.full((depth, 3), 17), 2, gates, word, mask)
+    assert np.array_equal(out.cpu().numpy(), want)
+    assert (history == 17).all()
+    seen = read_packed_history(out, dev([2]).expand(size), dev(word), dev(mask),
+                               dtype=torch.float64)
+    assert np.array_equal(seen.cpu().numpy(), gates)
+    owned = history.clone()
+    ptr = owned.data_ptr()
+    donated = source_history_record(owned, dev([2]), dev(gates), source_word_idx=dev(word),
+                                    source_bit_mask=dev(mask), _inplace=True)
+    assert donated.data_ptr() == ptr and torch.equal(donated, out)
+    # floating: only the unwritten history keeps its gradient
+    h = torch.arange(15, dtype=torch.float64, device="cuda").reshape(5, 3).requires_grad_()
+    g = dev([-0.2, 0.0, 0.8], torch.float64).requires_grad_()
+    for own in ({}, {"_clone_once": True}):
+        upd = source_history_record(h, dev([7]), g, **own)
+        exp = history_ref.record_float(np.arange(15.0).reshape(5, 3), 7, [-0.2, 0.0, 0.8])
+        assert np.array_equal(upd.detach().cpu().numpy(), exp)
+        gh, gg = torch.autograd.grad(upd.sum(), (h, g))
+        m = torch.ones(5, 3, dtype=torch.float64)
+        m[2] = 0
+        assert torch.equal(gh.cpu(), m) and torch.equal(gg.cpu(), torch.ones(3, dtype=torch.float64))
+    big = rng.standard_normal((201, 5000)).astype(np.float32)
+    gates32 = rng.standard_normal(5000).astype(np.float32)
+    upd = source_history_record(dev(big), dev([205]), dev(gates32))
+    assert np.array_equal(upd.cpu().numpy(), history_ref.record_float(big, 205, gates32))
+
+
+def test_empty_and_donated_outputs():
+    from dendra_amd.history import source_history_project, source_history_record
+
+    empty = torch.empty(0, dtype=torch.long, device="cuda")
+    for packed in (False, True):
+        history = torch.zeros((3, 1 if packed else 0),
+                              dtype=torch.int64 if packed else torch.float64, device="cuda")
+        meta = {"word_idx": empty, "bit_mask": empty} if packed else {"source_pos": empty}
+        out = source_history_project(history, dev([2]),
+                                     torch.empty(0, dtype=torch.float64, device="cuda"),
+                                     post_idx=empty, depth=3, width=4, delay_steps=empty, **meta)
+        assert torch.equal(out.cpu(), torch.zeros(4, dtype=torch.float64))
+        upd = source_history_record(history, dev([2]),
+                                    torch.empty(0, dtype=torch.float64, device="cuda"),
+                                    source_word_idx=empty if packed else None,
+                                    source_bit_mask=empty if packed else None)
+        assert torch.equal(upd, history)
+    scratch = torch.full((2,), 99.0, dtype=torch.float64, device="cuda")
+    res = source_history_project(dev(VALUES), dev([4]), dev(WEIGHTS), post_idx=dev(POST),
+                                 depth=5, width=2, delay_steps=dev(FIXED),
+                                 source_pos=dev(SOURCE_POS), _scratch=scratch)
+    assert res is scratch
+    want = history_ref.project(VALUES, 4, WEIGHTS, POST, 2, col=SOURCE_POS,
+                               offsets=FIXED[:, None])
+    assert np.array_equal(res.cpu().numpy(), want)
+    with pytest.raises(ValueError, match=r"source_pos entries must lie in \[0, 3\)"):
+        source_history_project(dev(VALUES), dev([4]), dev(WEIGHTS), post_idx=dev(POST),
+                               depth=5, width=2, delay_steps=dev(FIXED),
+                               source_pos=dev([0, 1, 2, 3, 0]))
