@@ -137,6 +137,12 @@ for _sfx in ("f32", "f64"):
         i32, [p, p, i32, i64, i64, p, i32, i64] + [p] * 9 + [p])
     _SIGNATURES[f"dn_history_record_{_sfx}"] = (i32, [p, i64, i64, p, p, p])
 _SIGNATURES["dn_history_record_packed"] = (i32, [p, i64, i64, p, i64, p, p, p, p])
+_SIGNATURES["dn_calendar_work_bytes"] = (i64, [i64, i64, i32])
+for _sfx in ("f32", "f64", "i32"):
+    _SIGNATURES[f"dn_calendar_append_{_sfx}"] = (
+        i32, [p, p, p, i64, p, p, p, p, i64, p, p, p, p, p, p, i64, p])
+    _SIGNATURES[f"dn_calendar_pop_{_sfx}"] = (
+        i32, [p, p, p, i64, p, p, i64, p, p, p, p, p, p, p, p, i64, p])
 
 
 def _bind(lib) -> None:

@@ -228,6 +228,64 @@ int dn_history_record_packed(int64_t* hist, int64_t depth, int64_t n_words,
                              const int64_t* slot, int64_t n_src, const uint8_t* active,
                              const int32_t* word, const int64_t* mask, hipStream_t stream);
 
+/* ---- pending calendar of the sparse NetCon backend ------------------------
+ * Replaces pending_calendar_append / pending_calendar_pop
+ * (dendra/models/networks/_pending_calendar.py:37-113), the storage of the
+ * sparse_calendar delay backend (networks/netcon.py:3309-3450, 5223-5300).
+ * A calendar is slots/indices (int64) and values (float, double or int32) of
+ * `cap` entries, a device int64 size and a device bool overflow flag.
+ *   dn_calendar_append_*: slots/indices/values hold a copy of the old
+ *     calendar and receive, in order, the candidates with active[i] != 0 at
+ *     old_size + rank -- unless old_ovf or old_size + count > cap, when
+ *     nothing is written; new_size/new_ovf receive the result; dest (may be
+ *     null) the position of each candidate or -1.
+ *   dn_calendar_pop_*: entries p < size with slots[p] == *slot are due;
+ *     payload[indices[p]] += values[p] in position order (payload holds its
+ *     initial value; indices outside [0, width) are dropped); the other valid
+ *     entries go, in order, to the front of n_* (zeroed by the caller);
+ *     new_size = their count; keys_out (may be null) gets index for due
+ *     entries and width otherwise, keep_dest (may be null) each entry's new
+ *     position or -1.
+ * work: dn_calendar_work_bytes(n, cap, kind) bytes (kind 0 f32, 1 f64,
+ * 2 i32; n = candidates, 0 for a pop). */
+int64_t dn_calendar_work_bytes(int64_t n, int64_t cap, int kind);
+int dn_calendar_append_f32(int64_t* slots, int64_t* indices, float* values, int64_t cap,
+                          const int64_t* old_size, const uint8_t* old_ovf,
+                          int64_t* new_size, uint8_t* new_ovf, int64_t n,
+                          const int64_t* c_slots, const int64_t* c_indices,
+                          const float* c_values, const uint8_t* active, int64_t* dest,
+                          void* work, int64_t work_bytes, hipStream_t stream);
+int dn_calendar_pop_f32(const int64_t* slots, const int64_t* indices, const float* values,
+                       int64_t cap, const int64_t* size, const int64_t* slot, int64_t width,
+                       int64_t* n_slots, int64_t* n_indices, float* n_values,
+                       int64_t* new_size, float* payload, int32_t* keys_out,
+                       int64_t* keep_dest, void* work, int64_t work_bytes,
+                       hipStream_t stream);
+int dn_calendar_append_f64(int64_t* slots, int64_t* indices, double* values, int64_t cap,
+                          const int64_t* old_size, const uint8_t* old_ovf,
+                          int64_t* new_size, uint8_t* new_ovf, int64_t n,
+                          const int64_t* c_slots, const int64_t* c_indices,
+                          const double* c_values, const uint8_t* active, int64_t* dest,
+                          void* work, int64_t work_bytes, hipStream_t stream);
+int dn_calendar_pop_f64(const int64_t* slots, const int64_t* indices, const double* values,
+                       int64_t cap, const int64_t* size, const int64_t* slot, int64_t width,
+                       int64_t* n_slots, int64_t* n_indices, double* n_values,
+                       int64_t* new_size, double* payload, int32_t* keys_out,
+                       int64_t* keep_dest, void* work, int64_t work_bytes,
+                       hipStream_t stream);
+int dn_calendar_append_i32(int64_t* slots, int64_t* indices, int32_t* values, int64_t cap,
+                          const int64_t* old_size, const uint8_t* old_ovf,
+                          int64_t* new_size, uint8_t* new_ovf, int64_t n,
+                          const int64_t* c_slots, const int64_t* c_indices,
+                          const int32_t* c_values, const uint8_t* active, int64_t* dest,
+                          void* work, int64_t work_bytes, hipStream_t stream);
+int dn_calendar_pop_i32(const int64_t* slots, const int64_t* indices, const int32_t* values,
+                       int64_t cap, const int64_t* size, const int64_t* slot, int64_t width,
+                       int64_t* n_slots, int64_t* n_indices, int32_t* n_values,
+                       int64_t* new_size, int32_t* payload, int32_t* keys_out,
+                       int64_t* keep_dest, void* work, int64_t work_bytes,
+                       hipStream_t stream);
+
 /* ---- fused HH + backward Euler, unbranched cables -------------------------
  * Replaces the per-step transition of _bwd_euler_ub
  * (dendra/models/integrators/implicit.py:561-664) in Integrator._step order
