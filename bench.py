@@ -614,6 +614,40 @@ def operator_bench(dev):
         res[f"history_projection_{taps}tap"] = {
             "connections": n, "sources": n_src, "targets": width, "depth": depth,
             **_rate(ms, nbytes)}
+
+    # sparse calendar in steady state: each step pops today's slot and appends
+    # the step's deliveries (750 per step, delays 40-200 steps)
+    from dendra_amd import calendar
+
+    cap, per_step, n_fill, reps = 262_144, 750, 400, 50
+    total_steps = n_fill + reps + 3
+    delay_s = torch.randint(40, 201, (total_steps, per_step), device=dev, generator=gen)
+    cand_idx = torch.randint(0, width, (total_steps, per_step), device=dev, generator=gen)
+    cand_val = u((total_steps, per_step), 1e-4, 2e-3)
+    slots_of = [torch.full((1,), k % depth, dtype=torch.int64, device=dev)
+                for k in range(total_steps)]
+    fut = (torch.arange(total_steps, device=dev)[:, None] + delay_s) % depth
+    st = calendar.pending_calendar_empty(cap, device=dev, dtype=torch.float32)
+    cur = [0]
+
+    def cal_step():
+        nonlocal st
+        k = cur[0]
+        st, _ = calendar.pending_calendar_pop(st, slots_of[k], width=width)
+        st = calendar.pending_calendar_append(st, fut[k], cand_idx[k], cand_val[k])
+        cur[0] += 1
+
+    with torch.no_grad():
+        for _ in range(n_fill):  # fill to steady state
+            cal_step()
+        ms = _time_op(cal_step, reps=reps)
+    size = int(st.size.item())
+    # a pop reads the valid prefix (slot, index, value: 20 B) and writes the
+    # capacity of fresh arrays; an append copies the capacity once more
+    nbytes = 20.0 * size + 2 * 20.0 * cap
+    res["sparse_calendar_step"] = {"capacity": cap, "pending": size,
+                                   "appended_per_step": per_step, "targets": width,
+                                   **_rate(ms, nbytes)}
     return res
 
 
