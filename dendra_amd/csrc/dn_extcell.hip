@@ -1,0 +1,208 @@
+// dn_extcell.hip -- HH cables with two extracellular layers (ExtCell), fused
+// backward-Euler steps.
+//
+// Replaces _bwd_euler_bt._step (dendra/models/integrators/implicit.py:
+// 1021-1136) for HH cables: per step and compartment k (unknowns
+// vc = (vi, ve0, ve1)), with v = vi - ve0,
+//   gates by cnexp at v -> G, J (mod/HH.py:109-116; the cable kernels' code)
+//   g = G area, d = (J - G v) area + intra
+//   A_k = maind_k + [[g, -g, 0], [-g, g, 0], [0, 0, 0]]
+//   R_k = (d, -d - xg0 (ve0 - ve1), xg0 (ve0 - ve1) + xg1 (e_bath - ve1))
+//         + f_k - f_{k-1},  f_k = -lower_k * (vc_{k+1} - vc_k)
+//   (lower at (k, k-1), upper at (k-1, k): diagonal 3x3 couplings)
+//   block Thomas for the increment, vc += increment.
+// maind, lower, upper, area, xg are the prepared workspace of
+// _prepare_workspace (implicit.py:800-872).
+//
+// One lane owns one cell for all steps (no synchronisation between lanes).
+// Every array is laid out [compartment][component][cell], so the accesses of
+// a wavefront are coalesced; the forward sweep keeps the previous block's
+// inverse in registers, prefetches the next compartment's inputs, and stores
+// (A'^-1, R') for the backward sweep in a [K][12][B] workspace.
+#include "dn_common.hpp"
+
+namespace dn {
+
+template <typename T>
+__device__ __forceinline__ void xc_inv3(const T* a, T* r) {
+  const T c0 = a[4] * a[8] - a[5] * a[7];
+  const T c1 = a[5] * a[6] - a[3] * a[8];
+  const T c2 = a[3] * a[7] - a[4] * a[6];
+  const T id = T(1) / (a[0] * c0 + a[1] * c1 + a[2] * c2);
+  r[0] = c0 * id;
+  r[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+  r[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+  r[3] = c1 * id;
+  r[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+  r[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+  r[6] = c2 * id;
+  r[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+  r[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+}
+
+template <typename T>
+struct XcIn {
+  T mai[9], lo[3], up[3], area, xg0, xg1, m, h, n, vn[3], lead, iamp, sna, sk;
+};
+
+template <typename T>
+struct XcArgs {
+  const T *maind, *lower, *upper, *area, *xg, *sna, *sk, *lead, *amp, *wave, *iamp, *iwave;
+  const T* hh;
+  T *vc, *m, *h, *n, *work, *rec;
+  int64_t B, K, n_steps, rec_every;
+  T dt;
+};
+
+template <typename T>
+__device__ __forceinline__ void xc_load(const XcArgs<T>& a, int64_t k, int64_t b, XcIn<T>& in) {
+  const int64_t B = a.B, K = a.K;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) in.mai[c] = a.maind[(k * 9 + c) * B + b];
+  const bool edge = k + 1 < K;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    in.lo[c] = edge ? a.lower[(k * 3 + c) * B + b] : T(0);
+    in.up[c] = edge ? a.upper[(k * 3 + c) * B + b] : T(0);
+    in.vn[c] = edge ? a.vc[((k + 1) * 3 + c) * B + b] : T(0);
+  }
+  in.area = a.area[k * B + b];
+  in.xg0 = a.xg[(k * 2) * B + b];
+  in.xg1 = a.xg[(k * 2 + 1) * B + b];
+  in.m = a.m[k * B + b];
+  in.h = a.h[k * B + b];
+  in.n = a.n[k * B + b];
+  in.lead = a.lead ? a.lead[k * B + b] : T(0);
+  in.iamp = a.iamp ? a.iamp[k * B + b] : T(0);
+  in.sna = a.sna ? a.sna[k * B + b] : T(1);
+  in.sk = a.sk ? a.sk[k * B + b] : T(1);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void extcell_run_kernel(XcArgs<T> a) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.B) return;
+  const int64_t B = a.B, K = a.K;
+  const HHParams<T> hp{a.hh[0], a.hh[1], a.hh[2], a.hh[3], a.hh[4], a.hh[5], a.hh[6]};
+  const T amp = (a.amp && a.wave && a.lead) ? a.amp[b] : T(0);
+  T* W = a.work;
+  for (int64_t s = 0; s < a.n_steps; ++s) {
+    const T bath_s = (a.amp && a.wave && a.lead) ? a.wave[s] * amp : T(0);
+    const T intra_s = (a.iamp && a.iwave) ? a.iwave[s] : T(0);
+    T vk[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) vk[c] = a.vc[c * B + b];
+    T Ainv[9], Rp[3], fprev[3] = {T(0), T(0), T(0)}, lprev[3], uprev[3];
+    XcIn<T> cur, nxt;
+    xc_load(a, 0, b, cur);
+    for (int64_t k = 0; k < K; ++k) {
+      if (k + 1 < K) xc_load(a, k + 1, b, nxt);
+      // membrane: gates at v, then G and J
+      const T v = vk[0] - vk[1];
+      T G, J;
+      hh_advance<T>(v, a.dt, hp, cur.sna, cur.sk, cur.m, cur.h, cur.n, G, J);
+      a.m[k * B + b] = cur.m;
+      a.h[k * B + b] = cur.h;
+      a.n[k * B + b] = cur.n;
+      const T g = G * cur.area;
+      const T d = (J - G * v) * cur.area + cur.iamp * intra_s;
+      T A[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) A[c] = cur.mai[c];
+      A[0] += g;
+      A[1] -= g;
+      A[3] -= g;
+      A[4] += g;
+      const T shell = cur.xg0 * (vk[1] - vk[2]);
+      const T bathf = cur.xg1 * (cur.lead * bath_s - vk[2]);
+      T R[3] = {d, -d - shell, shell + bathf};
+      T f[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        f[c] = -cur.lo[c] * (cur.vn[c] - vk[c]);
+        R[c] += f[c] - fprev[c];
+      }
+      if (k > 0) {
+        // A -= diag(l) Ainv_prev diag(u);  R -= diag(l) Ainv_prev Rp
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) A[r * 3 + c] -= lprev[r] * Ainv[r * 3 + c] * uprev[c];
+          R[r] -= lprev[r] * (Ainv[r * 3] * Rp[0] + Ainv[r * 3 + 1] * Rp[1] +
+                              Ainv[r * 3 + 2] * Rp[2]);
+        }
+      }
+      xc_inv3(A, Ainv);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) W[(k * 12 + c) * B + b] = Ainv[c];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        W[(k * 12 + 9 + c) * B + b] = R[c];
+        Rp[c] = R[c];
+        fprev[c] = f[c];
+        lprev[c] = cur.lo[c];
+        uprev[c] = cur.up[c];
+        vk[c] = cur.vn[c];
+      }
+      cur = nxt;
+    }
+    // back-substitution: x_k = Ainv_k (R'_k - diag(u_k) x_{k+1}); vc += x
+    T xn[3] = {T(0), T(0), T(0)};
+    for (int64_t k = K - 1; k >= 0; --k) {
+      T Ai[9], r[3], x[3];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) Ai[c] = W[(k * 12 + c) * B + b];
+      const bool edge = k + 1 < K;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        r[c] = W[(k * 12 + 9 + c) * B + b] - (edge ? a.upper[(k * 3 + c) * B + b] * xn[c] : T(0));
+#pragma unroll
+      for (int c = 0; c < 3; ++c) x[c] = Ai[c * 3] * r[0] + Ai[c * 3 + 1] * r[1] + Ai[c * 3 + 2] * r[2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        a.vc[(k * 3 + c) * B + b] += x[c];
+        xn[c] = x[c];
+      }
+    }
+    if (a.rec && a.rec_every > 0 && (s + 1) % a.rec_every == 0) {
+      const int64_t rr = (s + 1) / a.rec_every - 1;
+      for (int64_t k = 0; k < K; ++k)
+        a.rec[(rr * K + k) * B + b] = a.vc[(k * 3) * B + b] - a.vc[(k * 3 + 1) * B + b];
+    }
+  }
+}
+
+}  // namespace dn
+
+template <typename T>
+static int extcell_impl(const dn::XcArgs<T>& a, int64_t work_elems, hipStream_t stream) {
+  if (a.B < 1 || a.K < 1 || a.n_steps < 0 || a.B > (int64_t)INT32_MAX * 256) return DN_EINVAL;
+  if (!a.maind || !a.area || !a.xg || !a.hh || !a.vc || !a.m || !a.h || !a.n || !a.work)
+    return DN_EINVAL;
+  if (a.K > 1 && (!a.lower || !a.upper)) return DN_EINVAL;
+  if (work_elems < 12 * a.B * a.K) return DN_EINVAL;
+  if (a.rec && a.rec_every <= 0) return DN_EINVAL;
+  if (a.n_steps == 0) return DN_OK;
+  hipLaunchKernelGGL((dn::extcell_run_kernel<T>), dim3((unsigned)((a.B + 255) / 256)),
+                     dim3(256), 0, stream, a);
+  return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+}
+
+DN_EXPORT int64_t dn_extcell_work_elems(int64_t B, int64_t K) {
+  return (B < 0 || K < 0) ? -1 : 12 * B * K;
+}
+
+#define DN_EXTCELL_EXPORT(T, SFX)                                                            \
+  DN_EXPORT int dn_extcell_run_##SFX(                                                        \
+      int64_t B, int64_t K, const T* maind, const T* lower, const T* upper, const T* area,   \
+      const T* xg, const T* sna, const T* sk, const T* hh, const T* lead, const T* amp,      \
+      const T* wave, const T* iamp, const T* i_wave, T dt, int64_t n_steps, T* vc, T* m,     \
+      T* h, T* n, T* rec, int64_t rec_every, T* work, int64_t work_elems,                    \
+      hipStream_t stream) {                                                                  \
+    dn::XcArgs<T> a{maind, lower, upper, area, xg, sna, sk, lead, amp, wave, iamp, i_wave,  \
+                    hh, vc, m, h, n, work, rec, B, K, n_steps, rec_every, dt};              \
+    return extcell_impl<T>(a, work_elems, stream);                                           \
+  }
+
+DN_EXTCELL_EXPORT(float, f32)
+DN_EXTCELL_EXPORT(double, f64)

@@ -286,6 +286,33 @@ int dn_calendar_pop_i32(const int64_t* slots, const int64_t* indices, const int3
                        int64_t* keep_dest, void* work, int64_t work_bytes,
                        hipStream_t stream);
 
+/* ---- HH cables with two extracellular layers (ExtCell) --------------------
+ * Replaces _bwd_euler_bt._step (dendra/models/integrators/implicit.py:
+ * 1021-1136) with its block solve (thomas_solve_cuda_bt) for B HH cables of
+ * K compartments, all n_steps in one launch.  Layout [compartment][component]
+ * [cell]: maind (K,9,B), lower/upper (K-1,3,B) (diagonal couplings at
+ * (k,k-1) / (k-1,k)), area (K,B), xg radial conductances (K,2,B), optional
+ * sna/sk/lead/iamp (K,B), amp (B), wave/i_wave (n_steps); state vc (K,3,B)
+ * = (vi, ve0, ve1) and m/h/n (K,B) advance in place; rec (optional)
+ * receives vi - ve0 every rec_every steps, (n_rec,K,B).  work:
+ * dn_extcell_work_elems(B, K) elements. */
+int64_t dn_extcell_work_elems(int64_t B, int64_t K);
+int dn_extcell_run_f32(int64_t B, int64_t K, const float* maind, const float* lower,
+                       const float* upper, const float* area, const float* xg,
+                       const float* sna, const float* sk, const float* hh, const float* lead,
+                       const float* amp, const float* wave, const float* iamp,
+                       const float* i_wave, float dt, int64_t n_steps, float* vc, float* m,
+                       float* h, float* n, float* rec, int64_t rec_every, float* work,
+                       int64_t work_elems, hipStream_t stream);
+int dn_extcell_run_f64(int64_t B, int64_t K, const double* maind, const double* lower,
+                       const double* upper, const double* area, const double* xg,
+                       const double* sna, const double* sk, const double* hh,
+                       const double* lead, const double* amp, const double* wave,
+                       const double* iamp, const double* i_wave, double dt, int64_t n_steps,
+                       double* vc, double* m, double* h, double* n, double* rec,
+                       int64_t rec_every, double* work, int64_t work_elems,
+                       hipStream_t stream);
+
 /* ---- fused HH + backward Euler, unbranched cables -------------------------
  * Replaces the per-step transition of _bwd_euler_ub
  * (dendra/models/integrators/implicit.py:561-664) in Integrator._step order

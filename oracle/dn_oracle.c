@@ -630,3 +630,138 @@ ORC_API int orc_threshold(int64_t n_fibers, const int64_t* off, const double* di
   }
   return bad ? -1 : 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Double-layer extracellular (ExtCell) cables: 3x3 block backward Euler      */
+/* ------------------------------------------------------------------------- */
+
+/* Solve the 3x3 system a x = b by Gaussian elimination with partial pivoting
+ * (a and b are overwritten). */
+static void solve3_pivot(double* a, double* b, double* x) {
+  int p[3] = {0, 1, 2};
+  for (int c = 0; c < 3; ++c) {
+    int best = c;
+    for (int r = c + 1; r < 3; ++r)
+      if (fabs(a[p[r] * 3 + c]) > fabs(a[p[best] * 3 + c])) best = r;
+    int t = p[c]; p[c] = p[best]; p[best] = t;
+    for (int r = c + 1; r < 3; ++r) {
+      double f = a[p[r] * 3 + c] / a[p[c] * 3 + c];
+      for (int k = c; k < 3; ++k) a[p[r] * 3 + k] -= f * a[p[c] * 3 + k];
+      b[p[r]] -= f * b[p[c]];
+    }
+  }
+  for (int c = 2; c >= 0; --c) {
+    double s = b[p[c]];
+    for (int k = c + 1; k < 3; ++k) s -= a[p[c] * 3 + k] * x[k];
+    x[c] = s / a[p[c] * 3 + c];
+  }
+}
+
+/*
+ * Batch of B HH cables of K compartments with two extracellular layers,
+ * _bwd_euler_bt._step (dendra/models/integrators/implicit.py:1021-1136).
+ * Per step, for compartment k of cell b (unknowns vi, ve0, ve1 = vc[k][0..2]):
+ *   v = vi - ve0; gates by cnexp at v -> G, J (HH.py:109-116)
+ *   g = G area, d = (J - G v) area + intra
+ *   A_k = maind_k + [[g, -g, 0], [-g, g, 0], [0, 0, 0]]
+ *   R_k = (d, -d - xg0 (ve0 - ve1), xg0 (ve0 - ve1) + xg1 (e_bath - ve1))
+ *         (_block_voltage_residual, implicit.py:1139-1147)
+ *       + f_k - f_{k-1},  f_k = -lower_k (vc_{k+1} - vc_k)   (axial flux)
+ *   block tridiagonal (lower at (k, k-1), upper at (k-1, k), both diagonal)
+ *   solve for the increment; vc += increment.
+ * maind (B,K,9), lower/upper (B,K-1,3), area (B,K), xg (B,K,2) are the
+ * prepared workspace of _prepare_workspace (implicit.py:800-872); bath
+ * potential e = lead[b][k] amp[b] wave[s] (mV), intra = iamp[b][k] i_wave[s]
+ * (mA).  vc (B,K,3), m, h, n (B,K) advance in place; rec (optional):
+ * v = vi - ve0 every rec_every steps, [n_rec][B*K].
+ */
+ORC_API int orc_extcell_run(int64_t B, int64_t K, const double* maind, const double* lower,
+                            const double* upper, const double* area, const double* xg,
+                            const double* sna, const double* sk, const double* hhp,
+                            const double* lead, const double* amp, const double* wave,
+                            const double* iamp, const double* i_wave, double dt,
+                            int64_t n_steps, double* vc, double* M, double* H, double* N,
+                            double* rec, int64_t rec_every, int n_threads) {
+  orc_hh_t p = {hhp[0], hhp[1], hhp[2], hhp[3], hhp[4], hhp[5], hhp[6]};
+  int bad = 0;
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+#endif
+  for (int64_t b = 0; b < B; ++b) {
+    double* w = (double*)malloc(sizeof(double) * (size_t)K * 24);
+    if (!w) { bad |= 1; continue; }
+    double *Ak = w, *Rk = w + 9 * K, *Xk = w + 12 * K, *Fk = w + 15 * K;
+    double* V = vc + b * K * 3;
+    for (int64_t s = 0; s < n_steps; ++s) {
+      for (int64_t k = 0; k < K; ++k) {
+        const int64_t i = b * K + k;
+        double v = V[k * 3] - V[k * 3 + 1];
+        double G, J;
+        hh_advance(v, dt, &p, sna ? sna[i] : 1.0, sk ? sk[i] : 1.0, &M[i], &H[i], &N[i], &G,
+                   &J);
+        double g = G * area[i];
+        double d = (J - G * v) * area[i];
+        if (iamp && i_wave) d += iamp[i] * i_wave[s];
+        for (int c = 0; c < 9; ++c) Ak[k * 9 + c] = maind[i * 9 + c];
+        Ak[k * 9 + 0] += g;
+        Ak[k * 9 + 1] -= g;
+        Ak[k * 9 + 3] -= g;
+        Ak[k * 9 + 4] += g;
+        double bath = (lead && amp && wave) ? lead[i] * amp[b] * wave[s] : 0.0;
+        double shell = xg[i * 2] * (V[k * 3 + 1] - V[k * 3 + 2]);
+        double bathf = xg[i * 2 + 1] * (bath - V[k * 3 + 2]);
+        Rk[k * 3 + 0] = d;
+        Rk[k * 3 + 1] = -d - shell;
+        Rk[k * 3 + 2] = shell + bathf;
+      }
+      for (int64_t k = 0; k + 1 < K; ++k)
+        for (int c = 0; c < 3; ++c)
+          Fk[k * 3 + c] = -lower[(b * (K - 1) + k) * 3 + c] * (V[(k + 1) * 3 + c] - V[k * 3 + c]);
+      for (int64_t k = 0; k < K; ++k)
+        for (int c = 0; c < 3; ++c) {
+          if (k + 1 < K) Rk[k * 3 + c] += Fk[k * 3 + c];
+          if (k > 0) Rk[k * 3 + c] -= Fk[(k - 1) * 3 + c];
+        }
+      /* block Thomas: A'_k = A_k - diag(l) A'^-1_{k-1} diag(u), one 3x3 solve
+       * per column of A'^-1_{k-1} diag(u) and one for the right-hand side */
+      for (int64_t k = 1; k < K; ++k) {
+        const double* l = lower + (b * (K - 1) + k - 1) * 3;
+        const double* u = upper + (b * (K - 1) + k - 1) * 3;
+        double Y[9], y[3];
+        for (int c = 0; c < 3; ++c) {
+          double a[9], rhs[3] = {0.0, 0.0, 0.0}, col[3];
+          memcpy(a, Ak + (k - 1) * 9, sizeof a);
+          rhs[c] = u[c];
+          solve3_pivot(a, rhs, col);
+          for (int r = 0; r < 3; ++r) Y[r * 3 + c] = col[r];
+        }
+        {
+          double a[9], rhs[3];
+          memcpy(a, Ak + (k - 1) * 9, sizeof a);
+          memcpy(rhs, Rk + (k - 1) * 3, sizeof rhs);
+          solve3_pivot(a, rhs, y);
+        }
+        for (int r = 0; r < 3; ++r) {
+          for (int c = 0; c < 3; ++c) Ak[k * 9 + r * 3 + c] -= l[r] * Y[r * 3 + c];
+          Rk[k * 3 + r] -= l[r] * y[r];
+        }
+      }
+      for (int64_t k = K - 1; k >= 0; --k) {
+        double a[9], rhs[3];
+        memcpy(a, Ak + k * 9, sizeof a);
+        for (int c = 0; c < 3; ++c)
+          rhs[c] = Rk[k * 3 + c] -
+                   (k + 1 < K ? upper[(b * (K - 1) + k) * 3 + c] * Xk[(k + 1) * 3 + c] : 0.0);
+        solve3_pivot(a, rhs, Xk + k * 3);
+      }
+      for (int64_t k = 0; k < K * 3; ++k) V[k] += Xk[k];
+      if (rec && rec_every > 0 && (s + 1) % rec_every == 0) {
+        int64_t r = (s + 1) / rec_every - 1;
+        for (int64_t k = 0; k < K; ++k) rec[r * B * K + b * K + k] = V[k * 3] - V[k * 3 + 1];
+      }
+    }
+    free(w);
+  }
+  return bad ? -1 : 0;
+}

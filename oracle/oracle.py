@@ -79,6 +79,9 @@ def load():
         lib.orc_cable_run.restype = i32
         lib.orc_tree_run.argtypes = [i64, lp, lp] + run_args[2:]
         lib.orc_tree_run.restype = i32
+        lib.orc_extcell_run.argtypes = [i64, i64, dp, dp, dp, dp, dp, vp, vp, dp, vp, vp, vp,
+                                        vp, vp, f64, i64, dp, dp, dp, dp, vp, i64, i32]
+        lib.orc_extcell_run.restype = i32
         lib.orc_threshold.argtypes = [i64, lp, dp, dp, dp, dp, dp, dp, dp, i64, f64, f64, lp,
                                       i32, f64, f64, f64, i32, i32, dp, dp, ip, ip, i32]
         lib.orc_threshold.restype = i32
@@ -191,6 +194,31 @@ def cable_run(spec, hh_arr, dt, n_steps, ve_wave=None, i_wave=None, record_every
     arrays = (_c(spec.diam), _c(spec.dx), _c(spec.rhoa), _c(spec.cm))
     return _run(load().orc_cable_run, spec.n_cells, spec.off, (), arrays, spec, hh_arr,
                 dt, n_steps, ve_wave, i_wave, record_every, threads, state)
+
+
+def extcell_run(ws, hh_arr, dt, n_steps, state, *, sna=None, sk=None, lead=None, amp=None,
+                wave=None, iamp=None, i_wave=None, record_every=0, threads=0):
+    """Double-layer extracellular HH cables (orc_extcell_run).  ws: the
+    prepared workspace as numpy arrays -- maind (B,K,3,3), lower/upper
+    (B,K-1,3), area (B,K), xg (B,K,2); state: vc (B,K,3), m, h, n (B,K),
+    advanced in copies.  Returns (state, rec of v = vi - ve0)."""
+    maind = _c(ws["maind"])
+    B, K = maind.shape[:2]
+    vc = _c(state["vc"]).copy()
+    M, H, N = (_c(state[k]).copy() for k in ("m", "h", "n"))
+    opt = [_opt(a)[1] for a in (sna, sk)]
+    keep = [_opt(a) for a in (lead, amp, wave, iamp, i_wave)]
+    rec = rec_p = None
+    if record_every > 0:
+        rec = np.zeros((n_steps // record_every, B * K))
+        rec_p = rec.ctypes.data_as(vp)
+    st = load().orc_extcell_run(B, K, maind, _c(ws["lower"]), _c(ws["upper"]), _c(ws["area"]),
+                                _c(ws["xg"]), *opt, _c(hh_arr), *(k[1] for k in keep),
+                                float(dt), int(n_steps), vc, M, H, N, rec_p,
+                                int(record_every), int(threads))
+    if st != 0:
+        raise RuntimeError(f"oracle run failed ({st})")
+    return {"vc": vc, "m": M, "h": H, "n": N}, rec
 
 
 def tree_run(spec, hh_arr, dt, n_steps, ve_wave=None, i_wave=None, record_every=0,
