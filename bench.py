@@ -648,6 +648,76 @@ def operator_bench(dev):
     res["sparse_calendar_step"] = {"capacity": cap, "pending": size,
                                    "appended_per_step": per_step, "targets": width,
                                    **_rate(ms, nbytes)}
+    res["extcell_cables"] = extcell_bench(dev)
+    return res
+
+
+def _extcell_spec(B, K, seed):
+    from dendra_amd.extcell import ExtCellSpec
+
+    rng = np.random.default_rng(seed)
+    lead = 300.0 / (4 * np.pi * 1e-4 * np.hypot(
+        (np.arange(K) - K / 2) * 50.0, rng.uniform(200.0, 1500.0, (B, 1))))
+    return ExtCellSpec(diam=rng.uniform(4.0, 12.0, (B, K)).round(1), dx=np.full((B, K), 50.0),
+                       rhoa=np.full((B, K), 100.0), cm=np.ones((B, K)),
+                       xraxial=np.full((B, K, 2), 1e5), xc=np.zeros((B, K, 2)),
+                       xg=np.full((B, K, 2), 10.0), lead=lead,
+                       ve_amp=rng.uniform(-1.0, -0.05, B) * 1e-3)
+
+
+def extcell_bench(dev, B=32768, K=101, n_steps=2000, dt=0.005, cpu_cells=512,
+                  cpu_steps=2000):
+    """HH cables with two extracellular layers (the ExtCell integrator step):
+    B x K compartments, float32, a 0.1 ms cathodic point-source pulse.
+    Algorithmic bytes 124 B per compartment-step: state vc and gates read and
+    written (48 B) and the prepared workspace read (maind 9, couplings 6, area,
+    radial conductances 2, lead field: 76 B); the block elimination's own
+    factors (12 values written and read back) are not counted."""
+    import torch
+
+    from dendra_amd.extcell import ExtCellCables
+    from dendra_amd.model import HH
+
+    hh = HH(celsius=6.3)
+    spec = _extcell_spec(B, K, 21)
+    wave = np.zeros(n_steps)
+    wave[200:220] = 1.0
+    cells = ExtCellCables(spec, dt, hh, torch.float32, dev)
+    cells.run(50, wave=wave)
+    torch.cuda.synchronize()
+    cells.reset()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    cells.run(n_steps, wave=wave)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop)
+    cs = B * K * n_steps
+    res = {"cells": B, "compartments": K, "time_steps": n_steps,
+           "compartment_steps_per_s": cs / (ms * 1e-3),
+           **_rate(ms, 124.0 * cs)}
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+
+    from dendra_amd.extcell import extcell_workspace
+    from dendra_amd.model import hh_steady_state
+
+    small = _extcell_spec(cpu_cells, K, 21)
+    ws = {k: v.numpy() for k, v in extcell_workspace(small, dt).items()}
+
+    m0, h0, n0 = hh_steady_state(-65.0)
+    st = {"vc": np.zeros((cpu_cells, K, 3)), "m": np.full((cpu_cells, K), m0),
+          "h": np.full((cpu_cells, K), h0), "n": np.full((cpu_cells, K), n0)}
+    st["vc"][..., 0] = -65.0
+    threads = oracle.cpu_threads()
+    t = time.perf_counter()
+    oracle.extcell_run(ws, hh.array(), dt, cpu_steps, st, lead=small.lead, amp=small.ve_amp,
+                       wave=wave[:cpu_steps], threads=threads)
+    cpu_s = time.perf_counter() - t
+    res["cpu_baseline"] = {"value": cpu_cells * K * cpu_steps / cpu_s,
+                           "unit": "compartment-steps/s", "cores": threads, "kind": "port",
+                           "sample": f"{cpu_cells} cells x {K} compartments x {cpu_steps} "
+                                     f"steps, float64 C oracle, {cpu_s:.1f}s"}
     return res
 
 
