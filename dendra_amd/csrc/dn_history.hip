@@ -60,6 +60,7 @@ __global__ __launch_bounds__(256) void history_project_kernel(
   const int slot = hist_slot(slot_p, depth);
   T acc = T(0);
   const int e0 = in_ptr[t], e1 = in_ptr[t + 1];
+#pragma unroll 4
   for (int i = e0; i < e1; ++i) {
     const int64_t e = in_edge[i];
     const int32_t c = col[e];

@@ -66,6 +66,56 @@ __global__ __launch_bounds__(256) void interp_delay_enqueue_kernel(
   }
 }
 
+// Tiled form: a workgroup of 64 lanes owns 64 target columns for all depth
+// rows.  The tile of the input ring is staged in LDS (coalesced), today's row
+// cleared, each lane applies its target's contributions in the same order as
+// above to the LDS cells, and the tile is written to the output ring.  The
+// copy the pure transition returns is thus fused, and the per-cell updates
+// are LDS read-modify-writes, so the connection loads can run ahead.
+template <typename T, int TAPS>
+__global__ __launch_bounds__(64) void interp_delay_tile_kernel(
+    const T* __restrict__ qin, T* __restrict__ qout, int depth, int64_t width,
+    const int64_t* __restrict__ slot_p, const int32_t* __restrict__ in_ptr,
+    const int32_t* __restrict__ in_edge, const T* __restrict__ w, const T* __restrict__ g,
+    const T* __restrict__ f, const int32_t* __restrict__ base) {
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned char dn_smem_raw[];
+  T* tile = reinterpret_cast<T*>(dn_smem_raw);
+  const int lane = threadIdx.x;
+  const int64_t t0 = (int64_t)blockIdx.x * 64;
+  const int64_t t = t0 + lane;
+  const bool live = t < width;
+  const int slot = ring_slot(slot_p, depth);
+  for (int r = 0; r < depth; ++r)
+    tile[r * 64 + lane] = (live && r != slot) ? qin[(int64_t)r * width + t] : T(0);
+  if (live) {
+    const int e0 = in_ptr[t], e1 = in_ptr[t + 1];
+    if (TAPS == 2) {
+      for (int j = 0; j < 2; ++j) {
+#pragma unroll 4
+        for (int i = e0; i < e1; ++i) {
+          const int e = in_edge[i];
+          const T wg = w[e] * g[e];
+          const T tap = j == 0 ? T(1) - f[e] : f[e];
+          T* cell = tile + ring_row(slot, base[e], j, depth) * 64 + lane;
+          *cell = *cell + wg * tap;
+        }
+      }
+    } else {
+#pragma unroll 4
+      for (int i = e0; i < e1; ++i) {
+        const int e = in_edge[i];
+        const T wg = w[e] * g[e];
+        for (int j = 0; j < 3; ++j) {
+          T* cell = tile + ring_row(slot, base[e], j, depth) * 64 + lane;
+          *cell = *cell + wg * f[(int64_t)e * 3 + j];
+        }
+      }
+    }
+    for (int r = 0; r < depth; ++r) qout[(int64_t)r * width + t] = tile[r * 64 + lane];
+  }
+}
+
 // Per-connection cotangents for a ring cotangent G.  Two taps:
 //   dw = g ((1 - f) G_0 + f G_1),  dg = w (...),  df = w g (G_1 - G_0);
 // three taps: dw = g sum_j f_j G_j, dg = w sum_j f_j G_j, df_j = w g G_j.
@@ -100,12 +150,27 @@ __global__ __launch_bounds__(256) void interp_delay_vjp_kernel(
 }  // namespace dn
 
 template <typename T>
-static int interp_enqueue_impl(T* q, int64_t depth, int64_t width, const int64_t* slot,
-                               int taps, const int32_t* in_ptr, const int32_t* in_edge,
-                               const T* w, const T* g, const T* f, const int32_t* base,
-                               hipStream_t stream) {
+static int interp_enqueue_impl(const T* qin, T* q, int64_t depth, int64_t width,
+                               const int64_t* slot, int taps, const int32_t* in_ptr,
+                               const int32_t* in_edge, const T* w, const T* g, const T* f,
+                               const int32_t* base, hipStream_t stream) {
   if (!q || !slot || !in_ptr || depth < 1 || depth > (1 << 30) || width < 1) return DN_EINVAL;
   if ((taps != 2 && taps != 3) || width > (int64_t)INT32_MAX * 256) return DN_EINVAL;
+  const size_t lds = (size_t)depth * 64 * sizeof(T);
+  if (qin && qin != q && lds <= 64 * 1024) {
+    const dim3 tiles((unsigned)((width + 63) / 64));
+    if (taps == 2)
+      hipLaunchKernelGGL((dn::interp_delay_tile_kernel<T, 2>), tiles, dim3(64), lds, stream,
+                         qin, q, (int)depth, width, slot, in_ptr, in_edge, w, g, f, base);
+    else
+      hipLaunchKernelGGL((dn::interp_delay_tile_kernel<T, 3>), tiles, dim3(64), lds, stream,
+                         qin, q, (int)depth, width, slot, in_ptr, in_edge, w, g, f, base);
+    return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+  }
+  if (qin && qin != q &&
+      hipMemcpyAsync(q, qin, sizeof(T) * depth * width, hipMemcpyDeviceToDevice, stream) !=
+          hipSuccess)
+    return DN_ELAUNCH;
   const dim3 grid((unsigned)((width + 255) / 256));
   if (taps == 2)
     hipLaunchKernelGGL((dn::interp_delay_enqueue_kernel<T, 2>), grid, dim3(256), 0, stream,
@@ -137,21 +202,21 @@ static int interp_vjp_impl(const T* G, int64_t depth, int64_t width, const int64
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
 
-DN_EXPORT int dn_interp_delay_enqueue_f32(float* q, int64_t depth, int64_t width,
+DN_EXPORT int dn_interp_delay_enqueue_f32(const float* qin, float* q, int64_t depth, int64_t width,
                                           const int64_t* slot, int taps,
                                           const int32_t* in_ptr, const int32_t* in_edge,
                                           const float* w, const float* g, const float* f,
                                           const int32_t* base, hipStream_t stream) {
-  return interp_enqueue_impl<float>(q, depth, width, slot, taps, in_ptr, in_edge, w, g, f,
-                                    base, stream);
+  return interp_enqueue_impl<float>(qin, q, depth, width, slot, taps, in_ptr, in_edge, w, g,
+                                    f, base, stream);
 }
-DN_EXPORT int dn_interp_delay_enqueue_f64(double* q, int64_t depth, int64_t width,
+DN_EXPORT int dn_interp_delay_enqueue_f64(const double* qin, double* q, int64_t depth, int64_t width,
                                           const int64_t* slot, int taps,
                                           const int32_t* in_ptr, const int32_t* in_edge,
                                           const double* w, const double* g, const double* f,
                                           const int32_t* base, hipStream_t stream) {
-  return interp_enqueue_impl<double>(q, depth, width, slot, taps, in_ptr, in_edge, w, g, f,
-                                     base, stream);
+  return interp_enqueue_impl<double>(qin, q, depth, width, slot, taps, in_ptr, in_edge, w,
+                                     g, f, base, stream);
 }
 DN_EXPORT int dn_interp_delay_vjp_f32(const float* G, int64_t depth, int64_t width,
                                       const int64_t* slot, int taps, int64_t n_edge,

@@ -99,10 +99,11 @@ class _InterpEnqueue(torch.autograd.Function):
     @staticmethod
     def forward(ctx, queue, slot, weights, gates, fraction, base, in_ptr, in_edge, post, taps):
         depth, width = map(int, queue.shape)
-        out = queue.clone(memory_format=torch.contiguous_format)
+        qin = queue.contiguous()
+        out = torch.empty_like(qin)
         w, g, f = (v.contiguous() for v in (weights, gates, fraction))
         p = _lib.ptr
-        st = _fn("dn_interp_delay_enqueue", out)(p(out), depth, width, p(slot), taps,
+        st = _fn("dn_interp_delay_enqueue", out)(p(qin), p(out), depth, width, p(slot), taps,
                                                  p(in_ptr), p(in_edge), p(w), p(g), p(f),
                                                  p(base), _lib.stream_handle(out.device))
         _lib.check(st, "interpolated-delay enqueue")

@@ -150,8 +150,10 @@ int dn_bt3_tree_f64(const double* main, const double* edge, const double* rhs,
  * Replaces the ordered two-tap enqueue (dendra/models/networks/
  * _two_tap_enqueue_cpu.cpp:26-152, dendra_native::ordered_two_tap_enqueue)
  * and the three-tap scatter of dense_interpolated_delay_transition
- * (networks/_event_delivery.py:226-310).  q (depth, width) is updated in
- * place: row *slot (device int64, reduced modulo depth) is cleared; then
+ * (networks/_event_delivery.py:226-310).  q (depth, width) receives the
+ * transition of qin (qin null or equal to q: in place; else qin is read and
+ * left unchanged): row *slot (device int64, reduced modulo depth) is
+ * cleared; then
  *   taps 2: every connection adds w g (1 - f) at row slot + base and w g f at
  *           row slot + base + 1, all lower taps before all upper taps;
  *   taps 3: every connection adds w g f[3e + j] at row slot + base + j,
@@ -162,11 +164,11 @@ int dn_bt3_tree_f64(const double* main, const double* edge, const double* rhs,
  * (n_edge, or 3 n_edge) are per connection; base is the first tap's row
  * offset modulo depth.  dn_interp_delay_vjp_* writes the per-connection
  * cotangents dw, dg, df for a ring cotangent G. */
-int dn_interp_delay_enqueue_f32(float* q, int64_t depth, int64_t width, const int64_t* slot,
-                                int taps, const int32_t* in_ptr, const int32_t* in_edge,
-                                const float* w, const float* g, const float* f,
-                                const int32_t* base, hipStream_t stream);
-int dn_interp_delay_enqueue_f64(double* q, int64_t depth, int64_t width,
+int dn_interp_delay_enqueue_f32(const float* qin, float* q, int64_t depth, int64_t width,
+                                const int64_t* slot, int taps, const int32_t* in_ptr,
+                                const int32_t* in_edge, const float* w, const float* g,
+                                const float* f, const int32_t* base, hipStream_t stream);
+int dn_interp_delay_enqueue_f64(const double* qin, double* q, int64_t depth, int64_t width,
                                 const int64_t* slot, int taps, const int32_t* in_ptr,
                                 const int32_t* in_edge, const double* w, const double* g,
                                 const double* f, const int32_t* base, hipStream_t stream);
