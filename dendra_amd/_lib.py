@@ -139,6 +139,10 @@ for _sfx in ("f32", "f64"):
 _SIGNATURES["dn_history_record_packed"] = (i32, [p, i64, i64, p, i64, p, p, p, p])
 _SIGNATURES["dn_calendar_work_bytes"] = (i64, [i64, i64, i32])
 _SIGNATURES["dn_extcell_work_elems"] = (i64, [i64, i64])
+_SIGNATURES["dn_extcell_tree_work_elems"] = (i64, [i64, i64])
+for _sfx, _ft in (("f32", f32), ("f64", f64)):
+    _SIGNATURES[f"dn_extcell_tree_run_{_sfx}"] = (
+        i32, [i64, i64] + [p] * 8 + [i64] + [p] * 10 + [_ft, i64] + [p] * 5 + [i64, p, i64, p])
 for _sfx, _ft in (("f32", f32), ("f64", f64)):
     _SIGNATURES[f"dn_extcell_run_{_sfx}"] = (
         i32, [i64, i64] + [p] * 13 + [_ft, i64] + [p] * 5 + [i64, p, i64, p])

@@ -488,3 +488,145 @@ DN_EXPORT int dn_bt3_tree_f64(const double* main, const double* edge, const doub
   return bt3_tree_impl<double>(main, edge, rhs, parent, order, layer_ptr, n_layers, child_ptr,
                             child_idx, x, B, K, work, work_elems, stream);
 }
+
+// ---------------------------------------------------------------------------
+// ExtCell trees: the block tree backward-Euler step of _dhs_bt._step
+// (dendra/models/integrators/tree_bt.py:504-654) for B HH trees of one
+// topology, in node order.  Per step and node i (unknowns vc = (vi, ve0, ve1),
+// v = vi - ve0):
+//   gates by cnexp at v -> G, J;  g = G area, d = (J - G v) area + intra
+//   rhs_i = (d, -d - xg0 (ve0 - ve1), xg0 (ve0 - ve1) + xg1 (e_bath - ve1))
+//           - f_i + sum over children c (ascending) of f_c,
+//   f_i = e_i * (vc_i - vc_parent)   (e_i * vc_i at a root)
+//   D_i = main_i + [[g, -g, 0], [-g, g, 0], [0, 0, 0]]
+// then the block tree solve (bt3_tree_layer_kernel / bt3_tree_kernel) of
+// D, e, rhs for the increment, and vc += increment.
+// ---------------------------------------------------------------------------
+namespace dn {
+
+template <typename T>
+struct XtArgs {
+  const T *main, *edge, *area, *xg, *sna, *sk, *hh, *lead, *amp, *wave, *iamp, *iwave;
+  const int32_t *parent, *child_ptr, *child_idx;
+  T *vc, *m, *h, *n, *D, *R;
+  int64_t B, K;
+  T dt;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void extcell_tree_assemble_kernel(XtArgs<T> a, int64_t s) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.B * a.K) return;
+  const int64_t b = j / a.K, i = j - b * a.K;
+  const HHParams<T> hp{a.hh[0], a.hh[1], a.hh[2], a.hh[3], a.hh[4], a.hh[5], a.hh[6]};
+  const T* V = a.vc + b * a.K * 3;
+  const T* E = a.edge + b * a.K * 3;
+  const T vi[3] = {V[i * 3], V[i * 3 + 1], V[i * 3 + 2]};
+  const T v = vi[0] - vi[1];
+  T m = a.m[j], h = a.h[j], n = a.n[j], G, J;
+  hh_advance<T>(v, a.dt, hp, a.sna ? a.sna[j] : T(1), a.sk ? a.sk[j] : T(1), m, h, n, G, J);
+  a.m[j] = m;
+  a.h[j] = h;
+  a.n[j] = n;
+  const T area = a.area[j];
+  const T g = G * area;
+  const T intra = (a.iamp && a.iwave) ? a.iamp[j] * a.iwave[s] : T(0);
+  const T d = (J - G * v) * area + intra;
+  const T bath = (a.lead && a.amp && a.wave) ? a.lead[j] * (a.amp[b] * a.wave[s]) : T(0);
+  const T shell = a.xg[j * 2] * (vi[1] - vi[2]);
+  const T bathf = a.xg[j * 2 + 1] * (bath - vi[2]);
+  T r[3] = {d, -d - shell, shell + bathf};
+  const int32_t p = a.parent[i];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    r[c] -= E[i * 3 + c] * (p >= 0 ? vi[c] - V[p * 3 + c] : vi[c]);
+  for (int32_t q = a.child_ptr[i]; q < a.child_ptr[i + 1]; ++q) {
+    const int64_t ch = a.child_idx[q];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[c] += E[ch * 3 + c] * (V[ch * 3 + c] - vi[c]);
+  }
+  T* Dj = a.D + j * 9;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) Dj[c] = a.main[j * 9 + c];
+  Dj[0] += g;
+  Dj[1] -= g;
+  Dj[3] -= g;
+  Dj[4] += g;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) a.R[j * 3 + c] = r[c];
+}
+
+// one thread per node: vc += x, and the membrane potential vi - ve0 when recorded
+template <typename T>
+__global__ __launch_bounds__(256) void extcell_tree_update_kernel(T* __restrict__ vc,
+                                                                  const T* __restrict__ x,
+                                                                  int64_t n_nodes,
+                                                                  T* __restrict__ rec) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_nodes) return;
+  T u[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    u[c] = vc[j * 3 + c] + x[j * 3 + c];
+    vc[j * 3 + c] = u[c];
+  }
+  if (rec) rec[j] = u[0] - u[1];
+}
+
+}  // namespace dn
+
+template <typename T>
+static int extcell_tree_impl(const dn::XtArgs<T>& a0, const int32_t* parent32,
+                             const int64_t* parent, const int64_t* order,
+                             const int64_t* layer_ptr, int64_t n_layers, int64_t n_steps,
+                             T* rec, int64_t rec_every, T* work, int64_t work_elems,
+                             hipStream_t stream) {
+  const int64_t B = a0.B, K = a0.K;
+  if (B < 1 || K < 1 || n_steps < 0 || B * K > (int64_t)INT32_MAX * 256) return DN_EINVAL;
+  if (!a0.main || !a0.edge || !a0.area || !a0.xg || !a0.hh || !a0.vc || !a0.m || !a0.h ||
+      !a0.n || !parent32 || !a0.child_ptr || !parent || !order || !work)
+    return DN_EINVAL;
+  if (work_elems < 27 * B * K || (rec && rec_every <= 0)) return DN_EINVAL;
+  dn::XtArgs<T> a = a0;
+  a.parent = parent32;
+  a.D = work;
+  a.R = work + 9 * B * K;
+  T* X = work + 12 * B * K;
+  T* solve_work = work + 15 * B * K;
+  const dim3 nodes((unsigned)((B * K + 255) / 256));
+  for (int64_t s = 0; s < n_steps; ++s) {
+    hipLaunchKernelGGL((dn::extcell_tree_assemble_kernel<T>), nodes, dim3(256), 0, stream, a,
+                       s);
+    const int st = bt3_tree_impl<T>(a.D, a.edge, a.R, parent, order, layer_ptr, n_layers,
+                                    a.child_ptr, a.child_idx, X, B, K, solve_work, 12 * B * K,
+                                    stream);
+    if (st != DN_OK) return st;
+    T* r = (rec && (s + 1) % rec_every == 0) ? rec + ((s + 1) / rec_every - 1) * B * K : nullptr;
+    hipLaunchKernelGGL((dn::extcell_tree_update_kernel<T>), nodes, dim3(256), 0, stream, a.vc,
+                       (const T*)X, B * K, r);
+  }
+  return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+}
+
+DN_EXPORT int64_t dn_extcell_tree_work_elems(int64_t B, int64_t K) {
+  return (B < 0 || K < 0) ? -1 : 27 * B * K;
+}
+
+#define DN_EXTCELL_TREE_EXPORT(T, SFX)                                                       \
+  DN_EXPORT int dn_extcell_tree_run_##SFX(                                                   \
+      int64_t B, int64_t K, const T* main, const T* edge, const T* area, const T* xg,        \
+      const int32_t* parent32, const int64_t* parent, const int64_t* order,                  \
+      const int64_t* layer_ptr, int64_t n_layers, const int32_t* child_ptr,                  \
+      const int32_t* child_idx, const T* sna, const T* sk, const T* hh, const T* lead,       \
+      const T* amp, const T* wave, const T* iamp, const T* i_wave, T dt, int64_t n_steps,    \
+      T* vc, T* m, T* h, T* n, T* rec, int64_t rec_every, T* work, int64_t work_elems,       \
+      hipStream_t stream) {                                                                  \
+    dn::XtArgs<T> a{main, edge, area, xg,      sna, sk,      hh, lead, amp, wave, iamp,     \
+                    i_wave, nullptr, child_ptr, child_idx, vc, m, h, n, nullptr, nullptr,  \
+                    B, K, dt};                                                               \
+    return extcell_tree_impl<T>(a, parent32, parent, order, layer_ptr, n_layers, n_steps,    \
+                                rec, rec_every, work, work_elems, stream);                   \
+  }
+
+DN_EXTCELL_TREE_EXPORT(float, f32)
+DN_EXTCELL_TREE_EXPORT(double, f64)

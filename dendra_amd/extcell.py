@@ -16,6 +16,11 @@ length, the double-layer extracellular model of the stimulation workflows:
   increment) in one HIP launch: one lane per cell for all steps.  The bath
   potential is ``lead * ve_amp * wave[s]`` (mV), the intracellular current
   ``iamp * i_wave[s]`` (mA).
+* branched trees (``_dhs_bt``, integrators/tree_bt.py:91-654):
+  ``tree_layered_edge_conductance``, ``prepare_tree_workspace`` (the block-tree
+  workspace with the reference's keys) and ``ExtCellTrees(spec, dt, hh)``,
+  whose ``run`` takes per step an assembly launch, the block tree solve over
+  the DHS layers and an update.
 """
 
 from __future__ import annotations
@@ -210,3 +215,194 @@ class ExtCellCables:
         _lib.check(st, "ExtCell run")
         self.t_step += n_steps
         return None if rec is None else rec.permute(0, 2, 1)
+
+
+# -- branched trees (tree_bt._dhs_bt) -------------------------------------------
+
+def tree_layered_edge_conductance(xraxial, dx, edge_child_orig, edge_parent_orig):
+    """Extracellular edge conductance (S) of each tree edge: the two
+    half-segment resistances in series; invalid or non-positive resistance
+    gives a sealed edge (tree_bt.py:91-121)."""
+    cd, pd = dx.index_select(-1, edge_child_orig), dx.index_select(-1, edge_parent_orig)
+    cx = xraxial.index_select(-2, edge_child_orig)
+    px = xraxial.index_select(-2, edge_parent_orig)
+    r = 0.5 * (cx * cd.unsqueeze(-1) + px * pd.unsqueeze(-1)) * 1.0e2
+    valid = torch.isfinite(r) & (r > 0)
+    return torch.where(valid, r.reciprocal(), torch.zeros_like(r))
+
+
+def prepare_tree_workspace(dt, *, cm, area, intracellular_edge_conductance,
+                           extracellular_edge_conductance, xc, xg, edge_child_orig,
+                           solver_order):
+    """The block-tree workspace of _dhs_bt._prepare_workspace
+    (tree_bt.py:288-372): radial main blocks and per-node conductances to the
+    parent (intracellular, two shells) gathered into solver order; zero-area
+    branch points carry no radial terms."""
+    dt_s = dt.reshape(1) * 1.0e-3 if torch.is_tensor(dt) else dt * 1.0e-3
+    cm_dt = 1.0e-6 * cm * area / dt_s
+    xc_dt = 1.0e-6 * xc * area.unsqueeze(-1) / dt_s
+    radial_g = xg * area.unsqueeze(-1)
+    zero_area = area == 0
+    cm_dt = torch.where(zero_area, torch.zeros_like(cm_dt), cm_dt)
+    xc_dt = torch.where(zero_area.unsqueeze(-1), torch.zeros_like(xc_dt), xc_dt)
+    radial_g = torch.where(zero_area.unsqueeze(-1), torch.zeros_like(radial_g), radial_g)
+    shell_0 = xc_dt[..., 0] + radial_g[..., 0]
+    zero = torch.zeros_like(cm_dt)
+    main = torch.stack((torch.stack((cm_dt, -cm_dt, zero), dim=-1),
+                        torch.stack((-cm_dt, cm_dt + shell_0, -shell_0), dim=-1),
+                        torch.stack((zero, -shell_0, shell_0 + xc_dt[..., 1] + radial_g[..., 1]),
+                                    dim=-1)), dim=-2)
+    gi, gx = intracellular_edge_conductance, extracellular_edge_conductance
+    idx = edge_child_orig.reshape((1,) * (gi.ndim - 1) + (-1,)).expand_as(gi)
+    gi_node = torch.zeros_like(area).scatter(-1, idx, gi)
+    sidx = edge_child_orig.reshape((1,) * (gx.ndim - 2) + (-1, 1)).expand_as(gx)
+    gx_node = torch.zeros_like(xc).scatter(-2, sidx, gx)
+    g_to_parent = torch.cat((gi_node.unsqueeze(-1), gx_node), dim=-1).index_select(-2,
+                                                                                   solver_order)
+    return {"area": area.clone(), "cm_dt": cm_dt, "xc_dt": xc_dt,
+            "c_rad": torch.cat((cm_dt.unsqueeze(-1), xc_dt), dim=-1), "xg": radial_g,
+            "main_blocks": main.index_select(-3, solver_order), "g_to_parent": g_to_parent}
+
+
+@dataclass
+class ExtCellTreeSpec:
+    """B trees of one topology (parent (K,), -1 at the root): per node diam,
+    length (um), ra (ohm cm), cm (uF/cm^2) as (B, K); xraxial (MOhm/cm), xc,
+    xg as (B, K, 2); optional gNa/gK scales, lead field (B, K) with ve_amp
+    (B), intracellular amplitude iamp (B, K)."""
+
+    parent: np.ndarray
+    diam: np.ndarray
+    length: np.ndarray
+    ra: np.ndarray
+    cm: np.ndarray
+    xraxial: np.ndarray
+    xc: np.ndarray
+    xg: np.ndarray
+    sna: np.ndarray | None = None
+    sk: np.ndarray | None = None
+    lead: np.ndarray | None = None
+    ve_amp: np.ndarray | None = None
+    iamp: np.ndarray | None = None
+    v_init: float = -65.0
+
+    @property
+    def shape(self):
+        return tuple(np.shape(self.diam))
+
+
+def extcell_tree_workspace(spec: ExtCellTreeSpec, dt: float) -> dict:
+    """The prepared block-tree workspace (float64, CPU, node order): lateral
+    cylinder areas, intracellular conductance 1 / (R_half_child + R_half_parent)
+    with R_half = ra (L/2) / (pi r^2) (integrators/tree.py:560-585), and the
+    layered extracellular conductance of each edge."""
+    t = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64))
+    par = np.asarray(spec.parent, dtype=np.int64)
+    diam, length, ra = t(spec.diam), t(spec.length), t(spec.ra)
+    area = diam * 1e-4 * PI * length * 1e-4
+    r = 1e-4 * diam / 2.0
+    rhalf = ra * (1e-4 * length / 2.0) / (PI * r * r)
+    child = torch.as_tensor(np.flatnonzero(par >= 0))
+    parent = torch.as_tensor(par[par >= 0])
+    gi = 1.0 / (rhalf.index_select(-1, child) + rhalf.index_select(-1, parent))
+    gx = tree_layered_edge_conductance(t(spec.xraxial), length, child, parent)
+    K = len(par)
+    return prepare_tree_workspace(dt, cm=t(spec.cm), area=area, intracellular_edge_conductance=gi,
+                                  extracellular_edge_conductance=gx, xc=t(spec.xc),
+                                  xg=t(spec.xg), edge_child_orig=child,
+                                  solver_order=torch.arange(K))
+
+
+class ExtCellTrees:
+    """B HH trees of one topology with two extracellular layers: the
+    reference's block-tree step (tree_bt.py:504-654) for all steps in one
+    call: per step an assembly kernel (one thread per node: gates, membrane
+    block, shell/bath fluxes, axial fluxes pulled from the children), the
+    block tree solve over the DHS layers, and the update.  State in node
+    order, (B, K, 3)."""
+
+    def __init__(self, spec: ExtCellTreeSpec, dt: float, hh: HH | None = None,
+                 dtype=torch.float32, device="cuda", threads: int = 16,
+                 workspace: dict | None = None):
+        from .blocks import _children_csr
+        from .topology import build_dhs_layers, build_morphology
+
+        if dtype not in _DT:
+            raise TypeError("ExtCellTrees supports float32 and float64")
+        self.spec, self.dt, self.dtype = spec, float(dt), dtype
+        self.device = torch.device(device)
+        self.hh = hh or HH()
+        B, K = spec.shape
+        par = [int(p) for p in np.asarray(spec.parent)]
+        if len(par) != K:
+            raise ValueError("parent must hold one entry per node")
+        _, _, depth = build_morphology(par)
+        order, layers = build_dhs_layers(depth, threads)
+        self.B, self.K = B, K
+        ws = workspace if workspace is not None else extcell_tree_workspace(spec, dt)
+        self.workspace = ws
+        dev = lambda a, dt_=dtype: torch.as_tensor(np.asarray(a) if not torch.is_tensor(a) else a,
+                                                   dtype=dt_, device=self.device).contiguous()
+        self._main = dev(ws["main_blocks"]).reshape(B, K, 9)
+        self._edge = dev(ws["g_to_parent"])
+        self._area, self._xg = dev(ws["area"]), dev(ws["xg"])
+        self._parent = dev(par, torch.int64)
+        self._parent32 = dev(par, torch.int32)
+        self._order = dev(order, torch.int64)
+        self._layers = dev(layers, torch.int64)
+        self._child_ptr, self._child_idx = _children_csr(self._parent)
+        opt = lambda a: None if a is None else dev(a)
+        self._sna, self._sk, self._lead, self._iamp = (opt(spec.sna), opt(spec.sk),
+                                                       opt(spec.lead), opt(spec.iamp))
+        self._amp = opt(spec.ve_amp)
+        self._hh = torch.as_tensor(self.hh.array(), dtype=dtype, device=self.device)
+        n = int(_lib.load().dn_extcell_tree_work_elems(B, K))
+        self._work = torch.empty(n, dtype=dtype, device=self.device)
+        self.reset()
+
+    def reset(self, v_init: float | None = None):
+        v0 = self.spec.v_init if v_init is None else v_init
+        m0, h0, n0 = hh_steady_state(v0)
+        mk = lambda x: torch.full((self.B, self.K), float(x), dtype=self.dtype,
+                                  device=self.device)
+        self.vc = torch.zeros(self.B, self.K, 3, dtype=self.dtype, device=self.device)
+        self.vc[..., 0] = v0
+        self.m, self.h, self.n = mk(m0), mk(h0), mk(n0)
+        self.t_step = 0
+
+    @property
+    def v(self) -> torch.Tensor:
+        return self.vc[..., 0] - self.vc[..., 1]
+
+    def state(self) -> dict:
+        g = lambda t: t.double().cpu().numpy()
+        return {"vc": g(self.vc), "m": g(self.m), "h": g(self.h), "n": g(self.n)}
+
+    def run(self, n_steps: int, wave=None, i_wave=None, record_every: int = 0, stream=None):
+        """Advance n_steps; returns vi - ve0 every record_every steps as
+        (n_rec, B, K) when record_every > 0."""
+        lib = _lib.load()
+        t = lambda w: None if w is None else torch.as_tensor(
+            np.asarray(w)[self.t_step:self.t_step + n_steps], dtype=self.dtype,
+            device=self.device).contiguous()
+        wv, iw = t(wave), t(i_wave)
+        for w, name in ((wv, "wave"), (iw, "i_wave")):
+            if w is not None and w.numel() < n_steps:
+                raise ValueError(f"{name} is shorter than the run")
+        rec = None
+        if record_every > 0:
+            rec = torch.empty(n_steps // record_every, self.B, self.K, dtype=self.dtype,
+                              device=self.device)
+        p = _lib.ptr
+        fn = lib.dn_extcell_tree_run_f64 if self.dtype == torch.float64 else \
+            lib.dn_extcell_tree_run_f32
+        st = fn(self.B, self.K, p(self._main), p(self._edge), p(self._area), p(self._xg),
+                p(self._parent32), p(self._parent), p(self._order), p(self._layers),
+                self._layers.numel() - 1, p(self._child_ptr), p(self._child_idx), p(self._sna),
+                p(self._sk), p(self._hh), p(self._lead), p(self._amp), p(wv), p(self._iamp),
+                p(iw), self.dt, int(n_steps), p(self.vc), p(self.m), p(self.h), p(self.n),
+                p(rec), int(record_every), p(self._work), self._work.numel(),
+                _lib.stream_handle(self.device) if stream is None else stream.cuda_stream)
+        _lib.check(st, "ExtCell tree run")
+        self.t_step += n_steps
+        return rec

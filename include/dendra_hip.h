@@ -315,6 +315,39 @@ int dn_extcell_run_f64(int64_t B, int64_t K, const double* maind, const double* 
                        int64_t rec_every, double* work, int64_t work_elems,
                        hipStream_t stream);
 
+/* ---- HH trees with two extracellular layers (ExtCell trees) ---------------
+ * Replaces _dhs_bt._step (dendra/models/integrators/tree_bt.py:504-654) with
+ * its block tree solve (_dhs_bt3_adjoint) for B trees of one topology, all
+ * n_steps in one call, arrays in node order: main (B,K,9) radial main blocks
+ * and edge (B,K,3) conductances to the parent (tree_bt.py:288-372), area
+ * (B,K), xg radial conductances (B,K,2), parent32/parent (K, int32/int64,
+ * -1 at a root), order and layer_ptr (n_layers + 1) the DHS layers, child_ptr
+ * (K + 1) / child_idx the children CSR (ascending child index); optional
+ * sna/sk/lead/iamp (B,K), amp (B), wave/i_wave (n_steps); state vc (B,K,3),
+ * m/h/n (B,K) in place; rec (optional) vi - ve0 every rec_every steps,
+ * (n_rec,B,K).  work: dn_extcell_tree_work_elems(B, K) elements. */
+int64_t dn_extcell_tree_work_elems(int64_t B, int64_t K);
+int dn_extcell_tree_run_f32(int64_t B, int64_t K, const float* main, const float* edge,
+                            const float* area, const float* xg, const int32_t* parent32,
+                            const int64_t* parent, const int64_t* order,
+                            const int64_t* layer_ptr, int64_t n_layers,
+                            const int32_t* child_ptr, const int32_t* child_idx,
+                            const float* sna, const float* sk, const float* hh, const float* lead,
+                            const float* amp, const float* wave, const float* iamp,
+                            const float* i_wave, float dt, int64_t n_steps, float* vc, float* m,
+                            float* h, float* n, float* rec, int64_t rec_every, float* work,
+                            int64_t work_elems, hipStream_t stream);
+int dn_extcell_tree_run_f64(int64_t B, int64_t K, const double* main, const double* edge,
+                            const double* area, const double* xg, const int32_t* parent32,
+                            const int64_t* parent, const int64_t* order,
+                            const int64_t* layer_ptr, int64_t n_layers,
+                            const int32_t* child_ptr, const int32_t* child_idx,
+                            const double* sna, const double* sk, const double* hh, const double* lead,
+                            const double* amp, const double* wave, const double* iamp,
+                            const double* i_wave, double dt, int64_t n_steps, double* vc, double* m,
+                            double* h, double* n, double* rec, int64_t rec_every, double* work,
+                            int64_t work_elems, hipStream_t stream);
+
 /* ---- fused HH + backward Euler, unbranched cables -------------------------
  * Replaces the per-step transition of _bwd_euler_ub
  * (dendra/models/integrators/implicit.py:561-664) in Integrator._step order

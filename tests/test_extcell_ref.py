@@ -158,3 +158,71 @@ def test_shorted_layers_reduce_to_the_cable_oracle():
                                record_every=1)
     assert np.abs(rec - crec).max() <= 1e-6
     assert crec.max() > 0.0  # the drive fires the cables: the comparison covers spikes
+
+
+# -- branched trees ------------------------------------------------------------
+
+sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
+import extcell_tree_ref  # noqa: E402
+
+
+def tree_spec(rng, B, parent, *, shorted=False, stim=True):
+    from dendra_amd.extcell import ExtCellTreeSpec
+
+    K = len(parent)
+    c = spec(rng, B, K, shorted=shorted, stim=stim)
+    return ExtCellTreeSpec(np.asarray(parent), c.diam, c.dx, c.rhoa, c.cm, c.xraxial, c.xc,
+                           c.xg, lead=c.lead, ve_amp=c.ve_amp, iamp=c.iamp), c
+
+
+def random_parents(K, rng):
+    return [-1] + [int(rng.integers(max(0, i - 3), i)) for i in range(1, K)]
+
+
+def test_path_tree_equals_the_cable_oracle():
+    """An unbranched ExtCell tree (dense block solve, axial fluxes along parent
+    edges) equals the ExtCell cable (block Thomas, banded couplings)."""
+    from dendra_amd.extcell import extcell_tree_workspace
+
+    rng = np.random.default_rng(21)
+    B, K, dt, steps = 2, 12, 0.005, 300
+    ts, cs = tree_spec(rng, B, list(range(-1, K - 1)))
+    wave = np.zeros(steps)
+    wave[40:140] = 1.0
+    iw = np.zeros(steps)
+    iw[150:200] = 1.0
+    hh_arr = HH(celsius=6.3).array()
+    _, trec = extcell_tree_ref.run(np_ws(extcell_tree_workspace(ts, dt)), ts.parent, hh_arr,
+                                   dt, steps, init_state(cs), lead=ts.lead, amp=ts.ve_amp,
+                                   wave=wave, iamp=ts.iamp, i_wave=iw, record_every=10)
+    _, crec = oracle.extcell_run(np_ws(extcell_workspace(cs, dt)), hh_arr, dt, steps,
+                                 init_state(cs), lead=cs.lead, amp=cs.ve_amp, wave=wave,
+                                 iamp=cs.iamp, i_wave=iw, record_every=10)
+    assert np.abs(trec.reshape(trec.shape[0], -1) - crec).max() <= 1e-9
+    assert crec.max() > 0.0
+
+
+def test_shorted_tree_reduces_to_the_tree_oracle():
+    """Shorted layers on a branched tree: vi - ve0 follows the plain tree
+    oracle with the bath potential as its extracellular drive."""
+    from dendra_amd.extcell import extcell_tree_workspace
+    from dendra_amd.model import TreeSpec
+
+    rng = np.random.default_rng(22)
+    B, K, dt, steps = 2, 14, 0.005, 400
+    par = random_parents(K, rng)
+    ts, cs = tree_spec(rng, B, par, shorted=True)
+    wave = np.zeros(steps)
+    wave[40:200] = 1.0
+    iw = np.zeros(steps)
+    iw[220:300] = 1.0
+    hh_arr = HH(celsius=6.3).array()
+    _, trec = extcell_tree_ref.run(np_ws(extcell_tree_workspace(ts, dt)), par, hh_arr, dt,
+                                   steps, init_state(cs), lead=ts.lead, amp=ts.ve_amp,
+                                   wave=wave, iamp=ts.iamp, i_wave=iw, record_every=1)
+    tree = TreeSpec(np.arange(B + 1) * K, np.tile(par, B), ts.diam.ravel(), ts.length.ravel(),
+                    ts.ra.ravel(), ts.cm.ravel(), lead=ts.lead.ravel(), ve_amp=ts.ve_amp,
+                    iamp=ts.iamp.ravel())
+    _, orec = oracle.tree_run(tree, hh_arr, dt, steps, ve_wave=wave, i_wave=iw, record_every=1)
+    assert np.abs(trec.reshape(steps, -1) - orec).max() <= 1e-6
+    assert orec.max() > 0.0

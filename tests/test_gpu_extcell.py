@@ -91,3 +91,47 @@ def test_reset_state_round_trip_and_steps_continue():
     cells.run(4)
     cells.run(6)
     assert np.array_equal(cells.state()["vc"], a["vc"])
+
+
+@pytest.mark.parametrize("kind,K", [("random", 30), ("binary", 31), ("path", 9), ("single", 1)])
+def test_tree_float64_matches_dense_oracle(kind, K):
+    from dendra_amd.extcell import ExtCellTrees
+    from dendra_amd.model import HH
+    from test_extcell_ref import extcell_tree_ref, random_parents, tree_spec
+
+    rng = np.random.default_rng(50 + K)
+    par = {"random": lambda: random_parents(K, rng),
+           "binary": lambda: [-1] + [(i - 1) // 2 for i in range(1, K)],
+           "path": lambda: list(range(-1, K - 1)), "single": lambda: [-1]}[kind]()
+    B, dt, steps = 3, 0.005, 150
+    ts, cs = tree_spec(rng, B, par)
+    wave, iw = drive(steps)
+    hh = HH(celsius=6.3)
+    trees = ExtCellTrees(ts, dt, hh, torch.float64, "cuda")
+    rec = trees.run(steps, wave=wave, i_wave=iw, record_every=5).cpu().numpy()
+    st, orec = extcell_tree_ref.run(np_ws(trees.workspace), par, hh.array(), dt, steps,
+                                    init_state(cs), lead=ts.lead, amp=ts.ve_amp, wave=wave,
+                                    iamp=ts.iamp, i_wave=iw, record_every=5)
+    got = trees.state()
+    assert np.abs(rec - orec).max() <= 1e-8
+    assert np.abs(got["vc"] - st["vc"]).max() <= 1e-8
+    for g in ("m", "h", "n"):
+        assert np.abs(got[g] - st[g]).max() <= 1e-10
+
+
+def test_path_tree_equals_cable_on_the_gpu():
+    from dendra_amd.extcell import ExtCellCables, ExtCellTrees
+    from dendra_amd.model import HH
+    from test_extcell_ref import tree_spec
+
+    rng = np.random.default_rng(60)
+    B, K, dt, steps = 40, 45, 0.005, 500
+    ts, cs = tree_spec(rng, B, list(range(-1, K - 1)))
+    wave, iw = drive(steps)
+    hh = HH(celsius=6.3)
+    t = ExtCellTrees(ts, dt, hh, torch.float64, "cuda").run(steps, wave=wave, i_wave=iw,
+                                                            record_every=10)
+    c = ExtCellCables(cs, dt, hh, torch.float64, "cuda").run(steps, wave=wave, i_wave=iw,
+                                                             record_every=10)
+    assert torch.max(torch.abs(t - c)).item() <= 1e-8
+    assert c.max().item() > 0.0
