@@ -649,7 +649,48 @@ def operator_bench(dev):
                                    "appended_per_step": per_step, "targets": width,
                                    **_rate(ms, nbytes)}
     res["extcell_cables"] = extcell_bench(dev)
+    res["extcell_trees"] = extcell_tree_bench(dev)
     return res
+
+
+def extcell_tree_bench(dev, B=1024, n_steps=1000, dt=0.005):
+    """ExtCell trees: B copies of one 1000-node random binary tree (per-copy
+    diameters and stimulus), float32, per step an assembly launch, the block
+    tree solve and the update.  Algorithmic bytes 112 B per node-step: state
+    vc and gates read and written (48 B), the prepared radial block (9),
+    conductances to the parent (3), area, radial conductances (2) and the
+    lead field read (64 B)."""
+    import torch
+
+    from dendra_amd.extcell import ExtCellTrees, ExtCellTreeSpec
+    from dendra_amd.model import HH
+    from dendra_amd.workloads import random_binary_tree
+
+    rng = np.random.default_rng(23)
+    par, diam0, pos = random_binary_tree(1000, 0, 13)
+    K = len(par)
+    lead = 300.0 / (4 * np.pi * 1e-4 * np.hypot(
+        np.linalg.norm(pos - pos.mean(0), axis=1)[None, :], rng.uniform(200.0, 1500.0, (B, 1))))
+    spec = ExtCellTreeSpec(parent=par, diam=np.tile(diam0, (B, 1)) * rng.uniform(0.8, 1.2, (B, 1)),
+                           length=np.full((B, K), 10.0), ra=np.full((B, K), 100.0),
+                           cm=np.ones((B, K)), xraxial=np.full((B, K, 2), 1e5),
+                           xc=np.zeros((B, K, 2)), xg=np.full((B, K, 2), 10.0), lead=lead,
+                           ve_amp=rng.uniform(-1.0, -0.05, B) * 1e-3)
+    wave = np.zeros(n_steps)
+    wave[200:220] = 1.0
+    trees = ExtCellTrees(spec, dt, HH(celsius=6.3), torch.float32, dev)
+    trees.run(20, wave=wave)
+    torch.cuda.synchronize()
+    trees.reset()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    trees.run(n_steps, wave=wave)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop)
+    cs = B * K * n_steps
+    return {"trees": B, "nodes": K, "time_steps": n_steps,
+            "node_steps_per_s": cs / (ms * 1e-3), **_rate(ms, 112.0 * cs)}
 
 
 def _extcell_spec(B, K, seed):
