@@ -315,38 +315,77 @@ __global__ __launch_bounds__(256) void bt3_tree_kernel(const T* __restrict__ mai
 // then top-down x_i = Ainv_i (b_i + e_i * x_p).  work: [B][K][12] = Ainv (9)
 // and the eliminated right-hand side (3) per node.
 template <typename T>
-__global__ __launch_bounds__(64) void bt3_tree_layer_kernel(
+__global__ __launch_bounds__(256) void bt3_tree_layer_kernel(
     const T* __restrict__ main, const T* __restrict__ edge, const T* __restrict__ rhs,
     const int64_t* __restrict__ parent, const int64_t* __restrict__ order,
     const int64_t* __restrict__ layer_ptr, int n_layers, const int32_t* __restrict__ child_ptr,
     const int32_t* __restrict__ child_idx, T* __restrict__ x, T* __restrict__ work,
-    int64_t K) {
-  const int64_t s = blockIdx.x;
-  const T* M = main + s * K * 9;
-  const T* E = edge + s * K * 3;
-  const T* R = rhs + s * K * 3;
-  T* X = x + s * K * 3;
-  T* W = work + s * K * 12;
-  auto span = [&](int l, int64_t& lo, int64_t& hi) {
-    lo = layer_ptr[l];
-    hi = layer_ptr[l + 1];
-    lo = lo < 0 ? 0 : (lo > K ? K : lo);
-    hi = hi < lo ? lo : (hi > K ? K : hi);
+    int64_t B, int64_t K, int topo_lds) {
+  // G systems per workgroup (one wavefront each) walk the same layers, so the
+  // workgroup barriers line up; the shared topology is staged once in LDS as
+  // int32 (order, layer bounds, children CSR, parent) when it fits.
+  extern __shared__ __align__(16) unsigned char dn_smem_raw[];
+  const int lane = threadIdx.x & 63;
+  const int G = blockDim.x >> 6;
+  const int64_t s = (int64_t)blockIdx.x * G + (threadIdx.x >> 6);
+  const bool live = s < B;
+  int32_t* t_order = reinterpret_cast<int32_t*>(dn_smem_raw);
+  int32_t* t_par = t_order + K;
+  int32_t* t_cp = t_par + K;
+  int32_t* t_lp = t_cp + K + 1;
+  int32_t* t_ci = t_lp + n_layers + 1;
+  if (topo_lds) {
+    for (int64_t j = threadIdx.x; j < K; j += blockDim.x) {
+      const int64_t o = order[j], p = parent[j];
+      t_order[j] = (o >= 0 && o < K) ? (int32_t)o : -1;
+      t_par[j] = (p >= 0 && p < K) ? (int32_t)p : -1;
+    }
+    for (int64_t j = threadIdx.x; j <= K; j += blockDim.x) t_cp[j] = child_ptr[j];
+    for (int64_t j = threadIdx.x; j <= n_layers; j += blockDim.x) {
+      const int64_t v = layer_ptr[j];
+      t_lp[j] = (int32_t)(v < 0 ? 0 : (v > K ? K : v));
+    }
+    const int32_t nc = child_ptr[K];
+    for (int64_t j = threadIdx.x; j < nc; j += blockDim.x) t_ci[j] = child_idx[j];
+    __syncthreads();
+  }
+  auto bound = [&](int l) -> int64_t {
+    if (topo_lds) return t_lp[l];
+    const int64_t v = layer_ptr[l];
+    return v < 0 ? 0 : (v > K ? K : v);
   };
+  auto node = [&](int64_t n) -> int64_t {
+    if (topo_lds) return t_order[n];
+    const int64_t i = order[n];
+    return (i >= 0 && i < K) ? i : -1;
+  };
+  auto par = [&](int64_t i) -> int64_t {
+    if (topo_lds) return t_par[i];
+    const int64_t p = parent[i];
+    return (p >= 0 && p < K) ? p : -1;
+  };
+  const int64_t so = live ? s : 0;
+  const T* M = main + so * K * 9;
+  const T* E = edge + so * K * 3;
+  const T* R = rhs + so * K * 3;
+  T* X = x + so * K * 3;
+  T* W = work + so * K * 12;
   for (int l = 0; l < n_layers; ++l) {
-    int64_t lo, hi;
-    span(l, lo, hi);
-    for (int64_t n = lo + threadIdx.x; n < hi; n += 64) {
-      const int64_t i = order[n];
-      if (i < 0 || i >= K) continue;
+    int64_t lo = bound(l), hi = bound(l + 1);
+    if (hi < lo) hi = lo;
+    for (int64_t n = lo + lane; live && n < hi; n += 64) {
+      const int64_t i = node(n);
+      if (i < 0) continue;
       T a[9], b[3];
       for (int c = 0; c < 9; ++c) a[c] = M[i * 9 + c];
       for (int c = 0; c < 3; ++c) {
         a[c * 4] += E[i * 3 + c];
         b[c] = R[i * 3 + c];
       }
-      for (int j = child_ptr[i]; j < child_ptr[i + 1]; ++j) {
-        const int64_t ch = child_idx[j];
+      const int32_t c0 = topo_lds ? t_cp[i] : child_ptr[i];
+      const int32_t c1 = topo_lds ? t_cp[i + 1] : child_ptr[i + 1];
+      for (int32_t j = c0; j < c1; ++j) {
+        const int64_t ch = topo_lds ? t_ci[j] : child_idx[j];
         const T* Wc = W + ch * 12;
         const T e[3] = {E[ch * 3], E[ch * 3 + 1], E[ch * 3 + 2]};
         T iv[9], y[3];
@@ -366,15 +405,15 @@ __global__ __launch_bounds__(64) void bt3_tree_layer_kernel(
     __syncthreads();
   }
   for (int l = n_layers - 1; l >= 0; --l) {
-    int64_t lo, hi;
-    span(l, lo, hi);
-    for (int64_t n = lo + threadIdx.x; n < hi; n += 64) {
-      const int64_t i = order[n];
-      if (i < 0 || i >= K) continue;
-      const int64_t p = parent[i];
+    int64_t lo = bound(l), hi = bound(l + 1);
+    if (hi < lo) hi = lo;
+    for (int64_t n = lo + lane; live && n < hi; n += 64) {
+      const int64_t i = node(n);
+      if (i < 0) continue;
+      const int64_t p = par(i);
       T b[3], xi[3];
       for (int c = 0; c < 3; ++c)
-        b[c] = W[i * 12 + 9 + c] + ((p >= 0 && p < K) ? E[i * 3 + c] * X[p * 3 + c] : T(0));
+        b[c] = W[i * 12 + 9 + c] + (p >= 0 ? E[i * 3 + c] * X[p * 3 + c] : T(0));
       mv3(W + i * 12, b, xi);
       for (int c = 0; c < 3; ++c) X[i * 3 + c] = xi[c];
     }
@@ -435,9 +474,14 @@ static int bt3_tree_impl(const T* main, const T* edge, const T* rhs, const int64
   // children CSR is given; else one lane per system walks `order`.
   if (layer_ptr && child_ptr && child_idx && n_layers >= 1 && 2 * n_layers <= K &&
       B <= INT32_MAX) {
-    hipLaunchKernelGGL((dn::bt3_tree_layer_kernel<T>), dim3((unsigned)B), dim3(64), 0, stream,
-                       main, edge, rhs, parent, order, layer_ptr, (int)n_layers, child_ptr,
-                       child_idx, x, work, K);
+    // systems per workgroup: enough workgroups (~512) to fill the chip first
+    const int G = B >= 2048 ? 4 : (B >= 1024 ? 2 : 1);
+    const size_t topo = (size_t)4 * (3 * K + 1 + n_layers + 1 + (K > 0 ? K : 1));
+    const int topo_lds = topo <= 96 * 1024;
+    hipLaunchKernelGGL((dn::bt3_tree_layer_kernel<T>), dim3((unsigned)((B + G - 1) / G)),
+                       dim3(64 * G), topo_lds ? topo : 0, stream, main, edge, rhs, parent,
+                       order, layer_ptr, (int)n_layers, child_ptr, child_idx, x, work, B, K,
+                       topo_lds);
   } else {
     hipLaunchKernelGGL((dn::bt3_tree_kernel<T>), dim3(blocks_for<T>(B)), dim3(256), 0,
                        stream, main, edge, rhs, parent, order, x, work, B, K);
