@@ -80,48 +80,60 @@ struct NetArgs {
   const uint8_t* tape;     // delivery-only mode: [n_steps][n_src] source spikes
 };
 
+// Rising-edge detection of source i at this step (cells on the voltage
+// before their update, Poisson sources by the counter RNG, or a tape); a
+// spike sets the source's bit and appends it to the step's list.
+__device__ __forceinline__ void net_detect(const NetArgs& a, int i, int step) {
+  bool spike;
+  if (a.tape) {
+    spike = a.tape[(int64_t)step * a.n_src + i] != 0;
+  } else if (i < a.n_cells) {
+    const bool above = a.v[a.soma[i]] >= a.theta;
+    spike = above && !a.latch[i];
+    a.latch[i] = above;
+  } else {
+    spike = uniform01(a.seed, a.bg_stream,
+                      (uint64_t)step * (uint64_t)(a.n_src - a.n_cells) +
+                          (uint64_t)(i - a.n_cells)) < a.bg_p;
+  }
+  if (spike) {
+    atomicOr(&a.spk_bits[i >> 5], 1u << (i & 31));
+    const int k = atomicAdd(&a.ctr[2], 1);
+    a.spk_list[k] = i;
+    if (a.ev_log) {
+      const int q = atomicAdd(a.ev_count, 1);
+      if (q < a.ev_cap) {
+        a.ev_log[2 * (int64_t)q] = step;
+        a.ev_log[2 * (int64_t)q + 1] = i;
+      }
+    }
+  }
+}
+
+// Delivery of ring row cur to target i: the payload is read, the cell
+// cleared and (optionally) recorded; returns the payload.
+__device__ __forceinline__ float net_take(const NetArgs& a, int i, int cur, int step) {
+  float* row = a.ring + (int64_t)cur * a.n_cells;
+  const float pay = row[i];
+  row[i] = 0.0f;
+  if (a.delivered && step < a.rec_steps) a.delivered[(int64_t)step * a.n_cells + i] = pay;
+  return pay;
+}
+
 // Ka: deliver ring row cur to the synapses and detect this step's sources.
 __global__ __launch_bounds__(256) void net_deliver_detect(NetArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int cur = a.ctr[0];
   const int step = a.ctr[1];
   if (i < a.n_cells) {
-    float* row = a.ring + (int64_t)cur * a.n_cells;
-    const float pay = row[i];
-    row[i] = 0.0f;
-    if (a.delivered && step < a.rec_steps) a.delivered[(int64_t)step * a.n_cells + i] = pay;
+    const float pay = net_take(a, i, cur, step);
     if (a.A) {
       const float wf = pay * a.factor;
       a.A[i] = a.A[i] + wf;
       a.B[i] = a.B[i] + wf;
     }
   }
-  if (i < a.n_src) {
-    bool spike;
-    if (a.tape) {
-      spike = a.tape[(int64_t)step * a.n_src + i] != 0;
-    } else if (i < a.n_cells) {
-      const bool above = a.v[a.soma[i]] >= a.theta;
-      spike = above && !a.latch[i];
-      a.latch[i] = above;
-    } else {
-      spike = uniform01(a.seed, a.bg_stream,
-                        (uint64_t)step * (uint64_t)(a.n_src - a.n_cells) +
-                            (uint64_t)(i - a.n_cells)) < a.bg_p;
-    }
-    if (spike) {
-      atomicOr(&a.spk_bits[i >> 5], 1u << (i & 31));
-      const int k = atomicAdd(&a.ctr[2], 1);
-      a.spk_list[k] = i;
-      if (a.ev_log) {
-        const int q = atomicAdd(a.ev_count, 1);
-        if (q < a.ev_cap) {
-          a.ev_log[2 * (int64_t)q] = step;
-          a.ev_log[2 * (int64_t)q + 1] = i;
-        }
-      }
-    }
-  }
+  if (i < a.n_src) net_detect(a, i, step);
 }
 
 // Kb: every spiking source marks its targets; the first marker appends the
@@ -142,7 +154,7 @@ __global__ __launch_bounds__(256) void net_mark(NetArgs a) {
 
 // Kc: one wavefront per active target scans its incoming segment in
 // connection order and adds the hits sequentially (lane 0).
-__global__ __launch_bounds__(256) void net_enqueue(NetArgs a) {
+__device__ __forceinline__ void net_enqueue_body(const NetArgs& a) {
   const int lane = threadIdx.x & 63;
   const int wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int n_wv = (gridDim.x * blockDim.x) >> 6;
@@ -175,35 +187,87 @@ __global__ __launch_bounds__(256) void net_enqueue(NetArgs a) {
   }
 }
 
-// Kd (point cells, K = 1): synapse decay, HH, closed-form implicit update.
-__global__ __launch_bounds__(256) void net_point_step(NetArgs a, int c0, int c1) {
-  const int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= c1) return;
-  const int64_t i = a.off[c];
-  const float A = a.A[c] * a.decay1, B = a.B[c] * a.decay2;
-  a.A[c] = A;
-  a.B[c] = B;
-  const float v = a.v[i];
-  float m = a.m[i], h = a.h[i], n = a.n[i], G, J;
-  hh_advance<float>(v, a.dt, a.hh, a.sna ? a.sna[i] : 1.0f, a.sk ? a.sk[i] : 1.0f, m, h,
-                    n, G, J);
-  a.m[i] = m; a.h[i] = h; a.n[i] = n;
-  const float gs = (B - A) * 1e-6f / a.area[i];  // uS -> S/cm^2
-  G += gs;
-  J += gs * a.e_syn;
-  const float sc = a.scale[i];
-  const float dts = a.dt * 1e-3f;
-  const float f = (J - G * v) * sc;
-  a.v[i] = v + dts * f / (1.0f + dts * G * sc);
+__global__ __launch_bounds__(256) void net_enqueue(NetArgs a) { net_enqueue_body(a); }
+
+// Ke: advance the step counters and clear the step's spike bits.
+__device__ __forceinline__ void net_advance_body(const NetArgs& a) {
+  const int ns = a.ctr[2];
+  for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+    const int s = a.spk_list[k];
+    a.spk_bits[s >> 5] = 0u;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int c = a.ctr[0] + 1;
+    a.ctr[0] = c >= a.depth ? 0 : c;
+    a.ctr[1] += 1;
+    a.ctr[2] = 0;
+    a.ctr[3] = 0;
+  }
 }
 
-// Kd (multi-compartment cells, K <= 64 * NR): one wavefront per cell,
-// synapse at the soma (compartment 0), partition solve as in dn_cable.hip.
+__global__ void net_advance(NetArgs a) { net_advance_body(a); }
+
+// Kc + Ke: the enqueue, then the workgroup that finishes last (a counter in
+// ctr[4] that each workgroup increments once all its threads are done; no
+// workgroup waits on it) advances the counters and clears the spike bits.
+// The last workgroup reads only what earlier launches wrote (the spike list
+// and counts), every other workgroup has read the counters before its
+// increment, and the ring and flag writes reach later launches through the
+// launch boundary, so no device-wide fence is needed (one per workgroup
+// would write back the L2 of its XCD).
+__global__ __launch_bounds__(256) void net_enqueue_advance(NetArgs a) {
+  net_enqueue_body(a);
+  __shared__ int last;
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(&a.ctr[4], 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  net_advance_body(a);
+  if (threadIdx.x == 0) a.ctr[4] = 0;
+}
+
+// Ka + Kd: one launch delivers today's ring row to every synapse, detects
+// every source on the voltage before its update, and advances the cells.
+// Blocks [0, pb) own the point cells (one thread each), blocks [pb, pb + cb)
+// the multi-compartment cells (one wavefront each), the rest the Poisson
+// sources.  The synapse update is the two kernels' arithmetic: A = (A + w f)
+// decay1, B = (B + w f) decay2.
 template <int NR>
-__global__ __launch_bounds__(256) void net_cable_step(NetArgs a, int c0, int c1) {
+__global__ __launch_bounds__(256) void net_step_cells(NetArgs a, int n_point, int pb, int cb) {
+  const int cur = a.ctr[0];
+  const int step = a.ctr[1];
+  if ((int)blockIdx.x < pb) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_point) return;
+    const float wf = net_take(a, c, cur, step) * a.factor;
+    net_detect(a, c, step);
+    const int64_t i = a.off[c];
+    const float A = (a.A[c] + wf) * a.decay1, B = (a.B[c] + wf) * a.decay2;
+    a.A[c] = A;
+    a.B[c] = B;
+    const float v = a.v[i];
+    float m = a.m[i], h = a.h[i], n = a.n[i], G, J;
+    hh_advance<float>(v, a.dt, a.hh, a.sna ? a.sna[i] : 1.0f, a.sk ? a.sk[i] : 1.0f, m, h,
+                      n, G, J);
+    a.m[i] = m; a.h[i] = h; a.n[i] = n;
+    const float gs = (B - A) * 1e-6f / a.area[i];  // uS -> S/cm^2
+    G += gs;
+    J += gs * a.e_syn;
+    const float sc = a.scale[i];
+    const float dts = a.dt * 1e-3f;
+    const float f = (J - G * v) * sc;
+    a.v[i] = v + dts * f / (1.0f + dts * G * sc);
+    return;
+  }
+  if ((int)blockIdx.x >= pb + cb) {
+    const int i = a.n_cells + (blockIdx.x - pb - cb) * blockDim.x + threadIdx.x;
+    if (i < a.n_src) net_detect(a, i, step);
+    return;
+  }
   const int lane = threadIdx.x & 63;
-  const int c = c0 + ((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (c >= c1) return;
+  const int c = n_point + (((blockIdx.x - pb) * blockDim.x + threadIdx.x) >> 6);
+  if (c >= a.n_cells) return;
   const int64_t o = a.off[c];
   const int K = (int)(a.off[c + 1] - o);
   const int nr = (K + 63) / 64;
@@ -213,8 +277,10 @@ __global__ __launch_bounds__(256) void net_cable_step(NetArgs a, int c0, int c1)
   const float dts = a.dt * 1e-3f;
   float As = 0.0f, Bs = 0.0f;
   if (lane == 0) {
-    As = a.A[c] * a.decay1;
-    Bs = a.B[c] * a.decay2;
+    const float wf = net_take(a, c, cur, step) * a.factor;
+    net_detect(a, c, step);  // before any lane writes the cell's voltage
+    As = (a.A[c] + wf) * a.decay1;
+    Bs = (a.B[c] + wf) * a.decay2;
     a.A[c] = As;
     a.B[c] = Bs;
   }
@@ -251,38 +317,20 @@ __global__ __launch_bounds__(256) void net_cable_step(NetArgs a, int c0, int c1)
   for (int j = lane; j < K; j += 64) a.v[o + j] += sd[j];
 }
 
-// Ke: advance the step counters and clear the step's spike bits.
-__global__ void net_advance(NetArgs a) {
-  const int ns = a.ctr[2];
-  for (int k = threadIdx.x; k < ns; k += blockDim.x) {
-    const int s = a.spk_list[k];
-    a.spk_bits[s >> 5] = 0u;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int c = a.ctr[0] + 1;
-    a.ctr[0] = c >= a.depth ? 0 : c;
-    a.ctr[1] += 1;
-    a.ctr[2] = 0;
-    a.ctr[3] = 0;
-  }
-}
-
 constexpr int kNetCableNR = 4;  // multi-compartment cells up to 256 compartments
 
+// Three launches per step: cells and sources (Ka + Kd), marking (Kb), and
+// the enqueue with the counter advance (Kc + Ke).  The cells' update writes
+// only cell state, which marking and enqueueing never read, so it may run
+// before them.
 static void launch_step(const NetArgs& a, int n_point, int n_cable, int mark_blocks,
                         hipStream_t st) {
-  const int nsrc_blocks = (a.n_src + 255) / 256;
-  hipLaunchKernelGGL(net_deliver_detect, dim3(nsrc_blocks), dim3(256), 0, st, a);
+  const int pb = (n_point + 255) / 256, cb = (n_cable + 3) / 4;
+  const int sb = (a.n_src - a.n_cells + 255) / 256;
+  hipLaunchKernelGGL(net_step_cells<kNetCableNR>, dim3(pb + cb + sb), dim3(256), 0, st, a,
+                     n_point, pb, cb);
   hipLaunchKernelGGL(net_mark, dim3(mark_blocks), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(net_enqueue, dim3(mark_blocks), dim3(256), 0, st, a);
-  if (n_point > 0)
-    hipLaunchKernelGGL(net_point_step, dim3((n_point + 255) / 256), dim3(256), 0, st, a, 0,
-                       n_point);
-  if (n_cable > 0)
-    hipLaunchKernelGGL(net_cable_step<kNetCableNR>, dim3((n_cable + 3) / 4), dim3(256), 0,
-                       st, a, n_point, n_point + n_cable);
-  hipLaunchKernelGGL(net_advance, dim3(1), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(net_enqueue_advance, dim3(mark_blocks), dim3(256), 0, st, a);
 }
 
 }  // namespace dn
@@ -290,7 +338,7 @@ static void launch_step(const NetArgs& a, int n_point, int n_cable, int mark_blo
 // Advance the network n_steps.  Cells [0, n_point) are point cells (K = 1),
 // cells [n_point, n_cells) are multi-compartment (K <= 256).  hh (host):
 // {gnabar, gkbar, gl, ena, ek, el, q10}; syn (host): {factor, tau1_eff,
-// tau2, e}.  ctr holds {cur, step, 0, 0} on entry.  graph_steps > 0 captures
+// tau2, e}.  ctr (8 ints) holds {cur, step, 0, 0, 0, ...} on entry.  graph_steps > 0 captures
 // that many steps once as a HIP graph and replays it (n_steps must be a
 // multiple); 0 launches every kernel directly.
 DN_EXPORT int dn_net_run_f32(

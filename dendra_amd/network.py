@@ -85,7 +85,7 @@ class _Delivery:
         self.spk_list = torch.zeros(conn.n_src, dtype=torch.int32, device=device)
         self.tgt_flag = torch.zeros(conn.n_tgt, dtype=torch.int32, device=device)
         self.tgt_list = torch.zeros(conn.n_tgt, dtype=torch.int32, device=device)
-        self.ctr = torch.zeros(4, dtype=torch.int32, device=device)
+        self.ctr = torch.zeros(8, dtype=torch.int32, device=device)
 
 
 def deliver_tape(conn: Connectivity, tape, device="cuda") -> np.ndarray:

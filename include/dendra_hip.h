@@ -502,8 +502,9 @@ int dn_net_deliver_tape_f32(int n_targets, int n_src, const int32_t* out_ptr,
  * with p = bg_p per step (splitmix64 counter RNG of seed, bg_stream).
  * syn (host) = {factor, tau1_eff, tau2, e}.  Optional records: ev_log
  * (step, source) pairs up to ev_cap, delivered payloads for the first
- * rec_steps steps.  graph_steps > 0 captures that many steps as one HIP graph
- * and replays it (stream must not be the null stream). */
+ * rec_steps steps.  ctr: 8 int32 {cur, step, 0, ...}.  graph_steps > 0
+ * captures that many steps as one HIP graph and replays it (stream must not
+ * be the null stream). */
 int dn_net_run_f32(int n_cells, int n_point, const int64_t* off, const int32_t* soma,
                    float* v, float* m, float* h, float* n, const float* gL,
                    const float* gRp, const float* scale, const float* sna,
