@@ -14,7 +14,9 @@
 // maind, lower, upper, area, xg are the prepared workspace of
 // _prepare_workspace (implicit.py:800-872).
 //
-// One lane owns one cell for all steps (no synchronisation between lanes).
+// One lane owns one cell for all steps (no synchronisation between lanes),
+// or, for two or more compartments, two adjacent lanes share it and
+// eliminate from both ends (extcell_pair_kernel below).
 // Every array is laid out [compartment][component][cell], so the accesses of
 // a wavefront are coalesced; the forward sweep keeps the previous block's
 // inverse in registers, prefetches the next compartment's inputs, and stores
@@ -172,6 +174,191 @@ __global__ __launch_bounds__(256) void extcell_run_kernel(XcArgs<T> a) {
   }
 }
 
+// Two lanes per cell (adjacent lanes 2b, 2b + 1): elimination from both
+// ends.  The "down" lane eliminates compartments 0 .. s-1 top-down as above,
+// the "up" lane compartments K-1 .. s bottom-up,
+//   A''_k = A_k - diag(u_k) A''^-1_{k+1} diag(l_k),
+//   R''_k = R_k - diag(u_k) A''^-1_{k+1} R''_{k+1}.
+// At the interface, with y_d = A'^-1_{s-1} R'_{s-1}, C_d = A'^-1_{s-1} diag(u),
+// y_u = A''^-1_s R''_s, C_u = A''^-1_s diag(l) (u, l the couplings of edge
+// (s-1, s)), both lanes solve (I - C_d C_u) x_{s-1} = y_d - C_d y_u and
+// x_s = y_u - C_u x_{s-1} from the exchanged values, then back-substitute
+// outward.  Halves the sequential chain and doubles the lanes.
+template <typename T>
+__device__ __forceinline__ void xc_assemble(const XcArgs<T>& a, int64_t k, int64_t b,
+                                            const HHParams<T>& hp, T bath_s, T intra_s, T* A,
+                                            T* R) {
+  const int64_t B = a.B, K = a.K;
+  T vk[3], vp[3], vn[3], lp[3], ln[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    vk[c] = a.vc[(k * 3 + c) * B + b];
+    vp[c] = k > 0 ? a.vc[((k - 1) * 3 + c) * B + b] : vk[c];
+    vn[c] = k + 1 < K ? a.vc[((k + 1) * 3 + c) * B + b] : vk[c];
+    lp[c] = k > 0 ? a.lower[((k - 1) * 3 + c) * B + b] : T(0);
+    ln[c] = k + 1 < K ? a.lower[(k * 3 + c) * B + b] : T(0);
+  }
+  const int64_t i = k * B + b;
+  const T v = vk[0] - vk[1];
+  T m = a.m[i], h = a.h[i], n = a.n[i], G, J;
+  hh_advance<T>(v, a.dt, hp, a.sna ? a.sna[i] : T(1), a.sk ? a.sk[i] : T(1), m, h, n, G, J);
+  a.m[i] = m;
+  a.h[i] = h;
+  a.n[i] = n;
+  const T area = a.area[i];
+  const T g = G * area;
+  const T d = (J - G * v) * area + (a.iamp ? a.iamp[i] : T(0)) * intra_s;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) A[c] = a.maind[(k * 9 + c) * B + b];
+  A[0] += g;
+  A[1] -= g;
+  A[3] -= g;
+  A[4] += g;
+  const T xg0 = a.xg[(k * 2) * B + b], xg1 = a.xg[(k * 2 + 1) * B + b];
+  const T lead = a.lead ? a.lead[i] : T(0);
+  const T shell = xg0 * (vk[1] - vk[2]);
+  const T bathf = xg1 * (lead * bath_s - vk[2]);
+  R[0] = d;
+  R[1] = -d - shell;
+  R[2] = shell + bathf;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T f = -ln[c] * (vn[c] - vk[c]);
+    const T fprev = -lp[c] * (vk[c] - vp[c]);
+    R[c] += f - fprev;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void extcell_pair_kernel(XcArgs<T> a) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t b = g >> 1;
+  if (b >= a.B) return;  // both lanes of a pair leave together
+  const bool up = g & 1;
+  const int64_t B = a.B, K = a.K, sp = K / 2;
+  const HHParams<T> hp{a.hh[0], a.hh[1], a.hh[2], a.hh[3], a.hh[4], a.hh[5], a.hh[6]};
+  const T amp = (a.amp && a.wave && a.lead) ? a.amp[b] : T(0);
+  T* W = a.work;
+  for (int64_t s = 0; s < a.n_steps; ++s) {
+    const T bath_s = (a.amp && a.wave && a.lead) ? a.wave[s] * amp : T(0);
+    const T intra_s = (a.iamp && a.iwave) ? a.iwave[s] : T(0);
+    T Ainv[9], Rp[3], cprev[3];  // previous block's inverse, rhs, coupling pair
+    const int64_t n_own = up ? K - sp : sp;
+    for (int64_t t = 0; t < n_own; ++t) {
+      const int64_t k = up ? K - 1 - t : t;
+      T A[9], R[3];
+      xc_assemble(a, k, b, hp, bath_s, intra_s, A, R);
+      if (t > 0) {
+        // down: couplings (k, k-1) = diag(l_{k-1}), (k-1, k) = diag(u_{k-1});
+        // up:   couplings (k, k+1) = diag(u_k),    (k+1, k) = diag(l_k)
+        T cr[3], cc[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const int64_t e = up ? k : k - 1;
+          cr[c] = up ? a.upper[(e * 3 + c) * B + b] : a.lower[(e * 3 + c) * B + b];
+          cc[c] = cprev[c];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) A[r * 3 + c] -= cr[r] * Ainv[r * 3 + c] * cc[c];
+          R[r] -= cr[r] * (Ainv[r * 3] * Rp[0] + Ainv[r * 3 + 1] * Rp[1] + Ainv[r * 3 + 2] * Rp[2]);
+        }
+      }
+      xc_inv3(A, Ainv);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) W[(k * 12 + c) * B + b] = Ainv[c];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        W[(k * 12 + 9 + c) * B + b] = R[c];
+        Rp[c] = R[c];
+        // the coupling the next eliminated block multiplies from the right:
+        // down: diag(u_k) of edge (k, k+1); up: diag(l_{k-1}) of edge (k-1, k)
+        const int64_t e = up ? k - 1 : k;
+        cprev[c] = (e >= 0 && e + 1 < K)
+                       ? (up ? a.lower[(e * 3 + c) * B + b] : a.upper[(e * 3 + c) * B + b])
+                       : T(0);
+      }
+    }
+    // interface: exchange (Ainv, R') of compartments s-1 (down) and s (up)
+    T Ao[9], Ro[3];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) Ao[c] = __shfl_xor(Ainv[c], 1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Ro[c] = __shfl_xor(Rp[c], 1);
+    const T* Ad = up ? Ao : Ainv;
+    const T* Rd = up ? Ro : Rp;
+    const T* Au = up ? Ainv : Ao;
+    const T* Ru = up ? Rp : Ro;
+    T uu[3], ll[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      uu[c] = a.upper[((sp - 1) * 3 + c) * B + b];
+      ll[c] = a.lower[((sp - 1) * 3 + c) * B + b];
+    }
+    T yd[3], yu[3], Cd[9], Cu[9], M[9], Mi[9], rhs[3], xd[3], xu[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      yd[r] = Ad[r * 3] * Rd[0] + Ad[r * 3 + 1] * Rd[1] + Ad[r * 3 + 2] * Rd[2];
+      yu[r] = Au[r * 3] * Ru[0] + Au[r * 3 + 1] * Ru[1] + Au[r * 3 + 2] * Ru[2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        Cd[r * 3 + c] = Ad[r * 3 + c] * uu[c];
+        Cu[r * 3 + c] = Au[r * 3 + c] * ll[c];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        M[r * 3 + c] = (r == c ? T(1) : T(0)) -
+                       (Cd[r * 3] * Cu[c] + Cd[r * 3 + 1] * Cu[3 + c] + Cd[r * 3 + 2] * Cu[6 + c]);
+      rhs[r] = yd[r] - (Cd[r * 3] * yu[0] + Cd[r * 3 + 1] * yu[1] + Cd[r * 3 + 2] * yu[2]);
+    }
+    xc_inv3(M, Mi);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) xd[r] = Mi[r * 3] * rhs[0] + Mi[r * 3 + 1] * rhs[1] + Mi[r * 3 + 2] * rhs[2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) xu[r] = yu[r] - (Cu[r * 3] * xd[0] + Cu[r * 3 + 1] * xd[1] + Cu[r * 3 + 2] * xd[2]);
+    // back-substitution outward from the interface
+    T xn[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xn[c] = up ? xu[c] : xd[c];
+    for (int64_t t = n_own - 1; t >= 0; --t) {
+      const int64_t k = up ? K - 1 - t : t;
+      T x[3];
+      if (t == n_own - 1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[c] = xn[c];
+      } else {
+        T Ai[9], r[3];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) Ai[c] = W[(k * 12 + c) * B + b];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const T cpl = up ? a.lower[((k - 1) * 3 + c) * B + b] : a.upper[(k * 3 + c) * B + b];
+          r[c] = W[(k * 12 + 9 + c) * B + b] - cpl * xn[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          x[c] = Ai[c * 3] * r[0] + Ai[c * 3 + 1] * r[1] + Ai[c * 3 + 2] * r[2];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        a.vc[(k * 3 + c) * B + b] += x[c];
+        xn[c] = x[c];
+      }
+    }
+    if (a.rec && a.rec_every > 0 && (s + 1) % a.rec_every == 0) {
+      const int64_t rr = (s + 1) / a.rec_every - 1;
+      for (int64_t t = 0; t < n_own; ++t) {
+        const int64_t k = up ? K - 1 - t : t;
+        a.rec[(rr * K + k) * B + b] = a.vc[(k * 3) * B + b] - a.vc[(k * 3 + 1) * B + b];
+      }
+    }
+  }
+}
+
 }  // namespace dn
 
 template <typename T>
@@ -183,8 +370,12 @@ static int extcell_impl(const dn::XcArgs<T>& a, int64_t work_elems, hipStream_t 
   if (work_elems < 12 * a.B * a.K) return DN_EINVAL;
   if (a.rec && a.rec_every <= 0) return DN_EINVAL;
   if (a.n_steps == 0) return DN_OK;
-  hipLaunchKernelGGL((dn::extcell_run_kernel<T>), dim3((unsigned)((a.B + 255) / 256)),
-                     dim3(256), 0, stream, a);
+  if (a.K >= 2 && a.B <= (int64_t)INT32_MAX * 128)
+    hipLaunchKernelGGL((dn::extcell_pair_kernel<T>), dim3((unsigned)((2 * a.B + 255) / 256)),
+                       dim3(256), 0, stream, a);
+  else
+    hipLaunchKernelGGL((dn::extcell_run_kernel<T>), dim3((unsigned)((a.B + 255) / 256)),
+                       dim3(256), 0, stream, a);
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
 

@@ -13,7 +13,8 @@ length, the double-layer extracellular model of the stimulation workflows:
   (vi, ve0, ve1) and the HH gates; ``run(n_steps, wave=, i_wave=)`` advances
   every cell by the reference's ``_step`` (gates at v = vi - ve0, membrane
   block and residual, axial fluxes, block tridiagonal solve of the
-  increment) in one HIP launch: one lane per cell for all steps.  The bath
+  increment) in one HIP launch for all steps (two lanes per cell, eliminating
+  from both ends).  The bath
   potential is ``lead * ve_amp * wave[s]`` (mV), the intracellular current
   ``iamp * i_wave[s]`` (mA).
 * branched trees (``_dhs_bt``, integrators/tree_bt.py:91-654):
