@@ -523,6 +523,18 @@ def operator_bench(dev):
     u = lambda shape, lo, hi: torch.rand(shape, device=dev, generator=gen) * (hi - lo) + lo
     res = {}
 
+    # batched tridiagonal drop-ins (thomas_solve_cuda_t / pcr_solve_cuda_t):
+    # 20 B per row (read a, b, c, d; write x)
+    for name, (B, K) in (("short", (65536, 201)), ("long", (2048, 4001))):
+        a, c = u((B, K - 1), -0.4, -0.1), u((B, K - 1), -0.4, -0.1)
+        b, d = u((B, K), 1.0, 2.0), u((B, K), -1.0, 1.0)
+        line = {"systems": B, "rows": K}
+        with torch.no_grad():
+            for meth in ("thomas", "pcr", "auto"):
+                ms = _time_op(lambda: solvers.solve_tridiagonal(a, b, c, d, meth))
+                line[meth] = _rate(ms, 20.0 * B * K)
+        res[f"tridiagonal_{name}"] = line
+
     B, K = 2048, 2000
     trees = {"binary_tree": np.asarray(random_binary_tree(K, 0, 11)[0]),
              "path": np.arange(-1, K - 1)}
