@@ -425,6 +425,9 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
   const T dt = A.dt;
   for (int64_t st = 0; st < A.n_steps; ++st) {
     const T w = A.wave ? A.wave[A.step0 + st] : T(0);
+    // the drive is reloaded only on steps whose waveform value is nonzero
+    // (w is the same for the whole workgroup)
+    const bool drw = drp && w != T(0);
     T vtail = v[0];
 #pragma unroll
     for (int r = 1; r < kSecMax; ++r)
@@ -440,7 +443,7 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
         hh_advance<T>(v[r], dt, A.hh, sna, sk, m[r], h[r], n[r], G, J);
         const T ar = arp[r];
         M[r] = cdp[r] + G * ar;
-        F[r] = (J - G * v[r]) * ar + (drp ? drp[r] * w : T(0));
+        F[r] = (J - G * v[r]) * ar + (drw ? drp[r] * w : T(0));
       }
     }
     __syncthreads();
