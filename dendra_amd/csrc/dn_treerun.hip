@@ -319,7 +319,8 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
   extern __shared__ unsigned char dn_smem_raw[];
   const int nt = blockDim.x;
   T* gs = reinterpret_cast<T*>(dn_smem_raw);
-  T *vh = gs + node_cap, *vt = vh + nt, *cD = vh + 2 * nt, *cB = vh + 3 * nt,
+  T* ars = gs + node_cap;  // per-node membrane area, staged once
+  T *vh = ars + node_cap, *vt = vh + nt, *cD = vh + 2 * nt, *cB = vh + 3 * nt,
     *xt = vh + 4 * nt, *gh = vh + 5 * nt;
   const int s = threadIdx.x;
   const int g0 = grp_off[blockIdx.x], g1 = grp_off[blockIdx.x + 1];
@@ -403,11 +404,13 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
     const int t = order[k];
     const int64_t to = A.pl.node_off[t];
     const int nn = (int)(A.pl.node_off[t + 1] - to);
-    for (int i = s; i < nn; i += nt) gs[node_base + i] = A.g[to + i];
+    for (int i = s; i < nn; i += nt) {
+      gs[node_base + i] = A.g[to + i];
+      ars[node_base + i] = A.area[to + i];
+    }
     node_base += nn;
   }
   const int gb = tree_node_base + hd;  // block-local index of the section head
-  const T* arp = A.area + no + hd;
   const T* cdp = A.cmdt + no + hd;
   const T* drp = A.drive ? A.drive + no + hd : nullptr;
   T v[kSecMax], m[kSecMax], h[kSecMax], n[kSecMax];
@@ -441,7 +444,7 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
         const T sk = SCALES && A.sk ? A.sk[no + hd + r] : T(1);
         T G, J;
         hh_advance<T>(v[r], dt, A.hh, sna, sk, m[r], h[r], n[r], G, J);
-        const T ar = arp[r];
+        const T ar = ars[gb + r];
         M[r] = cdp[r] + G * ar;
         F[r] = (J - G * v[r]) * ar + (drw ? drp[r] * w : T(0));
       }
@@ -553,7 +556,7 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
 
 __host__ __device__ constexpr size_t tree_sec_lds_bytes(int threads, int node_cap,
                                                         size_t elem) {
-  return ((size_t)node_cap + 6 * (size_t)threads) * elem;
+  return (2 * (size_t)node_cap + 6 * (size_t)threads) * elem;
 }
 
 // ---------------------------------------------------------------------------
