@@ -301,6 +301,8 @@ __global__ __launch_bounds__(512) void hh_tree_run_kernel(TreeRunArgs<T> A,
 // section levels, instead of every section chain on a root-to-leaf path.
 // LDS carries:
 //   gs      axial conductance of every node of the block's trees
+//   ars     membrane area of every node, and cms its cm/dt when node_cap
+//           leaves room for it (cm_lds; else cm/dt is read each step)
 //   vh/vt   head and tail voltage (axial currents across section joins)
 //   cD/cB   a section's contribution to its parent tail's self term and
 //           right-hand side (compressed chain + Schur term of its tail)
@@ -311,7 +313,7 @@ __global__ __launch_bounds__(512) void hh_tree_run_kernel(TreeRunArgs<T> A,
 // Four waves per SIMD (two 512-thread blocks per CU) beat three waves without
 // spills: the level phases are latency-bound, so residency matters more than
 // the dozen registers the bound spills.
-template <typename T, bool SCALES>
+template <typename T, bool SCALES, bool CM_LDS>
 __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
                                                           const int32_t* order,
                                                           const int32_t* grp_off,
@@ -320,8 +322,9 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
   const int nt = blockDim.x;
   T* gs = reinterpret_cast<T*>(dn_smem_raw);
   T* ars = gs + node_cap;  // per-node membrane area, staged once
-  T *vh = ars + node_cap, *vt = vh + nt, *cD = vh + 2 * nt, *cB = vh + 3 * nt,
-    *xt = vh + 4 * nt, *gh = vh + 5 * nt;
+  T* cms = ars + node_cap;  // per-node cm/dt when CM_LDS
+  T *vh = cms + (CM_LDS ? node_cap : 0), *vt = vh + nt, *cD = vh + 2 * nt,
+    *cB = vh + 3 * nt, *xt = vh + 4 * nt, *gh = vh + 5 * nt;
   const int s = threadIdx.x;
   const int g0 = grp_off[blockIdx.x], g1 = grp_off[blockIdx.x + 1];
   // Sections of the group's trees, numbered tree by tree in breadth-first
@@ -407,6 +410,7 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
     for (int i = s; i < nn; i += nt) {
       gs[node_base + i] = A.g[to + i];
       ars[node_base + i] = A.area[to + i];
+      if (CM_LDS) cms[node_base + i] = A.cmdt[to + i];
     }
     node_base += nn;
   }
@@ -445,7 +449,7 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
         T G, J;
         hh_advance<T>(v[r], dt, A.hh, sna, sk, m[r], h[r], n[r], G, J);
         const T ar = ars[gb + r];
-        M[r] = cdp[r] + G * ar;
+        M[r] = (CM_LDS ? cms[gb + r] : cdp[r]) + G * ar;
         F[r] = (J - G * v[r]) * ar + (drw ? drp[r] * w : T(0));
       }
     }
@@ -555,8 +559,13 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
 }
 
 __host__ __device__ constexpr size_t tree_sec_lds_bytes(int threads, int node_cap,
-                                                        size_t elem) {
-  return (2 * (size_t)node_cap + 6 * (size_t)threads) * elem;
+                                                        size_t elem, int cm_lds = 0) {
+  return ((2 + (cm_lds ? 1 : 0)) * (size_t)node_cap + 6 * (size_t)threads) * elem;
+}
+// cm/dt joins the LDS node table while two blocks of the section kernel still
+// fit a CU (80 KiB each)
+__host__ __device__ constexpr int tree_sec_cm_lds(int threads, int node_cap, size_t elem) {
+  return tree_sec_lds_bytes(threads, node_cap, elem, 1) <= 80 * 1024 ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -647,8 +656,9 @@ static int tree_run_impl(dn::TreePlan P, const int32_t* order, const int32_t* gr
   if (layout == 1 && (!grp_off || kmax < 1)) return DN_EINVAL;
   if (layout != 0 && layout != 1) return DN_EINVAL;
   if (rec && rec_every < 1) return DN_EINVAL;
+  const int cm_lds = layout == 1 ? dn::tree_sec_cm_lds(threads, kmax, sizeof(T)) : 0;
   size_t lds = layout == 0 ? dn::tree_lds_bytes(kmax, P.smax, P.lmax, sizeof(T))
-                           : dn::tree_sec_lds_bytes(threads, kmax, sizeof(T));
+                           : dn::tree_sec_lds_bytes(threads, kmax, sizeof(T), cm_lds);
   if (lds > 160 * 1024) return DN_ERANGE;
   if (n_steps == 0) return DN_OK;
   dn::TreeRunArgs<T> A;
@@ -658,12 +668,15 @@ static int tree_run_impl(dn::TreePlan P, const int32_t* order, const int32_t* gr
   A.hh = {hh[0], hh[1], hh[2], hh[3], hh[4], hh[5], hh[6]};
   A.rec = rec; A.rec_every = rec_every; A.total = total;
   if (layout == 1) {
-    if (sna || sk)
-      hipLaunchKernelGGL((dn::hh_tree_sec_kernel<T, true>), dim3(n_groups),
-                         dim3(threads), lds, stream, A, order, grp_off, kmax);
-    else
-      hipLaunchKernelGGL((dn::hh_tree_sec_kernel<T, false>), dim3(n_groups),
-                         dim3(threads), lds, stream, A, order, grp_off, kmax);
+#define DN_TREE_SEC(SC, CM)                                                                  \
+  hipLaunchKernelGGL((dn::hh_tree_sec_kernel<T, SC, CM>), dim3(n_groups), dim3(threads), lds, \
+                     stream, A, order, grp_off, kmax)
+    const bool sc = sna || sk;
+    if (sc && cm_lds) DN_TREE_SEC(true, true);
+    else if (sc) DN_TREE_SEC(true, false);
+    else if (cm_lds) DN_TREE_SEC(false, true);
+    else DN_TREE_SEC(false, false);
+#undef DN_TREE_SEC
   } else if (sna || sk)
     hipLaunchKernelGGL((dn::hh_tree_run_kernel<T, true>), dim3(P.n_trees), dim3(threads),
                        lds, stream, A, order, kmax, P.smax, P.lmax);
