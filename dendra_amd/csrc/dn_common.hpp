@@ -120,6 +120,45 @@ __device__ __forceinline__ T cnexp(T x, T xa, T xb, T dt) {
   return far ? rel : inc;
 }
 
+// float32: when every lane of the wavefront has |z| <= 1/16 the series is
+// cut after z^4 (the dropped tail is below |z|^5/720 <= 1.3e-9, under half
+// an ulp of phi1 ~ 1); otherwise the full degree-9 form above.
+template <>
+__device__ __forceinline__ float cnexp<float>(float x, float xa, float xb, float dt) {
+  const float z = -xb * dt;
+  float ph;
+  if (__ballot(fabsf(z) > 0.0625f) == 0ull) {
+    ph = fmaf(z, 1.0f / 120.0f, 1.0f / 24.0f);
+    ph = fmaf(ph, z, 1.0f / 6.0f);
+    ph = fmaf(ph, z, 0.5f);
+    ph = fmaf(ph, z, 1.0f);
+  } else {
+    ph = phi1(z);
+  }
+  const float inc = x + dt * (xa - xb * x) * ph;
+  const bool far = fabsf(z) > 0.5f;
+  if (__ballot(far) == 0ull) return inc;
+  const float eq = xa * frcp(xb);
+  const float rel = eq + (x - eq) * fexp(z);
+  return far ? rel : inc;
+}
+
+// vtrap10(x) given e = exp(x/10).
+__device__ __forceinline__ float vtrap10e(float x, float e) {
+  const float q = x * 0.1f;
+  const bool near0 = fabsf(q) <= 1.0f;
+  const float direct = x * frcp(e - 1.0f);
+  if (__ballot(near0) == 0ull) return direct;
+  const float q2 = q * q;
+  float s = fmaf(q2, -691.0f / 1307674368000.0f, 1.0f / 47900160.0f);
+  s = fmaf(s, q2, -1.0f / 1209600.0f);
+  s = fmaf(s, q2, 1.0f / 30240.0f);
+  s = fmaf(s, q2, -1.0f / 720.0f);
+  s = fmaf(s, q2, 1.0f / 12.0f);
+  const float series = (10.0f - x * 0.5f) + x * (q * s);
+  return near0 ? series : direct;
+}
+
 // Gate advance at v_old, then the affine current terms from the new gates.
 // sna/sk scale gNa/gK per compartment (1 when the model has no scales).
 template <typename T>
@@ -137,6 +176,35 @@ __device__ __forceinline__ void hh_advance(T v, T dt, const HHParams<T>& p, T sn
   T gna = p.gnabar * sna * (m * m * m) * h;
   T n2 = n * n;
   T gk = p.gkbar * sk * (n2 * n2);
+  G = p.gl + gna + gk;
+  J = p.gl * p.el + gna * p.ena + gk * p.ek;
+}
+
+// float32: the six rate exponentials reduce to three, by
+//   exp(-(v+55)/10) = exp(-(v+40)/10) e^-1.5,
+//   exp(-(v+35)/10) = exp(-(v+40)/10) e^0.5,
+//   exp(-(v+65)/20) = exp(-(v+65)/80)^4,
+// each exact in real arithmetic and within a few ulp in float32.
+template <>
+__device__ __forceinline__ void hh_advance<float>(float v, float dt, const HHParams<float>& p,
+                                                  float sna, float sk, float& m, float& h,
+                                                  float& n, float& G, float& J) {
+  const float xm = -(v + 40.0f), xn = -(v + 55.0f);
+  const float e40 = fexp(xm * 0.1f);
+  const float e80 = fexp((v + 65.0f) * (-1.0f / 80.0f));
+  const float e80s = e80 * e80;
+  float am = 0.1f * vtrap10e(xm, e40);
+  float bm = 4.0f * fexp((v + 65.0f) * (-1.0f / 18.0f));
+  float ah = 0.07f * (e80s * e80s);
+  float bh = frcp(fmaf(e40, 1.6487212707001282f, 1.0f));
+  float an = 0.01f * vtrap10e(xn, e40 * 0.22313016014842982f);
+  float bn = 0.125f * e80;
+  m = cnexp(m, p.q10 * am, p.q10 * (am + bm), dt);
+  h = cnexp(h, p.q10 * ah, p.q10 * (ah + bh), dt);
+  n = cnexp(n, p.q10 * an, p.q10 * (an + bn), dt);
+  float gna = p.gnabar * sna * (m * m * m) * h;
+  float n2 = n * n;
+  float gk = p.gkbar * sk * (n2 * n2);
   G = p.gl + gna + gk;
   J = p.gl * p.el + gna * p.ena + gk * p.ek;
 }
