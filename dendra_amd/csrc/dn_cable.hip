@@ -51,6 +51,8 @@ template <typename T> struct CableRunArgs {
 
 // Per-row read-only coefficients staged once in LDS, row-major over lanes so
 // that each row is one conflict-free 16-byte (32-byte for float64) read.
+// They are stored multiplied by dt * 1e-3 (ms -> s with the membrane terms
+// per unit area), so the row's matrix entries need no further scaling.
 template <typename T> struct alignas(4 * sizeof(T)) RowCoef { T gl, grp, sc, dr; };
 template <typename T> struct alignas(2 * sizeof(T)) RowScale { T sna, sk; };
 
@@ -70,6 +72,8 @@ __device__ __forceinline__ void cable_run_wave(const CableRunArgs<T>& A, int tas
     K = (int)(A.off[cell + 1] - o);
   }
   const int j0 = pl * nr;
+  const T dt = A.dt;
+  const T dts = dt * T(1e-3);
   T v[RMAX], m[RMAX], h[RMAX], n[RMAX];
 #pragma unroll
   for (int r = 0; r < RMAX; ++r) {
@@ -82,10 +86,10 @@ __device__ __forceinline__ void cable_run_wave(const CableRunArgs<T>& A, int tas
     n[r] = in ? A.n[idx] : T(0);
     if (r < nr) {
       RowCoef<T> c;
-      c.gl = in ? A.gL[idx] : T(0);
-      c.grp = in ? A.gRp[idx] : T(0);
-      c.sc = in ? A.scale[idx] : T(0);
-      c.dr = (in && A.drive) ? A.drive[idx] : T(0);
+      c.gl = in ? dts * A.gL[idx] : T(0);
+      c.grp = in ? dts * A.gRp[idx] : T(0);
+      c.sc = in ? dts * A.scale[idx] : T(0);
+      c.dr = (in && A.drive) ? dts * A.drive[idx] : T(0);
       cf[r * 64 + lane] = c;
       if (SCALES) {
         RowScale<T> q;
@@ -95,8 +99,6 @@ __device__ __forceinline__ void cable_run_wave(const CableRunArgs<T>& A, int tas
       }
     }
   }
-  const T dt = A.dt;
-  const T dts = dt * T(1e-3);
   for (int64_t s = 0; s < A.n_steps; ++s) {
     const T w = A.wave ? A.wave[A.step0 + s] : T(0);
     T vlast = v[0];
@@ -123,12 +125,11 @@ __device__ __forceinline__ void cable_run_wave(const CableRunArgs<T>& A, int tas
         const T vl = (r == 0) ? vprev : v[r > 0 ? r - 1 : 0];
         T vr = vnext;
         if (r + 1 < RMAX) vr = (r + 1 < nr) ? v[r + 1 < RMAX ? r + 1 : r] : vnext;
-        const T f = (J - G * v[r]) * c.sc + c.gl * (vr - v[r]) -
-                    c.grp * (v[r] - vl) + c.dr * w;
-        const T bb = T(1) + dts * (c.gl + c.grp + G * c.sc);
-        const T dd = dts * f;
-        const T aa = -dts * c.grp;
-        const T cc = -dts * c.gl;
+        const T dd = (J - G * v[r]) * c.sc + c.gl * (vr - v[r]) -
+                     c.grp * (v[r] - vl) + c.dr * w;
+        const T bb = T(1) + c.gl + c.grp + G * c.sc;
+        const T aa = -c.grp;
+        const T cc = -c.gl;
         if (r == 0) {
           a0 = aa; b0 = bb; c0 = cc; d0 = dd;
         } else if (r == 1) {

@@ -120,45 +120,6 @@ __device__ __forceinline__ T cnexp(T x, T xa, T xb, T dt) {
   return far ? rel : inc;
 }
 
-// float32: when every lane of the wavefront has |z| <= 1/16 the series is
-// cut after z^4 (the dropped tail is below |z|^5/720 <= 1.3e-9, under half
-// an ulp of phi1 ~ 1); otherwise the full degree-9 form above.
-template <>
-__device__ __forceinline__ float cnexp<float>(float x, float xa, float xb, float dt) {
-  const float z = -xb * dt;
-  float ph;
-  if (__ballot(fabsf(z) > 0.0625f) == 0ull) {
-    ph = fmaf(z, 1.0f / 120.0f, 1.0f / 24.0f);
-    ph = fmaf(ph, z, 1.0f / 6.0f);
-    ph = fmaf(ph, z, 0.5f);
-    ph = fmaf(ph, z, 1.0f);
-  } else {
-    ph = phi1(z);
-  }
-  const float inc = x + dt * (xa - xb * x) * ph;
-  const bool far = fabsf(z) > 0.5f;
-  if (__ballot(far) == 0ull) return inc;
-  const float eq = xa * frcp(xb);
-  const float rel = eq + (x - eq) * fexp(z);
-  return far ? rel : inc;
-}
-
-// vtrap10(x) given e = exp(x/10).
-__device__ __forceinline__ float vtrap10e(float x, float e) {
-  const float q = x * 0.1f;
-  const bool near0 = fabsf(q) <= 1.0f;
-  const float direct = x * frcp(e - 1.0f);
-  if (__ballot(near0) == 0ull) return direct;
-  const float q2 = q * q;
-  float s = fmaf(q2, -691.0f / 1307674368000.0f, 1.0f / 47900160.0f);
-  s = fmaf(s, q2, -1.0f / 1209600.0f);
-  s = fmaf(s, q2, 1.0f / 30240.0f);
-  s = fmaf(s, q2, -1.0f / 720.0f);
-  s = fmaf(s, q2, 1.0f / 12.0f);
-  const float series = (10.0f - x * 0.5f) + x * (q * s);
-  return near0 ? series : direct;
-}
-
 // Gate advance at v_old, then the affine current terms from the new gates.
 // sna/sk scale gNa/gK per compartment (1 when the model has no scales).
 template <typename T>
@@ -180,28 +141,74 @@ __device__ __forceinline__ void hh_advance(T v, T dt, const HHParams<T>& p, T sn
   J = p.gl * p.el + gna * p.ena + gk * p.ek;
 }
 
-// float32: the six rate exponentials reduce to three, by
-//   exp(-(v+55)/10) = exp(-(v+40)/10) e^-1.5,
-//   exp(-(v+35)/10) = exp(-(v+40)/10) e^0.5,
-//   exp(-(v+65)/20) = exp(-(v+65)/80)^4,
-// each exact in real arithmetic and within a few ulp in float32.
+// float32 gate step of x' = q10 (a - s x) with s = a + b, given dq = q10 dt:
+// x + dq (a - s x) phi1(z), z = -s dq.  phi1 is the Taylor form for
+// |z| <= 0.5 -- cut after z^4 when every lane of the wavefront has
+// |z| <= 1/16 (the dropped tail is below |z|^5/720 <= 1.3e-9, under half an
+// ulp of phi1 ~ 1) -- and (e^z - 1)/z beyond, which makes the step the
+// exponential relaxation a/s + (x - a/s) e^z written without a/s.
+__device__ __forceinline__ float gate_step(float x, float a, float s, float dq) {
+  const float z = -s * dq;
+  float ph;
+  if (__ballot(fabsf(z) > 0.0625f) == 0ull) {
+    ph = fmaf(z, 1.0f / 120.0f, 1.0f / 24.0f);
+    ph = fmaf(ph, z, 1.0f / 6.0f);
+    ph = fmaf(ph, z, 0.5f);
+    ph = fmaf(ph, z, 1.0f);
+  } else {
+    ph = phi1(z);
+  }
+  const float pf = (fexp(z) - 1.0f) * frcp(z);
+  ph = fabsf(z) > 0.5f ? pf : ph;
+  return fmaf(dq * fmaf(-s, x, a), ph, x);
+}
+
+// B(q) = q / (e^q - 1) given e = e^q: the even Bernoulli series for
+// |q| <= 1 (evaluated only when some lane needs it), the quotient beyond.
+// 0.1 vtrap10(x) = B(x/10).
+__device__ __forceinline__ float bern(float q, float e) {
+  const bool near0 = fabsf(q) <= 1.0f;
+  const float direct = q * frcp(e - 1.0f);
+  if (__ballot(near0) == 0ull) return direct;
+  const float q2 = q * q;
+  float c = fmaf(q2, -691.0f / 1307674368000.0f, 1.0f / 47900160.0f);
+  c = fmaf(c, q2, -1.0f / 1209600.0f);
+  c = fmaf(c, q2, 1.0f / 30240.0f);
+  c = fmaf(c, q2, -1.0f / 720.0f);
+  c = fmaf(c, q2, 1.0f / 12.0f);
+  const float series = fmaf(q2, c, fmaf(q, -0.5f, 1.0f));
+  return near0 ? series : direct;
+}
+
+// float32: the six rate exponentials reduce to three, on centred arguments
+// (v + 40, v + 55 and v + 65 are exact in float32 near rest):
+//   e40 = exp(-(v+40)/10),  exp(-(v+55)/10) = e40 e^-1.5,
+//   exp(-(v+35)/10) = e40 e^0.5,
+//   e80 = exp(-(v+65)/80),  bn = e80/8,  ah = 0.07 e80^4,
+//   bm = 4 exp(-(v+65)/18),
+// and q10 enters once, through dq = q10 dt.  Each identity is exact in real
+// arithmetic; the float32 results move by a few ulp, with no systematic
+// bias at rest (the argument constants stay the reference's 1/10, 1/18,
+// 1/80 before the product with log2 e inside fexp).
 template <>
 __device__ __forceinline__ void hh_advance<float>(float v, float dt, const HHParams<float>& p,
                                                   float sna, float sk, float& m, float& h,
                                                   float& n, float& G, float& J) {
-  const float xm = -(v + 40.0f), xn = -(v + 55.0f);
-  const float e40 = fexp(xm * 0.1f);
-  const float e80 = fexp((v + 65.0f) * (-1.0f / 80.0f));
+  const float xm = -(v + 40.0f), xn = -(v + 55.0f), w = v + 65.0f;
+  const float qm = xm * 0.1f, qn = xn * 0.1f;
+  const float e40 = fexp(qm);
+  const float e80 = fexp(w * (-1.0f / 80.0f));
+  const float bm = 4.0f * fexp(w * (-1.0f / 18.0f));
   const float e80s = e80 * e80;
-  float am = 0.1f * vtrap10e(xm, e40);
-  float bm = 4.0f * fexp((v + 65.0f) * (-1.0f / 18.0f));
-  float ah = 0.07f * (e80s * e80s);
-  float bh = frcp(fmaf(e40, 1.6487212707001282f, 1.0f));
-  float an = 0.01f * vtrap10e(xn, e40 * 0.22313016014842982f);
-  float bn = 0.125f * e80;
-  m = cnexp(m, p.q10 * am, p.q10 * (am + bm), dt);
-  h = cnexp(h, p.q10 * ah, p.q10 * (ah + bh), dt);
-  n = cnexp(n, p.q10 * an, p.q10 * (an + bn), dt);
+  const float ah = 0.07f * (e80s * e80s);
+  const float bn = 0.125f * e80;
+  const float bh = frcp(fmaf(e40, 1.6487212707001282f, 1.0f));
+  const float am = bern(qm, e40);
+  const float an = 0.1f * bern(qn, e40 * 0.22313016014842982f);
+  const float dq = p.q10 * dt;
+  m = gate_step(m, am, am + bm, dq);
+  h = gate_step(h, ah, ah + bh, dq);
+  n = gate_step(n, an, an + bn, dq);
   float gna = p.gnabar * sna * (m * m * m) * h;
   float n2 = n * n;
   float gk = p.gkbar * sk * (n2 * n2);
