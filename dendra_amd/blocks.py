@@ -184,9 +184,54 @@ def _children_csr(parent):
     return csr
 
 
+_levels = WeakIdKeyDictionary()
+
+
+def _depth_levels(parent):
+    """Elimination order deepest -> root in whole depth levels (int64 order,
+    layer_ptr on the parent's device), cached per parent tensor; None when
+    the parent array is not a forest."""
+    stamp = (parent._version, parent.numel(), parent.device)
+    hit = _levels.get(parent)
+    if hit is not None and hit[0] == stamp:
+        return hit[1]
+    par = parent.detach().to("cpu", torch.int64).numpy().reshape(-1)
+    K = par.size
+    par = np.where((par >= 0) & (par < K), par, -1)
+    depth = np.full(K, -1, dtype=np.int64)
+    frontier = np.flatnonzero(par < 0)
+    depth[frontier] = 0
+    kids = np.flatnonzero(par >= 0)
+    child_order = kids[np.argsort(par[kids], kind="stable")]
+    ptr = np.zeros(K + 1, dtype=np.int64)
+    np.cumsum(np.bincount(par[kids], minlength=K), out=ptr[1:])
+    d = 0
+    while frontier.size:
+        nxt = np.concatenate([child_order[ptr[i]:ptr[i + 1]] for i in frontier]) \
+            if frontier.size else frontier
+        d += 1
+        nxt = nxt[depth[nxt] < 0]
+        depth[nxt] = d
+        frontier = nxt
+    res = None
+    if (depth >= 0).all():
+        order = np.concatenate([np.flatnonzero(depth == k) for k in range(d - 1, -1, -1)])
+        layer_ptr = np.concatenate([[0], np.cumsum(np.bincount(depth, minlength=d)[::-1])])
+        dev = parent.device
+        res = (torch.as_tensor(order, dtype=torch.int64, device=dev),
+               torch.as_tensor(layer_ptr, dtype=torch.int64, device=dev))
+    _levels[parent] = (stamp, res)
+    return res
+
+
 def _tree_raw(main, edge, rhs, parent, order, layer_ptr):
     main, edge, rhs = (t.contiguous() for t in (main, edge, rhs))
     child_ptr, child_idx = _children_csr(parent)
+    # whole depth levels let the kernel pass the children's terms through
+    # LDS; the solve's results do not depend on the layering
+    lv = _depth_levels(parent)
+    if lv is not None:
+        order, layer_ptr = lv
     parent, order, layer_ptr = (t.to(torch.int64).contiguous()
                                 for t in (parent, order, layer_ptr))
     B, K = map(int, rhs.shape[:2])

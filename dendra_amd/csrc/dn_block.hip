@@ -314,7 +314,19 @@ __global__ __launch_bounds__(256) void bt3_tree_kernel(const T* __restrict__ mai
 //   b_i = rhs_i + sum_c e_c * (Ainv_c b_c)
 // then top-down x_i = Ainv_i (b_i + e_i * x_p).  work: [B][K][12] = Ainv (9)
 // and the eliminated right-hand side (3) per node.
-template <typename T>
+//
+// RING: when every layer is one whole depth level of at most 64 nodes (each
+// node's parent in the next layer, checked at staging), a node's children
+// are exactly the previous layer, so their eliminated terms pass through a
+// two-layer LDS ring indexed by position in the layer instead of the
+// workspace, each lane owns at most one node per layer, and the next layer's
+// inputs are fetched while the current one is eliminated.  A ring slot holds
+// e_c (x) Ainv_c row-scaled (9), e_c (3) and Ainv_c b_c (3): the parent
+// evaluates exactly the expressions of the workspace form, so both give the
+// same bits.  Layers that fail the check take the workspace form.
+constexpr int kRingW = 64;     // nodes per layer the ring holds
+constexpr int kRingSlot = 15;  // values per node in the elimination ring
+template <typename T, bool RING>
 __global__ __launch_bounds__(256) void bt3_tree_layer_kernel(
     const T* __restrict__ main, const T* __restrict__ edge, const T* __restrict__ rhs,
     const int64_t* __restrict__ parent, const int64_t* __restrict__ order,
@@ -345,7 +357,8 @@ __global__ __launch_bounds__(256) void bt3_tree_layer_kernel(
       const int64_t v = layer_ptr[j];
       t_lp[j] = (int32_t)(v < 0 ? 0 : (v > K ? K : v));
     }
-    const int32_t nc = child_ptr[K];
+    int32_t nc = child_ptr[K];
+    if (RING && nc > K) nc = (int32_t)K;  // the ring check rejects such a CSR
     for (int64_t j = threadIdx.x; j < nc; j += blockDim.x) t_ci[j] = child_idx[j];
     __syncthreads();
   }
@@ -370,6 +383,167 @@ __global__ __launch_bounds__(256) void bt3_tree_layer_kernel(
   const T* R = rhs + so * K * 3;
   T* X = x + so * K * 3;
   T* W = work + so * K * 12;
+  if (RING && topo_lds) {
+    // inverse of order (t_inv, after K entries reserved for the children
+    // list), then the checks that make the ring exact
+    int32_t* t_inv = t_ci + K;
+    for (int64_t j = threadIdx.x; j < K; j += blockDim.x) t_inv[j] = -1;
+    __syncthreads();
+    for (int64_t n = threadIdx.x; n < K; n += blockDim.x) {
+      const int32_t i = t_order[n];
+      if (i >= 0) t_inv[i] = (int32_t)n;
+    }
+    __syncthreads();
+    int ok = 1;
+    if (threadIdx.x == 0 && (t_lp[0] != 0 || t_lp[n_layers] != K || t_cp[0] != 0 ||
+                             t_cp[K] < 0 || t_cp[K] > K))
+      ok = 0;
+    for (int l = threadIdx.x; l < n_layers; l += blockDim.x)
+      if (t_lp[l + 1] < t_lp[l] || t_lp[l + 1] - t_lp[l] > kRingW) ok = 0;
+    for (int64_t i = threadIdx.x; i < K; i += blockDim.x) {
+      const int32_t n = t_inv[i];
+      if (n < 0 || t_order[n] != i) { ok = 0; continue; }
+      const int32_t c0 = t_cp[i], c1 = t_cp[i + 1];
+      if (c0 < 0 || c1 < c0 || c1 > K) { ok = 0; continue; }
+      for (int32_t j = c0; j < c1; ++j) {
+        const int32_t ch = t_ci[j];
+        if (ch < 0 || ch >= K || t_par[ch] != i) ok = 0;
+      }
+      const int32_t p = t_par[i];
+      if (p < 0) continue;
+      // layer of n: the last l with t_lp[l] <= n
+      int lo = 0, hi = n_layers;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (t_lp[mid] <= n) lo = mid; else hi = mid;
+      }
+      const int32_t np = t_inv[p];
+      if (lo + 1 >= n_layers || np < t_lp[lo + 1] || np >= t_lp[lo + 2]) ok = 0;
+    }
+    if (__syncthreads_and(ok)) {
+      // Children and parents renamed to their positions in their layers
+      // (the workspace form, which needs the node indices, is not taken
+      // from here on).
+      for (int64_t i = threadIdx.x; i < K; i += blockDim.x) {
+        const int32_t n = t_inv[i];
+        int lo = 0, hi = n_layers;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (t_lp[mid] <= n) lo = mid; else hi = mid;
+        }
+        for (int32_t j = t_cp[i]; j < t_cp[i + 1]; ++j) t_ci[j] = t_inv[t_ci[j]] - t_lp[lo - 1];
+        const int32_t p = t_par[i];
+        t_par[i] = p >= 0 ? t_inv[p] - t_lp[lo + 1] : -1;
+      }
+      __syncthreads();
+      // rings of this wavefront: elimination [2][kRingW][kRingSlot], x [2][kRingW][3],
+      // after the staged topology (5K + n_layers + 2 int32, rounded to 16 B)
+      const size_t ring_off = ((size_t)4 * (5 * K + n_layers + 2) + 15) & ~(size_t)15;
+      T* rS = reinterpret_cast<T*>(dn_smem_raw + ring_off) +
+              (threadIdx.x >> 6) * (2 * kRingW * (kRingSlot + 3));
+      T* rX = rS + 2 * kRingW * kRingSlot;
+      // node of this lane in layer l, or -1 (branch-free: the read is
+      // clamped into the staged order and its value dropped when unused)
+      auto node_of = [&](int l) -> int32_t {
+        const int lc = l < 0 ? 0 : (l >= n_layers ? n_layers - 1 : l);
+        const int32_t lo = t_lp[lc], w = t_lp[lc + 1] - lo;
+        const int32_t at = lo + (lane < w ? lane : (w > 0 ? w - 1 : 0));
+        const int32_t v = t_order[at < K ? at : K - 1];
+        return (l >= 0 && l < n_layers && lane < w) ? v : -1;
+      };
+      // Bottom-up.  Two register sets alternate (the loop is unrolled by
+      // two), so the next layer's loads stay in flight while this layer is
+      // eliminated.
+      // Loads are issued by every lane (a lane without a node reads node 0 of
+      // its system, or of system 0 past the batch, and ignores it): with no
+      // branch around them the compiler can leave them in flight.
+      auto load_fwd = [&](int32_t i, T (&a)[9], T (&e)[3], T (&b)[3]) {
+        const int64_t k = i >= 0 ? i : 0;
+        for (int c = 0; c < 9; ++c) a[c] = M[k * 9 + c];
+        for (int c = 0; c < 3; ++c) { e[c] = E[k * 3 + c]; b[c] = R[k * 3 + c]; }
+      };
+      auto elim = [&](int l, int32_t i, T (&a)[9], const T (&e)[3], T (&b)[3]) {
+        if (live && i >= 0) {
+          for (int c = 0; c < 3; ++c) a[c * 4] += e[c];
+          const T* prev = rS + ((l + 1) & 1) * (kRingW * kRingSlot);
+          for (int32_t j = t_cp[i]; j < t_cp[i + 1]; ++j) {
+            const T* q = prev + t_ci[j] * kRingSlot;
+            for (int r = 0; r < 3; ++r) {
+              a[r * 4] += q[9 + r];
+              for (int c = 0; c < 3; ++c) a[r * 3 + c] -= q[r * 3 + c] * q[9 + c];
+              b[r] += q[9 + r] * q[12 + r];
+            }
+          }
+          T iv[9], y[3];
+          inv3(a, iv);
+          for (int c = 0; c < 9; ++c) W[(int64_t)i * 12 + c] = iv[c];
+          for (int c = 0; c < 3; ++c) W[(int64_t)i * 12 + 9 + c] = b[c];
+          mv3(iv, b, y);
+          T* q = rS + (l & 1) * (kRingW * kRingSlot) + lane * kRingSlot;
+          for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) q[r * 3 + c] = e[r] * iv[r * 3 + c];
+            q[9 + r] = e[r];
+            q[12 + r] = y[r];
+          }
+        }
+        // The rings belong to this wavefront alone and its LDS operations
+        // complete in issue order: a wavefront-scope fence orders them for
+        // the compiler without waiting for the prefetched global loads.
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      };
+      {
+        T a0[9], e0[3], b0[3], a1[9], e1[3], b1[3];
+        int32_t i0 = node_of(0), i1;
+        load_fwd(i0, a0, e0, b0);
+        for (int l = 0; l < n_layers; l += 2) {
+          i1 = node_of(l + 1);
+          load_fwd(i1, a1, e1, b1);
+          elim(l, i0, a0, e0, b0);
+          if (l + 1 >= n_layers) break;
+          i0 = node_of(l + 2);
+          load_fwd(i0, a0, e0, b0);
+          elim(l + 1, i1, a1, e1, b1);
+        }
+      }
+      // Top-down, the same alternation over (Ainv, b, e).
+      auto load_bwd = [&](int32_t i, T (&w)[12], T (&e)[3]) {
+        const int64_t k = i >= 0 ? i : 0;
+        for (int c = 0; c < 12; ++c) w[c] = W[k * 12 + c];
+        for (int c = 0; c < 3; ++c) e[c] = E[k * 3 + c];
+      };
+      auto back = [&](int l, int32_t i, const T (&w)[12], const T (&e)[3]) {
+        if (live && i >= 0) {
+          const int32_t p = t_par[i];  // the parent's position in layer l + 1
+          const T* xp = rX + ((l + 1) & 1) * (kRingW * 3) + (p >= 0 ? p : 0) * 3;
+          T bb[3], xi[3];
+          for (int c = 0; c < 3; ++c) bb[c] = w[9 + c] + (p >= 0 ? e[c] * xp[c] : T(0));
+          mv3(w, bb, xi);
+          for (int c = 0; c < 3; ++c) {
+            X[(int64_t)i * 3 + c] = xi[c];
+            rX[(l & 1) * (kRingW * 3) + lane * 3 + c] = xi[c];
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      };
+      {
+        T w0[12], e0[3], w1[12], e1[3];
+        int32_t i0 = node_of(n_layers - 1), i1;
+        load_bwd(i0, w0, e0);
+        for (int l = n_layers - 1; l >= 0; l -= 2) {
+          i1 = node_of(l - 1);
+          load_bwd(i1, w1, e1);
+          back(l, i0, w0, e0);
+          if (l - 1 < 0) break;
+          i0 = node_of(l - 2);
+          load_bwd(i0, w0, e0);
+          back(l - 1, i1, w1, e1);
+        }
+      }
+      return;
+    }
+  }
   for (int l = 0; l < n_layers; ++l) {
     int64_t lo = bound(l), hi = bound(l + 1);
     if (hi < lo) hi = lo;
@@ -478,10 +652,21 @@ static int bt3_tree_impl(const T* main, const T* edge, const T* rhs, const int64
     const int G = B >= 2048 ? 4 : (B >= 1024 ? 2 : 1);
     const size_t topo = (size_t)4 * (3 * K + 1 + n_layers + 1 + (K > 0 ? K : 1));
     const int topo_lds = topo <= 96 * 1024;
-    hipLaunchKernelGGL((dn::bt3_tree_layer_kernel<T>), dim3((unsigned)((B + G - 1) / G)),
-                       dim3(64 * G), topo_lds ? topo : 0, stream, main, edge, rhs, parent,
-                       order, layer_ptr, (int)n_layers, child_ptr, child_idx, x, work, B, K,
-                       topo_lds);
+    // ring form: the topology plus the inverse order, then per wavefront the
+    // two-layer elimination and solution rings (the kernel decides at
+    // staging whether the layers allow it)
+    const size_t ring_topo = ((topo + (size_t)4 * K) + 15) & ~(size_t)15;
+    const size_t ring = ring_topo + (size_t)G * 2 * dn::kRingW * (dn::kRingSlot + 3) * sizeof(T);
+    if (ring <= 96 * 1024) {
+      hipLaunchKernelGGL((dn::bt3_tree_layer_kernel<T, true>), dim3((unsigned)((B + G - 1) / G)),
+                         dim3(64 * G), ring, stream, main, edge, rhs, parent, order, layer_ptr,
+                         (int)n_layers, child_ptr, child_idx, x, work, B, K, 1);
+    } else {
+      hipLaunchKernelGGL((dn::bt3_tree_layer_kernel<T, false>),
+                         dim3((unsigned)((B + G - 1) / G)), dim3(64 * G), topo_lds ? topo : 0,
+                         stream, main, edge, rhs, parent, order, layer_ptr, (int)n_layers,
+                         child_ptr, child_idx, x, work, B, K, topo_lds);
+    }
   } else {
     hipLaunchKernelGGL((dn::bt3_tree_kernel<T>), dim3(blocks_for<T>(B)), dim3(256), 0,
                        stream, main, edge, rhs, parent, order, x, work, B, K);

@@ -338,7 +338,10 @@ class ExtCellTrees:
         if len(par) != K:
             raise ValueError("parent must hold one entry per node")
         _, _, depth = build_morphology(par)
-        order, layers = build_dhs_layers(depth, threads)
+        # whole depth levels: the block tree kernel then hands the children's
+        # eliminated terms over through LDS (its results do not depend on the
+        # layering; `threads` only bounds the reference's chunking)
+        order, layers = build_dhs_layers(depth, max(K, threads))
         self.B, self.K = B, K
         ws = workspace if workspace is not None else extcell_tree_workspace(spec, dt)
         self.workspace = ws
