@@ -321,9 +321,9 @@ __global__ __launch_bounds__(256) void bt3_tree_kernel(const T* __restrict__ mai
 // two-layer LDS ring indexed by position in the layer instead of the
 // workspace, each lane owns at most one node per layer, and the next layer's
 // inputs are fetched while the current one is eliminated.  A ring slot holds
-// e_c (x) Ainv_c row-scaled (9), e_c (3) and Ainv_c b_c (3): the parent
-// evaluates exactly the expressions of the workspace form, so both give the
-// same bits.  Layers that fail the check take the workspace form.
+// e_c-scaled rows of Ainv_c (9), e_c (3) and Ainv_c b_c (3), from which the
+// parent forms the same Schur terms as the workspace form (the two agree to
+// rounding).  Layers that fail the check take the workspace form.
 constexpr int kRingW = 64;     // nodes per layer the ring holds
 constexpr int kRingSlot = 15;  // values per node in the elimination ring
 template <typename T, bool RING>

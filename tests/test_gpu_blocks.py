@@ -194,3 +194,51 @@ def test_block_validation_messages():
         dhs_bt_solve(z(2, 4, 3, 3), z(2, 4), z(2, 4, 3), idx, idx, idx[:2])
     with pytest.raises(ValueError, match="parent_idx and order must each contain K"):
         dhs_bt_solve(z(2, 4, 3, 3), z(2, 4, 3), z(2, 4, 3), idx[:3], idx, idx[:2])
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_block_tree_ring_and_workspace_forms_agree_bitwise(dtype):
+    """Whole depth levels of <= 64 nodes take the LDS-ring form; the same
+    tree with its levels split into chunks of three fails the kernel's check
+    and takes the workspace form (still the layer kernel: at least two nodes
+    per layer on average).  Both follow the same elimination; the compiler
+    may contract their products into fused multiply-adds differently, so the
+    bar between them is a few units of rounding, and each matches a dense
+    float64 solve."""
+    from dendra_amd import _lib
+    from dendra_amd.blocks import _children_csr
+    from dendra_amd.topology import build_dhs_layers, build_morphology
+    from dendra_amd.workloads import random_binary_tree
+
+    parents = [int(p) for p in random_binary_tree(300, 0, 4)[0]]
+    K, B = len(parents), 5
+    _, _, depth = build_morphology(parents)
+    rng = np.random.default_rng(9)
+    main = torch.as_tensor(rng.uniform(-0.3, 0.3, (B, K, 3, 3)) + np.eye(3) * 2.0)
+    edge = torch.as_tensor(rng.uniform(0.1, 2.0, (B, K, 3)))
+    rhs = torch.as_tensor(rng.standard_normal((B, K, 3)))
+    want = dense_solve(block_tree_matrix(main, edge, parents), rhs)
+    par = torch.as_tensor(parents, dtype=torch.int64, device="cuda")
+    cp, ci = _children_csr(par)
+    args = [v.to("cuda", dtype).contiguous() for v in (main, edge, rhs)]
+    lib = _lib.load()
+    fn = lib.dn_bt3_tree_f64 if dtype == torch.float64 else lib.dn_bt3_tree_f32
+    n = int(lib.dn_block_work_elems(2, B, K))
+    outs = []
+    for chunk in (K, 3):
+        order, layers = build_dhs_layers(depth, chunk)
+        assert (np.diff(layers) <= 64).all() and 2 * (len(layers) - 1) <= K
+        o = torch.as_tensor(order, dtype=torch.int64, device="cuda")
+        lp = torch.as_tensor(layers, dtype=torch.int64, device="cuda")
+        x = torch.empty_like(args[2])
+        work = torch.empty(n, dtype=dtype, device="cuda")
+        p = _lib.ptr
+        st = fn(p(args[0]), p(args[1]), p(args[2]), p(par), p(o), p(lp), lp.numel() - 1,
+                p(cp), p(ci), p(x), B, K, p(work), n, _lib.stream_handle(args[2].device))
+        _lib.check(st, "block tree solve")
+        outs.append(x)
+    eps = torch.finfo(dtype).eps
+    scale = outs[1].abs().max().item()
+    assert (outs[0] - outs[1]).abs().max().item() <= 64 * eps * max(1.0, scale)
+    tol = 1e-11 if dtype == torch.float64 else 1e-4
+    assert (outs[0].double().cpu() - want).abs().max() <= tol * max(1.0, want.abs().max())
