@@ -117,7 +117,11 @@ int dn_tree_contract_f64(const double* d, const double* g, const double* rhs, do
  *                    child_idx; with the layers and CSR given (and layers
  *                    averaging two or more nodes) one wavefront solves a
  *                    system layer by layer, else (layer_ptr null) one lane
- *                    walks `order`.
+ *                    walks `order`.  When every layer is one whole depth
+ *                    level of at most 64 nodes (checked on the device), the
+ *                    children's terms pass through LDS instead of `work`
+ *                    (same solution to rounding); passing whole depth levels
+ *                    therefore pays.
  * Inputs are not modified; work holds dn_block_work_elems(kind, B, K)
  * elements (kind 0 Thomas, 1 SPD, 2 tree). */
 int64_t dn_block_work_elems(int kind, int64_t B, int64_t K);
