@@ -198,24 +198,24 @@ def _depth_levels(parent):
     par = parent.detach().to("cpu", torch.int64).numpy().reshape(-1)
     K = par.size
     par = np.where((par >= 0) & (par < K), par, -1)
-    depth = np.full(K, -1, dtype=np.int64)
-    frontier = np.flatnonzero(par < 0)
-    depth[frontier] = 0
-    kids = np.flatnonzero(par >= 0)
-    child_order = kids[np.argsort(par[kids], kind="stable")]
-    ptr = np.zeros(K + 1, dtype=np.int64)
-    np.cumsum(np.bincount(par[kids], minlength=K), out=ptr[1:])
-    d = 0
-    while frontier.size:
-        nxt = np.concatenate([child_order[ptr[i]:ptr[i + 1]] for i in frontier]) \
-            if frontier.size else frontier
-        d += 1
-        nxt = nxt[depth[nxt] < 0]
-        depth[nxt] = d
-        frontier = nxt
+    # depth by pointer jumping: after round r, anc is the 2^r-th ancestor and
+    # depth counts the edges passed; a pointer still live after
+    # ceil(log2 K) + 1 rounds means a cycle
+    depth = (par >= 0).astype(np.int64)
+    anc = par.copy()
+    for _ in range(max(1, int(K).bit_length()) + 1):
+        live = anc >= 0
+        if not live.any():
+            break
+        a = anc[live]
+        depth[live] += depth[a]
+        anc[live] = anc[a]
+    if (anc >= 0).any():
+        depth[:] = -1
+    d = int(depth.max()) + 1 if K else 0
     res = None
     if (depth >= 0).all():
-        order = np.concatenate([np.flatnonzero(depth == k) for k in range(d - 1, -1, -1)])
+        order = np.argsort(-depth, kind="stable")  # deepest first, ascending index within a level
         layer_ptr = np.concatenate([[0], np.cumsum(np.bincount(depth, minlength=d)[::-1])])
         dev = parent.device
         res = (torch.as_tensor(order, dtype=torch.int64, device=dev),

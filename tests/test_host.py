@@ -299,3 +299,43 @@ def test_entry_points_reject_invalid_arguments_before_any_launch():
                            None, None) == -1
     assert lib.dn_mlp_ensemble(10, 9, fake, 10, fake, 0.0, 1.0, 0.03, fake, fake,
                                None) == -1
+
+
+def test_depth_levels_are_whole_levels_deepest_first():
+    """The block tree solve's host layering: every node once, whole depth
+    levels from the deepest, ascending node index within a level, each
+    parent in the next layer; a cycle gives None.  Depths are checked
+    against a breadth-first search from the roots (networkx)."""
+    import torch
+
+    from dendra_amd.blocks import _depth_levels
+
+    rng = np.random.default_rng(4)
+    cases = [np.arange(-1, 299), np.array([-1]), np.array([-1, 0, 0, -1, 3, 4, 4])]
+    for K in (50, 400):
+        p = np.full(K, -1)
+        for i in range(1, K):
+            p[i] = rng.integers(max(0, i - 8), i)
+        perm = rng.permutation(K)
+        inv = np.argsort(perm)
+        cases.append(np.where(p[perm] >= 0, inv[np.maximum(p[perm], 0)], -1))
+    for par in cases:
+        K = len(par)
+        order, lp = (t.numpy() for t in _depth_levels(torch.as_tensor(par, dtype=torch.int64)))
+        assert sorted(order.tolist()) == list(range(K)) and lp[0] == 0 and lp[-1] == K
+        g = nx.DiGraph()
+        g.add_nodes_from(range(K))
+        g.add_edges_from((int(q), i) for i, q in enumerate(par) if q >= 0)
+        depth = np.zeros(K, int)
+        for r in np.flatnonzero(par < 0):
+            for node, dist in nx.single_source_shortest_path_length(g, int(r)).items():
+                depth[node] = dist
+        n_lvl = len(lp) - 1
+        for l in range(n_lvl):
+            seg = order[lp[l]:lp[l + 1]]
+            assert (depth[seg] == n_lvl - 1 - l).all() and (np.diff(seg) > 0).all()
+            for i in seg:
+                if par[i] >= 0:
+                    assert lp[l + 1] <= np.flatnonzero(order == par[i])[0] < lp[l + 2]
+    assert _depth_levels(torch.tensor([1, 2, 0])) is None
+    assert _depth_levels(torch.tensor([-1, 2, 1])) is None
