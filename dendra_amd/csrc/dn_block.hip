@@ -657,7 +657,15 @@ static int bt3_tree_impl(const T* main, const T* edge, const T* rhs, const int64
     // staging whether the layers allow it)
     const size_t ring_topo = ((topo + (size_t)4 * K) + 15) & ~(size_t)15;
     const size_t ring = ring_topo + (size_t)G * 2 * dn::kRingW * (dn::kRingSlot + 3) * sizeof(T);
-    if (ring <= 96 * 1024) {
+    // the ring form's wavefronts never wait on each other after staging, so
+    // four systems share a workgroup (and its staging) from B >= 256 on
+    const int Gr = B >= 256 ? 4 : G;
+    const size_t ring4 = ring_topo + (size_t)Gr * 2 * dn::kRingW * (dn::kRingSlot + 3) * sizeof(T);
+    if (ring4 <= 96 * 1024) {
+      hipLaunchKernelGGL((dn::bt3_tree_layer_kernel<T, true>), dim3((unsigned)((B + Gr - 1) / Gr)),
+                         dim3(64 * Gr), ring4, stream, main, edge, rhs, parent, order, layer_ptr,
+                         (int)n_layers, child_ptr, child_idx, x, work, B, K, 1);
+    } else if (ring <= 96 * 1024) {
       hipLaunchKernelGGL((dn::bt3_tree_layer_kernel<T, true>), dim3((unsigned)((B + G - 1) / G)),
                          dim3(64 * G), ring, stream, main, edge, rhs, parent, order, layer_ptr,
                          (int)n_layers, child_ptr, child_idx, x, work, B, K, 1);
