@@ -239,6 +239,17 @@ __global__ __launch_bounds__(256) void bt3_spd_kernel(
 // before parents).  work: [K][12][B] = accumulated diagonal block, then its
 // inverse (9), and the eliminated right-hand side (3).
 // ---------------------------------------------------------------------------
+// Optional membrane term of the ExtCell step: the solved block is
+// main + g [[1, -1, 0], [-1, 1, 0], [0, 0, 0]], added here in the order the
+// assembly used to write it (so the caller stores one g per node, not D).
+template <typename T>
+__device__ __forceinline__ void add_membrane(T* a, T g) {
+  a[0] += g;
+  a[1] -= g;
+  a[3] -= g;
+  a[4] += g;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void bt3_tree_kernel(const T* __restrict__ main,
                                                        const T* __restrict__ edge,
@@ -247,7 +258,7 @@ __global__ __launch_bounds__(256) void bt3_tree_kernel(const T* __restrict__ mai
                                                        const int64_t* __restrict__ order,
                                                        T* __restrict__ x,
                                                        T* __restrict__ work, int64_t B,
-                                                       int64_t K) {
+                                                       int64_t K, const T* __restrict__ gmem) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= B) return;
   const T* M = main + s * K * 9;
@@ -258,6 +269,12 @@ __global__ __launch_bounds__(256) void bt3_tree_kernel(const T* __restrict__ mai
   auto w = [&](int64_t k, int c) -> T& { return W[(k * 12 + c) * B]; };
   for (int64_t i = 0; i < K; ++i) {
     for (int c = 0; c < 9; ++c) w(i, c) = M[i * 9 + c];
+    if (gmem) {
+      T a[9];
+      for (int c = 0; c < 9; ++c) a[c] = w(i, c);
+      add_membrane(a, gmem[s * K + i]);
+      for (int c = 0; c < 9; ++c) w(i, c) = a[c];
+    }
     for (int c = 0; c < 3; ++c) {
       w(i, c * 4) += E[i * 3 + c];
       w(i, 9 + c) = R[i * 3 + c];
@@ -332,7 +349,7 @@ __global__ __launch_bounds__(256) void bt3_tree_layer_kernel(
     const int64_t* __restrict__ parent, const int64_t* __restrict__ order,
     const int64_t* __restrict__ layer_ptr, int n_layers, const int32_t* __restrict__ child_ptr,
     const int32_t* __restrict__ child_idx, T* __restrict__ x, T* __restrict__ work,
-    int64_t B, int64_t K, int topo_lds) {
+    int64_t B, int64_t K, int topo_lds, const T* __restrict__ gmem) {
   // G systems per workgroup (one wavefront each) walk the same layers, so the
   // workgroup barriers line up; the shared topology is staged once in LDS as
   // int32 (order, layer bounds, children CSR, parent) when it fits.
@@ -381,6 +398,7 @@ __global__ __launch_bounds__(256) void bt3_tree_layer_kernel(
   const T* M = main + so * K * 9;
   const T* E = edge + so * K * 3;
   const T* R = rhs + so * K * 3;
+  const T* Gm = gmem ? gmem + so * K : nullptr;
   T* X = x + so * K * 3;
   T* W = work + so * K * 12;
   if (RING && topo_lds) {
@@ -460,6 +478,7 @@ __global__ __launch_bounds__(256) void bt3_tree_layer_kernel(
       auto load_fwd = [&](int32_t i, T (&a)[9], T (&e)[3], T (&b)[3]) {
         const int64_t k = i >= 0 ? i : 0;
         for (int c = 0; c < 9; ++c) a[c] = M[k * 9 + c];
+        if (Gm) add_membrane(a, Gm[k]);
         for (int c = 0; c < 3; ++c) { e[c] = E[k * 3 + c]; b[c] = R[k * 3 + c]; }
       };
       auto elim = [&](int l, int32_t i, T (&a)[9], const T (&e)[3], T (&b)[3]) {
@@ -552,6 +571,7 @@ __global__ __launch_bounds__(256) void bt3_tree_layer_kernel(
       if (i < 0) continue;
       T a[9], b[3];
       for (int c = 0; c < 9; ++c) a[c] = M[i * 9 + c];
+      if (Gm) add_membrane(a, Gm[i]);
       for (int c = 0; c < 3; ++c) {
         a[c * 4] += E[i * 3 + c];
         b[c] = R[i * 3 + c];
@@ -638,7 +658,8 @@ template <typename T>
 static int bt3_tree_impl(const T* main, const T* edge, const T* rhs, const int64_t* parent,
                          const int64_t* order, const int64_t* layer_ptr, int64_t n_layers,
                          const int32_t* child_ptr, const int32_t* child_idx, T* x, int64_t B,
-                         int64_t K, T* work, int64_t work_elems, hipStream_t stream) {
+                         int64_t K, T* work, int64_t work_elems, hipStream_t stream,
+                         const T* gmem = nullptr) {
   if (!main || !edge || !rhs || !parent || !order || !x || !work || B < 1 || K < 1)
     return DN_EINVAL;
   if (work_elems < 12 * B * K || B > (int64_t)INT32_MAX * 256 || n_layers < 0 ||
@@ -664,20 +685,20 @@ static int bt3_tree_impl(const T* main, const T* edge, const T* rhs, const int64
     if (ring4 <= 96 * 1024) {
       hipLaunchKernelGGL((dn::bt3_tree_layer_kernel<T, true>), dim3((unsigned)((B + Gr - 1) / Gr)),
                          dim3(64 * Gr), ring4, stream, main, edge, rhs, parent, order, layer_ptr,
-                         (int)n_layers, child_ptr, child_idx, x, work, B, K, 1);
+                         (int)n_layers, child_ptr, child_idx, x, work, B, K, 1, gmem);
     } else if (ring <= 96 * 1024) {
       hipLaunchKernelGGL((dn::bt3_tree_layer_kernel<T, true>), dim3((unsigned)((B + G - 1) / G)),
                          dim3(64 * G), ring, stream, main, edge, rhs, parent, order, layer_ptr,
-                         (int)n_layers, child_ptr, child_idx, x, work, B, K, 1);
+                         (int)n_layers, child_ptr, child_idx, x, work, B, K, 1, gmem);
     } else {
       hipLaunchKernelGGL((dn::bt3_tree_layer_kernel<T, false>),
                          dim3((unsigned)((B + G - 1) / G)), dim3(64 * G), topo_lds ? topo : 0,
                          stream, main, edge, rhs, parent, order, layer_ptr, (int)n_layers,
-                         child_ptr, child_idx, x, work, B, K, topo_lds);
+                         child_ptr, child_idx, x, work, B, K, topo_lds, gmem);
     }
   } else {
     hipLaunchKernelGGL((dn::bt3_tree_kernel<T>), dim3(blocks_for<T>(B)), dim3(256), 0,
-                       stream, main, edge, rhs, parent, order, x, work, B, K);
+                       stream, main, edge, rhs, parent, order, x, work, B, K, gmem);
   }
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
@@ -737,7 +758,8 @@ DN_EXPORT int dn_bt3_tree_f64(const double* main, const double* edge, const doub
 //   f_i = e_i * (vc_i - vc_parent)   (e_i * vc_i at a root)
 //   D_i = main_i + [[g, -g, 0], [-g, g, 0], [0, 0, 0]]
 // then the block tree solve (bt3_tree_layer_kernel / bt3_tree_kernel) of
-// D, e, rhs for the increment, and vc += increment.
+// D, e, rhs for the increment, and vc += increment.  The assembly stores
+// only g; the solve adds it to the constant main block as it loads it.
 // ---------------------------------------------------------------------------
 namespace dn {
 
@@ -782,13 +804,7 @@ __global__ __launch_bounds__(256) void extcell_tree_assemble_kernel(XtArgs<T> a,
 #pragma unroll
     for (int c = 0; c < 3; ++c) r[c] += E[ch * 3 + c] * (V[ch * 3 + c] - vi[c]);
   }
-  T* Dj = a.D + j * 9;
-#pragma unroll
-  for (int c = 0; c < 9; ++c) Dj[c] = a.main[j * 9 + c];
-  Dj[0] += g;
-  Dj[1] -= g;
-  Dj[3] -= g;
-  Dj[4] += g;
+  a.D[j] = g;  // the solve adds g to the constant main block
 #pragma unroll
   for (int c = 0; c < 3; ++c) a.R[j * 3 + c] = r[c];
 }
@@ -834,9 +850,9 @@ static int extcell_tree_impl(const dn::XtArgs<T>& a0, const int32_t* parent32,
   for (int64_t s = 0; s < n_steps; ++s) {
     hipLaunchKernelGGL((dn::extcell_tree_assemble_kernel<T>), nodes, dim3(256), 0, stream, a,
                        s);
-    const int st = bt3_tree_impl<T>(a.D, a.edge, a.R, parent, order, layer_ptr, n_layers,
+    const int st = bt3_tree_impl<T>(a.main, a.edge, a.R, parent, order, layer_ptr, n_layers,
                                     a.child_ptr, a.child_idx, X, B, K, solve_work, 12 * B * K,
-                                    stream);
+                                    stream, a.D);
     if (st != DN_OK) return st;
     T* r = (rec && (s + 1) % rec_every == 0) ? rec + ((s + 1) / rec_every - 1) * B * K : nullptr;
     hipLaunchKernelGGL((dn::extcell_tree_update_kernel<T>), nodes, dim3(256), 0, stream, a.vc,
