@@ -38,6 +38,34 @@ def test_single_cable_fp64_records_match_oracle(orc, path):
     assert want.max() > 0.0, "the strongest pulses must fire action potentials"
 
 
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("path", ["persistent", "plain"])
+def test_warm_cable_takes_the_relaxation_branch_and_matches_oracle(orc, path, dtype):
+    """At 37 C (q10 = 3^3.07, about 29) and dt 0.025 ms the m gate's step
+    has |z| = q10 dt (am + bm) of about 3 at rest, beyond the Taylor range
+    (0.5): the exponential relaxation branch of the gate update runs every
+    step, and the n gate (|z| about 0.13) takes the full degree-9 series.
+    Bars: float64 1e-7 mV, float32 5e-3 mV, as for the cables at 6.3 C."""
+    from dendra_amd.model import HH
+    from dendra_amd.population import CablePopulation
+    from dendra_amd.workloads import single_cable, single_cable_wave
+
+    hh = HH(celsius=37.0)
+    v = -65.0
+    am = 0.1 * (v + 40) / (1 - np.exp(-(v + 40) / 10))
+    bm = 4 * np.exp(-(v + 65) / 18)
+    assert hh.q10 * 0.025 * (am + bm) > 0.5
+    spec = single_cable(8, seed=3)
+    wave = single_cable_wave(600, 0.025)
+    _, want = orc.cable_run(spec, hh.array(), 0.025, 600, i_wave=wave, record_every=1)
+    pop = CablePopulation(spec, 0.025, hh, dtype=dtype)
+    rec = pop.unpermute(pop.run(600, wave=wave, record_every=1, path=path))
+    err = np.abs(rec - want).max()
+    assert err <= (1e-7 if dtype == torch.float64 else 5e-3), err
+    assert np.ptp(want) > 10.0, "the pulses must move the membrane"
+
+
 def _uneven_sample():
     from dendra_amd.model import CableSpec
     from dendra_amd.workloads import uneven_cables
