@@ -42,8 +42,8 @@ _SIGNATURES = {
     "dn_tree_dhs_f32": (i32, [p, p, p, p, p, p, p, i64, i64, i64, p, i64, p]),
     "dn_tree_dhs_f64": (i32, [p, p, p, p, p, p, p, i64, i64, i64, p, i64, p]),
     "dn_hh_cable_rmax": (i32, []),
-    "dn_hh_cable_run_f32": (i32, _CABLE_RUN + [f32, p, p, i64, i64, i32, i32, p]),
-    "dn_hh_cable_run_f64": (i32, _CABLE_RUN + [f64, p, p, i64, i64, i32, i32, p]),
+    "dn_hh_cable_run_f32": (i32, _CABLE_RUN + [f32, p, p, i64, i64, i32, i32, p, p]),
+    "dn_hh_cable_run_f64": (i32, _CABLE_RUN + [f64, p, p, i64, i64, i32, i32, p, p]),
     "dn_hh_cable_step_f32": (
         i32,
         [i64, p, i64, i32, p, p, p, p, p, p, p, p, p, p, f32, f32, p, p, p, p],
@@ -58,14 +58,14 @@ _SIGNATURES = {
         _TREE_PLAN
         + [p, i32, p, i32, i32, i32, i32, p, p, p, p, p, p, p, p, p, p, p, i64, i64, f32]
         + [p, p]
-        + [i64, i64, p],
+        + [i64, i64, p, p],
     ),
     "dn_hh_tree_run_f64": (
         i32,
         _TREE_PLAN
         + [p, i32, p, i32, i32, i32, i32, p, p, p, p, p, p, p, p, p, p, p, i64, i64, f64]
         + [p, p]
-        + [i64, i64, p],
+        + [i64, i64, p, p],
     ),
     "dn_hh_tree_step_f32": (
         i32,

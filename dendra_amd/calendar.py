@@ -215,38 +215,46 @@ def pending_calendar_pop(state, slot, *, width, base=None, _scratch=None):
 
 
 def pending_calendar_from_legacy(chunks, *, capacity=None, device, dtype):
-    """Import slot -> [(indices, values)] chunks (values keep their graph)."""
-    slots, indices, values = [], [], []
-    for slot, bucket in chunks.items():
-        for idx, value in bucket:
-            idx = idx.to(device=device, dtype=torch.long).reshape(idx.numel())
-            value = value.to(device=device, dtype=dtype).reshape(value.numel())
-            if idx.numel() != value.numel():
-                raise ValueError("Pending calendar indices and values must align.")
-            slots.append(torch.full_like(idx, int(slot)))
-            indices.append(idx)
-            values.append(value)
-    count = sum(v.numel() for v in values)
-    capacity = count if capacity is None else capacity
-    state = pending_calendar_empty(capacity, device=device, dtype=dtype)
-    if count > capacity:
-        raise ValueError(f"Pending calendar capacity {capacity} cannot hold {count} entries.")
-    if count == 0:
-        return state
-    return pending_calendar_append(state, torch.cat(slots), torch.cat(indices),
-                                   torch.cat(values))
+    """Build a calendar from the legacy ``{slot: [(indices, values), ...]}``
+    buckets (networks/_pending_calendar.py:120-140 semantics): entries keep
+    the bucket order and, within a bucket, the chunk order; floating values
+    keep their autograd graph.  ``capacity`` defaults to the entry count."""
+    pieces = [(int(slot), idx, val) for slot, bucket in chunks.items() for idx, val in bucket]
+    idx_cols = [p[1].reshape(-1).to(device=device, dtype=torch.long) for p in pieces]
+    val_cols = [p[2].reshape(-1).to(device=device, dtype=dtype) for p in pieces]
+    if any(i.numel() != v.numel() for i, v in zip(idx_cols, val_cols)):
+        raise ValueError("legacy calendar chunk: index and value counts differ")
+    lengths = [i.numel() for i in idx_cols]
+    total = sum(lengths)
+    cap = total if capacity is None else capacity
+    empty = pending_calendar_empty(cap, device=device, dtype=dtype)
+    if total > cap:
+        raise ValueError(f"capacity {cap} cannot hold the {total} legacy calendar entries")
+    if total == 0:
+        return empty
+    slot_col = torch.repeat_interleave(
+        torch.tensor([p[0] for p in pieces], dtype=torch.long, device=device),
+        torch.tensor(lengths, dtype=torch.long, device=device))
+    return pending_calendar_append(empty, slot_col, torch.cat(idx_cols), torch.cat(val_cols))
 
 
 def pending_calendar_to_legacy(state):
-    """Export buckets in first-occurrence order, keeping value gradients."""
+    """Export the live entries as legacy buckets ``{slot: [(indices, values)]}``
+    (networks/_pending_calendar.py:143-161 semantics): one chunk per slot,
+    entries in position order, buckets ordered by their slot's first position;
+    values keep their gradients.  An overflowed calendar cannot be exported."""
     if bool(state.overflow.any()):
-        raise RuntimeError("Pending calendar overflow; increase pending_capacity.")
-    size = int(state.size.item())
-    buckets = {}
-    for position, slot in enumerate(state.slots[:size].detach().cpu().tolist()):
-        buckets.setdefault(int(slot), []).append(position)
-    return {
-        slot: [(state.indices.index_select(0, torch.tensor(pos, device=state.indices.device)),
-                state.values.index_select(0, torch.tensor(pos, device=state.values.device)))]
-        for slot, pos in buckets.items()
-    }
+        raise RuntimeError("pending calendar overflowed: raise its capacity before export")
+    live = state.slots[: int(state.size.item())].detach()
+    # A stable sort groups the positions by slot and keeps them in position
+    # order inside each group; the groups are then ranked by first position.
+    by_slot = torch.argsort(live, stable=True)
+    keys, counts = torch.unique_consecutive(live[by_slot], return_counts=True)
+    groups = torch.split(by_slot, counts.tolist())
+    heads = [int(g[0]) for g in groups]
+    out = {}
+    for k in sorted(range(len(groups)), key=heads.__getitem__):
+        pos = groups[k]
+        out[int(keys[k])] = [(state.indices.index_select(0, pos),
+                              state.values.index_select(0, pos))]
+    return out

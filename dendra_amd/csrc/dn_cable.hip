@@ -47,6 +47,7 @@ template <typename T> struct CableRunArgs {
   HHParams<T> hh;
   T* rec;                     // optional V record, [n_rec][total]
   int64_t rec_every, total;
+  int32_t* status;            // launch-contract status word (nullable)
 };
 
 // Per-row read-only coefficients staged once in LDS, row-major over lanes so
@@ -216,7 +217,10 @@ __global__ __launch_bounds__(256) void hh_cable_run_kernel(CableRunArgs<T> A) {
   const int wv = threadIdx.x >> 6;
   const int task = blockIdx.x * (blockDim.x >> 6) + wv;
   if (task >= A.n_tasks) return;
-  if (A.task_nr[task] > RMAX) return;  // launch contract violated: leave the cells
+  if (A.task_nr[task] > RMAX) {  // launch contract violated: report, leave the cells
+    flag_status(A.status, kStatusCableRows);
+    return;
+  }
   RowCoef<T>* cf = reinterpret_cast<RowCoef<T>*>(dn_smem_raw) + wv * RMAX * 64;
   RowScale<T>* cs = reinterpret_cast<RowScale<T>*>(
       reinterpret_cast<RowCoef<T>*>(dn_smem_raw) + (blockDim.x >> 6) * RMAX * 64) +
@@ -271,6 +275,7 @@ __global__ __launch_bounds__(64) void hh_cable_solve_kernel(
   const int64_t o = off[cell];
   const int64_t K = off[cell + 1] - o;
   const int nr = (int)((K + 63) / 64);
+  if (nr > max_nr) return;  // beyond the launch's LDS image: left unsolved
   const int64_t KP = 64 * (int64_t)nr;
   const int64_t KPmax = 64 * (int64_t)max_nr;
   T* sa = reinterpret_cast<T*>(dn_smem_raw);
@@ -352,7 +357,7 @@ static int cable_step_launch(int64_t n_cells, const int64_t* off, int64_t total,
       T *h, T *n, const T *gL, const T *gRp, const T *scale, const T *drive,         \
       const T *sna, const T *sk, const T *wave, int64_t step0, int64_t n_steps,      \
       T dt, const T *hh, T *rec, int64_t rec_every, int64_t total,                   \
-      int waves_per_block, int max_rows, hipStream_t stream
+      int waves_per_block, int max_rows, int32_t *status, hipStream_t stream
 
 #define DN_CABLE_RUN_BODY(T)                                                         \
   dn::CableRunArgs<T> A;                                                            \
@@ -363,7 +368,7 @@ static int cable_step_launch(int64_t n_cells, const int64_t* off, int64_t total,
   A.wave = wave; A.step0 = step0; A.n_steps = n_steps; A.dt = dt;                   \
   if (!hh) return DN_EINVAL;                                                        \
   A.hh = {hh[0], hh[1], hh[2], hh[3], hh[4], hh[5], hh[6]};                         \
-  A.rec = rec; A.rec_every = rec_every; A.total = total;                            \
+  A.rec = rec; A.rec_every = rec_every; A.total = total; A.status = status;         \
   return dn::cable_run_launch<T>(A, waves_per_block, max_rows, stream);
 
 // Persistent multi-step run of a planned ragged cable batch.  hh holds

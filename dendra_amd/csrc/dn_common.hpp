@@ -22,6 +22,18 @@
 
 namespace dn {
 
+// Launch-contract status word (caller-owned device int32, nullable).  A
+// persistent kernel that meets work outside what its launch promised sets
+// one of these bits and leaves that work unadvanced.
+enum : int32_t {
+  kStatusCableRows = 1,   // a cable task has more rows than the kernel instance
+  kStatusTreeGroup = 2,   // a tree group exceeds the block's threads or node capacity
+  kStatusTreeSize = 4,    // a tree exceeds the launch's node capacity (kmax)
+};
+__device__ __forceinline__ void flag_status(int32_t* status, int32_t bit) {
+  if (status) atomicOr(status, bit);
+}
+
 // ---------------------------------------------------------------------------
 // scalar helpers
 // ---------------------------------------------------------------------------

@@ -56,6 +56,7 @@ template <typename T> struct TreeRunArgs {
   HHParams<T> hh;
   T* rec;
   int64_t rec_every, total;
+  int32_t* status;  // launch-contract status word (nullable)
 };
 
 // LDS image of one tree: four node arrays and the section table.
@@ -216,6 +217,12 @@ __global__ __launch_bounds__(512) void hh_tree_run_kernel(TreeRunArgs<T> A,
   const int tid = threadIdx.x, nt = blockDim.x;
   const int64_t no = A.pl.node_off[tr];
   const int K = (int)(A.pl.node_off[tr + 1] - no);
+  const int n_sec = A.pl.sec_off[tr + 1] - A.pl.sec_off[tr];
+  const int n_lvl = A.pl.lvl_off[tr + 1] - A.pl.lvl_off[tr] - 1;
+  if (K > kmax || n_sec > smax || n_lvl > lmax) {  // launch contract violated
+    if (tid == 0) flag_status(A.status, kStatusTreeSize);
+    return;
+  }
   TreeLds<T> L = tree_lds_carve<T>(dn_smem_raw, kmax, smax, lmax);
   tree_stage_topology<T>(A.pl, tr, L);
   T v[kTreeSlots], m[kTreeSlots], h[kTreeSlots], n[kTreeSlots];
@@ -340,7 +347,10 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
     const int nl = A.pl.lvl_off[t + 1] - A.pl.lvl_off[t] - 1;
     nlvl_max = nl > nlvl_max ? nl : nlvl_max;
   }
-  if (sec_base > nt || node_base > node_cap) return;  // launch contract violated
+  if (sec_base > nt || node_base > node_cap) {  // launch contract violated
+    if (s == 0) flag_status(A.status, kStatusTreeGroup);
+    return;
+  }
   struct Loc { int tree, k, sec_base, node_base, ls, lvl; };
   auto locate = [&](int slot) {
     Loc L = {-1, -1, 0, 0, 0, -1};
@@ -605,6 +615,10 @@ __global__ __launch_bounds__(256) void hh_tree_solve_kernel(
   const int tr = blockIdx.x;
   const int64_t no = pl.node_off[tr];
   const int K = (int)(pl.node_off[tr + 1] - no);
+  // a tree beyond the launch's LDS image is left unsolved (block-uniform exit)
+  if (K > kmax || pl.sec_off[tr + 1] - pl.sec_off[tr] > smax ||
+      pl.lvl_off[tr + 1] - pl.lvl_off[tr] - 1 > lmax)
+    return;
   TreeLds<T> L = tree_lds_carve<T>(dn_smem_raw, kmax, smax, lmax);
   tree_stage_topology<T>(pl, tr, L);
   for (int i = threadIdx.x; i < K; i += blockDim.x) {
@@ -648,7 +662,7 @@ static int tree_run_impl(dn::TreePlan P, const int32_t* order, const int32_t* gr
                          const T* g, const T* drive, const T* sna, const T* sk,
                          const T* wave, int64_t step0, int64_t n_steps, T dt,
                          const T* hh, T* rec, int64_t rec_every, int64_t total,
-                         hipStream_t stream) {
+                         int32_t* status, hipStream_t stream) {
   if (!dn::plan_ok(P) || !v || !m || !h || !n || !area || !cmdt || !g || !hh)
     return DN_EINVAL;
   if (threads < 64 || threads > 512 || (threads & 63)) return DN_EINVAL;
@@ -666,7 +680,7 @@ static int tree_run_impl(dn::TreePlan P, const int32_t* order, const int32_t* gr
   A.g = g; A.drive = drive; A.sna = sna; A.sk = sk; A.wave = wave; A.step0 = step0;
   A.n_steps = n_steps; A.dt = dt;
   A.hh = {hh[0], hh[1], hh[2], hh[3], hh[4], hh[5], hh[6]};
-  A.rec = rec; A.rec_every = rec_every; A.total = total;
+  A.rec = rec; A.rec_every = rec_every; A.total = total; A.status = status;
   if (layout == 1) {
 #define DN_TREE_SEC(SC, CM)                                                                  \
   hipLaunchKernelGGL((dn::hh_tree_sec_kernel<T, SC, CM>), dim3(n_groups), dim3(threads), lds, \
@@ -699,13 +713,14 @@ DN_EXPORT int dn_hh_tree_run_f32(DN_TREE_PLAN_ARGS, const int32_t* order, int n_
                                  const float* g, const float* drive, const float* sna,
                                  const float* sk, const float* wave, int64_t step0,
                                  int64_t n_steps, float dt, const float* hh, float* rec,
-                                 int64_t rec_every, int64_t total, hipStream_t stream) {
+                                 int64_t rec_every, int64_t total, int32_t* status,
+                                 hipStream_t stream) {
   dn::TreePlan P;
   DN_TREE_PLAN_FILL(P)
   if (order) P.n_trees = n_launch;
   return tree_run_impl<float>(P, order, grp_off, n_groups, layout, kmax, threads, v, m, h, n, area, cmdt, g, drive,
                               sna, sk, wave, step0, n_steps, dt, hh, rec, rec_every,
-                              total, stream);
+                              total, status, stream);
 }
 
 DN_EXPORT int dn_hh_tree_run_f64(DN_TREE_PLAN_ARGS, const int32_t* order, int n_launch,
@@ -716,13 +731,13 @@ DN_EXPORT int dn_hh_tree_run_f64(DN_TREE_PLAN_ARGS, const int32_t* order, int n_
                                  const double* sk, const double* wave, int64_t step0,
                                  int64_t n_steps, double dt, const double* hh,
                                  double* rec, int64_t rec_every, int64_t total,
-                                 hipStream_t stream) {
+                                 int32_t* status, hipStream_t stream) {
   dn::TreePlan P;
   DN_TREE_PLAN_FILL(P)
   if (order) P.n_trees = n_launch;
   return tree_run_impl<double>(P, order, grp_off, n_groups, layout, kmax, threads, v, m, h, n, area, cmdt, g,
                                drive, sna, sk, wave, step0, n_steps, dt, hh, rec,
-                               rec_every, total, stream);
+                               rec_every, total, status, stream);
 }
 
 template <typename T>

@@ -88,48 +88,63 @@ def _threshold_transition(values, previous, thresholds, *, diff_spiking, tau):
     return ge.detach(), spiked.to(s.dtype) + (rise - rise.detach())
 
 
+def _netstim_gates(sel, *, binary, dtype, **_):
+    # generated events pass through; a hard gate is their truth value
+    return sel.to(torch.bool) if binary else sel.to(dtype)
+
+
+def _pre_var_gates(sel, *, binary, dtype, diff_spiking, **_):
+    if binary and sel.dtype == torch.bool:
+        return sel
+    sel = sel.to(dtype)
+    if diff_spiking:
+        return sel
+    positive = sel > 0
+    return positive if binary else positive.to(dtype)
+
+
+# mode -> (gate function, does it run the rising-edge detector)
+_GATE_MODES = {"netstim": _netstim_gates, "pre_var": _pre_var_gates, "threshold": None}
+
+
 def source_history_gates(source_values, has_spiked, *, source_idx, mode, threshold=None,
                          diff_spiking=False, tau=0.1, dtype=None, binary=False):
-    """One gate per unique source (_source_history.py:14-57)."""
+    """Observe one gate per unique source (semantics of
+    _source_history.py:14-57): returns (detector latch, gates).  Modes:
+    "netstim" (the source is an event stream), "pre_var" (gate = value > 0,
+    or the raw value under surrogate spiking), "threshold" (rising edge of
+    value >= threshold with the latch ``has_spiked``).  ``binary`` asks for
+    hard boolean gates and excludes surrogate spiking."""
+    if mode not in _GATE_MODES:
+        raise ValueError(f"unknown source-history mode {mode!r} "
+                         "(expected netstim, pre_var or threshold)")
     if binary and diff_spiking:
-        raise ValueError("binary source history cannot use surrogate spiking")
-    selected = source_values.reshape(source_values.numel()).index_select(0, source_idx)
-    if binary and mode == "netstim":
-        return has_spiked, selected.to(torch.bool)
-    if binary and mode == "pre_var" and selected.dtype == torch.bool:
-        return has_spiked, selected
-    dtype = source_values.dtype if dtype is None else dtype
-    selected = selected.to(dtype=dtype)
-    if mode == "netstim":
-        return has_spiked, selected
-    if mode == "pre_var":
-        if diff_spiking:
-            return has_spiked, selected
-        gates = selected > 0
-        return has_spiked, gates if binary else gates.to(dtype)
-    if mode == "threshold":
-        return _threshold_transition(selected, has_spiked, threshold,
-                                     diff_spiking=diff_spiking, tau=tau)
-    raise ValueError("source-history mode must be netstim, pre_var, or threshold")
+        raise ValueError("binary source-history gates exclude surrogate spiking")
+    sel = torch.index_select(source_values.flatten(), 0, source_idx)
+    work = source_values.dtype if dtype is None else dtype
+    fn = _GATE_MODES[mode]
+    if fn is not None:
+        return has_spiked, fn(sel, binary=binary, dtype=work, diff_spiking=diff_spiking)
+    return _threshold_transition(sel.to(work), has_spiked, threshold,
+                                 diff_spiking=diff_spiking, tau=tau)
+
+
+def _tap_columns(rows, cols):
+    """Broadcast a per-connection column index over the tap rows."""
+    return cols if rows.dim() == 1 else cols[:, None].expand(rows.shape)
 
 
 def read_float_history(history, rows, source_pos):
-    """Floating gates in connection order at one or several tap rows."""
-    if rows.ndim == 1:
-        return history[rows.reshape(rows.numel()), source_pos]
-    cols = source_pos.unsqueeze(-1).expand_as(rows)
-    return history[rows.reshape(-1), cols.reshape(-1)].reshape(rows.shape)
+    """Floating gates history[row, source] in connection order, for one tap
+    (rows of shape (n,)) or several (rows of shape (n, taps))."""
+    return history[rows, _tap_columns(rows, source_pos)]
 
 
 def read_packed_history(history, rows, word_idx, bit_mask, *, dtype):
-    """Hard gates from the signed-int64 packed word layout."""
-    if rows.ndim == 1:
-        words = history[rows.reshape(rows.numel()), word_idx]
-        return (torch.bitwise_and(words, bit_mask) != 0).to(dtype=dtype)
-    cols = word_idx.unsqueeze(-1).expand_as(rows)
-    mask = bit_mask.unsqueeze(-1).expand_as(rows)
-    words = history[rows.reshape(-1), cols.reshape(-1)].reshape(rows.shape)
-    return (torch.bitwise_and(words, mask) != 0).to(dtype=dtype)
+    """Hard gates from packed signed-int64 history words: bit ``bit_mask``
+    of word ``word_idx`` at each tap row, as 0/1 of ``dtype``."""
+    words = history[rows, _tap_columns(rows, word_idx)]
+    return (words & _tap_columns(rows, bit_mask)).ne(0).to(dtype)
 
 
 # -- projection --------------------------------------------------------------

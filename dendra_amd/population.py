@@ -38,13 +38,65 @@ _DT = {torch.float32: np.float32, torch.float64: np.float64}
 ROW_CLASSES = (10, 13)
 
 
+_STATUS_TEXT = {
+    1: "a cable task has more rows per lane than its kernel instance holds",
+    2: "a tree group exceeds its block's threads or node capacity",
+    4: "a tree exceeds its launch's node, section or level capacity",
+}
+
+
+def validate_cable_launches(task_nr, task_p, launches, rmax: int) -> None:
+    """Host check of a cable launch list before anything is enqueued: each
+    launch (t0, t1, rows) covers tasks t0..t1 whose row counts are <= rows
+    <= rmax, with 4..64 lanes per cable.  Raises ValueError otherwise (the
+    kernel would leave such a task unadvanced)."""
+    task_nr, task_p = np.asarray(task_nr), np.asarray(task_p)
+    if not set(np.unique(task_p).tolist()) <= {4, 8, 16, 32, 64}:
+        raise ValueError("cable tasks must use 4, 8, 16, 32 or 64 lanes per cable")
+    for t0, t1, rows in launches:
+        if not 1 <= rows <= rmax:
+            raise ValueError(f"launch row bound {rows} outside 1..{rmax}")
+        if len(task_nr[t0:t1]) and int(task_nr[t0:t1].max()) > rows:
+            raise ValueError(f"cable task with {int(task_nr[t0:t1].max())} rows per lane "
+                             f"in a launch bounded by {rows}")
+
+
+def validate_tree_groups(groups, n_sec, sizes, threads: int, node_cap: int) -> None:
+    """Host check of one layout-1 tree launch: every group's section count
+    fits the block's threads and its node count the LDS node capacity."""
+    for g in groups:
+        if int(np.sum(n_sec[g])) > threads:
+            raise ValueError(f"tree group of {int(np.sum(n_sec[g]))} sections exceeds "
+                             f"{threads} threads")
+        if int(np.sum(sizes[g])) > node_cap:
+            raise ValueError(f"tree group of {int(np.sum(sizes[g]))} nodes exceeds the "
+                             f"node capacity {node_cap}")
+
+
+class _Status:
+    """The launch-contract status word the persistent kernels report into."""
+
+    def _init_status(self):
+        self.status = torch.zeros(1, dtype=torch.int32, device=self.device)
+
+    def check(self) -> None:
+        """Raise RuntimeError if a kernel of this population reported a
+        launch-contract violation (work left unadvanced); waits for the
+        work already enqueued on the current stream."""
+        flags = int(self.status.item())
+        if flags:
+            self.status.zero_()
+            why = "; ".join(t for b, t in _STATUS_TEXT.items() if flags & b)
+            raise RuntimeError(f"persistent run left work unadvanced ({why})")
+
+
 def _dev(a, dtype, device):
     if a is None:
         return None
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)
 
 
-class CablePopulation:
+class CablePopulation(_Status):
     """Ragged batch of unbranched HH cables (backward Euler, increment form)."""
 
     def __init__(self, spec: CableSpec, dt: float, hh: HH | None = None,
@@ -84,6 +136,8 @@ class CablePopulation:
         for c in np.unique(cls):
             sel = np.flatnonzero(cls[t_order] == c)
             self._launches.append((int(sel[0]), int(sel[-1]) + 1, int(nr[t_order][sel].max())))
+        validate_cable_launches(nr[t_order], self.plan.task_p[t_order], self._launches,
+                                self.rmax)
         self.task_cell0 = i32(self.plan.task_cell0[t_order])
         self.task_ncell = i32(self.plan.task_ncell[t_order])
         self.task_nr = i32(nr[t_order])
@@ -91,6 +145,7 @@ class CablePopulation:
         self._side = None
         self.max_nr_plain = int(-(-int(sizes.max()) // 64)) if len(sizes) else 1
         self.hh_arr = self.hh.array().astype(_DT[dtype])
+        self._init_status()
         self.reset()
 
     @property
@@ -113,6 +168,7 @@ class CablePopulation:
 
     def state(self) -> dict:
         """State in user (spec) order as float64 numpy arrays."""
+        self.check()
         out = {}
         for name in ("v", "m", "h", "n"):
             a = np.empty(self.total)
@@ -121,6 +177,7 @@ class CablePopulation:
         return out
 
     def unpermute(self, rec: torch.Tensor) -> np.ndarray:
+        self.check()
         r = rec.double().cpu().numpy()
         out = np.empty_like(r)
         out[:, self.perm] = r
@@ -131,7 +188,14 @@ class CablePopulation:
             steps_per_launch: int | None = None, stream=None):
         """Advance n_steps.  ``wave`` is the per-step drive waveform (length
         >= t_step + n_steps) or None.  Returns the V record [n_rec, total] in
-        plan order when record_every > 0, else None."""
+        plan order when record_every > 0, else None.  ``stream`` (a
+        torch.cuda.Stream, None = the current stream) becomes the current
+        stream for the run: the run's tensors are allocated and its kernels
+        ordered on it."""
+        if stream is not None:
+            with torch.cuda.stream(stream):
+                return self.run(n_steps, wave, record_every, path, waves_per_block,
+                                steps_per_launch)
         lib = _lib.load()
         f64 = self.dtype == torch.float64
         wave_t = None
@@ -144,12 +208,10 @@ class CablePopulation:
             n_rec = (self.t_step + n_steps) // record_every - self.t_step // record_every
             rec_base = self.t_step // record_every
             rec = torch.empty((n_rec, self.total), dtype=self.dtype, device=self.device)
-        # `stream`: a torch.cuda.Stream (None = the current stream)
-        strm = (stream if stream is not None
-                else torch.cuda.current_stream(self.device)).cuda_stream
+        caller = torch.cuda.current_stream(self.device)
+        strm = caller.cuda_stream
         if path == "persistent":
             fn = lib.dn_hh_cable_run_f64 if f64 else lib.dn_hh_cable_run_f32
-            caller = stream if stream is not None else torch.cuda.current_stream(self.device)
             if len(self._launches) > 1 and self._side is None:
                 self._side = torch.cuda.Stream(self.device)
             chunk = steps_per_launch or n_steps
@@ -173,7 +235,7 @@ class CablePopulation:
                         _lib.ptr(self.scale), _lib.ptr(self.drive), _lib.ptr(self.sna),
                         _lib.ptr(self.sk), _lib.ptr(wave_t), self.t_step, k, self.dt,
                         self.hh_arr.ctypes.data, rec_ptr, max(record_every, 1), self.total,
-                        waves_per_block, rows, s_.cuda_stream,
+                        waves_per_block, rows, _lib.ptr(self.status), s_.cuda_stream,
                     )
                     _lib.check(st, "persistent cable run")
                 if len(self._launches) > 1:
@@ -215,7 +277,7 @@ class _TreeBucket:
     n_groups: int
 
 
-class TreePopulation:
+class TreePopulation(_Status):
     """Ragged batch of HH trees (backward Euler in current units, DHS)."""
 
     def __init__(self, spec: TreeSpec, dt: float, hh: HH | None = None,
@@ -274,6 +336,7 @@ class TreePopulation:
         self.kmax = int(sizes.max())
         self.bucket_edges = tuple(bucket_edges)
         self.slots = slots
+        self._init_status()
         self.set_block(block)
         self.reset()
 
@@ -321,6 +384,7 @@ class TreePopulation:
                 else:
                     groups.append([t]); loads.append(int(self.n_sec[t]))
                     nodes.append(int(sizes[t]))
+            validate_tree_groups(groups, self.n_sec, sizes, threads, int(max(nodes)))
             order = np.concatenate([np.asarray(g_) for g_ in groups])
             goff = np.zeros(len(groups) + 1, dtype=np.int64)
             goff[1:] = np.cumsum([len(g_) for g_ in groups])
@@ -354,6 +418,7 @@ class TreePopulation:
         return self._streams
 
     def state(self) -> dict:
+        self.check()
         out = {}
         for name in ("v", "m", "h", "n"):
             a = np.empty(self.total)
@@ -362,6 +427,7 @@ class TreePopulation:
         return out
 
     def unpermute(self, rec: torch.Tensor) -> np.ndarray:
+        self.check()
         r = rec.double().cpu().numpy()
         out = np.empty_like(r)
         out[:, self.perm] = r
@@ -369,6 +435,11 @@ class TreePopulation:
 
     def run(self, n_steps: int, wave=None, record_every: int = 0,
             path: str = "persistent", steps_per_launch: int | None = None, stream=None):
+        """Advance n_steps (as CablePopulation.run; ``stream`` likewise becomes
+        the current stream for the run)."""
+        if stream is not None:
+            with torch.cuda.stream(stream):
+                return self.run(n_steps, wave, record_every, path, steps_per_launch)
         lib = _lib.load()
         f64 = self.dtype == torch.float64
         wave_t = None
@@ -382,9 +453,8 @@ class TreePopulation:
             n_rec = (self.t_step + n_steps) // record_every - self.t_step // record_every
             rec_base = self.t_step // record_every
             rec = torch.empty((n_rec, self.total), dtype=self.dtype, device=self.device)
-        # `stream`: a torch.cuda.Stream (None = the current stream)
-        strm = (stream if stream is not None
-                else torch.cuda.current_stream(self.device)).cuda_stream
+        caller = torch.cuda.current_stream(self.device)
+        strm = caller.cuda_stream
         if path == "persistent":
             fn = lib.dn_hh_tree_run_f64 if f64 else lib.dn_hh_tree_run_f32
             chunk = steps_per_launch or n_steps
@@ -394,8 +464,6 @@ class TreePopulation:
                 rec_ptr = None
                 if rec is not None:
                     rec_ptr = rec.data_ptr() - rec_base * self.total * rec.element_size()
-                caller = stream if stream is not None else torch.cuda.current_stream(
-                    self.device)
                 # a single bucket runs on the caller's stream (no extra queue)
                 streams = [caller] if len(self.buckets) == 1 else self._bucket_streams()
                 start = torch.cuda.Event()
@@ -411,7 +479,7 @@ class TreePopulation:
                         _lib.ptr(self.g), _lib.ptr(self.drive), _lib.ptr(self.sna),
                         _lib.ptr(self.sk), _lib.ptr(wave_t), self.t_step, k, self.dt,
                         self.hh_arr.ctypes.data, rec_ptr, max(record_every, 1),
-                        self.total, bs.cuda_stream,
+                        self.total, _lib.ptr(self.status), bs.cuda_stream,
                     )
                     _lib.check(st, "persistent tree run")
                     if bs is not caller:

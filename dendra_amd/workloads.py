@@ -86,6 +86,24 @@ def uneven_cables(n: int, seed: int = 2, kmin: int = 21, kmax: int = 801,
     )
 
 
+def uneven_parity_sample() -> CableSpec:
+    """41 extracellularly driven cables for parity runs: 24 cables of the
+    uneven population (seed 5) plus edge-case sizes -- one compartment, the
+    lane and tile boundaries of the cable plan, and the largest plan (801,
+    832 compartments).  Geometry and lead fields repeat the first 24 cables'."""
+    base = uneven_cables(64, seed=5)
+    extra_k = [1, 2, 3, 4, 21, 52, 53, 64, 65, 104, 105, 208, 209, 416, 417, 801, 832]
+    sizes = np.concatenate([base.sizes[:24], extra_k])
+    off = np.zeros(len(sizes) + 1, dtype=np.int64)
+    off[1:] = np.cumsum(sizes)
+    total = int(off[-1])
+    cell = np.repeat(np.arange(len(sizes)), sizes)
+    src = base.off[cell % 24] + (np.arange(total) - off[cell]) % base.sizes[cell % 24]
+    return CableSpec(off, base.diam[src], base.dx[src], base.rhoa[src], base.cm[src],
+                     lead=base.lead[src], ve_amp=base.ve_amp[np.arange(len(sizes)) % 24],
+                     v_init=-65.0)
+
+
 def random_binary_tree(target: int, key: int, seed: int):
     """One tree: parent (parent < child), per-node diameter, x/y/z (um)."""
     u = lambda n, stream: rng.uniform_n(seed, stream, n, start=key * 1_000_003)

@@ -366,7 +366,11 @@ int dn_extcell_tree_run_f64(int64_t B, int64_t K, const double* main, const doub
  * step, row (step0+s+1)/rec_every-1 of a [rows][total] array.  max_rows
  * (1..dn_hh_cable_rmax()) bounds task_nr over the launch and selects the
  * kernel instance (8, 10 or 13 rows per lane); a task with more rows than
- * the instance holds is left unadvanced. */
+ * the instance holds is left unadvanced and sets bit 1 of *status.
+ * status (nullable): caller-owned device int32 that the kernel ORs launch-
+ * contract violations into (1 cable rows, 2 tree group over threads or node
+ * capacity, 4 tree over kmax/smax/lmax); the caller reads it after the stream
+ * has run (dendra_amd.population does, and raises DN_ERANGE's RuntimeError). */
 int dn_hh_cable_rmax(void);
 int dn_hh_cable_run_f32(int n_tasks, const int32_t* task_cell0,
                         const int32_t* task_ncell, const int32_t* task_nr,
@@ -376,7 +380,7 @@ int dn_hh_cable_run_f32(int n_tasks, const int32_t* task_cell0,
                         const float* sk, const float* wave, int64_t step0,
                         int64_t n_steps, float dt, const float* hh, float* rec,
                         int64_t rec_every, int64_t total, int waves_per_block,
-                        int max_rows, hipStream_t stream);
+                        int max_rows, int32_t* status, hipStream_t stream);
 int dn_hh_cable_run_f64(int n_tasks, const int32_t* task_cell0,
                         const int32_t* task_ncell, const int32_t* task_nr,
                         const int32_t* task_p, const int64_t* off, double* v, double* m,
@@ -385,7 +389,7 @@ int dn_hh_cable_run_f64(int n_tasks, const int32_t* task_cell0,
                         const double* sk, const double* wave, int64_t step0,
                         int64_t n_steps, double dt, const double* hh, double* rec,
                         int64_t rec_every, int64_t total, int waves_per_block,
-                        int max_rows, hipStream_t stream);
+                        int max_rows, int32_t* status, hipStream_t stream);
 /* One plain step (gating+assembly launch, solve launch); mainv/rhs are
  * caller scratch of `total` values; w is this step's waveform value. */
 int dn_hh_cable_step_f32(int64_t n_cells, const int64_t* off, int64_t total, int max_nr,
@@ -415,7 +419,9 @@ int dn_hh_cable_step_f64(int64_t n_cells, const int64_t* off, int64_t total, int
  * tree; layout 1 (one thread per section, state in registers) takes kmax =
  * the launch's largest section count <= threads <= 512, layout 0 (eight node
  * slots per thread) requires
- * kmax >= each tree's size and kmax <= 8 * threads. */
+ * kmax >= each tree's size and kmax <= 8 * threads.  Work outside those
+ * bounds is left unadvanced and reported in *status (nullable device int32,
+ * bits as for dn_hh_cable_run_*). */
 int dn_hh_tree_slots(void);
 int dn_hh_tree_run_f32(int n_trees, const int64_t* node_off, const int32_t* parent,
                        const int32_t* ch_ptr, const int32_t* ch_idx,
@@ -431,7 +437,7 @@ int dn_hh_tree_run_f32(int n_trees, const int64_t* node_off, const int32_t* pare
                        const float* drive, const float* sna, const float* sk,
                        const float* wave, int64_t step0, int64_t n_steps, float dt,
                        const float* hh, float* rec, int64_t rec_every, int64_t total,
-                       hipStream_t stream);
+                       int32_t* status, hipStream_t stream);
 int dn_hh_tree_run_f64(int n_trees, const int64_t* node_off, const int32_t* parent,
                        const int32_t* ch_ptr, const int32_t* ch_idx,
                        const int32_t* sec_off, const int32_t* sec_head,
@@ -446,7 +452,7 @@ int dn_hh_tree_run_f64(int n_trees, const int64_t* node_off, const int32_t* pare
                        const double* drive, const double* sna, const double* sk,
                        const double* wave, int64_t step0, int64_t n_steps, double dt,
                        const double* hh, double* rec, int64_t rec_every, int64_t total,
-                       hipStream_t stream);
+                       int32_t* status, hipStream_t stream);
 int dn_hh_tree_step_f32(int n_trees, const int64_t* node_off, const int32_t* parent,
                         const int32_t* ch_ptr, const int32_t* ch_idx,
                         const int32_t* sec_off, const int32_t* sec_head,

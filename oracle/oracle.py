@@ -77,6 +77,11 @@ def load():
                     dp, dp, dp, dp, vp, i64, i32]
         lib.orc_cable_run.argtypes = run_args
         lib.orc_cable_run.restype = i32
+        prec_args = [i64, lp, dp, dp, dp, dp, vp, vp, dp, vp, vp, vp, f64, i64,
+                     dp, dp, dp, dp, vp, i64, i32]
+        for fn in (lib.orc_cable_run_f32, lib.orc_cable_run_f32gates):
+            fn.argtypes = prec_args
+            fn.restype = i32
         lib.orc_tree_run.argtypes = [i64, lp, lp] + run_args[2:]
         lib.orc_tree_run.restype = i32
         lib.orc_extcell_run.argtypes = [i64, i64, dp, dp, dp, dp, dp, vp, vp, dp, vp, vp, vp,
@@ -196,6 +201,38 @@ def cable_run(spec, hh_arr, dt, n_steps, ve_wave=None, i_wave=None, record_every
                 dt, n_steps, ve_wave, i_wave, record_every, threads, state)
 
 
+def cable_run_f32(spec, hh_arr, dt, n_steps, ve_wave=None, record_every=0, threads=0,
+                  state=None, binary64_voltage=False):
+    """The reference's float32 cable step restated in binary32
+    (orc_cable_run_f32), or with binary32 gating and a binary64 voltage,
+    assembly and elimination (orc_cable_run_f32gates).  Same arguments and
+    results as cable_run (no intracellular current)."""
+    lib = load()
+    fn = lib.orc_cable_run_f32gates if binary64_voltage else lib.orc_cable_run_f32
+    total = spec.total
+    sna, sna_p = _opt(spec.sna)
+    sk, sk_p = _opt(spec.sk)
+    lead, lead_p = _opt(spec.lead)
+    vea, vea_p = _opt(spec.ve_amp)
+    vew, vew_p = _opt(ve_wave)
+    if state is None:
+        m0, h0, n0 = hh_steady(spec.v_init)
+        V = np.full(total, float(spec.v_init))
+        M, H, N = np.full(total, m0), np.full(total, h0), np.full(total, n0)
+    else:
+        V, M, H, N = (np.array(state[k], dtype=np.float64) for k in ("v", "m", "h", "n"))
+    rec = rec_p = None
+    if record_every > 0:
+        rec = np.zeros((n_steps // record_every, total))
+        rec_p = rec.ctypes.data_as(vp)
+    st = fn(spec.n_cells, _c(spec.off, np.int64), _c(spec.diam), _c(spec.dx), _c(spec.rhoa),
+            _c(spec.cm), sna_p, sk_p, _c(hh_arr), lead_p, vea_p, vew_p, float(dt),
+            int(n_steps), V, M, H, N, rec_p, int(record_every), int(threads))
+    if st != 0:
+        raise RuntimeError(f"oracle run failed ({st})")
+    return {"v": V, "m": M, "h": H, "n": N}, rec
+
+
 def extcell_run(ws, hh_arr, dt, n_steps, state, *, sna=None, sk=None, lead=None, amp=None,
                 wave=None, iamp=None, i_wave=None, record_every=0, threads=0):
     """Double-layer extracellular HH cables (orc_extcell_run).  ws: the
@@ -307,8 +344,11 @@ def uniform(seed, stream, i):
     return lib.orc_uniform(seed, stream, i)
 
 
-def network_run(spec, hh_arr, dt, n_steps, record_spikes=True):
-    """Binary64 network simulation; returns (state dict, spikes [n_steps][n_src])."""
+def network_run(spec, hh_arr, dt, n_steps, record_spikes=True, v0=None):
+    """Binary64 network simulation; returns (state dict, spikes [n_steps][n_src]).
+
+    v0 (optional, one value per compartment) replaces the initial voltage; the
+    gates still start at the steady state of spec.cells.v_init."""
     lib = load()
     i64, f64 = C.c_int64, C.c_double
     lib.orc_network_run.argtypes = [i64, lp, lp, dp, dp, dp, dp, vp, vp, dp, dp, f64, i64,
@@ -319,6 +359,8 @@ def network_run(spec, hh_arr, dt, n_steps, record_spikes=True):
     total, n_cells = cells.total, cells.n_cells
     m0, h0, n0 = hh_steady(cells.v_init)
     V = np.full(total, float(cells.v_init))
+    if v0 is not None:
+        V[:] = np.asarray(v0, dtype=np.float64)
     M, H, N = np.full(total, m0), np.full(total, h0), np.full(total, n0)
     A, B = np.zeros(n_cells), np.zeros(n_cells)
     latch = np.zeros(n_cells, dtype=np.uint8)

@@ -1,0 +1,17 @@
+"""Voltage bars (mV) of the GPU parity tests against the binary64 oracle.
+
+F64: the float64 kernels agree with the oracle to round-off.
+F32_NORTH_STAR: the voltage bar the float32 path is asked to meet.
+F32: the bar the float32 cable runs are held to on spiking workloads.  It is
+wider than F32_NORTH_STAR because no float32 state can meet 1e-3 mV there;
+tests/test_precision.py shows it on the parity sample of 41 uneven cables
+(4000 steps of 5 us): a binary64 run moved by half a float32 ulp of V at
+t = 0 already ends > 1e-3 mV away, and the reference's float32 step
+restated in binary32 -- with or without a binary64 voltage and elimination
+-- lands between 1e-3 and 5e-3 mV.  Spike steps must still agree within
+one dt.
+"""
+
+F64 = 1e-7
+F32_NORTH_STAR = 1e-3
+F32 = 5e-3

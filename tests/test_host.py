@@ -339,3 +339,33 @@ def test_depth_levels_are_whole_levels_deepest_first():
                     assert lp[l + 1] <= np.flatnonzero(order == par[i])[0] < lp[l + 2]
     assert _depth_levels(torch.tensor([1, 2, 0])) is None
     assert _depth_levels(torch.tensor([-1, 2, 1])) is None
+
+
+def test_cable_launch_list_is_checked_before_launch():
+    """A launch whose tasks need more rows per lane than its bound (or than
+    the largest kernel instance) is rejected on the host, before any work
+    is enqueued."""
+    from dendra_amd.model import plan_cables
+    from dendra_amd.population import validate_cable_launches
+
+    plan = plan_cables(np.array([21, 200, 801, 832, 5]), 13, 64)
+    nr, tp = plan.task_nr, plan.task_p
+    validate_cable_launches(nr, tp, [(0, len(nr), int(nr.max()))], 13)
+    with pytest.raises(ValueError, match="rows per lane"):
+        validate_cable_launches(nr, tp, [(0, len(nr), int(nr.max()) - 1)], 13)
+    with pytest.raises(ValueError, match="outside"):
+        validate_cable_launches(nr, tp, [(0, len(nr), 14)], 13)
+    with pytest.raises(ValueError, match="lanes"):
+        validate_cable_launches(nr, np.full_like(tp, 2), [(0, len(nr), 13)], 13)
+
+
+def test_tree_groups_are_checked_before_launch():
+    from dendra_amd.population import validate_tree_groups
+
+    n_sec = np.array([300, 200, 13])
+    sizes = np.array([2000, 1500, 100])
+    validate_tree_groups([[0, 1], [2]], n_sec, sizes, 512, 3500)
+    with pytest.raises(ValueError, match="threads"):
+        validate_tree_groups([[0, 1, 2]], n_sec, sizes, 512, 4000)
+    with pytest.raises(ValueError, match="node capacity"):
+        validate_tree_groups([[0, 1]], n_sec, sizes, 512, 3000)

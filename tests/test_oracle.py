@@ -301,3 +301,112 @@ def test_network_oracle_single_cell_fires_from_background(orc):
     n_cell = int(spikes[:, 0].sum())
     assert n_bg > 3 and n_cell >= 1
     assert spikes[:, 0].argmax() > spikes[:, 1].argmax()
+
+
+def _one_event_net(k_tgt, soma_local, delay, w):
+    """Two passive cells: cell 0 (one compartment, started at +20 mV above the
+    0 mV threshold) spikes exactly once, at step 0, and projects one event of
+    weight ``w`` (uS) with ``delay`` steps onto the exp2syn of cell 1
+    (``k_tgt`` compartments, synapse on compartment ``soma_local``)."""
+    from dendra_amd.model import CableSpec
+    from dendra_amd.network import Connectivity, NetworkSpec
+
+    off = np.array([0, 1, 1 + k_tgt], dtype=np.int64)
+    total = 1 + k_tgt
+    diam = np.concatenate([[20.0], np.linspace(2.0, 3.0, k_tgt)])
+    dx = np.concatenate([[20.0], np.full(k_tgt, 15.0)])
+    cells = CableSpec(off, diam, dx, np.full(total, 110.0), np.full(total, 1.3),
+                      sna=np.zeros(total), sk=np.zeros(total), v_init=-54.3)
+    conn = Connectivity(np.array([0], np.int32), np.array([1], np.int32),
+                        np.array([delay], np.int32), np.array([w], np.float64), 2, 2)
+    spec = NetworkSpec(cells, 1 if k_tgt > 1 else 2, np.array([0, 1 + soma_local]), conn,
+                       tau1=0.5, tau2=5.0, e_syn=-10.0, theta=0.0, bg_rate_hz=0.0, seed=5)
+    v0 = np.full(total, -54.3)
+    v0[0] = 20.0
+    return spec, v0
+
+
+@pytest.mark.parametrize("n_steps", [1, 4, 5, 6, 7, 20, 161])
+def test_network_oracle_exp2syn_single_event_closed_form(orc, n_steps):
+    """One event of weight w sent at step 0 with a 5-step delay is delivered
+    at the start of step 5 and then decays by the exact exponential in the
+    same step (synapses advance before the populations, net.py:335-377), so
+    after N steps B - A = w * factor * (exp(-k dt/tau2) - exp(-k dt/tau1)) with
+    k = N - 5 (zero for N <= 5); the factor makes the peak of that difference
+    equal to w."""
+    from dendra_amd.model import HH
+
+    d, w, dt = 5, 0.004, 0.025
+    spec, v0 = _one_event_net(1, 0, d, w)
+    st, spikes = orc.network_run(spec, HH().array(), dt, n_steps, v0=v0)
+    assert spikes[:, 0].tolist() == [1] + [0] * (n_steps - 1)
+    assert not spikes[:, 1].any()
+    k = n_steps - d
+    tau1, tau2 = 0.5, 5.0
+    # factor from the peak time of the two-exponential difference
+    tp = tau1 * tau2 / (tau2 - tau1) * math.log(tau2 / tau1)
+    factor = 1.0 / (math.exp(-tp / tau2) - math.exp(-tp / tau1))
+    a = w * factor * math.exp(-k * dt / tau1) if k > 0 else 0.0
+    b = w * factor * math.exp(-k * dt / tau2) if k > 0 else 0.0
+    assert st["A"][1] == pytest.approx(a, rel=1e-12, abs=1e-300)
+    assert st["B"][1] == pytest.approx(b, rel=1e-12, abs=1e-300)
+    assert st["A"][0] == 0.0 and st["B"][0] == 0.0
+
+
+def _geometry_dense_step(diam, dx, rhoa, cm, gl, el, e_syn, soma, dt):
+    """Backward-Euler matrix pieces of a passive cable from geometry alone:
+    conductances in S, capacitances in F, potentials in mV, time in s."""
+    area = np.pi * diam * 1e-4 * dx * 1e-4                 # cm^2
+    r = 0.5 * diam * 1e-4
+    R = rhoa * dx * 1e-4 / (np.pi * r * r)                 # ohm, each half-cable pair
+    g_ax = 2.0 / (R[:-1] + R[1:])
+    C = cm * 1e-6 * area
+    K = len(diam)
+    Lax = np.zeros((K, K))
+    for j, g in enumerate(g_ax):
+        Lax[j, j] -= g
+        Lax[j + 1, j + 1] -= g
+        Lax[j, j + 1] += g
+        Lax[j + 1, j] += g
+    dts = dt * 1e-3
+
+    def step(v, g_syn_uS):
+        gs = g_syn_uS * 1e-6                                # S
+        L = Lax.copy()
+        L[np.diag_indices(K)] -= gl * area
+        L[soma, soma] -= gs
+        b = gl * el * area
+        b[soma] += gs * e_syn
+        L /= C[:, None]
+        return np.linalg.solve(np.eye(K) - dts * L, v + dts * b / C)
+
+    return step
+
+
+@pytest.mark.parametrize("k_tgt,soma", [(1, 0), (6, 2)])
+def test_network_oracle_exp2syn_drives_passive_cell_like_dense_recurrence(orc, k_tgt, soma):
+    """The synaptic current i = (B - A)(v - e) nA enters the membrane as a
+    conductance (B - A) uS on the synapse's compartment (mod/EXP2SYN.py:147-152,
+    converted with the compartment area): a passive target cell, driven only by
+    one event, follows the dense backward-Euler recurrence built from its
+    geometry with g_syn = B - A after each step's decay."""
+    from dendra_amd.model import HH
+
+    d, w, dt, n = 3, 0.02, 0.025, 240
+    spec, v0 = _one_event_net(k_tgt, soma, d, w)
+    hh = HH()
+    st, _ = orc.network_run(spec, hh.array(), dt, n, v0=v0)
+    c = spec.cells
+    sl = slice(1, 1 + k_tgt)
+    step = _geometry_dense_step(c.diam[sl], c.dx[sl], c.rhoa[sl], c.cm[sl], hh.gl, hh.el,
+                                spec.e_syn, soma, dt)
+    tau1, tau2 = 0.5, 5.0
+    tp = tau1 * tau2 / (tau2 - tau1) * math.log(tau2 / tau1)
+    factor = 1.0 / (math.exp(-tp / tau2) - math.exp(-tp / tau1))
+    v = v0[sl].copy()
+    for s in range(n):
+        k = s - d + 1
+        g = w * factor * (math.exp(-k * dt / tau2) - math.exp(-k * dt / tau1)) if k > 0 else 0.0
+        v = step(v, g)
+    assert np.abs(v - v0[sl]).max() > 1e-3           # the event moved the cell
+    np.testing.assert_allclose(st["v"][sl], v, rtol=0, atol=1e-11)

@@ -1,0 +1,79 @@
+"""Why the float32 cable runs are held to 5e-3 mV, not 1e-3 mV (CPU).
+
+Action potentials amplify perturbations: on the 41-cable parity sample a
+binary64 run whose initial voltage is moved by half a float32 ulp at 65 mV
+(2^-18 mV) ends more than 1e-3 mV away from the unperturbed run.  Any float32
+state commits roundings of that size at every step.  Two restatements of the
+reference's float32 step (oracle/dn_oracle_prec.c) show it directly: all
+binary32, and binary32 gating with a binary64 voltage and Thomas elimination
+(the float64 elimination the path allows for long cables) both land between
+1e-3 and 5e-3 mV of the binary64 oracle, with spike steps within one dt.  So
+the float64 elimination does not buy the 1e-3 mV bar; float64 state does.
+"""
+
+import numpy as np
+import pytest
+
+import bars
+
+
+@pytest.fixture(scope="module")
+def sample(orc):
+    from dendra_amd.model import HH
+    from dendra_amd.workloads import extracellular_wave, uneven_parity_sample
+
+    spec = uneven_parity_sample()
+    n_steps, dt = 4000, 0.005
+    wave = extracellular_wave(n_steps, dt)
+    hh = HH(celsius=6.3)
+    _, want = orc.cable_run(spec, hh.array(), dt, n_steps, ve_wave=wave, record_every=1)
+    return spec, hh, wave, dt, n_steps, want
+
+
+def _first_crossing(rec, theta=0.0):
+    above = rec >= theta
+    return np.where(above.any(axis=0), above.argmax(axis=0), -1)
+
+
+def test_half_ulp_initial_perturbation_exceeds_north_star_bar(orc, sample):
+    spec, hh, wave, dt, n_steps, want = sample
+    m0, h0, n0 = orc.hh_steady(-65.0)
+    T = spec.total
+    state = {"v": np.full(T, -65.0 + 2.0**-18), "m": np.full(T, m0), "h": np.full(T, h0),
+             "n": np.full(T, n0)}
+    _, got = orc.cable_run(spec, hh.array(), dt, n_steps, ve_wave=wave, record_every=1,
+                           state=state)
+    err = np.abs(got - want).max()
+    assert err > bars.F32_NORTH_STAR, err
+    assert (_first_crossing(want) >= 0).sum() > 100
+
+
+@pytest.mark.parametrize("binary64_voltage", [False, True])
+def test_float32_restatements_land_between_the_bars(orc, sample, binary64_voltage):
+    spec, hh, wave, dt, n_steps, want = sample
+    _, got = orc.cable_run_f32(spec, hh.array(), dt, n_steps, ve_wave=wave, record_every=1,
+                               binary64_voltage=binary64_voltage)
+    err = np.abs(got - want).max()
+    assert bars.F32_NORTH_STAR < err <= bars.F32, err
+    fired, got_fired = _first_crossing(want), _first_crossing(got)
+    assert np.array_equal(fired >= 0, got_fired >= 0)
+    assert np.abs(fired - got_fired).max() <= 1
+
+
+def test_float32_restatement_is_binary32(orc):
+    """Pin of the restatement itself: on a resting uniform cable (no drive)
+    the float32 run stays within float32 round-off of the binary64 one, and
+    its recorded voltages are exactly representable in binary32."""
+    from dendra_amd.model import HH, CableSpec
+
+    K = 30
+    off = np.array([0, K])
+    spec = CableSpec(off, np.full(K, 2.0), np.full(K, 10.0), np.full(K, 100.0), np.ones(K))
+    hh = HH()
+    _, want = orc.cable_run(spec, hh.array(), 0.025, 200, record_every=1)
+    _, got = orc.cable_run_f32(spec, hh.array(), 0.025, 200, record_every=1)
+    assert np.array_equal(got.astype(np.float32).astype(np.float64), got)
+    assert np.abs(got - want).max() < 1e-4
+    _, mixed = orc.cable_run_f32(spec, hh.array(), 0.025, 200, record_every=1,
+                                 binary64_voltage=True)
+    assert not np.array_equal(mixed.astype(np.float32).astype(np.float64), mixed)
