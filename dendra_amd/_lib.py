@@ -46,11 +46,11 @@ _SIGNATURES = {
     "dn_hh_cable_run_f64": (i32, _CABLE_RUN + [f64, p, p, i64, i64, i32, i32, p, p]),
     "dn_hh_cable_step_f32": (
         i32,
-        [i64, p, i64, i32, p, p, p, p, p, p, p, p, p, p, f32, f32, p, p, p, p],
+        [i64, p, i64, i32, p, p, p, p, p, p, p, p, p, p, f32, f32, p, p, p, p, p, p, p],
     ),
     "dn_hh_cable_step_f64": (
         i32,
-        [i64, p, i64, i32, p, p, p, p, p, p, p, p, p, p, f64, f64, p, p, p, p],
+        [i64, p, i64, i32, p, p, p, p, p, p, p, p, p, p, f64, f64, p, p, p, p, p, p, p],
     ),
     "dn_hh_tree_slots": (i32, []),
     "dn_hh_tree_run_f32": (
@@ -69,11 +69,11 @@ _SIGNATURES = {
     ),
     "dn_hh_tree_step_f32": (
         i32,
-        _TREE_PLAN + [p, i32, i64, p, p, p, p, p, p, p, p, p, p, f32, f32, p, p, p, p],
+        _TREE_PLAN + [p, i32, i64, p, p, p, p, p, p, p, p, p, p, f32, f32, p, p, p, p, p, p],
     ),
     "dn_hh_tree_step_f64": (
         i32,
-        _TREE_PLAN + [p, i32, i64, p, p, p, p, p, p, p, p, p, p, f64, f64, p, p, p, p],
+        _TREE_PLAN + [p, i32, i64, p, p, p, p, p, p, p, p, p, p, f64, f64, p, p, p, p, p, p],
     ),
 }
 
@@ -153,6 +153,18 @@ for _sfx in ("f32", "f64", "i32"):
         i32, [p, p, p, i64, p, p, i64, p, p, p, p, p, p, p, p, i64, p])
 
 
+_SIGNATURES["dn_bitpack_pack"] = (i32, [p, i64, p, i64, i64, p, p])
+for _sfx in ("f32", "f64"):
+    _SIGNATURES[f"dn_bitpack_deliver_{_sfx}"] = (
+        i32, [p, i64, i64, p, p, i64, p, p, p, p, p, i64, i64, p])
+
+
+_SIGNATURES["dn_dense_event_step"] = (i32, [p, p, i64, i64, p, p, p, p])
+_SIGNATURES["dn_ordered_scatter_work_bytes"] = (i64, [i64, i64, i32])
+for _sfx in ("f32", "f64"):
+    _SIGNATURES[f"dn_ordered_scatter_add_{_sfx}"] = (i32, [p, i64, p, p, i64, p, i64, p])
+
+
 def _bind(lib) -> None:
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name, None)
@@ -183,6 +195,10 @@ def load():
             _bind(lib)
             _lib = lib
     return _lib
+
+
+def loaded() -> bool:
+    return _lib is not None
 
 
 def check(status: int, what: str) -> None:

@@ -245,6 +245,7 @@ __global__ __launch_bounds__(256) void hh_cable_assemble_kernel(
     T* __restrict__ n_, const T* __restrict__ gL, const T* __restrict__ gRp,
     const T* __restrict__ scale, const T* __restrict__ drive,
     const T* __restrict__ sna, const T* __restrict__ sk, T w, T dt, HHParams<T> hh,
+    const T* __restrict__ ve, const T* __restrict__ intra, const T* __restrict__ icap,
     T* __restrict__ mainv, T* __restrict__ rhs) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= total) return;
@@ -256,8 +257,15 @@ __global__ __launch_bounds__(256) void hh_cable_assemble_kernel(
   // Neighbours across a cable boundary carry a zero conductance.
   const T vr = i + 1 < total ? v_[i + 1] : v;
   const T vl = i > 0 ? v_[i - 1] : v;
-  const T f = (J - G * v) * sc + g_l * (vr - v) - g_r * (v - vl) +
-              (drive ? drive[i] * w : T(0));
+  T f = (J - G * v) * sc + g_l * (vr - v) - g_r * (v - vl) +
+        (drive ? drive[i] * w : T(0));
+  if (ve) {  // extracellular potential of this step: axial drive of its differences
+    const T e = ve[i];
+    const T er = i + 1 < total ? ve[i + 1] : e;
+    const T el = i > 0 ? ve[i - 1] : e;
+    f += g_l * (er - e) - g_r * (e - el);
+  }
+  if (intra) f += intra[i] * icap[i];  // injected current (mA) over C
   const T dts = dt * T(1e-3);
   mainv[i] = T(1) + dts * (g_l + g_r + G * sc);
   rhs[i] = dts * f;
@@ -332,18 +340,20 @@ template <typename T>
 static int cable_step_launch(int64_t n_cells, const int64_t* off, int64_t total,
                              int max_nr, T* v, T* m, T* h, T* n, const T* gL,
                              const T* gRp, const T* scale, const T* drive, const T* sna,
-                             const T* sk, T w, T dt, HHParams<T> hh, T* mainv, T* rhs,
+                             const T* sk, T w, T dt, HHParams<T> hh, const T* ve,
+                             const T* intra, const T* icap, T* mainv, T* rhs,
                              hipStream_t stream) {
   if (!off || !v || !m || !h || !n || !gL || !gRp || !scale || !mainv || !rhs)
     return DN_EINVAL;
+  if (intra && !icap) return DN_EINVAL;
   if (n_cells < 1 || total < 1 || max_nr < 1 || n_cells > 0x7fffffff)
     return DN_EINVAL;
   size_t lds = (size_t)4 * 64 * max_nr * sizeof(T);
   if (lds > 64 * 1024) return DN_ERANGE;
   int blocks = (int)((total + 255) / 256);
   hipLaunchKernelGGL(hh_cable_assemble_kernel<T>, dim3(blocks), dim3(256), 0, stream,
-                     total, v, m, h, n, gL, gRp, scale, drive, sna, sk, w, dt, hh, mainv,
-                     rhs);
+                     total, v, m, h, n, gL, gRp, scale, drive, sna, sk, w, dt, hh, ve, intra,
+                     icap, mainv, rhs);
   hipLaunchKernelGGL(hh_cable_solve_kernel<T>, dim3((unsigned)n_cells), dim3(64), lds,
                      stream, off, gL, gRp, mainv, rhs, v, dt, max_nr);
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
@@ -384,13 +394,14 @@ DN_EXPORT int dn_hh_cable_step_f32(int64_t n_cells, const int64_t* off, int64_t 
                                    int max_nr, float* v, float* m, float* h, float* n,
                                    const float* gL, const float* gRp, const float* scale,
                                    const float* drive, const float* sna, const float* sk,
-                                   float w, float dt, const float* hh, float* mainv,
+                                   float w, float dt, const float* hh, const float* ve,
+                                   const float* intra, const float* icap, float* mainv,
                                    float* rhs, hipStream_t stream) {
   if (!hh) return DN_EINVAL;
   dn::HHParams<float> p = {hh[0], hh[1], hh[2], hh[3], hh[4], hh[5], hh[6]};
   return dn::cable_step_launch<float>(n_cells, off, total, max_nr, v, m, h, n, gL, gRp,
-                                      scale, drive, sna, sk, w, dt, p, mainv, rhs,
-                                      stream);
+                                      scale, drive, sna, sk, w, dt, p, ve, intra, icap,
+                                      mainv, rhs, stream);
 }
 
 DN_EXPORT int dn_hh_cable_step_f64(int64_t n_cells, const int64_t* off, int64_t total,
@@ -398,11 +409,12 @@ DN_EXPORT int dn_hh_cable_step_f64(int64_t n_cells, const int64_t* off, int64_t 
                                    double* n, const double* gL, const double* gRp,
                                    const double* scale, const double* drive,
                                    const double* sna, const double* sk, double w,
-                                   double dt, const double* hh, double* mainv,
+                                   double dt, const double* hh, const double* ve,
+                                   const double* intra, const double* icap, double* mainv,
                                    double* rhs, hipStream_t stream) {
   if (!hh) return DN_EINVAL;
   dn::HHParams<double> p = {hh[0], hh[1], hh[2], hh[3], hh[4], hh[5], hh[6]};
   return dn::cable_step_launch<double>(n_cells, off, total, max_nr, v, m, h, n, gL,
-                                       gRp, scale, drive, sna, sk, w, dt, p, mainv, rhs,
-                                       stream);
+                                       gRp, scale, drive, sna, sk, w, dt, p, ve, intra,
+                                       icap, mainv, rhs, stream);
 }

@@ -11,12 +11,15 @@
 //   three taps wg f_j at row slot + k - 1 + j (j = 0, 1, 2; the fractions are
 //              the normalised Gaussian weights), connection-major, tap-minor
 //              (the order of the reference's flattened scatter).
+//   one tap    wg at row slot + delay: the fixed integer-delay transition
+//              dense_fixed_delay_transition (_event_delivery.py:125-199),
+//              whose scatter_add accumulates in connection order.
 // Rows wrap modulo depth.  Each ring cell receives its contributions in the
 // reference's order, so the result is bitwise the reference's: one lane per
 // target walks that target's connections (target-major CSR, ascending
 // connection index), and products and sums are rounded separately (no
 // contraction into FMA).  `base` is the first tap's row offset modulo depth
-// (k for two taps, k - 1 for three).
+// (the delay for one tap, k for two taps, k - 1 for three).
 #include "dn_common.hpp"
 
 namespace dn {
@@ -44,7 +47,13 @@ __global__ __launch_bounds__(256) void interp_delay_enqueue_kernel(
   const int slot = ring_slot(slot_p, depth);
   q[(int64_t)slot * width + t] = T(0);
   const int e0 = in_ptr[t], e1 = in_ptr[t + 1];
-  if (TAPS == 2) {
+  if (TAPS == 1) {
+    for (int i = e0; i < e1; ++i) {
+      const int e = in_edge[i];
+      T* cell = q + ring_row(slot, base[e], 0, depth) * width + t;
+      *cell = *cell + w[e] * g[e];
+    }
+  } else if (TAPS == 2) {
     for (int j = 0; j < 2; ++j) {
       for (int i = e0; i < e1; ++i) {
         const int e = in_edge[i];
@@ -90,7 +99,14 @@ __global__ __launch_bounds__(64) void interp_delay_tile_kernel(
     tile[r * 64 + lane] = (live && r != slot) ? qin[(int64_t)r * width + t] : T(0);
   if (live) {
     const int e0 = in_ptr[t], e1 = in_ptr[t + 1];
-    if (TAPS == 2) {
+    if (TAPS == 1) {
+#pragma unroll 4
+      for (int i = e0; i < e1; ++i) {
+        const int e = in_edge[i];
+        T* cell = tile + ring_row(slot, base[e], 0, depth) * 64 + lane;
+        *cell = *cell + w[e] * g[e];
+      }
+    } else if (TAPS == 2) {
       for (int j = 0; j < 2; ++j) {
 #pragma unroll 4
         for (int i = e0; i < e1; ++i) {
@@ -116,7 +132,8 @@ __global__ __launch_bounds__(64) void interp_delay_tile_kernel(
   }
 }
 
-// Per-connection cotangents for a ring cotangent G.  Two taps:
+// Per-connection cotangents for a ring cotangent G.  One tap: dw = g G,
+// dg = w G at the connection's cell.  Two taps:
 //   dw = g ((1 - f) G_0 + f G_1),  dg = w (...),  df = w g (G_1 - G_0);
 // three taps: dw = g sum_j f_j G_j, dg = w sum_j f_j G_j, df_j = w g G_j.
 template <typename T, int TAPS>
@@ -130,7 +147,9 @@ __global__ __launch_bounds__(256) void interp_delay_vjp_kernel(
   const int slot = ring_slot(slot_p, depth);
   const int64_t t = post[e];
   T mix;
-  if (TAPS == 2) {
+  if (TAPS == 1) {
+    mix = G[ring_row(slot, base[e], 0, depth) * width + t];
+  } else if (TAPS == 2) {
     const T lo = G[ring_row(slot, base[e], 0, depth) * width + t];
     const T hi = G[ring_row(slot, base[e], 1, depth) * width + t];
     mix = (T(1) - f[e]) * lo + f[e] * hi;
@@ -155,11 +174,14 @@ static int interp_enqueue_impl(const T* qin, T* q, int64_t depth, int64_t width,
                                const int32_t* in_edge, const T* w, const T* g, const T* f,
                                const int32_t* base, hipStream_t stream) {
   if (!q || !slot || !in_ptr || depth < 1 || depth > (1 << 30) || width < 1) return DN_EINVAL;
-  if ((taps != 2 && taps != 3) || width > (int64_t)INT32_MAX * 256) return DN_EINVAL;
+  if (taps < 1 || taps > 3 || width > (int64_t)INT32_MAX * 256) return DN_EINVAL;
   const size_t lds = (size_t)depth * 64 * sizeof(T);
   if (qin && qin != q && lds <= 64 * 1024) {
     const dim3 tiles((unsigned)((width + 63) / 64));
-    if (taps == 2)
+    if (taps == 1)
+      hipLaunchKernelGGL((dn::interp_delay_tile_kernel<T, 1>), tiles, dim3(64), lds, stream,
+                         qin, q, (int)depth, width, slot, in_ptr, in_edge, w, g, f, base);
+    else if (taps == 2)
       hipLaunchKernelGGL((dn::interp_delay_tile_kernel<T, 2>), tiles, dim3(64), lds, stream,
                          qin, q, (int)depth, width, slot, in_ptr, in_edge, w, g, f, base);
     else
@@ -172,7 +194,10 @@ static int interp_enqueue_impl(const T* qin, T* q, int64_t depth, int64_t width,
           hipSuccess)
     return DN_ELAUNCH;
   const dim3 grid((unsigned)((width + 255) / 256));
-  if (taps == 2)
+  if (taps == 1)
+    hipLaunchKernelGGL((dn::interp_delay_enqueue_kernel<T, 1>), grid, dim3(256), 0, stream,
+                       q, (int)depth, width, slot, in_ptr, in_edge, w, g, f, base);
+  else if (taps == 2)
     hipLaunchKernelGGL((dn::interp_delay_enqueue_kernel<T, 2>), grid, dim3(256), 0, stream,
                        q, (int)depth, width, slot, in_ptr, in_edge, w, g, f, base);
   else
@@ -188,12 +213,16 @@ static int interp_vjp_impl(const T* G, int64_t depth, int64_t width, const int64
                            hipStream_t stream) {
   if (!G || !slot || depth < 1 || depth > (1 << 30) || width < 1 || n_edge < 0)
     return DN_EINVAL;
-  if (taps != 2 && taps != 3) return DN_EINVAL;
+  if (taps < 1 || taps > 3) return DN_EINVAL;
   if (n_edge == 0) return DN_OK;
-  if (!post || !w || !g || !f || !base || !dw || !dg || !df) return DN_EINVAL;
+  if (!post || !w || !g || !base || !dw || !dg) return DN_EINVAL;
+  if (taps > 1 && (!f || !df)) return DN_EINVAL;
   if (n_edge > (int64_t)INT32_MAX * 256) return DN_EINVAL;
   const dim3 grid((unsigned)((n_edge + 255) / 256));
-  if (taps == 2)
+  if (taps == 1)
+    hipLaunchKernelGGL((dn::interp_delay_vjp_kernel<T, 1>), grid, dim3(256), 0, stream, G,
+                       (int)depth, width, slot, n_edge, post, w, g, f, base, dw, dg, df);
+  else if (taps == 2)
     hipLaunchKernelGGL((dn::interp_delay_vjp_kernel<T, 2>), grid, dim3(256), 0, stream, G,
                        (int)depth, width, slot, n_edge, post, w, g, f, base, dw, dg, df);
   else

@@ -40,6 +40,11 @@ class DenseTransition(NamedTuple):
     event_queue: torch.Tensor | None
 
 
+class DenseDue(NamedTuple):
+    payload: torch.Tensor
+    event_counts: torch.Tensor | None
+
+
 def linear_delay_taps(positions):
     """Lower index and upper-tap fraction of positions in steps
     (_event_delivery.py:200-203)."""
@@ -101,7 +106,8 @@ class _InterpEnqueue(torch.autograd.Function):
         depth, width = map(int, queue.shape)
         qin = queue.contiguous()
         out = torch.empty_like(qin)
-        w, g, f = (v.contiguous() for v in (weights, gates, fraction))
+        w, g = weights.contiguous(), gates.contiguous()
+        f = None if fraction is None else fraction.contiguous()  # None: one tap
         p = _lib.ptr
         st = _fn("dn_interp_delay_enqueue", out)(p(qin), p(out), depth, width, p(slot), taps,
                                                  p(in_ptr), p(in_edge), p(w), p(g), p(f),
@@ -120,7 +126,7 @@ class _InterpEnqueue(torch.autograd.Function):
         dq.index_fill_(0, slot.remainder(depth), 0)
         n = w.numel()
         dw, dg = torch.empty_like(w), torch.empty_like(w)
-        df = torch.empty_like(f)
+        df = None if f is None else torch.empty_like(f)
         p = _lib.ptr
         st = _fn("dn_interp_delay_vjp", grad)(p(grad), depth, width, p(slot), ctx.taps, n,
                                               p(post), p(w), p(g), p(f), p(base), p(dw),
@@ -159,3 +165,152 @@ def dense_interpolated_delay_transition(queue, slot, global_step, weights, conne
                                connection_gates.reshape(-1), fraction, base, in_ptr, in_edge,
                                post, int(taps))
     return DenseTransition(out, (slot + 1).remainder(depth), global_step + 1, None)
+
+
+# -- fixed integer delays: the default dense backend ------------------------
+#
+# Drop-ins for dense_due, gather_source_gates, connection_event_counts,
+# _fixed_delay_indices, _clear_dense_slot, _enqueue_dense_values and
+# dense_fixed_delay_transition (_event_delivery.py:30-199), which
+# NetCon.advance_non_diff_dense_uniform/_mixed and the dense training path
+# call (netcon.py:5183-5205).  The scatters run as HIP kernels whose every
+# ring cell receives its values in connection order, bitwise the sequential
+# scatter_add; rows and index arithmetic are plain tensor indexing.
+
+
+def dense_due(queue, slot, event_queue=None):
+    """Today's target payload (and per-connection event counts): independent
+    copies of row ``slot`` (_event_delivery.py:30-43)."""
+    payload = queue.index_select(0, slot.reshape(1)).squeeze(0)
+    events = None if event_queue is None else \
+        event_queue.index_select(0, slot.reshape(1)).squeeze(0)
+    return DenseDue(payload, events)
+
+
+def gather_source_gates(source_gates, pre_idx):
+    """Source gates in connection order, values and gradients unchanged."""
+    return source_gates.flatten().index_select(0, pre_idx)
+
+
+def connection_event_counts(intrinsic_gates, scheduled_counts=None):
+    """int32 event counts: positive intrinsic gates plus scheduled events."""
+    counts = intrinsic_gates.gt(0).to(torch.int32)
+    return counts if scheduled_counts is None else counts + scheduled_counts
+
+
+def _fixed_delay_indices(slot, post_idx, delay_steps, *, depth, width, flat_offsets=None,
+                         uniform_delay=None):
+    """Flat ring index of each connection's destination cell: row
+    (slot + delay) mod depth, column post (_event_delivery.py:46-59)."""
+    if uniform_delay is not None:
+        return ((slot + uniform_delay) % depth) * width + post_idx
+    if flat_offsets is not None:
+        return (slot * width + flat_offsets) % (depth * width)
+    return ((slot + delay_steps) % depth) * width + post_idx
+
+
+def _clear_dense_slot(queue, slot, *, inplace=False):
+    """Zero row ``slot``: in place on caller-owned storage, else a new ring."""
+    if inplace:
+        return queue.index_fill_(0, slot.reshape(-1), 0)
+    keep = torch.arange(queue.shape[0], device=queue.device).ne(slot.reshape(1))
+    return queue * keep.to(queue.dtype).reshape(-1, *([1] * (queue.dim() - 1)))
+
+
+class _OrderedScatter(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, queue, destination, values, inplace):
+        out = queue if inplace else queue.contiguous().clone()
+        n = destination.numel()
+        lib = _lib.load()
+        f64 = out.dtype == torch.float64
+        nbytes = lib.dn_ordered_scatter_work_bytes(n, out.numel(), int(f64))
+        work = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=out.device)
+        fn = lib.dn_ordered_scatter_add_f64 if f64 else lib.dn_ordered_scatter_add_f32
+        st = fn(_lib.ptr(out), out.numel(), _lib.ptr(destination), _lib.ptr(values), n,
+                _lib.ptr(work), work.numel(), _lib.stream_handle(out.device))
+        _lib.check(st, "ordered scatter-add into the delay ring")
+        ctx.save_for_backward(destination)
+        if inplace:
+            ctx.mark_dirty(out)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad):
+        (destination,) = ctx.saved_tensors
+        return grad, None, grad.reshape(-1).index_select(0, destination), None
+
+
+def _enqueue_dense_values(queue, destination, values, *, inplace=False):
+    """queue.flat[destination[i]] += values[i] in index order, bitwise the
+    sequential scatter_add (_event_delivery.py:61-76).  ``inplace`` writes
+    into the caller's contiguous ring; otherwise a new ring is returned."""
+    if queue.device.type != "cuda":
+        raise ValueError("the GPU ring scatter requires GPU tensors")
+    if queue.dtype not in (torch.float32, torch.float64):
+        raise TypeError("the GPU ring scatter supports float32 and float64 rings")
+    if inplace and not queue.is_contiguous():
+        raise ValueError("an in-place ring must be contiguous")
+    dest = destination.reshape(-1).to(torch.int64).contiguous()
+    vals = values.reshape(-1).to(queue.dtype).contiguous()
+    if vals.numel() != dest.numel():
+        raise ValueError("destination and values must have one entry each")
+    return _OrderedScatter.apply(queue, dest, vals, bool(inplace))
+
+
+def _event_ring_step(event_queue, slot, delay_steps, event_counts):
+    if event_queue.dtype != torch.int32 or event_counts.dtype != torch.int32:
+        raise TypeError("event_queue and event_counts must be int32")
+    n = event_counts.numel()
+    if event_queue.ndim != 2 or event_queue.shape[1] != n:
+        raise ValueError("event_queue must be [depth, connections]")
+    qin = event_queue.contiguous()
+    out = torch.empty_like(qin)
+    delay = delay_steps.reshape(-1).to(torch.int64).contiguous()
+    st = _lib.load().dn_dense_event_step(
+        _lib.ptr(qin), _lib.ptr(out), qin.shape[0], n, _lib.ptr(slot), _lib.ptr(delay),
+        _lib.ptr(event_counts.reshape(-1).contiguous()), _lib.stream_handle(qin.device))
+    _lib.check(st, "event ring step")
+    return out
+
+
+def dense_fixed_delay_transition(queue, slot, global_step, weights, connection_gates, *,
+                                 post_idx, delay_steps, depth, width, event_queue=None,
+                                 event_counts=None, _clone_once=False):
+    """Clear today's row, enqueue ``weights * connection_gates`` at row
+    (slot + delay) mod depth of each connection's target, advance the
+    counters (_event_delivery.py:125-199).  Pure: inputs are not modified and
+    a new ring is returned (``_clone_once`` gives the same result).  Each ring
+    cell receives its connections in connection order, so the ring is
+    bitwise the reference's.  Gradients flow to the queue (zero on the
+    cleared row), the weights and the gates.  With event tracking, the int32
+    ``event_queue`` [depth, connections] gets each connection's count in its
+    own column."""
+    if (event_queue is None) != (event_counts is None):
+        raise ValueError("event tracking needs both event_queue and event_counts")
+    if queue.ndim != 2 or tuple(queue.shape) != (int(depth), int(width)):
+        raise ValueError(f"queue must have shape ({depth}, {width}), got {tuple(queue.shape)}")
+    if queue.device.type != "cuda":
+        raise ValueError("the GPU delay transition requires GPU tensors")
+    if queue.dtype not in (torch.float32, torch.float64):
+        raise TypeError("the GPU delay transition supports float32 and float64")
+    n = post_idx.numel()
+    for name, v in (("weights", weights), ("connection_gates", connection_gates)):
+        if v.numel() != n or v.dtype != queue.dtype or v.device != queue.device:
+            raise ValueError(f"{name} must hold one {queue.dtype} value per connection "
+                             f"on {queue.device}")
+    delay = torch.as_tensor(delay_steps, device=queue.device).reshape(-1)
+    if delay.numel() == 1 and n != 1:
+        delay = delay.expand(n)
+    if delay.numel() != n:
+        raise ValueError("delay_steps must hold one delay per connection (or one for all)")
+    in_ptr, in_edge, post = delay_plan(post_idx, int(width))
+    base = delay.to(torch.int64).remainder(int(depth)).to(torch.int32).contiguous()
+    slot_t = slot.reshape(1).to(torch.int64)
+    out = _InterpEnqueue.apply(queue, slot_t, weights.reshape(-1),
+                               connection_gates.reshape(-1), None, base, in_ptr, in_edge,
+                               post, 1)
+    events = None
+    if event_queue is not None:
+        events = _event_ring_step(event_queue, slot_t, delay, event_counts)
+    return DenseTransition(out, (slot + 1).remainder(depth), global_step + 1, events)

@@ -158,6 +158,7 @@ int dn_bt3_tree_f64(const double* main, const double* edge, const double* rhs,
  * transition of qin (qin null or equal to q: in place; else qin is read and
  * left unchanged): row *slot (device int64, reduced modulo depth) is
  * cleared; then
+ *   taps 1: every connection adds w g at row slot + base (f unused);
  *   taps 2: every connection adds w g (1 - f) at row slot + base and w g f at
  *           row slot + base + 1, all lower taps before all upper taps;
  *   taps 3: every connection adds w g f[3e + j] at row slot + base + j,
@@ -390,18 +391,25 @@ int dn_hh_cable_run_f64(int n_tasks, const int32_t* task_cell0,
                         int64_t n_steps, double dt, const double* hh, double* rec,
                         int64_t rec_every, int64_t total, int waves_per_block,
                         int max_rows, int32_t* status, hipStream_t stream);
-/* One plain step (gating+assembly launch, solve launch); mainv/rhs are
- * caller scratch of `total` values; w is this step's waveform value. */
+/* One plain step (gating+assembly launch, solve launch) of
+ * _bwd_euler_ub._step (integrators/core.py:769-834, implicit.py:569-664);
+ * mainv/rhs are caller scratch of `total` values; w is this step's waveform
+ * value for `drive`.  ve (nullable): this step's extracellular potential
+ * per compartment (mV), entering through its axial differences; intra
+ * (nullable): injected current per compartment (mA), scaled by icap = 1/C
+ * (1/F; required with intra).  v is advanced in place. */
 int dn_hh_cable_step_f32(int64_t n_cells, const int64_t* off, int64_t total, int max_nr,
                          float* v, float* m, float* h, float* n, const float* gL,
                          const float* gRp, const float* scale, const float* drive,
                          const float* sna, const float* sk, float w, float dt,
-                         const float* hh, float* mainv, float* rhs, hipStream_t stream);
+                         const float* hh, const float* ve, const float* intra,
+                         const float* icap, float* mainv, float* rhs, hipStream_t stream);
 int dn_hh_cable_step_f64(int64_t n_cells, const int64_t* off, int64_t total, int max_nr,
                          double* v, double* m, double* h, double* n, const double* gL,
                          const double* gRp, const double* scale, const double* drive,
                          const double* sna, const double* sk, double w, double dt,
-                         const double* hh, double* mainv, double* rhs,
+                         const double* hh, const double* ve, const double* intra,
+                         const double* icap, double* mainv, double* rhs,
                          hipStream_t stream);
 
 /* ---- fused HH + backward Euler, branched trees ----------------------------
@@ -453,6 +461,11 @@ int dn_hh_tree_run_f64(int n_trees, const int64_t* node_off, const int32_t* pare
                        const double* wave, int64_t step0, int64_t n_steps, double dt,
                        const double* hh, double* rec, int64_t rec_every, int64_t total,
                        int32_t* status, hipStream_t stream);
+/* One plain step of _dhs._step (integrators/tree.py:1202-1357): per-node
+ * gating and the current-form residual (mA), then the section-chain solve.
+ * ve (nullable): this step's extracellular potential per node (mV), as edge
+ * currents g (ve_child - ve_parent); intra (nullable): injected current per
+ * node (mA).  v is advanced in place. */
 int dn_hh_tree_step_f32(int n_trees, const int64_t* node_off, const int32_t* parent,
                         const int32_t* ch_ptr, const int32_t* ch_idx,
                         const int32_t* sec_off, const int32_t* sec_head,
@@ -464,8 +477,8 @@ int dn_hh_tree_step_f32(int n_trees, const int64_t* node_off, const int32_t* par
                         int64_t total, float* v, float* m, float* h, float* n,
                         const float* area, const float* cmdt, const float* g,
                         const float* drive, const float* sna, const float* sk, float w,
-                        float dt, const float* hh, float* mainv, float* rhs,
-                        hipStream_t stream);
+                        float dt, const float* hh, const float* ve, const float* intra,
+                        float* mainv, float* rhs, hipStream_t stream);
 int dn_hh_tree_step_f64(int n_trees, const int64_t* node_off, const int32_t* parent,
                         const int32_t* ch_ptr, const int32_t* ch_idx,
                         const int32_t* sec_off, const int32_t* sec_head,
@@ -477,8 +490,9 @@ int dn_hh_tree_step_f64(int n_trees, const int64_t* node_off, const int32_t* par
                         int64_t total, double* v, double* m, double* h, double* n,
                         const double* area, const double* cmdt, const double* g,
                         const double* drive, const double* sna, const double* sk,
-                        double w, double dt, const double* hh, double* mainv,
-                        double* rhs, hipStream_t stream);
+                        double w, double dt, const double* hh, const double* ve,
+                        const double* intra, double* mainv, double* rhs,
+                        hipStream_t stream);
 
 /* ---- event delivery ---------------------------------------------------------
  * Delivery from a spike tape.  Replaces the bitpacked source history and its
@@ -664,6 +678,59 @@ int dn_mlp_gather(int nz, int Bn, int F, const int32_t* idx, const void* X, cons
 int dn_mlp_ensemble(int64_t n, int nz, const float* pred, int64_t sp, const float* lmag,
                     float ymean, float ystd, float spread, float* thr, int32_t* flag,
                     hipStream_t stream);
+
+/* ---- packed source history and dense delivery (NetCon bitpack) -----------
+ * Replaces the reference's netcon_bitpack extension: pack_source_spikes
+ * (networks/netcon_bitpack_kernel.cu:20-57,130-148; .cpp:32-48) and
+ * build_delivery / build_delivery_uniform (.cu:59-126,150-228; .cpp:50-143),
+ * called through networks/netcon_bitpack_ops.py:559-700 by
+ * netcon.py:3758-3827.  history: int64 [depth][n_words], n_words =
+ * ceil(n_source / 63); word w holds sources 63w..63w+62 in bits 0..62.
+ * cur: device int64[1], the ring row of this step.
+ * dn_bitpack_pack writes row cur from spikes (bool bytes, n_source).
+ * dn_bitpack_deliver_* adds weight[e] into out[post[e]] for every connection
+ * whose bit (word_idx[e], bit_mask[e]) is set in row (cur - delay[e]) mod
+ * depth; delay NULL means one delay for all, uniform_delay (below 1 -> 1, at
+ * or beyond depth -> modulo depth, as the reference).  out (n_out values) is
+ * added to, not cleared; hits on one target add in an unspecified order.
+ * Words, targets or rows out of range are skipped. */
+int dn_bitpack_pack(const uint8_t* spikes, int64_t n_source, int64_t* history,
+                    int64_t depth, int64_t n_words, const int64_t* cur, hipStream_t stream);
+int dn_bitpack_deliver_f32(const int64_t* history, int64_t depth, int64_t n_words,
+                           const int64_t* cur, const int64_t* delay, int64_t uniform_delay,
+                           const int64_t* word_idx, const int64_t* bit_mask,
+                           const int64_t* post, const float* weight, float* out,
+                           int64_t n_out, int64_t n_conn, hipStream_t stream);
+int dn_bitpack_deliver_f64(const int64_t* history, int64_t depth, int64_t n_words,
+                           const int64_t* cur, const int64_t* delay, int64_t uniform_delay,
+                           const int64_t* word_idx, const int64_t* bit_mask,
+                           const int64_t* post, const double* weight, double* out,
+                           int64_t n_out, int64_t n_conn, hipStream_t stream);
+
+/* ---- dense delay ring with fixed integer delays ---------------------------
+ * The fixed-delay transition dense_fixed_delay_transition
+ * (networks/_event_delivery.py:125-199, the default dense backend of
+ * NetCon.advance_non_diff_dense_uniform/_mixed, netcon.py:5183-5205) is
+ * dn_interp_delay_enqueue_* with taps = 1, base[e] = delay[e] mod depth and
+ * f = NULL: every ring cell receives w[e] * g[e] in connection order; its
+ * cotangent is dn_interp_delay_vjp_* with taps = 1 (f, df NULL).
+ * dn_dense_event_step: the int32 event-count ring [depth][n_conn] of the same
+ * transition (:175-190): qout = qin with row slot cleared, then
+ * qout[(slot + delay[e]) mod depth][e] += counts[e].
+ * dn_ordered_scatter_add_*: _enqueue_dense_values (:61-76), q[dest[i]] +=
+ * values[i] for i in index order per cell (bitwise the sequential
+ * scatter_add); dest outside [0, n_cells) is dropped; `work` holds
+ * dn_ordered_scatter_work_bytes(n, n_cells, f64) bytes. */
+int dn_dense_event_step(const int32_t* qin, int32_t* qout, int64_t depth, int64_t n_conn,
+                        const int64_t* slot, const int64_t* delay, const int32_t* counts,
+                        hipStream_t stream);
+int64_t dn_ordered_scatter_work_bytes(int64_t n, int64_t n_cells, int f64);
+int dn_ordered_scatter_add_f32(float* q, int64_t n_cells, const int64_t* dest,
+                               const float* values, int64_t n, void* work, int64_t work_bytes,
+                               hipStream_t stream);
+int dn_ordered_scatter_add_f64(double* q, int64_t n_cells, const int64_t* dest,
+                               const double* values, int64_t n, void* work,
+                               int64_t work_bytes, hipStream_t stream);
 
 #ifdef __cplusplus
 }

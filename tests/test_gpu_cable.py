@@ -49,7 +49,8 @@ def test_warm_cable_takes_the_relaxation_branch_and_matches_oracle(orc, path, dt
     has |z| = q10 dt (am + bm) of about 3 at rest, beyond the Taylor range
     (0.5): the exponential relaxation branch of the gate update runs every
     step, and the n gate (|z| about 0.13) takes the full degree-9 series.
-    Bars: float64 1e-7 mV; float32 5e-3 mV."""
+    Bars: float64 1e-7 mV; float32 1e-3 mV (observed on MI355X: 8.4e-5 mV
+    persistent, 7.6e-5 mV plain; float64 1e-13 mV)."""
     from dendra_amd.model import HH
     from dendra_amd.population import CablePopulation
     from dendra_amd.workloads import single_cable, single_cable_wave
@@ -66,7 +67,7 @@ def test_warm_cable_takes_the_relaxation_branch_and_matches_oracle(orc, path, dt
     rec = pop.unpermute(pop.run(600, wave=wave, record_every=1, path=path))
     err = np.abs(rec - want).max()
     print(f"37 C cable {path} {dtype}: max |dV| {err:.3g} mV")
-    assert err <= (bars.F64 if dtype == torch.float64 else bars.F32), err
+    assert err <= (bars.F64 if dtype == torch.float64 else bars.F32_NORTH_STAR), err
     assert np.ptp(want) > 10.0, "the pulses must move the membrane"
 
 
@@ -84,7 +85,8 @@ def test_uneven_cables_match_oracle(orc, path, dtype):
 
     float64: max |dV| <= 1e-7 mV against the binary64 oracle.
     float32: spike (0 mV crossing) steps within one dt everywhere, and
-    max |dV| <= 5e-3 mV: on this sample half a float32 ulp of V at t = 0
+    max |dV| <= 5e-3 mV (observed on MI355X: 3.9e-3 mV persistent, 3.0e-3 mV
+    plain; float64 5e-12 / 1.4e-11 mV): on this sample half a float32 ulp of V at t = 0
     already moves a binary64 run by more than 1e-3 mV, and the reference's
     float32 step restated in binary32 reaches 1.6e-3 mV (2.8e-3 mV with a
     binary64 voltage and elimination); tests/test_precision.py.
