@@ -35,6 +35,13 @@ from .population import CablePopulation, TreePopulation
 _NP = {torch.float32: np.float32, torch.float64: np.float64}
 
 
+def _device(device) -> torch.device:
+    d = torch.device(device)
+    if d.type == "cuda" and d.index is None:
+        d = torch.device("cuda", torch.cuda.current_device())
+    return d
+
+
 def _hh_array(hh: HH, temp, dtype):
     arr = hh.array()
     if temp is not None:
@@ -108,7 +115,7 @@ class CableStepper(_Stepper):
     def __init__(self, spec: CableSpec, hh: HH | None = None, dtype=torch.float32,
                  device="cuda"):
         self.spec, self.hh = spec, hh or HH()
-        self.dtype, self.device = dtype, torch.device(device)
+        self.dtype, self.device = dtype, _device(device)
         _lib.require_gpu(torch.empty(0, device=self.device))
         self._pop = CablePopulation(spec, 0.025, self.hh, dtype=dtype, device=device)
         self.total = spec.total
@@ -151,7 +158,7 @@ class TreeStepper(_Stepper):
     def __init__(self, spec: TreeSpec, hh: HH | None = None, dtype=torch.float32,
                  device="cuda"):
         self.spec, self.hh = spec, hh or HH()
-        self.dtype, self.device = dtype, torch.device(device)
+        self.dtype, self.device = dtype, _device(device)
         _lib.require_gpu(torch.empty(0, device=self.device))
         self._pop = TreePopulation(spec, 0.025, self.hh, dtype=dtype, device=device)
         self.total = spec.total

@@ -43,7 +43,7 @@ def test_fixed_delay_transition_replays_the_oracle_ring_float32(orc):
     gates = ((rs.random((steps, n)) < 0.2) * rs.uniform(0.5, 1.5, (steps, n))).astype(np.float32)
     ring0 = rs.standard_normal((depth, width)).astype(np.float32)
     want_pay, want_ring, want_cur = orc.ring_replay_gates(width, post, delay, w, depth, gates,
-                                                           ring=ring0, cur=37)
+                                                           ring=ring0.copy(), cur=37)
     q, slot, step = dev(ring0), dev([37], torch.int64), dev([0], torch.int64)
     post_t, delay_t, w_t = dev(post, torch.int64), dev(delay, torch.int64), dev(w)
     for s in range(steps):

@@ -20,8 +20,8 @@ def _drives(rs, n_steps, total):
     base = np.cumsum(rs.standard_normal((n_steps, total)), axis=1) * 0.4
     ve = base - base.mean(axis=1, keepdims=True)
     intra = np.zeros((n_steps, total))
-    hits = rs.random((n_steps, total)) < 0.002
-    intra[hits] = rs.uniform(0.2e-6, 2e-6, hits.sum())
+    hits = rs.random((n_steps, total)) < 0.004
+    intra[hits] = rs.uniform(2e-6, 1e-5, hits.sum())
     return ve, intra
 
 
@@ -66,7 +66,8 @@ def test_cable_stepper_matches_oracle_with_arbitrary_drives(orc, dtype):
     err = np.abs(got - want).max()
     print(f"cable stepper {dtype}: max |dV| {err:.3g} mV, range {np.ptp(want):.1f} mV")
     assert np.ptp(want) > 20.0
-    assert err <= (bars.F64 if dtype == torch.float64 else bars.F32), err
+    # observed on MI355X: 7.4e-13 mV (float64), 1.9e-4 mV (float32)
+    assert err <= (bars.F64 if dtype == torch.float64 else bars.F32_NORTH_STAR), err
 
 
 def test_cable_stepper_contract():
@@ -115,4 +116,5 @@ def test_tree_stepper_matches_oracle_with_arbitrary_drives(orc, dtype):
     err = np.abs(np.array(got) - want).max()
     print(f"tree stepper {dtype}: max |dV| {err:.3g} mV, range {np.ptp(want):.1f} mV")
     assert np.ptp(want) > 20.0
-    assert err <= (bars.F64 if dtype == torch.float64 else bars.F32), err
+    # observed on MI355X: 3.5e-14 mV (float64), 4.7e-5 mV (float32)
+    assert err <= (bars.F64 if dtype == torch.float64 else bars.F32_NORTH_STAR), err
