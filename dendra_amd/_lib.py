@@ -165,6 +165,10 @@ for _sfx in ("f32", "f64"):
     _SIGNATURES[f"dn_ordered_scatter_add_{_sfx}"] = (i32, [p, i64, p, p, i64, p, i64, p])
 
 
+_SIGNATURES["dn_tridiag_noalloc_max_scratch_elems"] = (i64, [])
+_SIGNATURES["dn_tridiag_solve_noalloc_f32"] = (i32, [p, p, p, p, p, i64, i64, i32, p])
+
+
 def _bind(lib) -> None:
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name, None)

@@ -20,6 +20,8 @@
 //    when 4*64*nr values fit, else it runs in the caller's global scratch.
 #include "dn_wave.hpp"
 
+#include <mutex>
+
 namespace dn {
 
 template <typename T>
@@ -137,4 +139,62 @@ DN_EXPORT int dn_tridiag_solve_f64(const double* a, const double* b, const doubl
                                    hipStream_t stream) {
   return dn::tridiag_launch<double>(a, b, c, d, x, B, K, method, scratch,
                                     scratch_elems, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant (float32): the elimination workspace is a fixed
+// device array owned by the library instead of caller scratch.  Each GPU
+// holds its own copy for the life of the process; its address is resolved
+// once per device.  Every call on one GPU shares the array, so calls that
+// overlap on one GPU (different streams, or threads) race on it.  The
+// kernels write every workspace value before reading it, so nothing needs
+// clearing between calls.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int64_t kNoAllocScratchElems = (int64_t)2 << 20;  // 8 MiB of float32
+constexpr int kNoAllocMaxDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) float scratch[kNoAllocScratchElems];
+
+namespace {
+
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of `scratch` on the current device (the host-side name is
+// only a stand-in), resolved on the first call on that device.
+int scratch_on_current_device(float** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices)
+    return DN_EINVAL;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return DN_ELAUNCH;
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return DN_OK;
+}
+
+}  // namespace
+
+DN_EXPORT int64_t dn_tridiag_noalloc_max_scratch_elems(void) { return kNoAllocScratchElems; }
+
+DN_EXPORT int dn_tridiag_solve_noalloc_f32(const float* a, const float* b, const float* c,
+                                           const float* d, float* x, int64_t B, int64_t K,
+                                           int method, hipStream_t stream) {
+  if (B < 1 || K < 1 || method < 0 || method > 2) return DN_EINVAL;
+  const int64_t need = dn_tridiag_scratch_elems(B, K, method, (int)sizeof(float));
+  if (need > kNoAllocScratchElems) return DN_ERANGE;
+  float* work = nullptr;
+  if (need > 0) {
+    const int st = scratch_on_current_device(&work);
+    if (st != DN_OK) return st;
+  }
+  return dn::tridiag_launch<float>(a, b, c, d, x, B, K, method, work,
+                                   need > 0 ? kNoAllocScratchElems : 0, stream);
 }

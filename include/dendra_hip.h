@@ -732,6 +732,20 @@ int dn_ordered_scatter_add_f64(double* q, int64_t n_cells, const int64_t* dest,
                                const double* values, int64_t n, void* work,
                                int64_t work_bytes, hipStream_t stream);
 
+/* ---- allocation-free tridiagonal solve (float32) --------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same solve and arguments as dn_tridiag_solve_f32
+ * without the caller scratch: the workspace is a library-owned device array
+ * of dn_tridiag_noalloc_max_scratch_elems() floats (8 MiB) per GPU, held
+ * until the process ends.  Returns DN_ERANGE when the solve would need more
+ * (dn_tridiag_scratch_elems(B, K, method, 4) above that size).  Calls no
+ * allocating routine.  Calls that overlap on one GPU share the array and race. */
+int64_t dn_tridiag_noalloc_max_scratch_elems(void);
+int dn_tridiag_solve_noalloc_f32(const float* a, const float* b, const float* c,
+                                 const float* d, float* x, int64_t B, int64_t K, int method,
+                                 hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
