@@ -46,11 +46,11 @@ _SIGNATURES = {
     "dn_hh_cable_run_f64": (i32, _CABLE_RUN + [f64, p, p, i64, i64, i32, i32, p, p]),
     "dn_hh_cable_step_f32": (
         i32,
-        [i64, p, i64, i32, p, p, p, p, p, p, p, p, p, p, f32, f32, p, p, p, p, p, p, p],
+        [i64, p, i64, i32, p, p, p, p, p, p, p, p, p, p, f32, f32, p, p, p, p, p, p, p, p, p],
     ),
     "dn_hh_cable_step_f64": (
         i32,
-        [i64, p, i64, i32, p, p, p, p, p, p, p, p, p, p, f64, f64, p, p, p, p, p, p, p],
+        [i64, p, i64, i32, p, p, p, p, p, p, p, p, p, p, f64, f64, p, p, p, p, p, p, p, p, p],
     ),
     "dn_hh_tree_slots": (i32, []),
     "dn_hh_tree_run_f32": (
@@ -69,11 +69,11 @@ _SIGNATURES = {
     ),
     "dn_hh_tree_step_f32": (
         i32,
-        _TREE_PLAN + [p, i32, i64, p, p, p, p, p, p, p, p, p, p, f32, f32, p, p, p, p, p, p],
+        _TREE_PLAN + [p, i32, i64, p, p, p, p, p, p, p, p, p, p, f32, f32, p, p, p, p, p, p, p, p],
     ),
     "dn_hh_tree_step_f64": (
         i32,
-        _TREE_PLAN + [p, i32, i64, p, p, p, p, p, p, p, p, p, p, f64, f64, p, p, p, p, p, p],
+        _TREE_PLAN + [p, i32, i64, p, p, p, p, p, p, p, p, p, p, f64, f64, p, p, p, p, p, p, p, p],
     ),
 }
 
@@ -169,6 +169,13 @@ _SIGNATURES["dn_tridiag_noalloc_max_scratch_elems"] = (i64, [])
 _SIGNATURES["dn_tridiag_solve_noalloc_f32"] = (i32, [p, p, p, p, p, i64, i64, i32, p])
 
 
+for _sfx, _ft in (("f32", f32), ("f64", f64)):
+    _SIGNATURES[f"dn_exp2syn_receive_{_sfx}"] = (i32, [p, p, p, p, _ft, i64, p])
+    _SIGNATURES[f"dn_exp2syn_advance_{_sfx}"] = (i32, [p, p, p, _ft, p, _ft, _ft, i64, p])
+    _SIGNATURES[f"dn_exp2syn_membrane_{_sfx}"] = (
+        i32, [p, p, p, _ft, p, p, p, p, i64, p, p, p])
+
+
 def _bind(lib) -> None:
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name, None)
@@ -220,6 +227,14 @@ def ptr(t) -> int | None:
 
 def stream_handle(device=None) -> int:
     return torch.cuda.current_stream(device).cuda_stream
+
+
+def device_of(device) -> torch.device:
+    """torch.device with the index of the current GPU filled in."""
+    d = torch.device(device)
+    if d.type == "cuda" and d.index is None:
+        d = torch.device("cuda", torch.cuda.current_device())
+    return d
 
 
 def require_gpu(*tensors) -> None:

@@ -246,13 +246,18 @@ __global__ __launch_bounds__(256) void hh_cable_assemble_kernel(
     const T* __restrict__ scale, const T* __restrict__ drive,
     const T* __restrict__ sna, const T* __restrict__ sk, T w, T dt, HHParams<T> hh,
     const T* __restrict__ ve, const T* __restrict__ intra, const T* __restrict__ icap,
-    T* __restrict__ mainv, T* __restrict__ rhs) {
+    const T* __restrict__ gx, const T* __restrict__ gex, T* __restrict__ mainv,
+    T* __restrict__ rhs) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= total) return;
   const T v = v_[i];
   T m = m_[i], h = h_[i], n = n_[i], G, J;
   hh_advance<T>(v, dt, hh, sna ? sna[i] : T(1), sk ? sk[i] : T(1), m, h, n, G, J);
   m_[i] = m; h_[i] = h; n_[i] = n;
+  if (gx) {  // further affine membrane terms (point processes): g, g e per unit area
+    G += gx[i];
+    J += gex[i];
+  }
   const T g_l = gL[i], g_r = gRp[i], sc = scale[i];
   // Neighbours across a cable boundary carry a zero conductance.
   const T vr = i + 1 < total ? v_[i + 1] : v;
@@ -341,11 +346,11 @@ static int cable_step_launch(int64_t n_cells, const int64_t* off, int64_t total,
                              int max_nr, T* v, T* m, T* h, T* n, const T* gL,
                              const T* gRp, const T* scale, const T* drive, const T* sna,
                              const T* sk, T w, T dt, HHParams<T> hh, const T* ve,
-                             const T* intra, const T* icap, T* mainv, T* rhs,
-                             hipStream_t stream) {
+                             const T* intra, const T* icap, const T* gx, const T* gex,
+                             T* mainv, T* rhs, hipStream_t stream) {
   if (!off || !v || !m || !h || !n || !gL || !gRp || !scale || !mainv || !rhs)
     return DN_EINVAL;
-  if (intra && !icap) return DN_EINVAL;
+  if ((intra && !icap) || (!gx != !gex)) return DN_EINVAL;
   if (n_cells < 1 || total < 1 || max_nr < 1 || n_cells > 0x7fffffff)
     return DN_EINVAL;
   size_t lds = (size_t)4 * 64 * max_nr * sizeof(T);
@@ -353,7 +358,7 @@ static int cable_step_launch(int64_t n_cells, const int64_t* off, int64_t total,
   int blocks = (int)((total + 255) / 256);
   hipLaunchKernelGGL(hh_cable_assemble_kernel<T>, dim3(blocks), dim3(256), 0, stream,
                      total, v, m, h, n, gL, gRp, scale, drive, sna, sk, w, dt, hh, ve, intra,
-                     icap, mainv, rhs);
+                     icap, gx, gex, mainv, rhs);
   hipLaunchKernelGGL(hh_cable_solve_kernel<T>, dim3((unsigned)n_cells), dim3(64), lds,
                      stream, off, gL, gRp, mainv, rhs, v, dt, max_nr);
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
@@ -395,13 +400,14 @@ DN_EXPORT int dn_hh_cable_step_f32(int64_t n_cells, const int64_t* off, int64_t 
                                    const float* gL, const float* gRp, const float* scale,
                                    const float* drive, const float* sna, const float* sk,
                                    float w, float dt, const float* hh, const float* ve,
-                                   const float* intra, const float* icap, float* mainv,
+                                   const float* intra, const float* icap, const float* gx,
+                                   const float* gex, float* mainv,
                                    float* rhs, hipStream_t stream) {
   if (!hh) return DN_EINVAL;
   dn::HHParams<float> p = {hh[0], hh[1], hh[2], hh[3], hh[4], hh[5], hh[6]};
   return dn::cable_step_launch<float>(n_cells, off, total, max_nr, v, m, h, n, gL, gRp,
                                       scale, drive, sna, sk, w, dt, p, ve, intra, icap,
-                                      mainv, rhs, stream);
+                                      gx, gex, mainv, rhs, stream);
 }
 
 DN_EXPORT int dn_hh_cable_step_f64(int64_t n_cells, const int64_t* off, int64_t total,
@@ -410,11 +416,12 @@ DN_EXPORT int dn_hh_cable_step_f64(int64_t n_cells, const int64_t* off, int64_t 
                                    const double* scale, const double* drive,
                                    const double* sna, const double* sk, double w,
                                    double dt, const double* hh, const double* ve,
-                                   const double* intra, const double* icap, double* mainv,
+                                   const double* intra, const double* icap, const double* gx,
+                                   const double* gex, double* mainv,
                                    double* rhs, hipStream_t stream) {
   if (!hh) return DN_EINVAL;
   dn::HHParams<double> p = {hh[0], hh[1], hh[2], hh[3], hh[4], hh[5], hh[6]};
   return dn::cable_step_launch<double>(n_cells, off, total, max_nr, v, m, h, n, gL,
                                        gRp, scale, drive, sna, sk, w, dt, p, ve, intra,
-                                       icap, mainv, rhs, stream);
+                                       icap, gx, gex, mainv, rhs, stream);
 }

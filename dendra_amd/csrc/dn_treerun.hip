@@ -590,7 +590,8 @@ __global__ __launch_bounds__(256) void hh_tree_assemble_kernel(
     const T* __restrict__ area, const T* __restrict__ cmdt, const T* __restrict__ g,
     const T* __restrict__ drive, const T* __restrict__ sna, const T* __restrict__ sk,
     T w, T dt, HHParams<T> hh, const T* __restrict__ ve, const T* __restrict__ intra,
-    T* __restrict__ mainv, T* __restrict__ rhs) {
+    const T* __restrict__ gx, const T* __restrict__ gex, T* __restrict__ mainv,
+    T* __restrict__ rhs) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= total) return;
   const int64_t no = node_off[node_tree[i]];
@@ -598,6 +599,10 @@ __global__ __launch_bounds__(256) void hh_tree_assemble_kernel(
   T m = m_[i], h = h_[i], n = n_[i], G, J;
   hh_advance<T>(v, dt, hh, sna ? sna[i] : T(1), sk ? sk[i] : T(1), m, h, n, G, J);
   m_[i] = m; h_[i] = h; n_[i] = n;
+  if (gx) {  // further affine membrane terms (point processes): g, g e per unit area
+    G += gx[i];
+    J += gex[i];
+  }
   const int p = parent[i];
   T ax = p >= 0 ? -g[i] * (v - v_[no + p]) : T(0);
   for (int32_t q = ch_ptr[i]; q < ch_ptr[i + 1]; ++q) {
@@ -756,7 +761,9 @@ static int tree_step_impl(dn::TreePlan P, const int32_t* node_tree, int kmax,
                           int64_t total, T* v, T* m, T* h, T* n, const T* area,
                           const T* cmdt, const T* g, const T* drive, const T* sna,
                           const T* sk, T w, T dt, const T* hh, const T* ve,
-                          const T* intra, T* mainv, T* rhs, hipStream_t stream) {
+                          const T* intra, const T* gx, const T* gex, T* mainv, T* rhs,
+                          hipStream_t stream) {
+  if (!gx != !gex) return DN_EINVAL;
   if (!dn::plan_ok(P) || !node_tree || !v || !m || !h || !n || !area || !cmdt || !g ||
       !hh || !mainv || !rhs)
     return DN_EINVAL;
@@ -767,7 +774,8 @@ static int tree_step_impl(dn::TreePlan P, const int32_t* node_tree, int kmax,
   int blocks = (int)((total + 255) / 256);
   hipLaunchKernelGGL(dn::hh_tree_assemble_kernel<T>, dim3(blocks), dim3(256), 0, stream,
                      total, node_tree, P.node_off, P.parent, P.ch_ptr, P.ch_idx, v, m, h,
-                     n, area, cmdt, g, drive, sna, sk, w, dt, p, ve, intra, mainv, rhs);
+                     n, area, cmdt, g, drive, sna, sk, w, dt, p, ve, intra, gx, gex, mainv,
+                     rhs);
   hipLaunchKernelGGL(dn::hh_tree_solve_kernel<T>, dim3(P.n_trees), dim3(256), lds,
                      stream, P, g, mainv, rhs, v, kmax, P.smax, P.lmax);
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
@@ -779,12 +787,13 @@ DN_EXPORT int dn_hh_tree_step_f32(DN_TREE_PLAN_ARGS, const int32_t* node_tree, i
                                   const float* area, const float* cmdt, const float* g,
                                   const float* drive, const float* sna, const float* sk,
                                   float w, float dt, const float* hh, const float* ve,
-                                  const float* intra, float* mainv, float* rhs,
-                                  hipStream_t stream) {
+                                  const float* intra, const float* gx, const float* gex,
+                                  float* mainv, float* rhs, hipStream_t stream) {
   dn::TreePlan P;
   DN_TREE_PLAN_FILL(P)
   return tree_step_impl<float>(P, node_tree, kmax, total, v, m, h, n, area, cmdt, g,
-                               drive, sna, sk, w, dt, hh, ve, intra, mainv, rhs, stream);
+                               drive, sna, sk, w, dt, hh, ve, intra, gx, gex, mainv, rhs,
+                               stream);
 }
 
 DN_EXPORT int dn_hh_tree_step_f64(DN_TREE_PLAN_ARGS, const int32_t* node_tree, int kmax,
@@ -792,12 +801,14 @@ DN_EXPORT int dn_hh_tree_step_f64(DN_TREE_PLAN_ARGS, const int32_t* node_tree, i
                                   double* n, const double* area, const double* cmdt,
                                   const double* g, const double* drive, const double* sna,
                                   const double* sk, double w, double dt, const double* hh,
-                                  const double* ve, const double* intra, double* mainv,
-                                  double* rhs, hipStream_t stream) {
+                                  const double* ve, const double* intra, const double* gx,
+                                  const double* gex, double* mainv, double* rhs,
+                                  hipStream_t stream) {
   dn::TreePlan P;
   DN_TREE_PLAN_FILL(P)
   return tree_step_impl<double>(P, node_tree, kmax, total, v, m, h, n, area, cmdt, g,
-                                drive, sna, sk, w, dt, hh, ve, intra, mainv, rhs, stream);
+                                drive, sna, sk, w, dt, hh, ve, intra, gx, gex, mainv, rhs,
+                               stream);
 }
 
 DN_EXPORT int dn_hh_tree_slots(void) { return dn::kTreeSlots; }

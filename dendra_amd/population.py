@@ -254,8 +254,8 @@ class CablePopulation(_Status):
                     _lib.ptr(self.v), _lib.ptr(self.m), _lib.ptr(self.h), _lib.ptr(self.n),
                     _lib.ptr(self.gL), _lib.ptr(self.gRp), _lib.ptr(self.scale),
                     _lib.ptr(self.drive), _lib.ptr(self.sna), _lib.ptr(self.sk),
-                    w, self.dt, self.hh_arr.ctypes.data, None, None, None, _lib.ptr(mainv),
-                    _lib.ptr(rhs), strm,
+                    w, self.dt, self.hh_arr.ctypes.data, None, None, None, None, None,
+                    _lib.ptr(mainv), _lib.ptr(rhs), strm,
                 )
                 _lib.check(st, "plain cable step")
                 self.t_step += 1
@@ -500,8 +500,8 @@ class TreePopulation(_Status):
                     _lib.ptr(self.v), _lib.ptr(self.m), _lib.ptr(self.h), _lib.ptr(self.n),
                     _lib.ptr(self.area), _lib.ptr(self.cm_dt), _lib.ptr(self.g),
                     _lib.ptr(self.drive), _lib.ptr(self.sna), _lib.ptr(self.sk), w,
-                    self.dt, self.hh_arr.ctypes.data, None, None, _lib.ptr(mainv),
-                    _lib.ptr(rhs), strm,
+                    self.dt, self.hh_arr.ctypes.data, None, None, None, None,
+                    _lib.ptr(mainv), _lib.ptr(rhs), strm,
                 )
                 _lib.check(st, "plain tree step")
                 self.t_step += 1

@@ -20,7 +20,9 @@ one solve launch (``dn_hh_cable_step_*`` / ``dn_hh_tree_step_*``); ``temp``
 sets the rate factor q10 = 3^((temp - 6.3)/10).  Voltages are in user order
 (cell by cell, compartments in spec order); a uniform batch may pass
 ``(B, K)`` tensors.  The membrane-current output of the reference's
-``imem`` option is not produced (``i_membrane`` is None).
+``imem`` option is not produced (``i_membrane`` is None).  Point processes
+(``dendra_amd.synapse.Exp2Syn``) attached with ``add_point_process`` are
+advanced inside each step and enter its membrane frame.
 """
 
 from __future__ import annotations
@@ -33,13 +35,6 @@ from .model import HH, CableSpec, TreeSpec, cable_workspace, hh_steady_state, tr
 from .population import CablePopulation, TreePopulation
 
 _NP = {torch.float32: np.float32, torch.float64: np.float64}
-
-
-def _device(device) -> torch.device:
-    d = torch.device(device)
-    if d.type == "cuda" and d.index is None:
-        d = torch.device("cuda", torch.cuda.current_device())
-    return d
 
 
 def _hh_array(hh: HH, temp, dtype):
@@ -56,7 +51,30 @@ class _Stepper:
         self._perm = torch.as_tensor(perm, dtype=torch.int64, device=dev)
         inv = np.empty_like(perm)
         inv[perm] = np.arange(len(perm))
+        self._inv_np = inv
         self._inv = torch.as_tensor(inv, dtype=torch.int64, device=dev)
+        self._pp = None
+
+    def add_point_process(self, syn):
+        """Attach an ``Exp2Syn`` whose synapses sit on this population's
+        compartments: each step advances it (cnexp) and adds its
+        conductances to the membrane frame, as the reference's mechanism
+        handler does for point processes."""
+        if self._pp is not None:
+            raise ValueError("one point-process object per stepper")
+        if syn.dtype != self.dtype or syn.device != self.device:
+            raise ValueError("the synapses must share the stepper's dtype and device")
+        syn._attach(self._inv_np, self._area_plan, self.total)
+        self._gx = torch.zeros(self.total, dtype=self.dtype, device=self.device)
+        self._gex = torch.zeros_like(self._gx)
+        self._pp = syn
+
+    def _point_terms(self, dt):
+        if self._pp is None:
+            return None, None
+        self._pp.advance(dt)
+        self._pp.membrane_terms(self._gx, self._gex)
+        return _lib.ptr(self._gx), _lib.ptr(self._gex)
 
     def _to_plan(self, x, name):
         if x is None:
@@ -115,7 +133,7 @@ class CableStepper(_Stepper):
     def __init__(self, spec: CableSpec, hh: HH | None = None, dtype=torch.float32,
                  device="cuda"):
         self.spec, self.hh = spec, hh or HH()
-        self.dtype, self.device = dtype, _device(device)
+        self.dtype, self.device = dtype, _lib.device_of(device)
         _lib.require_gpu(torch.empty(0, device=self.device))
         self._pop = CablePopulation(spec, 0.025, self.hh, dtype=dtype, device=device)
         self.total = spec.total
@@ -123,6 +141,9 @@ class CableStepper(_Stepper):
         ws = cable_workspace(spec)
         self._icap = torch.as_tensor(ws.cm_inv[self._pop.perm], dtype=dtype,
                                      device=self.device)
+        area = np.pi * spec.diam * 1e-4 * spec.dx * 1e-4
+        self._area_plan = torch.as_tensor(area[self._pop.perm], dtype=dtype,
+                                          device=self.device)
         self.m, self.h, self.n = self._pop.m, self._pop.h, self._pop.n
         self._mainv = torch.empty(self.total, dtype=dtype, device=self.device)
         self._rhs = torch.empty_like(self._mainv)
@@ -137,6 +158,7 @@ class CableStepper(_Stepper):
         if vn.data_ptr() == v.data_ptr():
             vn = vn.clone()
         ve_p, in_p = self._to_plan(ve, "ve"), self._to_plan(intra, "intra")
+        gx, gex = self._point_terms(dt)
         lib = _lib.load()
         fn = lib.dn_hh_cable_step_f64 if self.dtype == torch.float64 else \
             lib.dn_hh_cable_step_f32
@@ -145,7 +167,7 @@ class CableStepper(_Stepper):
                 _lib.ptr(self.m), _lib.ptr(self.h), _lib.ptr(self.n), _lib.ptr(p.gL),
                 _lib.ptr(p.gRp), _lib.ptr(p.scale), None, _lib.ptr(p.sna), _lib.ptr(p.sk),
                 0.0, float(dt), hh.ctypes.data, _lib.ptr(ve_p), _lib.ptr(in_p),
-                _lib.ptr(self._icap), _lib.ptr(self._mainv), _lib.ptr(self._rhs),
+                _lib.ptr(self._icap), gx, gex, _lib.ptr(self._mainv), _lib.ptr(self._rhs),
                 _lib.stream_handle(self.device))
         _lib.check(st, "cable step")
         return self._from_plan(vn, v), None
@@ -158,12 +180,13 @@ class TreeStepper(_Stepper):
     def __init__(self, spec: TreeSpec, hh: HH | None = None, dtype=torch.float32,
                  device="cuda"):
         self.spec, self.hh = spec, hh or HH()
-        self.dtype, self.device = dtype, _device(device)
+        self.dtype, self.device = dtype, _lib.device_of(device)
         _lib.require_gpu(torch.empty(0, device=self.device))
         self._pop = TreePopulation(spec, 0.025, self.hh, dtype=dtype, device=device)
         self.total = spec.total
         self._setup_order(self._pop.perm)
         self.m, self.h, self.n = self._pop.m, self._pop.h, self._pop.n
+        self._area_plan = self._pop.area
         self._mainv = torch.empty(self.total, dtype=dtype, device=self.device)
         self._rhs = torch.empty_like(self._mainv)
         self._dt = None
@@ -186,6 +209,7 @@ class TreeStepper(_Stepper):
         if vn.data_ptr() == v.data_ptr():
             vn = vn.clone()
         ve_p, in_p = self._to_plan(ve, "ve"), self._to_plan(intra, "intra")
+        gx, gex = self._point_terms(dt)
         lib = _lib.load()
         fn = lib.dn_hh_tree_step_f64 if self.dtype == torch.float64 else \
             lib.dn_hh_tree_step_f32
@@ -194,7 +218,7 @@ class TreeStepper(_Stepper):
                 _lib.ptr(self.m), _lib.ptr(self.h), _lib.ptr(self.n), _lib.ptr(p.area),
                 _lib.ptr(self._cm_dt(dt)), _lib.ptr(p.g), None, _lib.ptr(p.sna),
                 _lib.ptr(p.sk), 0.0, float(dt), hh.ctypes.data, _lib.ptr(ve_p),
-                _lib.ptr(in_p), _lib.ptr(self._mainv), _lib.ptr(self._rhs),
+                _lib.ptr(in_p), gx, gex, _lib.ptr(self._mainv), _lib.ptr(self._rhs),
                 _lib.stream_handle(self.device))
         _lib.check(st, "tree step")
         return self._from_plan(vn, v), None

@@ -397,19 +397,24 @@ int dn_hh_cable_run_f64(int n_tasks, const int32_t* task_cell0,
  * value for `drive`.  ve (nullable): this step's extracellular potential
  * per compartment (mV), entering through its axial differences; intra
  * (nullable): injected current per compartment (mA), scaled by icap = 1/C
- * (1/F; required with intra).  v is advanced in place. */
+ * (1/F; required with intra).  gx, gex (nullable, together): further affine
+ * membrane terms per compartment, G += gx (S/cm^2), J += gex (S/cm^2 mV),
+ * e.g. exp2syn conductances from dn_exp2syn_membrane_*.  v is advanced in
+ * place. */
 int dn_hh_cable_step_f32(int64_t n_cells, const int64_t* off, int64_t total, int max_nr,
                          float* v, float* m, float* h, float* n, const float* gL,
                          const float* gRp, const float* scale, const float* drive,
                          const float* sna, const float* sk, float w, float dt,
                          const float* hh, const float* ve, const float* intra,
-                         const float* icap, float* mainv, float* rhs, hipStream_t stream);
+                         const float* icap, const float* gx, const float* gex, float* mainv,
+                         float* rhs, hipStream_t stream);
 int dn_hh_cable_step_f64(int64_t n_cells, const int64_t* off, int64_t total, int max_nr,
                          double* v, double* m, double* h, double* n, const double* gL,
                          const double* gRp, const double* scale, const double* drive,
                          const double* sna, const double* sk, double w, double dt,
                          const double* hh, const double* ve, const double* intra,
-                         const double* icap, double* mainv, double* rhs,
+                         const double* icap, const double* gx, const double* gex, double* mainv,
+                         double* rhs,
                          hipStream_t stream);
 
 /* ---- fused HH + backward Euler, branched trees ----------------------------
@@ -465,7 +470,8 @@ int dn_hh_tree_run_f64(int n_trees, const int64_t* node_off, const int32_t* pare
  * gating and the current-form residual (mA), then the section-chain solve.
  * ve (nullable): this step's extracellular potential per node (mV), as edge
  * currents g (ve_child - ve_parent); intra (nullable): injected current per
- * node (mA).  v is advanced in place. */
+ * node (mA); gx, gex (nullable, together): affine membrane terms per node as
+ * for dn_hh_cable_step_*.  v is advanced in place. */
 int dn_hh_tree_step_f32(int n_trees, const int64_t* node_off, const int32_t* parent,
                         const int32_t* ch_ptr, const int32_t* ch_idx,
                         const int32_t* sec_off, const int32_t* sec_head,
@@ -478,7 +484,8 @@ int dn_hh_tree_step_f32(int n_trees, const int64_t* node_off, const int32_t* par
                         const float* area, const float* cmdt, const float* g,
                         const float* drive, const float* sna, const float* sk, float w,
                         float dt, const float* hh, const float* ve, const float* intra,
-                        float* mainv, float* rhs, hipStream_t stream);
+                        const float* gx, const float* gex, float* mainv, float* rhs,
+                        hipStream_t stream);
 int dn_hh_tree_step_f64(int n_trees, const int64_t* node_off, const int32_t* parent,
                         const int32_t* ch_ptr, const int32_t* ch_idx,
                         const int32_t* sec_off, const int32_t* sec_head,
@@ -491,8 +498,8 @@ int dn_hh_tree_step_f64(int n_trees, const int64_t* node_off, const int32_t* par
                         const double* area, const double* cmdt, const double* g,
                         const double* drive, const double* sna, const double* sk,
                         double w, double dt, const double* hh, const double* ve,
-                        const double* intra, double* mainv, double* rhs,
-                        hipStream_t stream);
+                        const double* intra, const double* gx, const double* gex,
+                        double* mainv, double* rhs, hipStream_t stream);
 
 /* ---- event delivery ---------------------------------------------------------
  * Delivery from a spike tape.  Replaces the bitpacked source history and its
@@ -731,6 +738,38 @@ int dn_ordered_scatter_add_f32(float* q, int64_t n_cells, const int64_t* dest,
 int dn_ordered_scatter_add_f64(double* q, int64_t n_cells, const int64_t* dest,
                                const double* values, int64_t n, void* work,
                                int64_t work_bytes, hipStream_t stream);
+
+/* ---- exp2syn point processes, per step ------------------------------------
+ * Replaces the reference's exp2syn mechanism (mod/EXP2SYN.py) inside
+ * Network.step (networks/net.py:335-377) for callers that keep that order:
+ * dn_exp2syn_receive_*: net_receive_values (:154-156, Synapse.net_receive,
+ * mechanisms/_mechanism.py:4518-4524), A += payload * factor, B likewise
+ * (factor per synapse, or factor_all when NULL);
+ * dn_exp2syn_advance_*: the cnexp step of A' = -A/tau1, B' = -B/tau2
+ * (mechanisms/_cnexp.py:96-132), taus per synapse or the *_all scalars;
+ * dn_exp2syn_membrane_*: the affine current (B - A)(v - e) nA (:147-152) as
+ * membrane densities for the step's gx, gex: for each listed compartment
+ * comp[c] (CSR comp_ptr over syn_order, synapse order within), g = sum
+ * (B - A) 1e-6 / area[comp] and g e; other compartments are not written. */
+int dn_exp2syn_receive_f32(float* A, float* B, const float* payload, const float* factor,
+                           float factor_all, int64_t n, hipStream_t stream);
+int dn_exp2syn_receive_f64(double* A, double* B, const double* payload,
+                           const double* factor, double factor_all, int64_t n,
+                           hipStream_t stream);
+int dn_exp2syn_advance_f32(float* A, float* B, const float* tau1, float tau1_all,
+                           const float* tau2, float tau2_all, float dt, int64_t n,
+                           hipStream_t stream);
+int dn_exp2syn_advance_f64(double* A, double* B, const double* tau1, double tau1_all,
+                           const double* tau2, double tau2_all, double dt, int64_t n,
+                           hipStream_t stream);
+int dn_exp2syn_membrane_f32(const float* A, const float* B, const float* e, float e_all,
+                            const float* area, const int32_t* comp_ptr, const int32_t* comp,
+                            const int32_t* syn_order, int64_t n_comp, float* g_out,
+                            float* ge_out, hipStream_t stream);
+int dn_exp2syn_membrane_f64(const double* A, const double* B, const double* e,
+                            double e_all, const double* area, const int32_t* comp_ptr,
+                            const int32_t* comp, const int32_t* syn_order, int64_t n_comp,
+                            double* g_out, double* ge_out, hipStream_t stream);
 
 /* ---- allocation-free tridiagonal solve (float32) --------------------------
  * No counterpart in the reference: for callers that cannot allocate device
