@@ -176,6 +176,12 @@ for _sfx, _ft in (("f32", f32), ("f64", f64)):
         i32, [p, p, p, _ft, p, p, p, p, i64, p, p, p])
 
 
+_MULTI_PLAN = [p, p, i64, p, i64, p, p, p, p, p, p, i64, i64, i64, i64]
+for _sfx in ("f32", "f64"):
+    _SIGNATURES[f"dn_tree_dhs_multi_{_sfx}"] = (i32, [p, p, p] + _MULTI_PLAN + [i32, p, p, p])
+    _SIGNATURES[f"dn_tree_dhs_multi_grad_a_{_sfx}"] = (i32, [p, p] + _MULTI_PLAN + [p, p])
+
+
 def _bind(lib) -> None:
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name, None)

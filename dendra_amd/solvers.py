@@ -228,3 +228,132 @@ def dhs_solve(d_mem, a_geom, b, parent_idx, order, layer_ptr, threads: int = 16)
 
 
 dhs_solve_cuda = dhs_solve
+
+
+# -- packed multi-morphology tree solve (_dhs_multi) ------------------------
+
+_PLAN_NAMES = ("P_cat", "ORDER_cat", "LAYER_PTR_cat", "WARP_P_OFF", "WARP_ORDER_OFF",
+               "WARP_LPTR_OFF", "WARP_L", "WARP_ROW_BASE", "WARP_ROW_COUNT")
+
+
+def validate_tree_multi(d_mem, a_geom, b, plan, K_stride, L_max, threads, grid_x):
+    """Checks of the reference's packed launch contract (triton/_contracts.py:
+    271-341): (B, K_stride) planes of one float dtype, integer plan tables on
+    the planes' device, one entry per plan row in each WARP_* table, grid_x
+    equal to that count.  Reads no tensor values."""
+    data = (("d_mem", d_mem), ("a_geom", a_geom), ("b", b))
+    for name, t in data + tuple(zip(_PLAN_NAMES, plan)):
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"{name} must be a torch.Tensor")
+    if d_mem.ndim != 2:
+        raise ValueError(f"d_mem must have shape (B, K_stride), got {tuple(d_mem.shape)}")
+    if a_geom.shape != d_mem.shape or b.shape != d_mem.shape:
+        raise ValueError("d_mem, a_geom and b must share one (B, K_stride) shape")
+    batch, width = map(int, d_mem.shape)
+    if batch < 1 or width < 1:
+        raise ValueError("the planes need at least one row and one column")
+    for name, v in (("K_stride", K_stride), ("L_max", L_max)):
+        if isinstance(v, bool) or not isinstance(v, int):
+            raise TypeError(f"{name} must be an integer")
+    if K_stride != width:
+        raise ValueError(f"K_stride {K_stride} differs from the plane width {width}")
+    if L_max < 1:
+        raise ValueError("L_max must be at least 1")
+    _float_group(data)
+    for name, t in zip(_PLAN_NAMES, plan):
+        if t.dtype not in _INDEX:
+            raise TypeError(f"{name} must have dtype int32 or int64")
+        if t.device != d_mem.device:
+            raise ValueError(f"{name} must be on device {d_mem.device}; got {t.device}")
+    P_cat, order_cat, lptr_cat = plan[:3]
+    if P_cat.numel() != order_cat.numel() or P_cat.numel() < 1:
+        raise ValueError("P_cat and ORDER_cat must be non-empty and of equal length")
+    if lptr_cat.numel() < 2:
+        raise ValueError("LAYER_PTR_cat needs at least two entries")
+    n_warps = int(plan[7].numel())
+    if n_warps < 1:
+        raise ValueError("the warp plan has no entries")
+    for name, t in zip(_PLAN_NAMES[3:], plan[3:]):
+        if t.numel() != n_warps:
+            raise ValueError(f"{name} must hold one entry per plan row ({n_warps})")
+    if grid_x is not None:
+        if isinstance(grid_x, bool) or not isinstance(grid_x, int):
+            raise TypeError("grid_x must be an integer or None")
+        if grid_x != n_warps:
+            raise ValueError(f"grid_x {grid_x} differs from the plan's {n_warps} entries")
+    return batch, n_warps, validate_threads(threads)
+
+
+def _multi_args(plan, B_total, K_stride, L_max):
+    P_cat, order_cat, lptr_cat = (t.to(torch.int64).contiguous() for t in plan[:3])
+    warps = [t.to(torch.int64).contiguous() for t in plan[3:]]
+    keep = (P_cat, order_cat, lptr_cat, *warps)
+    args = (_lib.ptr(P_cat), _lib.ptr(order_cat), P_cat.numel(), _lib.ptr(lptr_cat),
+            lptr_cat.numel(), *(_lib.ptr(w) for w in warps), warps[0].numel(), B_total,
+            K_stride, L_max)
+    return keep, args
+
+
+def _multi_solve_raw(d_mem, a_geom, b, plan, K_stride, L_max, max_rows):
+    d_mem, a_geom, b = (t.contiguous() for t in (d_mem, a_geom, b))
+    B = d_mem.shape[0]
+    keep, args = _multi_args(plan, B, K_stride, L_max)
+    x = torch.zeros_like(b)
+    work = torch.empty_like(d_mem)
+    lib = _lib.load()
+    fn = lib.dn_tree_dhs_multi_f64 if b.dtype == torch.float64 else lib.dn_tree_dhs_multi_f32
+    st = fn(_lib.ptr(d_mem), _lib.ptr(a_geom), _lib.ptr(b), *args, int(max_rows),
+            _lib.ptr(work), _lib.ptr(x), _lib.stream_handle(b.device))
+    _lib.check(st, "packed multi-tree solve")
+    del keep
+    return x
+
+
+class _DHSSolveMulti(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, d_mem, a_geom, b, plan, K_stride, L_max, max_rows):
+        x = _multi_solve_raw(d_mem, a_geom, b, plan, K_stride, L_max, max_rows)
+        ctx.save_for_backward(d_mem, a_geom, x, *plan)
+        ctx.meta = (K_stride, L_max, max_rows)
+        return x
+
+    @staticmethod
+    def backward(ctx, grad_x):
+        d_mem, a_geom, x, *plan = ctx.saved_tensors
+        K_stride, L_max, max_rows = ctx.meta
+        need = ctx.needs_input_grad
+        # symmetric matrix: the adjoint is the same solve; padding stays zero
+        y = _multi_solve_raw(d_mem, a_geom, grad_x.contiguous(), plan, K_stride, L_max,
+                             max_rows)
+        gd = -y * x if need[0] else None
+        ga = None
+        if need[1]:
+            keep, args = _multi_args(plan, d_mem.shape[0], K_stride, L_max)
+            ga = torch.zeros_like(a_geom)
+            lib = _lib.load()
+            fn = lib.dn_tree_dhs_multi_grad_a_f64 if x.dtype == torch.float64 else \
+                lib.dn_tree_dhs_multi_grad_a_f32
+            st = fn(_lib.ptr(x), _lib.ptr(y), *args, _lib.ptr(ga), _lib.stream_handle(x.device))
+            _lib.check(st, "packed multi-tree gradient")
+            del keep
+        gb = y if need[2] else None
+        return gd, ga, gb, None, None, None, None
+
+
+def dhs_solve_multi(d_mem, a_geom, b, P_cat, ORDER_cat, LAYER_PTR_cat, WARP_P_OFF,
+                    WARP_ORDER_OFF, WARP_LPTR_OFF, WARP_L, WARP_ROW_BASE, WARP_ROW_COUNT,
+                    K_stride: int, L_max: int, threads: int = 16, grid_x: int | None = None):
+    """Packed multi-morphology tree solve over row-padded planes, with the
+    reference's arguments (dhs_kernel_multi.py:831-888).  Entries the plan
+    does not cover -- padding columns and rows -- come back as zero
+    (_cuda_functional.py:207-211).  Gradients: b (the adjoint solve), d_mem
+    (-y x) and a_geom (-(y_i - y_p)(x_i - x_p), -y x at roots)."""
+    plan = (P_cat, ORDER_cat, LAYER_PTR_cat, WARP_P_OFF, WARP_ORDER_OFF, WARP_LPTR_OFF,
+            WARP_L, WARP_ROW_BASE, WARP_ROW_COUNT)
+    _, _, threads = validate_tree_multi(d_mem, a_geom, b, plan, K_stride, L_max, threads,
+                                        grid_x)
+    max_rows = max(1, 32 // threads)
+    return _DHSSolveMulti.apply(d_mem, a_geom, b, plan, int(K_stride), int(L_max), max_rows)
+
+
+dhs_solve_multi_cuda = dhs_solve_multi

@@ -177,3 +177,37 @@ def section_plan(parent_idx, sec_max: int = SEC_MAX) -> SectionPlan:
         sec_parent=sec_parent,
         lvl_ptr=lvl_ptr.astype(np.int64),
     )
+
+
+def multi_dhs_plan(parents, rows, threads: int = 16):
+    """Packed plan of the multi-morphology tree solve for morphologies in
+    solver order (parent < child): one entry of ``parents`` per morphology,
+    ``rows[g]`` consecutive plane rows for morphology g (in order), each plan
+    row covering up to 32 // threads of them (the reference's neurons per
+    warp, _dhs_multi._get_tiled_plan, integrators/tree.py:2017-2084).
+    Returns the tables as int64 numpy arrays with K_stride and L_max."""
+    P, O, LP, p_off, o_off, l_off, nl, row0, rcount = ([] for _ in range(9))
+    per_warp = max(1, 32 // threads)
+    row = 0
+    for par, n in zip(parents, rows):
+        par = np.asarray(par, dtype=np.int64)
+        _, _, depth = build_morphology(par)
+        order, lptr = build_dhs_layers(depth, threads)
+        base_p, base_l = len(P), len(LP)
+        P.extend(par.tolist())
+        O.extend(order.tolist())
+        LP.extend(lptr.tolist())
+        for k in range(0, int(n), per_warp):
+            p_off.append(base_p)
+            o_off.append(base_p)
+            l_off.append(base_l)
+            nl.append(len(lptr) - 1)
+            row0.append(row + k)
+            rcount.append(min(per_warp, int(n) - k))
+        row += int(n)
+    a = lambda v: np.asarray(v, dtype=np.int64)
+    return dict(P_cat=a(P), ORDER_cat=a(O), LAYER_PTR_cat=a(LP), WARP_P_OFF=a(p_off),
+                WARP_ORDER_OFF=a(o_off), WARP_LPTR_OFF=a(l_off), WARP_L=a(nl),
+                WARP_ROW_BASE=a(row0), WARP_ROW_COUNT=a(rcount),
+                K_stride=max(len(p) for p in parents), L_max=int(max(nl)),
+                grid_x=len(row0), rows_total=row)

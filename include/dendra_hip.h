@@ -771,6 +771,53 @@ int dn_exp2syn_membrane_f64(const double* A, const double* B, const double* e,
                             const int32_t* comp, const int32_t* syn_order, int64_t n_comp,
                             double* g_out, double* ge_out, hipStream_t stream);
 
+/* ---- packed multi-morphology tree solve ----------------------------------
+ * Replaces dhs_solve_multi_cuda / DHSSolveMultiPacked and
+ * _multi_dhs_grad_a_kernel (integrators/triton/dhs_kernel_multi.py:16-206,
+ * 629-888; caller _dhs_multi._step, integrators/tree.py:2284-2300).  d, a, b,
+ * x, D_work: (B_total, K_stride) row-padded planes in solver order.  Topology
+ * tables (int64): P_cat / order_cat (n_cat) and lptr_cat (n_lcat), local to
+ * each morphology; plan entry w (int64 tables of n_warps) covers rows
+ * warp_row_base[w] .. + warp_row_count[w] - 1 of the morphology at offsets
+ * warp_p_off / warp_order_off / warp_lptr_off with warp_l layers (at most
+ * L_max visited).  x must be zero on entry: entries no plan row covers stay
+ * zero.  D_work: caller workspace of the same shape (overwritten).  max_rows
+ * sizes the workgroup (rows per entry; larger entries loop).  grad_a: the
+ * gradient with respect to a for the solution V and the adjoint solution G,
+ * written on covered entries (dA zero on entry). */
+int dn_tree_dhs_multi_f32(const float* d, const float* a, const float* b,
+                          const int64_t* P_cat, const int64_t* order_cat, int64_t n_cat,
+                          const int64_t* lptr_cat, int64_t n_lcat, const int64_t* warp_p_off,
+                          const int64_t* warp_order_off, const int64_t* warp_lptr_off,
+                          const int64_t* warp_l, const int64_t* warp_row_base,
+                          const int64_t* warp_row_count, int64_t n_warps, int64_t B_total,
+                          int64_t K_stride, int64_t L_max, int max_rows, float* D_work,
+                          float* x, hipStream_t stream);
+int dn_tree_dhs_multi_f64(const double* d, const double* a, const double* b,
+                          const int64_t* P_cat, const int64_t* order_cat, int64_t n_cat,
+                          const int64_t* lptr_cat, int64_t n_lcat, const int64_t* warp_p_off,
+                          const int64_t* warp_order_off, const int64_t* warp_lptr_off,
+                          const int64_t* warp_l, const int64_t* warp_row_base,
+                          const int64_t* warp_row_count, int64_t n_warps, int64_t B_total,
+                          int64_t K_stride, int64_t L_max, int max_rows, double* D_work,
+                          double* x, hipStream_t stream);
+int dn_tree_dhs_multi_grad_a_f32(const float* V, const float* G, const int64_t* P_cat,
+                                 const int64_t* order_cat, int64_t n_cat,
+                                 const int64_t* lptr_cat, int64_t n_lcat,
+                                 const int64_t* warp_p_off, const int64_t* warp_order_off,
+                                 const int64_t* warp_lptr_off, const int64_t* warp_l,
+                                 const int64_t* warp_row_base, const int64_t* warp_row_count,
+                                 int64_t n_warps, int64_t B_total, int64_t K_stride,
+                                 int64_t L_max, float* dA, hipStream_t stream);
+int dn_tree_dhs_multi_grad_a_f64(const double* V, const double* G, const int64_t* P_cat,
+                                 const int64_t* order_cat, int64_t n_cat,
+                                 const int64_t* lptr_cat, int64_t n_lcat,
+                                 const int64_t* warp_p_off, const int64_t* warp_order_off,
+                                 const int64_t* warp_lptr_off, const int64_t* warp_l,
+                                 const int64_t* warp_row_base, const int64_t* warp_row_count,
+                                 int64_t n_warps, int64_t B_total, int64_t K_stride,
+                                 int64_t L_max, double* dA, hipStream_t stream);
+
 /* ---- allocation-free tridiagonal solve (float32) --------------------------
  * No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

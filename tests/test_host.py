@@ -369,3 +369,28 @@ def test_tree_groups_are_checked_before_launch():
         validate_tree_groups([[0, 1, 2]], n_sec, sizes, 512, 4000)
     with pytest.raises(ValueError, match="node capacity"):
         validate_tree_groups([[0, 1]], n_sec, sizes, 512, 3000)
+
+
+def test_multi_dhs_plan_covers_each_row_once():
+    from dendra_amd.topology import multi_dhs_plan
+
+    morphs = [np.array([-1, 0, 0, 1, 1, 2]), np.arange(-1, 9), np.array([-1])]
+    for threads in (1, 4, 32):
+        pl = multi_dhs_plan(morphs, (70, 3, 1), threads)
+        per = max(1, 32 // threads)
+        assert pl["rows_total"] == 74 and pl["K_stride"] == 10 and pl["grid_x"] == len(
+            pl["WARP_ROW_BASE"])
+        covered = np.concatenate([np.arange(r, r + c) for r, c in
+                                  zip(pl["WARP_ROW_BASE"], pl["WARP_ROW_COUNT"])])
+        assert np.array_equal(np.sort(covered), np.arange(74))
+        assert pl["WARP_ROW_COUNT"].max() <= per
+        for w in range(pl["grid_x"]):
+            lp = pl["LAYER_PTR_cat"][pl["WARP_LPTR_OFF"][w]:][: pl["WARP_L"][w] + 1]
+            K = lp[-1]
+            order = pl["ORDER_cat"][pl["WARP_ORDER_OFF"][w]:][:K]
+            par = pl["P_cat"][pl["WARP_P_OFF"][w]:][:K]
+            assert sorted(order.tolist()) == list(range(K))
+            pos = np.empty(K, int)
+            pos[order] = np.arange(K)
+            assert all(pos[i] < pos[p] for i, p in enumerate(par) if p >= 0)  # children first
+            assert (np.diff(lp) <= threads).all()
