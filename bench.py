@@ -55,7 +55,7 @@ def _parse():
                     help="autotune the launch parameters before the timed region (0 = defaults)")
     ap.add_argument("--only", choices=["all", "cables", "trees", "none"], default="all",
                     help="parts of the uneven population to run (none: only the other lines)")
-    ap.add_argument("--network-steps", type=int, default=4000,
+    ap.add_argument("--network-steps", type=int, default=40000,
                     help="time steps of the network workload line (0 = skip)")
     ap.add_argument("--surrogate-train", type=int, default=2_000_000,
                     help="fibers in the surrogate training set (0 = skip the surrogate)")

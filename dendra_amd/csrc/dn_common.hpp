@@ -208,9 +208,16 @@ __device__ __forceinline__ void hh_advance<float>(float v, float dt, const HHPar
                                                   float& n, float& G, float& J) {
   const float xm = -(v + 40.0f), xn = -(v + 55.0f), w = v + 65.0f;
   const float qm = xm * 0.1f, qn = xn * 0.1f;
-  const float e40 = fexp(qm);
-  const float e80 = fexp(w * (-1.0f / 80.0f));
-  const float bm = 4.0f * fexp(w * (-1.0f / 18.0f));
+  // Exponent arguments are capped (e^80 = 5.5e34; e80 at e^20, so that
+  // ah = 0.07 e80^4 stays below e^80): below about -1660 mV bm, and below
+  // about -1840 mV ah, would overflow binary32 and turn the gate update into
+  // inf - inf or inf / inf (NaN).  A rate that large takes its gate to the
+  // equilibrium within one step with or without the cap (e^{-rate dt} is 0),
+  // so the cap changes no result above those voltages and keeps the state
+  // finite below them, as binary64 stays.
+  const float e40 = fexp(fminf(qm, 80.0f));
+  const float e80 = fexp(fminf(w * (-1.0f / 80.0f), 20.0f));
+  const float bm = 4.0f * fexp(fminf(w * (-1.0f / 18.0f), 80.0f));
   const float e80s = e80 * e80;
   const float ah = 0.07f * (e80s * e80s);
   const float bn = 0.125f * e80;

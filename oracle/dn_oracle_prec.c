@@ -21,6 +21,7 @@
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
+#include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -99,3 +100,124 @@ static void hh_advance_f(float v, float dt, const float* p, float sna, float sk,
 #include "dn_oracle_prec.inc"
 #undef PREC_NAME
 #undef VT
+
+/*
+ * orc_tree_run_f32: the reference's float32 tree step (integrators/
+ * tree.py:1202-1357 with the float32 mechanism branches above and the DHS
+ * elimination of triton/dhs_kernel.py:14-112) in binary32, the tree
+ * counterpart of orc_cable_run_f32: same arguments and order as orc_tree_run
+ * (dn_oracle.c) minus the intracellular current.  Geometry in binary64,
+ * rounded once.
+ */
+ORC_API int orc_tree_run_f32(int64_t n_trees, const int64_t* off, const int64_t* parent,
+                             const double* diam, const double* len, const double* ra,
+                             const double* cm, const double* sna, const double* sk,
+                             const double* hhp, const double* lead, const double* ve_amp,
+                             const double* ve_wave, double dt, int64_t n_steps, double* V,
+                             double* M, double* H, double* N, double* rec,
+                             int64_t rec_every, int n_threads) {
+  float pf[7];
+  for (int i = 0; i < 7; ++i) pf[i] = (float)hhp[i];
+  int64_t total = off[n_trees];
+  const double dt_s = dt * 1.0e-3;
+  const float dtf = (float)dt;
+  int bad = 0;
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+#endif
+  for (int64_t t = 0; t < n_trees; ++t) {
+    int64_t o = off[t], K = off[t + 1] - off[t];
+    const int64_t* par = parent + o;
+    float* w = (float*)malloc(sizeof(float) * (size_t)K * 13);
+    int64_t* iw = (int64_t*)malloc(sizeof(int64_t) * (size_t)(2 * K + 2));
+    if (!w || !iw) { bad |= 1; free(w); free(iw); continue; }
+    float *area = w, *cmdt = w + K, *ag = w + 2 * K, *f = w + 3 * K, *mainv = w + 4 * K,
+          *ve = w + 5 * K, *dv = w + 6 * K, *D = w + 7 * K, *Bv = w + 8 * K, *v = w + 9 * K,
+          *m = w + 10 * K, *h = w + 11 * K, *n = w + 12 * K;
+    int64_t *depth = iw, *order = iw + K;
+    int64_t maxd = 0;
+    for (int64_t i = 0; i < K; ++i) {
+      double a = PREC_PI * diam[o + i] * len[o + i] * 1.0e-8;
+      area[i] = (float)a;
+      cmdt[i] = (float)(1.0e-6 * cm[o + i] * a / dt_s);
+      v[i] = (float)V[o + i];
+      m[i] = (float)M[o + i];
+      h[i] = (float)H[o + i];
+      n[i] = (float)N[o + i];
+      if (par[i] < 0) { ag[i] = 0.0f; depth[i] = 0; continue; }
+      if (par[i] >= i) bad |= 2;
+      int64_t q = par[i];
+      double ri = 0.5 * diam[o + i] * 1.0e-4, rp = 0.5 * diam[o + q] * 1.0e-4;
+      double Ri = ra[o + i] * (len[o + i] * 1.0e-4 / 2.0) / (PREC_PI * ri * ri);
+      double Rp = ra[o + q] * (len[o + q] * 1.0e-4 / 2.0) / (PREC_PI * rp * rp);
+      ag[i] = (float)(1.0 / (Ri + Rp));
+      depth[i] = depth[q] + 1;
+      if (depth[i] > maxd) maxd = depth[i];
+    }
+    if (bad) { free(w); free(iw); continue; }
+    int64_t pos = 0;
+    for (int64_t dd = maxd; dd >= 0; --dd)
+      for (int64_t i = 0; i < K; ++i)
+        if (depth[i] == dd) order[pos++] = i;
+    for (int64_t s = 0; s < n_steps; ++s) {
+      for (int64_t i = 0; i < K; ++i) {
+        float G, J;
+        hh_advance_f(v[i], dtf, pf, sna ? (float)sna[o + i] : 1.0f,
+                     sk ? (float)sk[o + i] : 1.0f, &m[i], &h[i], &n[i], &G, &J);
+        f[i] = (J - G * v[i]) * area[i];
+        mainv[i] = cmdt[i] + G * area[i];
+        ve[i] = (lead && ve_wave) ? (float)lead[o + i] * (float)ve_amp[t] * (float)ve_wave[s]
+                                  : 0.0f;
+      }
+      for (int64_t i = 0; i < K; ++i) {
+        int64_t q = par[i];
+        if (q < 0) continue;
+        float flow = ag[i] * (v[i] - v[q]);
+        f[i] -= flow;
+        f[q] += flow;
+      }
+      if (lead && ve_wave) {
+        for (int64_t i = 0; i < K; ++i) {
+          int64_t q = par[i];
+          if (q < 0) continue;
+          float flow = ag[i] * (ve[i] - ve[q]);
+          f[i] -= flow;
+          f[q] += flow;
+        }
+      }
+      memcpy(D, mainv, sizeof(float) * (size_t)K);
+      memcpy(Bv, f, sizeof(float) * (size_t)K);
+      for (int64_t k = 0; k < K; ++k) {
+        int64_t i = order[k];
+        float old_d = D[i];
+        D[i] = old_d + ag[i];
+        int64_t q = par[i];
+        if (q >= 0) {
+          float fac = ag[i] / D[i];
+          D[q] += fac * old_d;
+          Bv[q] += fac * Bv[i];
+        }
+      }
+      for (int64_t k = K - 1; k >= 0; --k) {
+        int64_t i = order[k];
+        int64_t q = par[i];
+        dv[i] = (Bv[i] + ag[i] * (q >= 0 ? dv[q] : 0.0f)) / D[i];
+      }
+      for (int64_t i = 0; i < K; ++i) v[i] += dv[i];
+      if (rec && rec_every > 0 && (s + 1) % rec_every == 0) {
+        int64_t r = (s + 1) / rec_every - 1;
+        for (int64_t i = 0; i < K; ++i) rec[r * total + o + i] = (double)v[i];
+      }
+    }
+    for (int64_t i = 0; i < K; ++i) {
+      V[o + i] = v[i];
+      M[o + i] = m[i];
+      H[o + i] = h[i];
+      N[o + i] = n[i];
+    }
+    free(w);
+    free(iw);
+  }
+  return bad ? -1 : 0;
+}

@@ -82,6 +82,8 @@ def load():
         for fn in (lib.orc_cable_run_f32, lib.orc_cable_run_f32gates):
             fn.argtypes = prec_args
             fn.restype = i32
+        lib.orc_tree_run_f32.argtypes = [i64, lp, lp] + prec_args[2:]
+        lib.orc_tree_run_f32.restype = i32
         lib.orc_tree_run.argtypes = [i64, lp, lp] + run_args[2:]
         lib.orc_tree_run.restype = i32
         lib.orc_extcell_run.argtypes = [i64, i64, dp, dp, dp, dp, dp, vp, vp, dp, vp, vp, vp,
@@ -228,6 +230,37 @@ def cable_run_f32(spec, hh_arr, dt, n_steps, ve_wave=None, record_every=0, threa
     st = fn(spec.n_cells, _c(spec.off, np.int64), _c(spec.diam), _c(spec.dx), _c(spec.rhoa),
             _c(spec.cm), sna_p, sk_p, _c(hh_arr), lead_p, vea_p, vew_p, float(dt),
             int(n_steps), V, M, H, N, rec_p, int(record_every), int(threads))
+    if st != 0:
+        raise RuntimeError(f"oracle run failed ({st})")
+    return {"v": V, "m": M, "h": H, "n": N}, rec
+
+
+def tree_run_f32(spec, hh_arr, dt, n_steps, ve_wave=None, record_every=0, threads=0,
+                 state=None):
+    """The reference's float32 tree step restated in binary32
+    (orc_tree_run_f32); arguments and results as tree_run (no intracellular
+    current)."""
+    lib = load()
+    total = spec.total
+    sna, sna_p = _opt(spec.sna)
+    sk, sk_p = _opt(spec.sk)
+    lead, lead_p = _opt(spec.lead)
+    vea, vea_p = _opt(spec.ve_amp)
+    vew, vew_p = _opt(ve_wave)
+    if state is None:
+        m0, h0, n0 = hh_steady(spec.v_init)
+        V = np.full(total, float(spec.v_init))
+        M, H, N = np.full(total, m0), np.full(total, h0), np.full(total, n0)
+    else:
+        V, M, H, N = (np.array(state[k], dtype=np.float64) for k in ("v", "m", "h", "n"))
+    rec = rec_p = None
+    if record_every > 0:
+        rec = np.zeros((n_steps // record_every, total))
+        rec_p = rec.ctypes.data_as(vp)
+    st = lib.orc_tree_run_f32(spec.n_trees, _c(spec.off, np.int64), _c(spec.parent, np.int64),
+                              _c(spec.diam), _c(spec.length), _c(spec.ra), _c(spec.cm), sna_p,
+                              sk_p, _c(hh_arr), lead_p, vea_p, vew_p, float(dt), int(n_steps),
+                              V, M, H, N, rec_p, int(record_every), int(threads))
     if st != 0:
         raise RuntimeError(f"oracle run failed ({st})")
     return {"v": V, "m": M, "h": H, "n": N}, rec

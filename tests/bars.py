@@ -10,8 +10,14 @@ t = 0 already ends > 1e-3 mV away, and the reference's float32 step
 restated in binary32 -- with or without a binary64 voltage and elimination
 -- lands between 1e-3 and 5e-3 mV.  Spike steps must still agree within
 one dt.
+
+On the largest cells of the benchmark population the amplification is
+larger still (half a float32 ulp at t = 0 moves the 12 longest cables by
+2e-2 mV): there the float32 runs are held to F32_VS_BINARY32 times the error
+of the binary32 restatement of the same cells (oracle/dn_oracle_prec.c).
 """
 
 F64 = 1e-7
 F32_NORTH_STAR = 1e-3
 F32 = 5e-3
+F32_VS_BINARY32 = 2.0

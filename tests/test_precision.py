@@ -77,3 +77,24 @@ def test_float32_restatement_is_binary32(orc):
     _, mixed = orc.cable_run_f32(spec, hh.array(), 0.025, 200, record_every=1,
                                  binary64_voltage=True)
     assert not np.array_equal(mixed.astype(np.float32).astype(np.float64), mixed)
+
+
+def test_tree_float32_restatement(orc):
+    """Pins of the binary32 tree restatement: a resting tree without drive
+    stays within float32 round-off of the binary64 run, its voltages are
+    binary32 values, and on driven spiking trees it lands within float32's
+    reach of the binary64 oracle but not on it."""
+    from dendra_amd.model import HH
+    from dendra_amd.workloads import extracellular_wave, random_trees
+
+    hh = HH(celsius=6.3)
+    spec = random_trees(4, seed=9, tmin=200, tmax=900)
+    _, want = orc.tree_run(spec, hh.array(), 0.005, 300, record_every=1)
+    _, got = orc.tree_run_f32(spec, hh.array(), 0.005, 300, record_every=1)
+    assert np.array_equal(got.astype(np.float32).astype(np.float64), got)
+    assert np.abs(got - want).max() < 1e-4
+    wave = extracellular_wave(2000, 0.005)
+    _, want = orc.tree_run(spec, hh.array(), 0.005, 2000, ve_wave=wave, record_every=10)
+    _, got = orc.tree_run_f32(spec, hh.array(), 0.005, 2000, ve_wave=wave, record_every=10)
+    err = np.abs(got - want).max()
+    assert np.ptp(want) > 20.0 and 1e-6 < err < 1e-2, err
