@@ -8,10 +8,13 @@ dendra_amd/workloads.py).  Inputs are resident in HBM before the timed region.
 
 Printed JSON line: metric compartment-steps/s of the persistent path (cable
 and tree kernels on two streams), with
-  * roofline: the dominant kernel (the persistent cable kernel) -- algorithmic
-    bytes 48 B per compartment-step (read+write V,m,h,n and read four
-    coefficient streams, float32) times its compartment-steps per launch,
-    over its average launch time measured with HIP events on its stream;
+  * roofline: the binding bound, VALU issue: the two concurrent persistent
+    kernels' VALU instructions (per compartment-step from rocprofv3
+    SQ_INSTS_VALU) over the run time, against 6.144e11 wave64 instructions/s;
+  * hbm_roofline: the cable kernel's per-step algorithmic bytes (48 B per
+    compartment-step: read+write V,m,h,n and read four coefficient streams,
+    float32) over its launch time measured with HIP events on its stream --
+    a secondary figure, as the kernel keeps its state on chip;
   * plain: the per-step GPU path (2 launches per cable step, 2 per tree step)
     timed on the same workload;
   * cpu_baseline: the binary64 C oracle (OpenMP) on a bounded sample of the
@@ -36,6 +39,16 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CS = 48.0       # float32 cable compartment-step
 BYTES_PER_CS_TREE = 52.0  # float32 tree compartment-step (+ int32 parent index)
+
+# VALU issue bound of the persistent kernels (state stays on chip, so they
+# are not HBM-bound).  Peak: 256 CUs x 4 SIMDs, each issuing one wave64
+# VALU instruction per 4 cycles (16 lanes per cycle) at 2.4 GHz = 6.144e11
+# wave-instructions/s (the packed-FP32 forms, which double the published
+# 157.3 TFLOP/s figure's 78.6, are disabled in this build).  Instructions per
+# compartment-step: SQ_INSTS_VALU of each kernel divided by its
+# compartment-steps, profiles/hh_population_uneven_valu.md.
+VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 4
+VALU_PER_CS = {"cable": 2.379, "tree": 3.560}
 
 
 def _parse():
@@ -213,17 +226,37 @@ def population_bench(args, dev, hh):
                            "note": "algorithmic bytes of the per-step path (state and "
                                    "coefficients read and written every step); the persistent "
                                    "kernels keep the state on chip, so frac can exceed 1"}
+    # VALU bound of the job: both persistent kernels share the GPU for the
+    # whole run, so their instructions are counted together over the run.
+    valu = (VALU_PER_CS["cable"] * (work.cables.total if use_c else 0)
+            + VALU_PER_CS["tree"] * (work.trees.total if use_t else 0)) * work.n_steps
+    out["roofline"] = {
+        "bound": "valu", "achieved": valu / wall, "peak": VALU_PEAK_PER_S,
+        "unit": "wave64 VALU instructions/s", "frac": valu / wall / VALU_PEAK_PER_S,
+        "traffic": _measured_traffic("hh_cable_run_kernel"),
+        "kernel": "hh_cable_run_kernel<float> + hh_tree_sec_kernel<float> (concurrent)",
+        "valu_instructions_per_run": valu,
+        "valu_per_compartment_step": VALU_PER_CS,
+        "note": "instructions per compartment-step from rocprofv3 SQ_INSTS_VALU "
+                "(profiles/hh_population_uneven_valu.md); time = the run",
+    }
     if c_ms:
         c_avg = float(np.mean(c_ms)) * 1e-3
         c_bytes = BYTES_PER_CS * work.cables.total * work.n_steps
         ach = c_bytes / c_avg / 1e9
-        out["roofline"] = {
+        out["hbm_roofline"] = {
             "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS, "traffic": _measured_traffic("hh_cable_run_kernel"),
             "kernel": "hh_cable_run_kernel<float>",
             "kernel_ms": c_avg * 1e3,
             "algorithmic_bytes_per_launch": c_bytes,
+            "note": "per-step algorithmic bytes (48 B per compartment-step) over the cable "
+                    "kernel's launch time; a secondary figure, the kernel keeps its state "
+                    "on chip and moves `traffic` bytes",
         }
+        cv = VALU_PER_CS["cable"] * work.cables.total * work.n_steps / c_avg
+        out["cable_valu"] = {"achieved": cv, "frac_of_gpu": cv / VALU_PEAK_PER_S,
+                             "kernel_ms": c_avg * 1e3}
         out["cable_cs_per_s"] = work.cables.total * work.n_steps / c_avg
     if t_ms:
         t_avg = float(np.mean(t_ms)) * 1e-3
@@ -232,7 +265,11 @@ def population_bench(args, dev, hh):
         t_bytes = BYTES_PER_CS_TREE * work.trees.total * work.n_steps
         out["tree_roofline"] = {"achieved": t_bytes / t_avg / 1e9, "unit": "GB/s",
                                 "frac": t_bytes / t_avg / 1e9 / HBM_PEAK_GBS,
-                                "kernel": "hh_tree_sec_kernel<float>"}
+                                "kernel": "hh_tree_sec_kernel<float>",
+                                "note": "per-step algorithmic bytes, secondary"}
+        tv = VALU_PER_CS["tree"] * work.trees.total * work.n_steps / t_avg
+        out["tree_valu"] = {"achieved": tv, "frac_of_gpu": tv / VALU_PEAK_PER_S,
+                            "kernel_ms": t_avg * 1e3}
         if c_ms and t_avg > c_avg:
             out["roofline_note"] = "tree kernel is the longer of the two concurrent kernels"
     if args.plain > 0:
@@ -287,6 +324,17 @@ def network_bench(n_steps, hh, dev, with_cpu):
         "deliveries_per_s": deliveries / wall,
         "compartment_steps_per_s": spec.cells.total * n_steps / wall,
     }
+    # per-step algorithmic bytes: every compartment's state and coefficients
+    # (48 B), today's ring row read and cleared (8 B per cell), 16 B per
+    # delivered event (target, weight, ring write)
+    net_bytes = (48.0 * spec.cells.total + 8.0 * spec.cells.n_cells) * n_steps + \
+        16.0 * deliveries
+    res["roofline"] = {"bound": "hbm", "achieved": net_bytes / wall / 1e9,
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": net_bytes / wall / 1e9 / HBM_PEAK_GBS,
+                       "us_per_step": wall / n_steps * 1e6,
+                       "note": "three dependent launches per step (HIP graphs of 100 "
+                               "steps): launch latency, not bandwidth, sets the step time"}
     if with_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
         import oracle
@@ -801,13 +849,34 @@ def cpu_baseline(work, hh, frac, n_steps):
     oracle.tree_run(ts, hh.array(), work.dt, n_steps, ve_wave=wave, threads=threads)
     dt = time.perf_counter() - t
     n_cs = (cs.total + ts.total) * n_steps
+    # one thread on an eighth of that sample (every 8th cell of it)
+    cs1 = cs.subset(np.arange(0, cs.n_cells, 8))
+    ts1 = ts.subset(np.arange(0, ts.n_trees, 8))
+    t = time.perf_counter()
+    oracle.cable_run(cs1, hh.array(), work.dt, n_steps, ve_wave=wave, threads=1)
+    oracle.tree_run(ts1, hh.array(), work.dt, n_steps, ve_wave=wave, threads=1)
+    dt1 = time.perf_counter() - t
     return {
         "value": n_cs / dt, "unit": "compartment-steps/s", "cores": threads,
         "kind": "port",
         "sample": f"every {stride}th cable and tree ({cs.n_cells} cables, {ts.n_trees} "
                   f"trees, {cs.total + ts.total} compartments) x {n_steps} steps, float64 "
                   f"C oracle, OpenMP {threads} threads, {dt:.1f}s",
+        "value_1_thread": (cs1.total + ts1.total) * n_steps / dt1,
+        "sample_1_thread": f"{cs1.n_cells} cables and {ts1.n_trees} trees of that sample, "
+                           f"{n_steps} steps, 1 thread, {dt1:.1f}s",
+        "cpu_model": _cpu_model(),
     }
+
+
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 if __name__ == "__main__":
