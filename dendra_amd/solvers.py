@@ -14,6 +14,7 @@ t_kernel_thomas.py:147-189; the symmetric tree adjoint with
 from __future__ import annotations
 
 import torch
+from torch.utils.weak import WeakIdKeyDictionary
 
 from . import _lib
 
@@ -221,8 +222,49 @@ class _DHSSolve(torch.autograd.Function):
         return gd, ga, gb, None, None, None
 
 
+# Contraction plans of the tree topologies the drop-in has seen, cached per
+# parent_idx tensor object (weakly) and checked against its version and size.
+_contract_plans = WeakIdKeyDictionary()
+
+
+def _contraction_plan(parent_idx):
+    """The topology's contraction plan, or None for a forest (several roots)
+    or a tree beyond the contraction kernel's section capacity.  Building it
+    reads parent_idx to the host once per tensor."""
+    stamp = (parent_idx._version, parent_idx.numel(), parent_idx.device)
+    hit = _contract_plans.get(parent_idx)
+    if hit is not None and hit[0] == stamp:
+        return hit[1]
+    from .contraction import TreeContractionPlan
+
+    par = parent_idx.detach().to("cpu", torch.int64).numpy().reshape(-1)
+    try:
+        plan = TreeContractionPlan(par, max_segment_interior_nodes=16)
+    except ValueError:
+        plan = None
+    _contract_plans[parent_idx] = (stamp, plan)
+    return plan
+
+
 def dhs_solve(d_mem, a_geom, b, parent_idx, order, layer_ptr, threads: int = 16):
-    """Tree (Hines) solve over the caller's DHS layers; see the module doc."""
+    """Tree (Hines) solve; see the module doc.  The solution does not depend
+    on the caller's layering, so a single tree is solved by contraction over
+    the topology that parent_idx gives (O(log) dependent rounds instead of one
+    per layer; same gradients, including the root's a_geom as a conductance
+    to ground); a forest, or a tree beyond the contraction kernel's capacity,
+    is eliminated over the caller's layers."""
+    validate_tree(d_mem, a_geom, b, parent_idx, order, layer_ptr, threads)
+    plan = _contraction_plan(parent_idx)
+    if plan is not None and plan.n_sections <= plan.section_cap(d_mem.element_size()):
+        from .contraction import _TreeContraction
+
+        return _TreeContraction.apply(d_mem, a_geom, b, plan)
+    return _DHSSolve.apply(d_mem, a_geom, b, parent_idx, order, layer_ptr)
+
+
+def dhs_solve_layers(d_mem, a_geom, b, parent_idx, order, layer_ptr, threads: int = 16):
+    """The same solve always eliminated over the caller's layers
+    (dn_tree_dhs_*, one wavefront per system)."""
     validate_tree(d_mem, a_geom, b, parent_idx, order, layer_ptr, threads)
     return _DHSSolve.apply(d_mem, a_geom, b, parent_idx, order, layer_ptr)
 

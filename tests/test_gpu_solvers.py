@@ -114,11 +114,14 @@ def _dense_tree(d_mem, a_geom, rhs, parent):
 PARENT7 = [-1, 0, 0, 1, 1, 2, 5]  # tests/test_cuda_solver_parity.py:32
 
 
+@pytest.mark.parametrize("layers", [False, True], ids=["drop-in", "layer-kernel"])
 @pytest.mark.parametrize("threads", [1, 8, 32])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-def test_dhs_matches_dense_and_oracle(orc, threads, dtype):
-    from dendra_amd.solvers import dhs_solve
+def test_dhs_matches_dense_and_oracle(orc, threads, dtype, layers):
+    from dendra_amd import solvers
     from dendra_amd.topology import build_dhs_layers, build_morphology
+
+    dhs_solve = solvers.dhs_solve_layers if layers else solvers.dhs_solve
 
     _, _, depth = build_morphology(PARENT7)
     order, layer_ptr = build_dhs_layers(depth, threads)
@@ -146,10 +149,13 @@ def test_dhs_two_node_known_answer():
         [[1.5, -0.5], [-0.5, 2.5]], [5.0, 3.0]), atol=1e-6)
 
 
-def test_dhs_large_random_tree_and_gradients(orc):
-    from dendra_amd.solvers import dhs_solve
+@pytest.mark.parametrize("layers", [False, True], ids=["drop-in", "layer-kernel"])
+def test_dhs_large_random_tree_and_gradients(orc, layers):
+    from dendra_amd import solvers
     from dendra_amd.topology import build_dhs_layers, build_morphology
     from dendra_amd.workloads import random_binary_tree
+
+    dhs_solve = solvers.dhs_solve_layers if layers else solvers.dhs_solve
 
     parent, _, _ = random_binary_tree(3000, 0, 11)
     _, _, depth = build_morphology(parent)
@@ -191,3 +197,74 @@ def test_dhs_large_random_tree_and_gradients(orc):
     want_g = torch.autograd.grad((xd.square() * w.cpu()).sum(), (dc, ac, rc))
     for gg, wg in zip(got_g, want_g):
         np.testing.assert_allclose(gg.cpu().numpy(), wg.numpy(), rtol=3e-10, atol=3e-10)
+
+
+def _chain_matrix(main, edge, parents):
+    """Dense A = diag(main + edge + sum of children's edges) - offdiag(edge)."""
+    B, K = main.shape
+    A = torch.diag_embed(main + edge)
+    for ch, p in enumerate(parents):
+        if p < 0:
+            continue
+        e = edge[:, ch]
+        A = A.clone()
+        A[:, p, p] = A[:, p, p] + e
+        A[:, ch, p] = A[:, ch, p] - e
+        A[:, p, ch] = A[:, p, ch] - e
+    return A
+
+
+def _chain_topology(parents):
+    dev = lambda x: torch.tensor(x, dtype=torch.int64, device="cuda")
+    K = len(parents)
+    return dev(list(parents)), dev(list(range(K - 1, -1, -1))), dev(list(range(K + 1)))
+
+
+@pytest.mark.parametrize("layers", [False, True], ids=["drop-in", "layer-kernel"])
+def test_dhs_float32_stiff_edge_keeps_the_uniform_solution(layers):
+    """tests/func/test_cuda_solver_precision.py:46-65: a 2-node chain with an
+    axial conductance of 2^26 keeps the passive uniform solution (all ones)
+    finite and accurate in float32."""
+    from dendra_amd import solvers
+
+    solve = solvers.dhs_solve_layers if layers else solvers.dhs_solve
+    main = torch.ones(1, 2, dtype=torch.float32, device="cuda")
+    edge = torch.tensor([[0.0, float(2**26)]], dtype=torch.float32, device="cuda")
+    x = solve(main, edge, main.clone(), *_chain_topology((-1, 0)), threads=32)
+    assert x.dtype == torch.float32 and bool(torch.isfinite(x).all())
+    torch.testing.assert_close(x.cpu().double(), torch.ones(1, 2, dtype=torch.float64),
+                               rtol=2e-6, atol=2e-7)
+
+
+@pytest.mark.parametrize("layers", [False, True], ids=["drop-in", "layer-kernel"])
+@pytest.mark.parametrize("signed", [False, True], ids=["spd", "indefinite"])
+def test_dhs_float32_schur_primal_gradients_and_root_placeholder(layers, signed):
+    """tests/func/test_cuda_solver_precision.py:68-105: a 3-node chain, SPD or
+    nonsingular indefinite (negative internal diagonal), against dense float64
+    solves and autograd; the root's edge coefficient acts on its own diagonal,
+    so its gradient equals the root diagonal's exactly."""
+    from dendra_amd import solvers
+
+    solve = solvers.dhs_solve_layers if layers else solvers.dhs_solve
+    if signed:
+        main = [[2.0, -0.5, 1.0], [2.4, -0.6, 1.3]]
+        edge = [[0.2, 0.2, 0.1], [0.3, 0.15, 0.2]]
+    else:
+        main = [[1.0, 2.0, 1.5], [1.2, 1.8, 2.4]]
+        edge = [[0.2, 0.3, 0.4], [0.15, 0.4, 0.2]]
+    rhs = [[0.7, -1.2, 0.3], [-0.2, 0.9, 1.1]]
+    values = [torch.tensor(v, dtype=torch.float32) for v in (main, edge, rhs)]
+    parents = (-1, 0, 1)
+    ref = [v.double().requires_grad_() for v in values]
+    want = torch.linalg.solve(_chain_matrix(ref[0], ref[1], parents),
+                              ref[2].unsqueeze(-1)).squeeze(-1)
+    w = torch.tensor([[0.3, -0.7, 0.9], [-0.6, 0.8, 0.2]], dtype=torch.float64)
+    want_g = torch.autograd.grad((want * w).sum(), ref)
+    args = [v.cuda().requires_grad_() for v in values]
+    got = solve(*args, *_chain_topology(parents), threads=32)
+    got_g = torch.autograd.grad((got * w.cuda().float()).sum(), args)
+    torch.testing.assert_close(got.detach().cpu().double(), want.detach(), rtol=1e-5,
+                               atol=1e-6)
+    for g_, wg in zip(got_g, want_g):
+        torch.testing.assert_close(g_.cpu().double(), wg, rtol=2e-5, atol=2e-6)
+    assert torch.equal(got_g[0][:, 0], got_g[1][:, 0])
