@@ -331,14 +331,22 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
   T* ars = gs + node_cap;  // per-node membrane area, staged once
   T* cms = ars + node_cap;  // per-node cm/dt when CM_LDS
   T *vh = cms + (CM_LDS ? node_cap : 0), *vt = vh + nt, *cD = vh + 2 * nt,
-    *cB = vh + 3 * nt, *xt = vh + 4 * nt, *gh = vh + 5 * nt;
+    *cB = vh + 3 * nt, *xt = vh + 4 * nt, *gh = vh + 5 * nt, *tM = vh + 6 * nt,
+    *tF = vh + 7 * nt;
   const int s = threadIdx.x;
   const int g0 = grp_off[blockIdx.x], g1 = grp_off[blockIdx.x + 1];
   // Sections of the group's trees, numbered tree by tree in breadth-first
-  // order ("BFS slot"), are redistributed over the threads in level-major
-  // order (level, tree, section): the wavefronts then hold one or two levels
-  // each, so a level's elimination and back-substitution run on few
-  // wavefronts.  locate() resolves a BFS slot to its tree and section.
+  // order ("BFS slot"), get two numberings:
+  //  * level-major rank (level, tree, section): the section-boundary tables
+  //    in LDS (vh, vt, gh, cD, cB, xt, tM, tF) are indexed by it, and thread
+  //    j plays tail node j in the level phases, so a level's elimination and
+  //    back-substitution run on one or two wavefronts;
+  //  * length-major thread (longest section first, level-major among equal
+  //    lengths): the thread that holds the section's nodes in registers for
+  //    the per-node phases (gating, axial currents, chain compression and
+  //    recovery), whose loops run as long as the longest section of the
+  //    wavefront -- sorting by length keeps the lanes of a wavefront busy.
+  // locate() resolves a BFS slot to its tree and section.
   int sec_base = 0, node_base = 0, nlvl_max = 0;
   for (int k = g0; k < g1; ++k) {
     const int t = order[k];
@@ -371,9 +379,13 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
     return L;
   };
   // level-major rank of this thread's BFS slot
-  int32_t* new_of = reinterpret_cast<int32_t*>(vh);   // BFS slot -> thread
-  int32_t* who = reinterpret_cast<int32_t*>(vt);      // thread -> BFS slot
+  int32_t* new_of = reinterpret_cast<int32_t*>(vh);   // BFS slot -> rank
+  int32_t* who = reinterpret_cast<int32_t*>(vt);      // rank -> BFS slot
+  int32_t* len_of = reinterpret_cast<int32_t*>(cD);   // rank -> section length
+  int32_t* owned = reinterpret_cast<int32_t*>(cB);    // thread -> rank it holds
   who[s] = -1;
+  len_of[s] = 0;
+  owned[s] = -1;
   __syncthreads();
   {
     const Loc L = locate(s);
@@ -389,14 +401,42 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
       rank += L.ls - A.pl.lvl_ptr[A.pl.lvl_off[L.tree] + L.lvl];
       new_of[s] = rank;
       who[rank] = s;
+      len_of[rank] = A.pl.sec_len[A.pl.sec_off[L.tree] + L.ls];
     }
   }
   __syncthreads();
-  const int bfs = who[s];
-  const Loc L = bfs >= 0 ? locate(bfs) : Loc{-1, -1, 0, 0, 0, -1};
+  // tail node s of the level phases (rank s)
+  int cf_t = 0, nc_t = 0, psec_t = -1, my_lvl = -1;
+  {
+    const int bfs = who[s];
+    const Loc L = bfs >= 0 ? locate(bfs) : Loc{-1, -1, 0, 0, 0, -1};
+    if (L.tree >= 0) {
+      const int32_t so = A.pl.sec_off[L.tree];
+      nc_t = A.pl.sec_nchild[so + L.ls];
+      // children are consecutive at the next level, so they stay consecutive
+      cf_t = nc_t > 0 ? new_of[A.pl.sec_child[so + L.ls] + L.sec_base] : 0;
+      const int ps = A.pl.sec_parent[so + L.ls];
+      psec_t = ps >= 0 ? new_of[ps + L.sec_base] : -1;
+      my_lvl = L.lvl;
+      // length-major thread of rank s: sections longer than this one, then
+      // sections of equal length and lower rank (ranks without a section
+      // have length 0 and come after every section)
+      const int ln = len_of[s];
+      int pos = 0;
+      for (int k = 0; k < nt; ++k) {
+        const int lk = len_of[k];
+        pos += (lk > ln) || (lk == ln && k < s);
+      }
+      owned[pos] = s;
+    }
+  }
+  __syncthreads();
+  // the section this thread holds in registers (rank q)
+  const int q = owned[s];
+  const Loc L = q >= 0 ? locate(who[q]) : Loc{-1, -1, 0, 0, 0, -1};
   const int my_tree = L.tree, tree_node_base = L.node_base;
   const bool own = my_tree >= 0;
-  int hd = 0, len = 0, cf = 0, nc = 0, psec = -1, my_lvl = -1;
+  int hd = 0, len = 0, cf = 0, nc = 0, psec = -1;
   int64_t no = 0;
   if (own) {
     const int32_t so = A.pl.sec_off[my_tree];
@@ -404,11 +444,9 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
     hd = A.pl.sec_head[so + L.ls];
     len = A.pl.sec_len[so + L.ls];
     nc = A.pl.sec_nchild[so + L.ls];
-    // children are consecutive at the next level, so they stay consecutive
     cf = nc > 0 ? new_of[A.pl.sec_child[so + L.ls] + L.sec_base] : 0;
     const int ps = A.pl.sec_parent[so + L.ls];
     psec = ps >= 0 ? new_of[ps + L.sec_base] : -1;
-    my_lvl = L.lvl;
   }
   __syncthreads();
   // stage the group's axial conductances into the block's node table
@@ -438,7 +476,7 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
     n[r] = in ? A.n[idx] : T(0);
   }
   __syncthreads();
-  if (own) gh[s] = gs[gb];
+  if (own) gh[q] = gs[gb];
   const T dt = A.dt;
   for (int64_t st = 0; st < A.n_steps; ++st) {
     const T w = A.wave ? A.wave[A.step0 + st] : T(0);
@@ -449,7 +487,7 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
 #pragma unroll
     for (int r = 1; r < kSecMax; ++r)
       if (r < len) vtail = v[r];
-    if (own) { vh[s] = v[0]; vt[s] = vtail; }
+    if (own) { vh[q] = v[0]; vt[q] = vtail; }
     T M[kSecMax], F[kSecMax];
 #pragma unroll
     for (int r = 0; r < kSecMax; ++r) {
@@ -515,32 +553,44 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
         e *= ce;
       }
     }
-    // The sections' tails form a tree with the section tree's shape: node s
+    // The sections' tails form a tree with the section tree's shape: node j
     // has self term Mc + sum over child sections c of (dM_c + fac_c D_c),
     // right-hand side Fc + sum of (dF_c + fac_c B_c), edge e to its parent.
+    // The holder of each section hands (Mc, Fc, e, dM, dF) to tail node j =
+    // its rank: tM, tF, xt (e until the tails are solved), cD and cB (dM and
+    // dF until node j adds its Schur term).
+    if (own) {
+      tM[q] = Mc;
+      tF[q] = Fc;
+      xt[q] = e;
+      cD[q] = dM;
+      cB[q] = dF;
+    }
+    __syncthreads();
     // Elimination, deepest level first; one node per section per level.
     T fac = T(0), Binv = T(0);
     for (int l = nlvl_max - 1; l >= 0; --l) {
       if (my_lvl == l) {
-        T D = Mc, B = Fc;
-        for (int c = cf; c < cf + nc; ++c) { D += cD[c]; B += cB[c]; }
-        const T inv = frcp(D + e);
-        fac = e * inv;
+        T D = tM[s], B = tF[s];
+        for (int c = cf_t; c < cf_t + nc_t; ++c) { D += cD[c]; B += cB[c]; }
+        const T et = xt[s];
+        const T inv = frcp(D + et);
+        fac = et * inv;
         Binv = B * inv;
-        cD[s] = dM + fac * D;
-        cB[s] = dF + fac * B;
+        cD[s] += fac * D;
+        cB[s] += fac * B;
       }
       __syncthreads();
     }
     // tails, root level first
     for (int l = 0; l < nlvl_max; ++l) {
-      if (my_lvl == l) xt[s] = Binv + fac * (psec >= 0 ? xt[psec] : T(0));
+      if (my_lvl == l) xt[s] = Binv + fac * (psec_t >= 0 ? xt[psec_t] : T(0));
       __syncthreads();
     }
     // interior nodes of every section at once, tail first
     if (own) {
       const T xp = psec >= 0 ? xt[psec] : T(0);
-      T x = xt[s];
+      T x = xt[q];
 #pragma unroll
       for (int r = kSecMax - 1; r >= 0; --r) {
         if (r < len) {
@@ -570,7 +620,7 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
 
 __host__ __device__ constexpr size_t tree_sec_lds_bytes(int threads, int node_cap,
                                                         size_t elem, int cm_lds = 0) {
-  return ((2 + (cm_lds ? 1 : 0)) * (size_t)node_cap + 6 * (size_t)threads) * elem;
+  return ((2 + (cm_lds ? 1 : 0)) * (size_t)node_cap + 8 * (size_t)threads) * elem;
 }
 // cm/dt joins the LDS node table while two blocks of the section kernel still
 // fit a CU (80 KiB each)
