@@ -567,18 +567,40 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
       cB[q] = dF;
     }
     __syncthreads();
-    // Elimination, deepest level first; one node per section per level.
+    // Elimination, deepest level first; one node per section per level.  A
+    // tail's own terms are read once, before the level loop; at its level it
+    // reads its children's terms (two at a time, so the loads overlap) and
+    // adds them in child order.
     T fac = T(0), Binv = T(0);
+    T D0 = T(0), B0 = T(0), et = T(0), dM0 = T(0), dF0 = T(0);
+    if (my_lvl >= 0) {
+      D0 = tM[s];
+      B0 = tF[s];
+      et = xt[s];
+      dM0 = cD[s];
+      dF0 = cB[s];
+    }
     for (int l = nlvl_max - 1; l >= 0; --l) {
       if (my_lvl == l) {
-        T D = tM[s], B = tF[s];
-        for (int c = cf_t; c < cf_t + nc_t; ++c) { D += cD[c]; B += cB[c]; }
-        const T et = xt[s];
+        T D = D0, B = B0;
+        const int ce = cf_t + nc_t;
+        int c = cf_t;
+        for (; c + 1 < ce; c += 2) {
+          const T d0 = cD[c], d1 = cD[c + 1], b0 = cB[c], b1 = cB[c + 1];
+          D += d0;
+          D += d1;
+          B += b0;
+          B += b1;
+        }
+        if (c < ce) {
+          D += cD[c];
+          B += cB[c];
+        }
         const T inv = frcp(D + et);
         fac = et * inv;
         Binv = B * inv;
-        cD[s] += fac * D;
-        cB[s] += fac * B;
+        cD[s] = dM0 + fac * D;
+        cB[s] = dF0 + fac * B;
       }
       __syncthreads();
     }
