@@ -159,6 +159,9 @@ __device__ __forceinline__ void hh_advance(T v, T dt, const HHParams<T>& p, T sn
 // |z| <= 1/16 (the dropped tail is below |z|^5/720 <= 1.3e-9, under half an
 // ulp of phi1 ~ 1) -- and (e^z - 1)/z beyond, which makes the step the
 // exponential relaxation a/s + (x - a/s) e^z written without a/s.
+// The quotient is evaluated only when some lane of the wavefront has
+// |z| > 0.5, which implies the full series branch; the per-lane selection is
+// unchanged, so results do not depend on either skip.
 __device__ __forceinline__ float gate_step(float x, float a, float s, float dq) {
   const float z = -s * dq;
   float ph;
@@ -169,9 +172,11 @@ __device__ __forceinline__ float gate_step(float x, float a, float s, float dq) 
     ph = fmaf(ph, z, 1.0f);
   } else {
     ph = phi1(z);
+    if (__ballot(fabsf(z) > 0.5f) != 0ull) {
+      const float pf = (fexp(z) - 1.0f) * frcp(z);
+      ph = fabsf(z) > 0.5f ? pf : ph;
+    }
   }
-  const float pf = (fexp(z) - 1.0f) * frcp(z);
-  ph = fabsf(z) > 0.5f ? pf : ph;
   return fmaf(dq * fmaf(-s, x, a), ph, x);
 }
 
