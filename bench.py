@@ -48,7 +48,7 @@ BYTES_PER_CS_TREE = 52.0  # float32 tree compartment-step (+ int32 parent index)
 # compartment-step: SQ_INSTS_VALU of each kernel divided by its
 # compartment-steps, profiles/hh_population_uneven_valu.md.
 VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 4
-VALU_PER_CS = {"cable": 2.379, "tree": 3.560}
+VALU_PER_CS = {"cable": 2.035, "tree": 2.289}
 
 
 def _parse():
