@@ -244,39 +244,89 @@ __device__ __forceinline__ void hh_advance<float>(float v, float dt, const HHPar
 // cross-lane moves inside a segment of W lanes (W a power of two <= 64).
 // Out-of-segment sources return the caller's own value, which the
 // eliminations below rely on (the corresponding couplings are exactly 0).
+// Where one instruction does the move it is used: DPP row shifts inside
+// 16-lane rows (W <= 16, with a select at the segment edge for W < 16), the
+// DPP whole-wavefront shift by one lane (W = 32 with an edge select, W = 64),
+// and the half swaps v_permlane16_swap (stride 16 at W = 32) and
+// v_permlane32_swap (stride 32 at W = 64).  Lanes without a DPP source keep
+// the old operand, which is the caller's own value.  The other strides go
+// through ds_bpermute.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int bperm_from(int src_lane, int x) {
+  return __builtin_amdgcn_ds_bpermute(src_lane << 2, x);
+}
+template <int W>
+__device__ __forceinline__ int seg_up_i(int x, int s) {
+  const int lane = __lane_id();
+  const int seg_lane = lane & (W - 1);
+  if constexpr (W <= 16) {
+    int y;
+    if (s == 1) y = __builtin_amdgcn_update_dpp(x, x, 0x111, 0xF, 0xF, false);
+    else if (s == 2) y = __builtin_amdgcn_update_dpp(x, x, 0x112, 0xF, 0xF, false);
+    else if (s == 4) y = __builtin_amdgcn_update_dpp(x, x, 0x114, 0xF, 0xF, false);
+    else if (s == 8) y = __builtin_amdgcn_update_dpp(x, x, 0x118, 0xF, 0xF, false);
+    else return bperm_from(seg_lane >= s ? lane - s : lane, x);
+    if constexpr (W < 16) y = seg_lane >= s ? y : x;
+    return y;
+  } else {
+    if (s == 1) {
+      const int y = __builtin_amdgcn_update_dpp(x, x, 0x138, 0xF, 0xF, false);
+      if constexpr (W == 32) return seg_lane != 0 ? y : x;
+      return y;
+    }
+    if constexpr (W == 32)
+      if (s == 16) return __builtin_amdgcn_permlane16_swap(x, x, false, false)[0];
+    if constexpr (W == 64)
+      if (s == 32) return __builtin_amdgcn_permlane32_swap(x, x, false, false)[0];
+    return bperm_from(seg_lane >= s ? lane - s : lane, x);
+  }
+}
+template <int W>
+__device__ __forceinline__ int seg_down_i(int x, int s) {
+  const int lane = __lane_id();
+  const int seg_lane = lane & (W - 1);
+  if constexpr (W <= 16) {
+    int y;
+    if (s == 1) y = __builtin_amdgcn_update_dpp(x, x, 0x101, 0xF, 0xF, false);
+    else if (s == 2) y = __builtin_amdgcn_update_dpp(x, x, 0x102, 0xF, 0xF, false);
+    else if (s == 4) y = __builtin_amdgcn_update_dpp(x, x, 0x104, 0xF, 0xF, false);
+    else if (s == 8) y = __builtin_amdgcn_update_dpp(x, x, 0x108, 0xF, 0xF, false);
+    else return bperm_from(seg_lane + s < W ? lane + s : lane, x);
+    if constexpr (W < 16) y = seg_lane + s < W ? y : x;
+    return y;
+  } else {
+    if (s == 1) {
+      const int y = __builtin_amdgcn_update_dpp(x, x, 0x130, 0xF, 0xF, false);
+      if constexpr (W == 32) return seg_lane != W - 1 ? y : x;
+      return y;
+    }
+    if constexpr (W == 32)
+      if (s == 16) return __builtin_amdgcn_permlane16_swap(x, x, false, false)[1];
+    if constexpr (W == 64)
+      if (s == 32) return __builtin_amdgcn_permlane32_swap(x, x, false, false)[1];
+    return bperm_from(seg_lane + s < W ? lane + s : lane, x);
+  }
+}
 template <int W>
 __device__ __forceinline__ float shfl_up_seg(float x, int s) {
-  int lane = __lane_id();
-  int seg_lane = lane & (W - 1);
-  int src = seg_lane >= s ? lane - s : lane;
-  return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(x)));
+  return __int_as_float(seg_up_i<W>(__float_as_int(x), s));
 }
 template <int W>
 __device__ __forceinline__ float shfl_down_seg(float x, int s) {
-  int lane = __lane_id();
-  int seg_lane = lane & (W - 1);
-  int src = seg_lane + s < W ? lane + s : lane;
-  return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(x)));
+  return __int_as_float(seg_down_i<W>(__float_as_int(x), s));
 }
 template <int W>
 __device__ __forceinline__ double shfl_up_seg(double x, int s) {
-  int lane = __lane_id();
-  int seg_lane = lane & (W - 1);
-  int src = (seg_lane >= s ? lane - s : lane) << 2;
   int2 u = *reinterpret_cast<int2*>(&x);
-  u.x = __builtin_amdgcn_ds_bpermute(src, u.x);
-  u.y = __builtin_amdgcn_ds_bpermute(src, u.y);
+  u.x = seg_up_i<W>(u.x, s);
+  u.y = seg_up_i<W>(u.y, s);
   return *reinterpret_cast<double*>(&u);
 }
 template <int W>
 __device__ __forceinline__ double shfl_down_seg(double x, int s) {
-  int lane = __lane_id();
-  int seg_lane = lane & (W - 1);
-  int src = (seg_lane + s < W ? lane + s : lane) << 2;
   int2 u = *reinterpret_cast<int2*>(&x);
-  u.x = __builtin_amdgcn_ds_bpermute(src, u.x);
-  u.y = __builtin_amdgcn_ds_bpermute(src, u.y);
+  u.x = seg_down_i<W>(u.x, s);
+  u.y = seg_down_i<W>(u.y, s);
   return *reinterpret_cast<double*>(&u);
 }
 
