@@ -53,6 +53,7 @@ _SIGNATURES = {
         [i64, p, i64, i32, p, p, p, p, p, p, p, p, p, p, f64, f64, p, p, p, p, p, p, p, p, p],
     ),
     "dn_hh_tree_slots": (i32, []),
+    "dn_hh_tree_sec_max": (i32, []),
     "dn_hh_tree_run_f32": (
         i32,
         _TREE_PLAN

@@ -26,7 +26,10 @@
 namespace dn {
 
 constexpr int kTreeSlots = 8;  // nodes per thread in the persistent kernel
-constexpr int kSecMax = 16;    // nodes per section (the host splits longer chains)
+// nodes per section (the host splits longer chains): 13 keeps the section
+// kernel's register spill small at 128 VGPRs while the bench-size groups
+// still fit two 512-thread blocks per CU in one round
+constexpr int kSecMax = 13;
 
 struct TreePlan {
   int n_trees;
@@ -448,7 +451,11 @@ __global__ __launch_bounds__(512, 4) void hh_tree_sec_kernel(TreeRunArgs<T> A,
     const int ps = A.pl.sec_parent[so + L.ls];
     psec = ps >= 0 ? new_of[ps + L.sec_base] : -1;
   }
-  __syncthreads();
+  // a section longer than the kernel holds: the group is left unadvanced
+  if (__syncthreads_or(own && len > kSecMax)) {
+    if (s == 0) flag_status(A.status, kStatusTreeGroup);
+    return;
+  }
   // stage the group's axial conductances into the block's node table
   node_base = 0;
   for (int k = g0; k < g1; ++k) {
@@ -884,3 +891,4 @@ DN_EXPORT int dn_hh_tree_step_f64(DN_TREE_PLAN_ARGS, const int32_t* node_tree, i
 }
 
 DN_EXPORT int dn_hh_tree_slots(void) { return dn::kTreeSlots; }
+DN_EXPORT int dn_hh_tree_sec_max(void) { return dn::kSecMax; }

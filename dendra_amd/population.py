@@ -40,7 +40,8 @@ ROW_CLASSES = (10, 13)
 
 _STATUS_TEXT = {
     1: "a cable task has more rows per lane than its kernel instance holds",
-    2: "a tree group exceeds its block's threads or node capacity",
+    2: "a tree group exceeds its block's threads or node capacity, or holds a "
+       "section longer than the kernel takes",
     4: "a tree exceeds its launch's node, section or level capacity",
 }
 
@@ -288,6 +289,7 @@ class TreePopulation(_Status):
         self.dtype, self.device = dtype, torch.device(device)
         dev = self.device
         slots = lib.dn_hh_tree_slots()
+        sec_max = lib.dn_hh_tree_sec_max()  # the kernels' section capacity
         ws = tree_workspace(spec, dt)
         perm, parent, ch_ptr, ch_idx = [], [], [0], []
         sec = {k: [] for k in ("head", "len", "child", "nchild", "ptail", "parent")}
@@ -295,7 +297,7 @@ class TreePopulation(_Status):
         n_sec, n_lvl = [], []
         for t in range(spec.n_trees):
             o, e = int(spec.off[t]), int(spec.off[t + 1])
-            sp = section_plan(spec.parent[o:e])
+            sp = section_plan(spec.parent[o:e], sec_max=sec_max)
             perm.append(sp.perm + o)
             parent.append(sp.parent)
             ch_ptr.extend((sp.ch_ptr[1:] + len(ch_idx)).tolist())

@@ -77,7 +77,7 @@ def build_dhs_layers(depth, k_threads: int):
     return np.asarray(order, dtype=np.int32), np.asarray(layer_ptr, dtype=np.int32)
 
 
-SEC_MAX = 16  # longest section the fused tree kernels hold in registers
+SEC_MAX = 13  # longest section the fused tree kernels hold (dn_hh_tree_sec_max)
 
 
 @dataclass

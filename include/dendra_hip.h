@@ -436,6 +436,10 @@ int dn_hh_cable_step_f64(int64_t n_cells, const int64_t* off, int64_t total, int
  * bounds is left unadvanced and reported in *status (nullable device int32,
  * bits as for dn_hh_cable_run_*). */
 int dn_hh_tree_slots(void);
+/* Longest section (unbranched chain) the tree kernels take; the host's
+ * section plan must split longer chains.  Layout-1 runs report longer
+ * sections through *status (bit 2) and leave their group unadvanced. */
+int dn_hh_tree_sec_max(void);
 int dn_hh_tree_run_f32(int n_trees, const int64_t* node_off, const int32_t* parent,
                        const int32_t* ch_ptr, const int32_t* ch_idx,
                        const int32_t* sec_off, const int32_t* sec_head,

@@ -100,3 +100,35 @@ def test_cable_run_reports_too_many_rows_in_status():
     pop.run(10)
     pop.check()
     assert not torch.equal(pop.v, v1)
+
+
+def test_tree_run_reports_sections_longer_than_the_kernel_takes(monkeypatch):
+    """A host plan that splits chains at 16 nodes against a kernel that takes
+    dn_hh_tree_sec_max() (13): the section kernel leaves the group
+    unadvanced, sets bit 2 of the status word, and the population's check
+    raises; with the plan the kernel takes, the same tree advances."""
+    from dendra_amd import _lib, population, topology
+    from dendra_amd.model import HH, TreeSpec
+
+    cap = _lib.load().dn_hh_tree_sec_max()
+    assert cap == topology.SEC_MAX and cap < 16
+    K = 40  # one unbranched chain: sections of 16, 16, 8 under the wider plan
+    spec = TreeSpec(np.array([0, K]), np.arange(-1, K - 1), np.full(K, 2.0),
+                    np.full(K, 10.0), np.full(K, 100.0), np.ones(K), v_init=-60.0)
+    real = population.section_plan
+    monkeypatch.setattr(population, "section_plan",
+                        lambda parent, sec_max=None: real(parent, sec_max=16))
+    pop = population.TreePopulation(spec, 0.005, HH(), dtype=torch.float32)
+    assert int(pop.sec["len"].max()) == 16
+    v0 = pop.v.clone()
+    pop.run(20)
+    torch.cuda.synchronize()
+    assert torch.equal(pop.v, v0)
+    with pytest.raises(RuntimeError, match="section longer"):
+        pop.check()
+    monkeypatch.setattr(population, "section_plan", real)
+    ok = population.TreePopulation(spec, 0.005, HH(), dtype=torch.float32)
+    assert int(ok.sec["len"].max()) <= cap
+    ok.run(20)
+    ok.check()
+    assert not torch.equal(ok.v, v0)
