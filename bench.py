@@ -48,7 +48,7 @@ BYTES_PER_CS_TREE = 52.0  # float32 tree compartment-step (+ int32 parent index)
 # compartment-step: SQ_INSTS_VALU of each kernel divided by its
 # compartment-steps, profiles/hh_population_uneven_valu.md.
 VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 4
-VALU_PER_CS = {"cable": 2.035, "tree": 2.289}
+VALU_PER_CS = {"cable": 2.099, "tree": 2.304}
 
 
 def _parse():
@@ -233,7 +233,7 @@ def population_bench(args, dev, hh):
     out["roofline"] = {
         "bound": "valu", "achieved": valu / wall, "peak": VALU_PEAK_PER_S,
         "unit": "wave64 VALU instructions/s", "frac": valu / wall / VALU_PEAK_PER_S,
-        "traffic": _measured_traffic("hh_cable_run_kernel"),
+        "traffic": _job_traffic(use_c, use_t),
         "kernel": "hh_cable_run_kernel<float> + hh_tree_sec_kernel<float> (concurrent)",
         "valu_instructions_per_run": valu,
         "valu_per_compartment_step": VALU_PER_CS,
@@ -264,6 +264,7 @@ def population_bench(args, dev, hh):
         out["tree_cs_per_s"] = work.trees.total * work.n_steps / t_avg
         t_bytes = BYTES_PER_CS_TREE * work.trees.total * work.n_steps
         out["tree_roofline"] = {"achieved": t_bytes / t_avg / 1e9, "unit": "GB/s",
+                                "traffic": _measured_traffic("hh_tree_sec_kernel"),
                                 "frac": t_bytes / t_avg / 1e9 / HBM_PEAK_GBS,
                                 "kernel": "hh_tree_sec_kernel<float>",
                                 "note": "per-step algorithmic bytes, secondary"}
@@ -837,6 +838,13 @@ def _measured_traffic(kernel):
         return float(json.loads(path.read_text())["traffic_bytes_per_launch"][kernel])
     except (OSError, KeyError, ValueError):
         return None
+
+
+def _job_traffic(use_c, use_t):
+    """HBM bytes of one run's persistent launches (the kernels that run)."""
+    parts = ([_measured_traffic("hh_cable_run_kernel")] if use_c else []) + \
+            ([_measured_traffic("hh_tree_sec_kernel")] if use_t else [])
+    return None if any(p is None for p in parts) else float(sum(parts))
 
 
 def cpu_baseline(work, hh, frac, n_steps):
