@@ -229,8 +229,28 @@ __device__ void fit_forward_wave(const FitArgs<T>& A, int task) {
   const T dt = A.dt, dts = dt * T(1e-3);
   int kc = -1;
   T lmse = T(0), lmar = T(0);
+  // per-step inputs are fetched one step ahead, so that their global-memory
+  // latency overlaps the step before the one that uses them
+  T tg[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    tg[r] = (r < nr && site[r] >= 0) ? A.target[A.step0 * A.tstride + site[r]] : T(0);
+  int kn = A.n_steps > 0 ? clamp_knot(A.knot_k[A.step0], A.n_knots) : 0;
+  T aln = A.n_steps > 0 ? A.knot_a[A.step0] : T(0);
   for (int64_t s = 0; s < A.n_steps; ++s) {
     const int64_t ns = A.step0 + s;
+    const int k = kn;
+    const T al = aln;
+    const bool more = s + 1 < A.n_steps;
+    T tgn[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      tgn[r] = (more && r < nr && site[r] >= 0) ? A.target[(ns + 1) * A.tstride + site[r]]
+                                                 : T(0);
+    if (more) {
+      kn = clamp_knot(A.knot_k[ns + 1], A.n_knots);
+      aln = A.knot_a[ns + 1];
+    }
     if (A.traj) {
       T* tr = A.traj + s * 4 * S;
 #pragma unroll
@@ -249,8 +269,6 @@ __device__ void fit_forward_wave(const FitArgs<T>& A, int task) {
           ck[i] = v[r]; ck[S + i] = m[r]; ck[2 * S + i] = h[r]; ck[3 * S + i] = n[r];
         }
     }
-    const int k = clamp_knot(A.knot_k[ns], A.n_knots);
-    const T al = A.knot_a[ns];
     if (k != kc) {
 #pragma unroll
       for (int r = 0; r < R; ++r)
@@ -289,11 +307,13 @@ __device__ void fit_forward_wave(const FitArgs<T>& A, int task) {
       if (r < nr) {
         v[r] += d[r];
         if (site[r] >= 0) {
-          const T df = v[r] - A.target[ns * A.tstride + site[r]];
+          const T df = v[r] - tg[r];
           lmse += df * df;
         }
         if (mw[r] != T(0)) lmar += mw[r] * sigm((v[r] - A.theta) * A.inv_tau);
       }
+#pragma unroll
+    for (int r = 0; r < R; ++r) tg[r] = tgn[r];
   }
   const int64_t nend = A.step0 + A.n_steps;
   if (A.traj) {
@@ -387,10 +407,54 @@ __device__ void fit_adjoint_wave(const FitArgs<T>& A, int task) {
   }
   const T dt = A.dt, dts = dt * T(1e-3);
   int kc = clamp_knot(A.knot_k[A.step0 + A.n_steps - 1], A.n_knots);
+  // the inputs of step s -- knots, targets and the trajectory row (pre-step
+  // state) -- are fetched during step s + 1, so that their global-memory
+  // latency overlaps the step before the one that uses them
+  int kn = kc;
+  T aln = A.knot_a[A.step0 + A.n_steps - 1];
+  T tg[R], q0[R], q1[R], q2[R], q3[R];
+  {
+    const int64_t ns = A.step0 + A.n_steps - 1;
+    const T* tr = A.traj + (A.n_steps - 1) * 4 * S;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int64_t i = base + r * 64;
+      const bool in = r < nr;
+      tg[r] = (in && site[r] >= 0) ? A.target[ns * A.tstride + site[r]] : T(0);
+      q0[r] = in ? tr[i] : T(0);
+      q1[r] = in ? tr[S + i] : T(0);
+      q2[r] = in ? tr[2 * S + i] : T(0);
+      q3[r] = in ? tr[3 * S + i] : T(0);
+    }
+  }
   for (int64_t s = A.n_steps - 1; s >= 0; --s) {
     const int64_t ns = A.step0 + s;
-    const int k = clamp_knot(A.knot_k[ns], A.n_knots);
-    const T al = A.knot_a[ns];
+    const int k = kn;
+    const T al = aln;
+    T tgc[R], v0r[R], m0r[R], h0r[R], n0r[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      tgc[r] = tg[r];
+      v0r[r] = q0[r];
+      m0r[r] = q1[r];
+      h0r[r] = q2[r];
+      n0r[r] = q3[r];
+    }
+    if (s > 0) {
+      kn = clamp_knot(A.knot_k[ns - 1], A.n_knots);
+      aln = A.knot_a[ns - 1];
+      const T* tr = A.traj + (s - 1) * 4 * S;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int64_t i = base + r * 64;
+        const bool in = r < nr;
+        tg[r] = (in && site[r] >= 0) ? A.target[(ns - 1) * A.tstride + site[r]] : T(0);
+        q0[r] = in ? tr[i] : T(0);
+        q1[r] = in ? tr[S + i] : T(0);
+        q2[r] = in ? tr[2 * S + i] : T(0);
+        q3[r] = in ? tr[3 * S + i] : T(0);
+      }
+    }
     if (k != kc) {
 #pragma unroll
       for (int r = 0; r < R; ++r)
@@ -413,8 +477,7 @@ __device__ void fit_adjoint_wave(const FitArgs<T>& A, int task) {
       a[r] = T(0); b[r] = T(1); c[r] = T(0); d[r] = T(0);
       if (r < nr) {
         T lt = lam[r];
-        if (site[r] >= 0)
-          lt += T(2) * A.mse_scale * (vp[r] - A.target[ns * A.tstride + site[r]]);
+        if (site[r] >= 0) lt += T(2) * A.mse_scale * (vp[r] - tgc[r]);
         if (mw[r] != T(0)) {
           const T sg = sigm((vp[r] - A.theta) * A.inv_tau);
           lt += mw[r] * sg * (T(1) - sg) * A.inv_tau;
@@ -429,12 +492,10 @@ __device__ void fit_adjoint_wave(const FitArgs<T>& A, int task) {
       }
     }
     seg_solve<T, P, R>(nr, a, b, c, d);
-    const T* tr = A.traj + s * 4 * S;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (r < nr) {
-        const int64_t i = base + r * 64;
-        const T v0 = tr[i], m0 = tr[S + i], h0 = tr[2 * S + i], n0 = tr[3 * S + i];
+        const T v0 = v0r[r], m0 = m0r[r], h0 = h0r[r], n0 = n0r[r];
         const T y = d[r];
         const T w = dts * sc[r] * y;
         const T tna = w * (hp.ena - vp[r]);
