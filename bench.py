@@ -49,6 +49,10 @@ BYTES_PER_CS_TREE = 52.0  # float32 tree compartment-step (+ int32 parent index)
 # compartment-steps, profiles/hh_population_uneven_valu.md.
 VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 4
 VALU_PER_CS = {"cable": 2.099, "tree": 2.304}
+# The trace-fitting kernels (four wavefronts per cell, one row per lane):
+# SQ_INSTS_VALU per compartment-step of the forward (also the recomputation)
+# and of the adjoint, profiles/fit_adjoint_512x201_kernels.md.
+FIT_VALU_PER_CS = {"forward": 6.74, "adjoint": 10.22}
 
 
 def _parse():
@@ -355,7 +359,11 @@ def fit_bench(dev, with_cpu, reps=3):
     """Secondary line: loss and hand-written adjoint of 512 HH cables x 201
     compartments, 10,000 steps of 0.01 ms in float32, gradients w.r.t. the
     per-compartment gNa/gK scales and the 201 x 64 Ve knot table, state
-    checkpoints every 500 steps.  Algorithmic bytes: 56 per compartment-step
+    checkpoints every 500 steps.  Each cell runs on a workgroup of four
+    wavefronts, one compartment row per lane (2048 wavefronts).  Roofline:
+    VALU issue (the kernels keep their state on chip), from the VALU
+    instructions per compartment-step of the forward, the recomputation and
+    the adjoint; secondary: algorithmic bytes, 56 per compartment-step
     forward, 144 per compartment-step forward + adjoint."""
     import torch
 
@@ -367,7 +375,7 @@ def fit_bench(dev, with_cpu, reps=3):
     fit = CableFit(w.spec, w.dt, w.n_steps, w.sites, w.targets, knot_times=w.knot_times,
                    margin_site=w.margin_site, margin_sign=w.margin_sign,
                    margin_weight=w.margin_weight, theta=w.theta, tau=w.tau,
-                   hh=HH(celsius=6.3), dtype=torch.float32, device=dev)
+                   hh=HH(celsius=6.3), dtype=torch.float32, device=dev, rmax=1, waves=4)
     p = [torch.tensor(np.asarray(x).reshape(-1), dtype=torch.float32, device=dev)
          for x in (w.sna, w.sk)] + [torch.tensor(w.ve, dtype=torch.float32, device=dev)]
     fit.forward(*p)
@@ -392,13 +400,25 @@ def fit_bench(dev, with_cpu, reps=3):
                     "compartment gNa/gK scales, 201x64 Ve knot table, MSE at 8 sites + "
                     "sigmoid margin, checkpoints every 500 steps",
         "dtype": "f32", "loss": loss.item(),
+        "layout": "one cell per workgroup of 4 wavefronts, one row per lane (2048 wavefronts)",
         "forward": {"ms": tf, "compartment_steps_per_s": cs / (tf * 1e-3)},
         "forward_adjoint": {"ms": both, "compartment_steps_per_s": cs / (both * 1e-3)},
-        "roofline": {"bound": "hbm", "bytes_per_cs": 144,
-                     "achieved": 144 * cs / (both * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s",
-                     "frac": 144 * cs / (both * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
+    # forward + recomputation + adjoint VALU instructions over the evaluation
+    valu = (2 * FIT_VALU_PER_CS["forward"] + FIT_VALU_PER_CS["adjoint"]) * cs
+    res["roofline"] = {
+        "bound": "valu", "achieved": valu / (both * 1e-3), "peak": VALU_PEAK_PER_S,
+        "unit": "wave64 VALU instructions/s", "frac": valu / (both * 1e-3) / VALU_PEAK_PER_S,
+        "valu_per_compartment_step": FIT_VALU_PER_CS,
+        "forward_frac": FIT_VALU_PER_CS["forward"] * cs / (tf * 1e-3) / VALU_PEAK_PER_S,
+        "note": "instructions per compartment-step from rocprofv3 SQ_INSTS_VALU "
+                "(profiles/fit_adjoint_512x201_kernels.md); forward + adjoint = forward, "
+                "recomputation and adjoint sweeps"}
+    res["hbm_roofline"] = {
+        "bound": "hbm", "bytes_per_cs": 144, "achieved": 144 * cs / (both * 1e-3) / 1e9,
+        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": 144 * cs / (both * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "note": "per-step algorithmic bytes; a secondary figure, the state stays on chip"}
     if with_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
         import fit_ref

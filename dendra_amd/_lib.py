@@ -90,11 +90,12 @@ _SIGNATURES["dn_net_deliver_tape_f32"] = (
     i32, [i32, i32] + [p] * 6 + [i32] + [p] * 8 + [i64, p],
 )
 
-# n_tasks, rcap, task_nr, task_p, gl, grp, scale, sna, sk, act, n_knots, knot_k,
-# knot_a, site, mw, target, tstride, mse_scale, theta, tau, step0, n_steps, dt, hh
+# n_tasks, rcap, waves, task_nr, task_p, gl, grp, scale, sna, sk, act, n_knots,
+# knot_k, knot_a, site, mw, target, tstride, mse_scale, theta, tau, step0, n_steps,
+# dt, hh
 def _fit_common(f):
-    return [i32, i32, p, p, p, p, p, p, p, p, i32, p, p, p, p, p, i64, f, f, f, i64, i64,
-            f, p]
+    return [i32, i32, i32, p, p, p, p, p, p, p, p, i32, p, p, p, p, p, i64, f, f, f, i64,
+            i64, f, p]
 
 
 _SIGNATURES.update({

@@ -30,8 +30,11 @@
 // starts from y.  Every derivative is of the arithmetic the forward runs
 // (the Taylor polynomial of phi1, the Bernoulli polynomial of vtrap).
 //
-// Layout ("slots"): a task (wavefront) owns 64 * R slots; slot
-// task*64*R + r*64 + lane holds row r of that lane.  Every per-row array --
+// Layout ("slots"): a task (one workgroup of NW wavefronts, W = 64 NW
+// threads) owns W * R slots; slot task*W*R + r*W + tid holds row r of
+// thread tid.  With NW = 1 a wavefront holds 64 / P cells of P lanes; with
+// NW = 2 or 4 (R = 2 or 1) one cell spans the W threads, so that a
+// population of few cells still fills every SIMD.  Every per-row array --
 // coefficients, state [4][slots], trajectory [(steps+1)][4][slots],
 // activating knots [Q][slots] and their gradients -- uses it, so each row
 // access of a wavefront is one coalesced 256-byte (512-byte) transaction.
@@ -201,17 +204,18 @@ __device__ __forceinline__ int clamp_knot(int k, int n_knots) {
 // ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
-template <typename T, int P, int R>
-__device__ void fit_forward_wave(const FitArgs<T>& A, int task) {
+template <typename T, int P, int R, int NW>
+__device__ __forceinline__ void fit_forward_wave(const FitArgs<T>& A, int task, T* xs) {
+  constexpr int W = 64 * NW;  // threads of the task
   const int lane = __lane_id();
   const int nr = A.task_nr[task];
   const int64_t S = A.slots;
-  const int64_t base = (int64_t)task * 64 * R + lane;
+  const int64_t base = (int64_t)task * W * R + threadIdx.x;
   T v[R], m[R], h[R], n[R], gl[R], grp[R], sc[R], sna[R], sk[R], e0[R], e1[R], mw[R];
   int site[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int64_t i = base + r * 64;
+    const int64_t i = base + r * W;
     const bool in = r < nr;
     v[r] = in ? A.x0[i] : T(0);
     m[r] = in ? A.x0[S + i] : T(0);
@@ -256,7 +260,7 @@ __device__ void fit_forward_wave(const FitArgs<T>& A, int task) {
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (r < nr) {
-          const int64_t i = base + r * 64;
+          const int64_t i = base + r * W;
           tr[i] = v[r]; tr[S + i] = m[r]; tr[2 * S + i] = h[r]; tr[3 * S + i] = n[r];
         }
     }
@@ -265,7 +269,7 @@ __device__ void fit_forward_wave(const FitArgs<T>& A, int task) {
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (r < nr) {
-          const int64_t i = base + r * 64;
+          const int64_t i = base + r * W;
           ck[i] = v[r]; ck[S + i] = m[r]; ck[2 * S + i] = h[r]; ck[3 * S + i] = n[r];
         }
     }
@@ -273,8 +277,8 @@ __device__ void fit_forward_wave(const FitArgs<T>& A, int task) {
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (r < nr) {
-          e0[r] = A.act[(int64_t)k * S + base + r * 64];
-          e1[r] = A.act[(int64_t)(k + 1) * S + base + r * 64];
+          e0[r] = A.act[(int64_t)k * S + base + r * W];
+          e1[r] = A.act[(int64_t)(k + 1) * S + base + r * W];
         }
       kc = k;
     }
@@ -282,8 +286,13 @@ __device__ void fit_forward_wave(const FitArgs<T>& A, int task) {
 #pragma unroll
     for (int r = 1; r < R; ++r)
       if (r < nr) vlast = v[r];
-    const T vprev = shfl_up_seg<P>(vlast, 1);
-    const T vnext = shfl_down_seg<P>(v[0], 1);
+    T vprev, vnext;
+    if constexpr (NW == 1) {
+      vprev = shfl_up_seg<P>(vlast, 1);
+      vnext = shfl_down_seg<P>(v[0], 1);
+    } else {
+      blk_neighbours<T, NW>(vlast, v[0], xs + 24 * NW, vprev, vnext);
+    }
     T a[R], b[R], c[R], d[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -301,7 +310,8 @@ __device__ void fit_forward_wave(const FitArgs<T>& A, int task) {
         d[r] = dts * f;
       }
     }
-    seg_solve<T, P, R>(nr, a, b, c, d);
+    if constexpr (NW == 1) seg_solve<T, P, R>(nr, a, b, c, d);
+    else blk_solve<T, R, NW>(nr, a, b, c, d, xs + (s & 1) * 12 * NW);
 #pragma unroll
     for (int r = 0; r < R; ++r)
       if (r < nr) {
@@ -321,7 +331,7 @@ __device__ void fit_forward_wave(const FitArgs<T>& A, int task) {
 #pragma unroll
     for (int r = 0; r < R; ++r)
       if (r < nr) {
-        const int64_t i = base + r * 64;
+        const int64_t i = base + r * W;
         tr[i] = v[r]; tr[S + i] = m[r]; tr[2 * S + i] = h[r]; tr[3 * S + i] = n[r];
       }
   }
@@ -330,7 +340,7 @@ __device__ void fit_forward_wave(const FitArgs<T>& A, int task) {
 #pragma unroll
     for (int r = 0; r < R; ++r)
       if (r < nr) {
-        const int64_t i = base + r * 64;
+        const int64_t i = base + r * W;
         ck[i] = v[r]; ck[S + i] = m[r]; ck[2 * S + i] = h[r]; ck[3 * S + i] = n[r];
       }
   }
@@ -338,7 +348,7 @@ __device__ void fit_forward_wave(const FitArgs<T>& A, int task) {
 #pragma unroll
     for (int r = 0; r < R; ++r)
       if (r < nr) {
-        const int64_t i = base + r * 64;
+        const int64_t i = base + r * W;
         A.xout[i] = v[r]; A.xout[S + i] = m[r]; A.xout[2 * S + i] = h[r];
         A.xout[3 * S + i] = n[r];
       }
@@ -347,8 +357,9 @@ __device__ void fit_forward_wave(const FitArgs<T>& A, int task) {
     lmse = wave_sum(lmse);
     lmar = wave_sum(lmar);
     if (lane == 0) {
-      A.loss[2 * task] = lmse * A.mse_scale;
-      A.loss[2 * task + 1] = lmar;
+      const int64_t o = 2 * ((int64_t)task * NW + (threadIdx.x >> 6));
+      A.loss[o] = lmse * A.mse_scale;
+      A.loss[o + 1] = lmar;
     }
   }
 }
@@ -356,13 +367,17 @@ __device__ void fit_forward_wave(const FitArgs<T>& A, int task) {
 // ---------------------------------------------------------------------------
 // adjoint over one segment [step0, step0 + n_steps), trajectory from traj
 // ---------------------------------------------------------------------------
-template <typename T, int P, int R>
-__device__ void fit_adjoint_wave(const FitArgs<T>& A, int task) {
+template <typename T, int P, int R, int NW>
+__device__ __forceinline__ void fit_adjoint_wave(const FitArgs<T>& A, int task, T* xs) {
+  constexpr int W = 64 * NW;  // threads of the task
   const int lane = __lane_id();
-  const int pl = lane & (P - 1);
+  // position in the cell's lane segment (P lanes in a wavefront, or all W
+  // threads of the workgroup)
+  const int pl = NW == 1 ? (lane & (P - 1)) : (int)threadIdx.x;
+  constexpr int PL = NW == 1 ? P : W;
   const int nr = A.task_nr[task];
   const int64_t S = A.slots;
-  const int64_t base = (int64_t)task * 64 * R + lane;
+  const int64_t base = (int64_t)task * W * R + threadIdx.x;
   const HHParams<T> hp = A.hh;
   T gl[R], grp[R], sc[R], sna[R], sk[R], mw[R], glm[R], grp1[R];
   int site[R];
@@ -372,7 +387,7 @@ __device__ void fit_adjoint_wave(const FitArgs<T>& A, int task) {
   const T* tre = A.traj + A.n_steps * 4 * S;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int64_t i = base + r * 64;
+    const int64_t i = base + r * W;
     const bool in = r < nr;
     gl[r] = in ? A.gl[i] : T(0);
     grp[r] = in ? A.grp[i] : T(0);
@@ -397,12 +412,17 @@ __device__ void fit_adjoint_wave(const FitArgs<T>& A, int task) {
 #pragma unroll
     for (int r = 1; r < R; ++r)
       if (r < nr) gl_last = gl[r];
-    const T glp = shfl_up_seg<P>(gl_last, 1);
-    const T grn = shfl_down_seg<P>(grp[0], 1);
+    T glp, grn;
+    if constexpr (NW == 1) {
+      glp = shfl_up_seg<P>(gl_last, 1);
+      grn = shfl_down_seg<P>(grp[0], 1);
+    } else {
+      blk_neighbours<T, NW>(gl_last, grp[0], xs + 24 * NW, glp, grn);
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       glm[r] = r == 0 ? (pl == 0 ? T(0) : glp) : gl[r > 0 ? r - 1 : 0];
-      grp1[r] = (r + 1 < nr) ? grp[r + 1 < R ? r + 1 : r] : (pl == P - 1 ? T(0) : grn);
+      grp1[r] = (r + 1 < nr) ? grp[r + 1 < R ? r + 1 : r] : (pl == PL - 1 ? T(0) : grn);
     }
   }
   const T dt = A.dt, dts = dt * T(1e-3);
@@ -418,7 +438,7 @@ __device__ void fit_adjoint_wave(const FitArgs<T>& A, int task) {
     const T* tr = A.traj + (A.n_steps - 1) * 4 * S;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const int64_t i = base + r * 64;
+      const int64_t i = base + r * W;
       const bool in = r < nr;
       tg[r] = (in && site[r] >= 0) ? A.target[ns * A.tstride + site[r]] : T(0);
       q0[r] = in ? tr[i] : T(0);
@@ -446,7 +466,7 @@ __device__ void fit_adjoint_wave(const FitArgs<T>& A, int task) {
       const T* tr = A.traj + (s - 1) * 4 * S;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int64_t i = base + r * 64;
+        const int64_t i = base + r * W;
         const bool in = r < nr;
         tg[r] = (in && site[r] >= 0) ? A.target[(ns - 1) * A.tstride + site[r]] : T(0);
         q0[r] = in ? tr[i] : T(0);
@@ -459,7 +479,7 @@ __device__ void fit_adjoint_wave(const FitArgs<T>& A, int task) {
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (r < nr) {
-          const int64_t i = base + r * 64;
+          const int64_t i = base + r * W;
           A.g_act[(int64_t)(kc + 1) * S + i] += acc1[r];
           if (k == kc - 1) {
             acc1[r] = acc0[r];
@@ -491,7 +511,8 @@ __device__ void fit_adjoint_wave(const FitArgs<T>& A, int task) {
         d[r] = lt;
       }
     }
-    seg_solve<T, P, R>(nr, a, b, c, d);
+    if constexpr (NW == 1) seg_solve<T, P, R>(nr, a, b, c, d);
+    else blk_solve<T, R, NW>(nr, a, b, c, d, xs + (s & 1) * 12 * NW);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (r < nr) {
@@ -526,7 +547,7 @@ __device__ void fit_adjoint_wave(const FitArgs<T>& A, int task) {
 #pragma unroll
   for (int r = 0; r < R; ++r)
     if (r < nr) {
-      const int64_t i = base + r * 64;
+      const int64_t i = base + r * W;
       A.adj[i] = lam[r];
       A.adj[S + i] = mum[r];
       A.adj[2 * S + i] = muh[r];
@@ -538,15 +559,25 @@ __device__ void fit_adjoint_wave(const FitArgs<T>& A, int task) {
     }
 }
 
-template <typename T, int R, bool ADJ>
-__global__ __launch_bounds__(64) void fit_kernel(FitArgs<T> A) {
+// One task per workgroup: NW = 1, a wavefront of cells of task_p lanes each;
+// NW > 1, one cell over the 64 NW threads.
+template <typename T, int R, bool ADJ, int NW>
+__global__ __launch_bounds__(64 * NW) void fit_kernel(FitArgs<T> A) {
+  // cross-wavefront exchange areas (NW > 1): two alternating solve areas of
+  // 12 NW values and one neighbour area of 2 NW
+  __shared__ T xs[NW > 1 ? 26 * NW : 1];
   const int task = blockIdx.x;
   if (task >= A.n_tasks) return;
+  if constexpr (NW > 1) {
+    if constexpr (ADJ) fit_adjoint_wave<T, 64 * NW, R, NW>(A, task, xs);
+    else fit_forward_wave<T, 64 * NW, R, NW>(A, task, xs);
+    return;
+  }
   switch (A.task_p[task]) {
 #define DN_FIT_CASE(PP)                                             \
   case PP:                                                          \
-    if constexpr (ADJ) fit_adjoint_wave<T, PP, R>(A, task);         \
-    else fit_forward_wave<T, PP, R>(A, task);                       \
+    if constexpr (ADJ) fit_adjoint_wave<T, PP, R, 1>(A, task, xs);  \
+    else fit_forward_wave<T, PP, R, 1>(A, task, xs);                \
     break;
     DN_FIT_CASE(4)
     DN_FIT_CASE(8)
@@ -578,12 +609,15 @@ __global__ __launch_bounds__(256) void csr_knots_kernel(int64_t n_out, int n_k,
 }
 
 template <typename T>
-static int fit_launch(const FitArgs<T>& A, int rcap, bool adj, hipStream_t stream) {
+static int fit_launch(const FitArgs<T>& A, int rcap, int waves, bool adj,
+                      hipStream_t stream) {
   if (A.n_tasks < 0 || !A.task_nr || !A.task_p || !A.gl || !A.grp || !A.sc || !A.sna ||
       !A.sk || !A.act || !A.knot_k || !A.knot_a || !A.site || !A.mw || !A.target)
     return DN_EINVAL;
-  if (rcap != 4 && rcap != 8) return DN_EINVAL;
-  if (A.slots != (int64_t)A.n_tasks * 64 * rcap || A.n_knots < 2 || A.tstride < 0 ||
+  if (!((waves == 1 && (rcap == 4 || rcap == 8)) || (waves == 2 && rcap == 2) ||
+        (waves == 4 && rcap == 1)))
+    return DN_EINVAL;
+  if (A.slots != (int64_t)A.n_tasks * 64 * waves * rcap || A.n_knots < 2 || A.tstride < 0 ||
       A.step0 < 0 || A.n_steps < 0)
     return DN_EINVAL;
   if (adj) {
@@ -592,14 +626,17 @@ static int fit_launch(const FitArgs<T>& A, int rcap, bool adj, hipStream_t strea
     if (!A.x0 || (A.ckpt && A.ckpt_every < 1)) return DN_EINVAL;
   }
   if (A.n_tasks == 0 || A.n_steps == 0) return DN_OK;
-  const dim3 grid(A.n_tasks), block(64);
-  if (rcap == 4) {
-    if (adj) hipLaunchKernelGGL((fit_kernel<T, 4, true>), grid, block, 0, stream, A);
-    else hipLaunchKernelGGL((fit_kernel<T, 4, false>), grid, block, 0, stream, A);
-  } else {
-    if (adj) hipLaunchKernelGGL((fit_kernel<T, 8, true>), grid, block, 0, stream, A);
-    else hipLaunchKernelGGL((fit_kernel<T, 8, false>), grid, block, 0, stream, A);
+  const dim3 grid(A.n_tasks), block(64 * waves);
+#define DN_FIT_GO(RR, NW)                                                                \
+  {                                                                                     \
+    if (adj) hipLaunchKernelGGL((fit_kernel<T, RR, true, NW>), grid, block, 0, stream, A); \
+    else hipLaunchKernelGGL((fit_kernel<T, RR, false, NW>), grid, block, 0, stream, A);    \
   }
+  if (waves == 4) DN_FIT_GO(1, 4)
+  else if (waves == 2) DN_FIT_GO(2, 2)
+  else if (rcap == 4) DN_FIT_GO(4, 1)
+  else DN_FIT_GO(8, 1)
+#undef DN_FIT_GO
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
 
@@ -621,7 +658,7 @@ static int csr_knots_launch(int64_t n_out, int n_k, const int64_t* ptr, const in
 }  // namespace dn
 
 #define DN_FIT_COMMON(T)                                                              \
-  int n_tasks, int rcap, const int32_t *task_nr, const int32_t *task_p,              \
+  int n_tasks, int rcap, int waves, const int32_t *task_nr, const int32_t *task_p,   \
       const T *gl, const T *grp, const T *scale, const T *sna, const T *sk,          \
       const T *act, int n_knots, const int32_t *knot_k, const T *knot_a,             \
       const int32_t *site, const T *mw, const T *target, int64_t tstride,            \
@@ -631,7 +668,7 @@ static int csr_knots_launch(int64_t n_out, int n_k, const int64_t* ptr, const in
   dn::FitArgs<T> A = {};                                                              \
   if (!hh || !(tau > T(0))) return DN_EINVAL;                                         \
   A.n_tasks = n_tasks; A.task_nr = task_nr; A.task_p = task_p;                        \
-  A.slots = (int64_t)n_tasks * 64 * rcap;                                             \
+  A.slots = (int64_t)n_tasks * 64 * waves * rcap;                                     \
   A.gl = gl; A.grp = grp; A.sc = scale; A.sna = sna; A.sk = sk;                       \
   A.act = act; A.n_knots = n_knots; A.knot_k = knot_k; A.knot_a = knot_a;             \
   A.site = site; A.mw = mw; A.target = target; A.tstride = tstride;                   \
@@ -645,7 +682,7 @@ static int csr_knots_launch(int64_t n_out, int n_k, const int64_t* ptr, const in
     DN_FIT_FILL(T)                                                                    \
     A.x0 = x0; A.xout = xout; A.ckpt = ckpt; A.ckpt_every = ckpt_every;               \
     A.traj = traj; A.loss = loss;                                                     \
-    return dn::fit_launch<T>(A, rcap, false, stream);                                 \
+    return dn::fit_launch<T>(A, rcap, waves, false, stream);                                 \
   }
 
 #define DN_FIT_ADJ(T, NAME)                                                           \
@@ -654,7 +691,7 @@ static int csr_knots_launch(int64_t n_out, int n_k, const int64_t* ptr, const in
     DN_FIT_FILL(T)                                                                    \
     A.traj = const_cast<T *>(traj); A.adj = adj;                                      \
     A.g_sna = g_sna; A.g_sk = g_sk; A.g_act = g_act;                                  \
-    return dn::fit_launch<T>(A, rcap, true, stream);                                  \
+    return dn::fit_launch<T>(A, rcap, waves, true, stream);                                  \
   }
 
 DN_FIT_FWD(float, dn_fit_forward_f32)

@@ -33,7 +33,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .model import HH, CableSpec, cable_workspace, hh_steady_state, plan_cables, slot_map
+from .model import (HH, CableSpec, cable_workspace, hh_steady_state, plan_cables,
+                    plan_cells_per_task, slot_map)
 
 _NP = {torch.float32: np.float32, torch.float64: np.float64}
 
@@ -58,6 +59,10 @@ class CableFit:
     targets: (n_steps, n_cells, S) target voltages after each step.
     ve (at call time): (K, Q) shared by all cells (``shared_ve``, equal K)
     or (total, Q) per compartment in spec order; knot_times (Q,) in ms.
+    (waves, rmax): (1, 4) or (1, 8) packs 64/P cells of P lanes into a
+    wavefront with up to rmax rows per lane; (2, 2) or (4, 1) spreads each
+    cell over a workgroup of that many wavefronts, so that a few hundred
+    cells still fill every SIMD.
     """
 
     def __init__(self, spec: CableSpec, dt: float, n_steps: int, sites, targets, *,
@@ -65,12 +70,12 @@ class CableFit:
                  margin_sign=None, margin_weight: float = 0.0, theta: float = 0.0,
                  tau: float = 1.0, ckpt_every: int = 500, hh: HH | None = None,
                  shared_ve: bool = True, dtype=torch.float32, device="cuda",
-                 rmax: int = 4):
+                 rmax: int = 4, waves: int = 1):
         lib = _lib.load()
         if spec.lead is not None or spec.iamp is not None:
             raise ValueError("CableFit drives cables through the Ve table only")
-        if rmax not in (4, 8):
-            raise ValueError("rmax must be 4 or 8")
+        if (waves, rmax) not in ((1, 4), (1, 8), (2, 2), (4, 1)):
+            raise ValueError("(waves, rmax) must be (1, 4), (1, 8), (2, 2) or (4, 1)")
         if n_steps < 1 or ckpt_every < 1:
             raise ValueError("n_steps and ckpt_every must be positive")
         if not tau > 0:
@@ -80,21 +85,25 @@ class CableFit:
         self.hh = hh or HH()
         self.dtype, self.device = dtype, torch.device(device)
         _lib.require_gpu(torch.empty(0, device=self.device))
-        self.rmax, self.ckpt_every = rmax, int(ckpt_every)
+        self.rmax, self.waves, self.ckpt_every = rmax, int(waves), int(ckpt_every)
         self.theta, self.tau = float(theta), float(tau)
         sizes = spec.sizes
         B = spec.n_cells
         self.shared_ve = bool(shared_ve)
         if self.shared_ve and len(set(sizes.tolist())) != 1:
             raise ValueError("a shared Ve table needs equal compartment counts")
-        plan = plan_cables(sizes, rmax, 64)
+        width = 64 * self.waves
+        # one wavefront per task holds 64 / P cells; with several wavefronts
+        # per task each cell spans all of them
+        plan = (plan_cables(sizes, rmax, 64) if self.waves == 1
+                else plan_cells_per_task(sizes, rmax, width))
         self.plan = plan
         nt = len(plan.task_p)
         R = rmax
-        self.n_tasks, self.slots = nt, nt * 64 * R
+        self.n_tasks, self.slots = nt, nt * width * R
 
         # slot of every compartment (spec order)
-        comp_slot = slot_map(plan, sizes, spec.off, R)
+        comp_slot = slot_map(plan, sizes, spec.off, R, width)
         self.comp_slot = comp_slot
 
         ws = cable_workspace(spec)
@@ -180,7 +189,7 @@ class CableFit:
         self.n_seg = -(-self.n_steps // self.ckpt_every)
         self.ckpt = torch.empty((self.n_steps // self.ckpt_every + 1, 4, self.slots),
                                 dtype=dtype, device=dev)
-        self.loss_part = torch.zeros((nt, 2), dtype=dtype, device=dev)
+        self.loss_part = torch.zeros((nt * self.waves, 2), dtype=dtype, device=dev)
         self.hh_arr = self.hh.array().astype(npd)
         self._traj = None
 
@@ -212,7 +221,8 @@ class CableFit:
         _lib.check(st, "knot CSR product")
 
     def _common(self, sna_s, sk_s, act, step0, n):
-        return [self.n_tasks, self.rmax, _lib.ptr(self.task_nr), _lib.ptr(self.task_p),
+        return [self.n_tasks, self.rmax, self.waves, _lib.ptr(self.task_nr),
+                _lib.ptr(self.task_p),
                 _lib.ptr(self.gl), _lib.ptr(self.grp), _lib.ptr(self.sc), _lib.ptr(sna_s),
                 _lib.ptr(sk_s), _lib.ptr(act), self.n_knots, _lib.ptr(self.knot_k),
                 _lib.ptr(self.knot_a), _lib.ptr(self.site), _lib.ptr(self.mw),

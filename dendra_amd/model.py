@@ -310,10 +310,30 @@ def plan_cables(sizes: np.ndarray, rmax: int = 13, max_lanes: int = 64) -> Cable
                      task_nr[t_order], task_p[t_order], lanes[order])
 
 
-def slot_map(plan: CablePlan, sizes: np.ndarray, off: np.ndarray, rcap: int) -> np.ndarray:
+def plan_cells_per_task(sizes: np.ndarray, rcap: int, width: int) -> CablePlan:
+    """One cell per task of `width` threads (several wavefronts of a
+    workgroup), nr = ceil(K / width) <= rcap rows per thread; tasks in cell
+    order."""
+    sizes = np.asarray(sizes, dtype=np.int64)
+    if (sizes < 1).any():
+        raise ValueError("every cable needs at least one compartment")
+    nr = -(-sizes // width)
+    if (nr > rcap).any():
+        raise ValueError(f"cable of {int(sizes.max())} compartments exceeds {width * rcap}")
+    n = len(sizes)
+    order = np.arange(n)
+    off = np.zeros(n + 1, dtype=np.int64)
+    off[1:] = np.cumsum(sizes)
+    return CablePlan(order, off, np.arange(n), np.ones(n, dtype=np.int64), nr,
+                     np.full(n, width, dtype=np.int64), np.full(n, width, dtype=np.int64))
+
+
+def slot_map(plan: CablePlan, sizes: np.ndarray, off: np.ndarray, rcap: int,
+             width: int = 64) -> np.ndarray:
     """Slot of every compartment (spec order) in the row-slot layout of the
-    fitting and threshold kernels: task t owns 64*rcap slots and row r of
-    lane `sub*P + pl` is slot t*64*rcap + r*64 + sub*P + pl, where
+    fitting and threshold kernels: task t owns width*rcap slots (width = the
+    task's threads: a wavefront, or several when one cell spans them) and row
+    r of lane `sub*P + pl` is slot t*width*rcap + r*width + sub*P + pl, where
     compartment j of the task's sub-th cell sits on lane pl = j // nr, row
     r = j % nr."""
     sizes = np.asarray(sizes, dtype=np.int64)
@@ -332,4 +352,4 @@ def slot_map(plan: CablePlan, sizes: np.ndarray, off: np.ndarray, rcap: int) -> 
     c_of = np.repeat(np.arange(len(sizes)), sizes)
     j = np.arange(int(sizes.sum())) - np.asarray(off, dtype=np.int64)[c_of]
     nr, P = nr_c[c_of], P_c[c_of]
-    return task_c[c_of] * 64 * rcap + (j % nr) * 64 + sub_c[c_of] * P + j // nr
+    return task_c[c_of] * width * rcap + (j % nr) * width + sub_c[c_of] * P + j // nr

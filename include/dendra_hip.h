@@ -562,16 +562,21 @@ int dn_net_run_f32(int n_cells, int n_point, const int64_t* off, const int32_t* 
  * gating vector-Jacobian products autograd would record
  * (mechanisms/_cnexp.py:96-132, mod/HH.py:24-56,109-116), with checkpointed
  * recomputation as Population.longrun_checkpointed (core.py:7802).
- * Slot layout: task t owns 64*rcap slots, slot t*64*rcap + r*64 + lane is
- * row r of that lane (rcap 4 or 8; task_p in {4,8,16,32,64}); per-slot
+ * Slot layout: task t is one workgroup of `waves` wavefronts (W = 64*waves
+ * threads) and owns W*rcap slots; slot t*W*rcap + r*W + tid is row r of
+ * thread tid.  (rcap, waves) is (4, 1) or (8, 1) -- a wavefront of cells of
+ * task_p in {4,8,16,32,64} lanes -- or (2, 2) or (1, 4): one cell over the
+ * W threads (task_p = W).  Per-slot
  * arrays are [slots], state/adjoint [4][slots] (v, m, h, n), knots
  * [n_knots][slots], trajectory [n_steps+1][4][slots].  act holds the
  * activating function of each Ve knot (dn_fit_csr_knots_*).  knot_k/knot_a
  * (length >= step0+n_steps) give the segment and weight of step n; target
  * is [steps][tstride], site[] its column for recorded rows (-1 otherwise),
  * mw[] the margin weight.  hh (host) = {gnabar, gkbar, gl, ena, ek, el, q10}.
- * loss (optional, [n_tasks][2]) receives per-wavefront (mse, margin) sums. */
-int dn_fit_forward_f32(int n_tasks, int rcap, const int32_t* task_nr, const int32_t* task_p,
+ * loss (optional, [n_tasks*waves][2]) receives per-wavefront (mse, margin)
+ * sums. */
+int dn_fit_forward_f32(int n_tasks, int rcap, int waves, const int32_t* task_nr,
+                       const int32_t* task_p,
                        const float* gl, const float* grp, const float* scale,
                        const float* sna, const float* sk, const float* act, int n_knots,
                        const int32_t* knot_k, const float* knot_a, const int32_t* site,
@@ -580,7 +585,8 @@ int dn_fit_forward_f32(int n_tasks, int rcap, const int32_t* task_nr, const int3
                        int64_t n_steps, float dt, const float* hh, const float* x0,
                        float* xout, float* ckpt, int64_t ckpt_every, float* traj,
                        float* loss, hipStream_t stream);
-int dn_fit_forward_f64(int n_tasks, int rcap, const int32_t* task_nr, const int32_t* task_p,
+int dn_fit_forward_f64(int n_tasks, int rcap, int waves, const int32_t* task_nr,
+                       const int32_t* task_p,
                        const double* gl, const double* grp, const double* scale,
                        const double* sna, const double* sk, const double* act,
                        int n_knots, const int32_t* knot_k, const double* knot_a,
@@ -592,7 +598,8 @@ int dn_fit_forward_f64(int n_tasks, int rcap, const int32_t* task_nr, const int3
 /* Reverse sweep over steps [step0, step0+n_steps) from the trajectory the
  * forward wrote for that segment; adj carries (dL/dv, dL/dm, dL/dh, dL/dn)
  * in and out; g_sna, g_sk, g_act are accumulated (+=). */
-int dn_fit_adjoint_f32(int n_tasks, int rcap, const int32_t* task_nr, const int32_t* task_p,
+int dn_fit_adjoint_f32(int n_tasks, int rcap, int waves, const int32_t* task_nr,
+                       const int32_t* task_p,
                        const float* gl, const float* grp, const float* scale,
                        const float* sna, const float* sk, const float* act, int n_knots,
                        const int32_t* knot_k, const float* knot_a, const int32_t* site,
@@ -601,7 +608,8 @@ int dn_fit_adjoint_f32(int n_tasks, int rcap, const int32_t* task_nr, const int3
                        int64_t n_steps, float dt, const float* hh, const float* traj,
                        float* adj, float* g_sna, float* g_sk, float* g_act,
                        hipStream_t stream);
-int dn_fit_adjoint_f64(int n_tasks, int rcap, const int32_t* task_nr, const int32_t* task_p,
+int dn_fit_adjoint_f64(int n_tasks, int rcap, int waves, const int32_t* task_nr,
+                       const int32_t* task_p,
                        const double* gl, const double* grp, const double* scale,
                        const double* sna, const double* sk, const double* act,
                        int n_knots, const int32_t* knot_k, const double* knot_a,

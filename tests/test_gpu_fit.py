@@ -27,14 +27,14 @@ from dendra_amd.workloads import fit_cables
 pytestmark = pytest.mark.gpu
 
 
-def _fit(w, dtype, ckpt_every=500, rmax=4, shared=True, spec=None):
+def _fit(w, dtype, ckpt_every=500, rmax=4, shared=True, spec=None, waves=1):
     from dendra_amd.fit import CableFit
 
     return CableFit(spec or w.spec, w.dt, w.n_steps, w.sites, w.targets,
                     knot_times=w.knot_times, margin_site=w.margin_site,
                     margin_sign=w.margin_sign, margin_weight=w.margin_weight,
                     theta=w.theta, tau=w.tau, ckpt_every=ckpt_every, hh=HH(celsius=6.3),
-                    shared_ve=shared, dtype=dtype, rmax=rmax)
+                    shared_ve=shared, dtype=dtype, rmax=rmax, waves=waves)
 
 
 def _ref(w, spec, sna, sk, ve_rows, shared):
@@ -83,12 +83,13 @@ def _rel(a, b):
     return np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b))
 
 
-@pytest.mark.parametrize("rmax,ckpt", [(4, 70), (8, 300), (4, 1)])
-def test_fp64_adjoint_equals_autograd_restatement(rmax, ckpt):
+@pytest.mark.parametrize("rmax,ckpt,waves", [(4, 70, 1), (8, 300, 1), (4, 1, 1), (2, 70, 2),
+                                             (2, 1, 2), (1, 70, 4)])
+def test_fp64_adjoint_equals_autograd_restatement(rmax, ckpt, waves):
     w = fit_cables(n_cells=5, n_comp=31, n_steps=300, dt=0.01, n_knots=9, seed=3,
                    ve_gain=20.0)
-    print(f"shared table, rmax={rmax}, ckpt_every={ckpt}", flush=True)
-    fit = _fit(w, torch.float64, ckpt_every=ckpt, rmax=rmax)
+    print(f"shared table, rmax={rmax}, ckpt_every={ckpt}, waves={waves}", flush=True)
+    fit = _fit(w, torch.float64, ckpt_every=ckpt, rmax=rmax, waves=waves)
     got = _gpu(fit, w.sna, w.sk, w.ve, torch.float64)
     want = _ref(w, w.spec, w.sna, w.sk, w.ve, True)
     assert abs(got[0] - want[0]) <= 1e-10 * abs(want[0])
@@ -96,14 +97,20 @@ def test_fp64_adjoint_equals_autograd_restatement(rmax, ckpt):
         assert _rel(g, r) <= 1e-9, _rel(g, r)
 
 
-def test_fp64_adjoint_ragged_cells_per_compartment_table():
-    base = fit_cables(n_cells=6, n_comp=45, n_steps=240, dt=0.01, n_knots=7, seed=8,
-                      ve_gain=20.0)
-    sizes = np.array([45, 12, 33, 20, 7, 45])
+@pytest.mark.parametrize("waves", [1, 2])
+def test_fp64_adjoint_ragged_cells_per_compartment_table(waves):
+    """Ragged cells; with two wavefronts per cell the sizes straddle the
+    one-row (<= 128) and two-row (129..256) forms of the cross-wavefront solve
+    and the interface rows of the 201-compartment cells."""
+    K = 45 if waves == 1 else 201
+    base = fit_cables(n_cells=6, n_comp=K, n_steps=240, dt=0.01, n_knots=7, seed=8,
+                      ve_gain=20.0 if waves == 1 else 3.0)
+    sizes = (np.array([45, 12, 33, 20, 7, 45]) if waves == 1
+             else np.array([201, 128, 129, 64, 65, 200]))
     off = np.zeros(7, dtype=np.int64)
     off[1:] = np.cumsum(sizes)
     total = int(off[-1])
-    pick = np.concatenate([np.arange(c * 45, c * 45 + k) for c, k in enumerate(sizes)])
+    pick = np.concatenate([np.arange(c * K, c * K + k) for c, k in enumerate(sizes)])
     spec = CableSpec(off=off, diam=base.spec.diam[pick], dx=base.spec.dx[pick],
                      rhoa=base.spec.rhoa[pick], cm=base.spec.cm[pick], v_init=-65.0)
     sna = base.sna.reshape(-1)[pick]
@@ -114,8 +121,9 @@ def test_fp64_adjoint_ragged_cells_per_compartment_table():
     w.sites = np.array([0, 3, 6])
     w.margin_site = 2
     w.targets = base.targets[:, :, :3].copy()
-    print("ragged cells, per-compartment table", flush=True)
-    fit = _fit(w, torch.float64, ckpt_every=50, shared=False, spec=spec)
+    print(f"ragged cells, per-compartment table, waves={waves}", flush=True)
+    fit = _fit(w, torch.float64, ckpt_every=50, shared=False, spec=spec,
+               rmax=4 if waves == 1 else 2, waves=waves)
     got = _gpu(fit, sna, sk, ve, torch.float64)
     want = _ref(w, spec, sna, sk, ve, False)
     assert abs(got[0] - want[0]) <= 1e-10 * abs(want[0])
@@ -186,11 +194,61 @@ def test_full_size_fp64_adjoint_vs_central_differences(full_workload):
     assert len(ratios) >= 8 and all(3.0 <= r <= 5.5 for r in ratios)
 
 
-def test_fp64_adjoint_201_compartments_equals_restatement():
-    """The 64-lane, 4-row layout of the 201-compartment workload."""
+def test_full_size_fp32_adjoint_vs_central_differences(full_workload):
+    """The benchmarked float32 adjoint (four wavefronts per cell) against
+    central differences of the float64 forward at h = 1e-5 (which agree with
+    the float64 adjoint within 1e-4, test above), one direction per cell:
+    32 seeded cells, each a random unit direction in that cell's gNa/gK
+    scales.  The float32 error of a cell's gradient is that of its float32
+    trajectory over 10,000 steps; a cell near a spike threshold is chaotic in
+    float32 (whether a spike appears depends on the last bits), so the bars
+    are on the distribution, as in the per-cell comparison below: median
+    <= 1e-3 and at least 29 of the 32 within 1e-2."""
+    w = full_workload
+    fit64 = _fit(w, torch.float64)
+    fit32 = _fit(w, torch.float32, rmax=1, waves=4)
+    dev = "cuda"
+    mk = lambda dt: [torch.tensor(np.asarray(x).reshape(-1), dtype=dt, device=dev)
+                     for x in (w.sna, w.sk)] + [torch.tensor(w.ve, dtype=dt, device=dev)]
+    p64, p32 = mk(torch.float64), mk(torch.float32)
+    fit32.forward(*p32)
+    g32 = [x.double() for x in fit32.backward()]
+    B = w.spec.n_cells
+    K = p64[0].numel() // B
+    rs = np.random.default_rng(77)
+    cells = rs.choice(B, 32, replace=False)
+    h = 1e-5
+    err = []
+    for c in cells:
+        u = [torch.zeros_like(x) for x in p64]
+        d = rs.normal(size=2 * K)
+        d /= np.linalg.norm(d)
+        u[0][c * K:(c + 1) * K] = torch.as_tensor(d[:K], device=dev)
+        u[1][c * K:(c + 1) * K] = torch.as_tensor(d[K:], device=dev)
+        an = sum((gi * di).sum() for gi, di in zip(g32, u)).item()
+        lp = fit64.forward(*[x + h * di for x, di in zip(p64, u)]).item()
+        lm = fit64.forward(*[x - h * di for x, di in zip(p64, u)]).item()
+        D = (lp - lm) / (2 * h)
+        err.append(abs(an - D) / abs(D))
+        print(f"cell {c}: float32 adjoint={an:.7e} D(1e-5)={D:.9e} rel {err[-1]:.2e}",
+              flush=True)
+    err = np.asarray(err)
+    print(f"float32 adjoint vs float64 central differences over 32 cells: median "
+          f"{np.median(err):.2e}, 90th percentile {np.quantile(err, 0.9):.2e}, "
+          f"worst {err.max():.2e}", flush=True)
+    assert np.median(err) <= 1e-3
+    assert (err <= 1e-2).sum() >= 29
+
+
+@pytest.mark.parametrize("rmax,waves", [(4, 1), (2, 2), (1, 4)])
+def test_fp64_adjoint_201_compartments_equals_restatement(rmax, waves):
+    """The layouts of the 201-compartment workload: 64 lanes x 4 rows, two
+    wavefronts x 2 rows, four wavefronts x 1 row."""
     w = fit_cables(n_cells=2, n_comp=201, n_steps=300, dt=0.01, n_knots=9, seed=3,
                    ve_gain=3.0)
-    got = _gpu(_fit(w, torch.float64, ckpt_every=70), w.sna, w.sk, w.ve, torch.float64)
+    print(f"201 compartments, rmax={rmax}, waves={waves}", flush=True)
+    got = _gpu(_fit(w, torch.float64, ckpt_every=70, rmax=rmax, waves=waves), w.sna, w.sk,
+               w.ve, torch.float64)
     want = _ref(w, w.spec, w.sna, w.sk, w.ve, True)
     assert abs(got[0] - want[0]) <= 1e-10 * abs(want[0])
     for g, r in zip(got[1:], want[1:]):
@@ -207,14 +265,16 @@ def test_fp32_adjoint_small_against_fp64():
         assert _rel(a, b) <= 2e-3, _rel(a, b)
 
 
-def test_full_size_fp32_against_fp64(full_workload):
+@pytest.mark.parametrize("rmax,waves", [(4, 1), (1, 4)])
+def test_full_size_fp32_against_fp64(full_workload, rmax, waves):
     """Per-cell comparison: over 10,000 steps a cell sitting on a spike
     threshold has a gradient orders of magnitude above the others (its spike
     appears or vanishes with the last bits of the trajectory), so the bar is
     on the distribution of per-cell relative errors, not on the global norm."""
     w = full_workload
     g64 = _gpu(_fit(w, torch.float64), w.sna, w.sk, w.ve, torch.float64)
-    g32 = _gpu(_fit(w, torch.float32), w.sna, w.sk, w.ve, torch.float32)
+    g32 = _gpu(_fit(w, torch.float32, rmax=rmax, waves=waves), w.sna, w.sk, w.ve,
+               torch.float32)
     rl = abs(g32[0] - g64[0]) / abs(g64[0])
     B = w.spec.n_cells
     out = []

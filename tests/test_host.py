@@ -240,6 +240,30 @@ def test_slot_map_matches_definition():
                 assert np.array_equal(got[off[cell] + j], want)
 
 
+def test_cell_per_task_slot_map():
+    """Cells spanning several wavefronts: one cell per task of `width`
+    threads, compartment j on thread j // nr, row j % nr, with
+    nr = ceil(K / width); rows beyond a cell stay free."""
+    from dendra_amd.model import plan_cells_per_task, slot_map
+
+    rs = np.random.default_rng(6)
+    sizes = np.concatenate([[1, 128, 129, 256, 255], rs.integers(1, 257, 40)])
+    off = np.zeros(len(sizes) + 1, dtype=np.int64)
+    off[1:] = np.cumsum(sizes)
+    for R, width in ((2, 128), (1, 256)):
+        plan = plan_cells_per_task(sizes, R, width)
+        assert np.array_equal(plan.task_nr, -(-sizes // width))
+        assert (plan.task_p == width).all() and (plan.task_ncell == 1).all()
+        got = slot_map(plan, sizes, off, R, width)
+        assert len(np.unique(got)) == len(got) and got.max() < len(sizes) * width * R
+        for c in range(len(sizes)):
+            nr = -(-int(sizes[c]) // width)
+            j = np.arange(sizes[c])
+            assert np.array_equal(got[off[c] + j], c * width * R + (j % nr) * width + j // nr)
+    with pytest.raises(ValueError):
+        plan_cells_per_task(np.array([257]), 1, 256)
+
+
 def test_entry_points_reject_invalid_arguments_before_any_launch():
     """Argument validation happens on the host before any HIP call, so the
     error contract (DN_EINVAL = -1, DN_ERANGE = -3) is checkable without a GPU."""
@@ -249,22 +273,27 @@ def test_entry_points_reject_invalid_arguments_before_any_launch():
     hh = np.array([0.12, 0.036, 3e-4, 50.0, -77.0, -54.3, 1.0])
     hp = hh.ctypes.data
     fake = 1 << 20  # never dereferenced: validation fails first
-    # fit forward: rcap must be 4 or 8; null task arrays
-    args = [1, 5, fake, fake] + [fake] * 6 + [4, fake, fake, fake, fake, fake, 8, 1.0, 0.0,
-                                              1.0, 0, 10, 0.01, hp, fake, None, None, 1,
-                                              None, None, None]
+    # fit forward: (rcap, waves) in {(4, 1), (8, 1), (2, 2), (1, 4)}; null task arrays
+    args = [1, 5, 1, fake, fake] + [fake] * 6 + [4, fake, fake, fake, fake, fake, 8, 1.0,
+                                                 0.0, 1.0, 0, 10, 0.01, hp, fake, None,
+                                                 None, 1, None, None, None]
     assert lib.dn_fit_forward_f32(*args) == -1
-    args[1] = 4
-    args[2] = None
+    for rcap, waves in ((2, 1), (4, 2), (1, 2), (2, 4), (8, 4)):
+        args[1], args[2] = rcap, waves
+        assert lib.dn_fit_forward_f32(*args) == -1
+    args[1], args[2] = 4, 1
+    args[3] = None
     assert lib.dn_fit_forward_f32(*args) == -1
     # tau must be positive
-    args[2] = fake
-    args[19] = 0.0
+    args[3] = fake
+    args[20] = 0.0
     assert lib.dn_fit_forward_f64(*args) == -1
     # adjoint needs a trajectory
-    adj = [1, 4, fake, fake] + [fake] * 6 + [4, fake, fake, fake, fake, fake, 8, 1.0, 0.0,
-                                             1.0, 0, 10, 0.01, hp, None, fake, fake, fake,
-                                             fake, None]
+    adj = [1, 4, 1, fake, fake] + [fake] * 6 + [4, fake, fake, fake, fake, fake, 8, 1.0,
+                                                0.0, 1.0, 0, 10, 0.01, hp, None, fake, fake,
+                                                fake, fake, None]
+    assert lib.dn_fit_adjoint_f32(*adj) == -1
+    adj[1], adj[2] = 1, 4
     assert lib.dn_fit_adjoint_f32(*adj) == -1
     # knot CSR: null pointer table
     assert lib.dn_fit_csr_knots_f32(4, 2, None, fake, fake, fake, 1, 1, fake, 1, 1, None) == -1
