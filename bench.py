@@ -501,6 +501,28 @@ def surrogate_bench(n_train, n_test, epochs, with_cpu):
     sur.predict(te.profile, d)
     torch.cuda.synchronize()
     t_net = time.perf_counter() - t
+    # kernel-level rates: the fused forward alone on a 65,536-row chunk (one
+    # launch), and training steps on one minibatch, timed with HIP events on
+    # the stream they run on
+    Xc, _, _ = sur.featurize(te.profile[:65536], d[:65536])
+    strm = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    sur._forward_chunk(Xc, 65536)
+    ev[0].record(strm)
+    for _ in range(10):
+        sur._forward_chunk(Xc, 65536)
+    ev[1].record(strm)
+    g = torch.Generator(device=Xc.device).manual_seed(5)
+    idx_b = torch.stack([torch.randperm(sur.X.shape[0], generator=g, device=Xc.device)[:8192]
+                         for _ in range(4)]).to(torch.int32).contiguous()
+    sur.train_step(idx_b, 8192, 0.0)
+    ev[2].record(strm)
+    for _ in range(20):
+        sur.train_step(idx_b, 8192, 0.0)
+    ev[3].record(strm)
+    torch.cuda.synchronize()
+    t_fwd_k = ev[0].elapsed_time(ev[1]) / 10 * 1e-3
+    t_step_k = ev[2].elapsed_time(ev[3]) / 20 * 1e-3
     ns = min(50_000, n_test)
     idx = np.arange(ns)
     ref = judge(idx)
@@ -524,7 +546,24 @@ def surrogate_bench(n_train, n_test, epochs, with_cpu):
                      "achieved_train": train_flops / t_train / 1e12,
                      "frac_train": train_flops / t_train / 1e12 / 2500.0,
                      "achieved_inference": inf_flops / t_net / 1e12,
-                     "frac_inference": inf_flops / t_net / 1e12 / 2500.0},
+                     "frac_inference": inf_flops / t_net / 1e12 / 2500.0,
+                     "kernel_inference": {
+                         "kernel": "mlp_fwd_kernel (fused forward, 64-row tiles), 65536 rows "
+                                   "x 4 members per launch",
+                         "us": t_fwd_k * 1e6,
+                         "achieved": 65536 * 4 * _mlp_flops_per_sample(False) / t_fwd_k / 1e12,
+                         "frac": 65536 * 4 * _mlp_flops_per_sample(False) / t_fwd_k / 1e12
+                                 / 2500.0},
+                     "kernel_train_step": {
+                         "launches": "gather, fused forward, fused backward, partial sums, "
+                                     "3 weight-gradient products + reductions, one Adam",
+                         "us": t_step_k * 1e6,
+                         "achieved": 8192 * 4 * _mlp_flops_per_sample(True) / t_step_k / 1e12,
+                         "frac": 8192 * 4 * _mlp_flops_per_sample(True) / t_step_k / 1e12
+                                 / 2500.0},
+                     "note": "achieved_train / achieved_inference are whole-loop rates (host "
+                             "loop, features, ensemble included); kernel_* are the GPU-timed "
+                             "launches"},
     }
     if with_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))

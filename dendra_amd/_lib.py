@@ -105,7 +105,12 @@ _SIGNATURES.update({
     "dn_mlp_out_bwd": (i32, [i32, i32, i32, p, i64, p, i64, p, i64, p, p, i64, p]),
     "dn_mlp_row_dot": (i32, [i32, i32, i32, i32, p, i64, p, i64, p, i64, p]),
     "dn_mlp_splitk_reduce": (i32, [i32, i64, i32, p, p, p]),
-    "dn_mlp_adam": (i32, [i64, i32, i32, p, p, p, p, f32, f32, f32, f32, p, p, p]),
+    "dn_mlp_adam": (i32, [i64, i32, i32, p, p, p, p, f32, f32, f32, f32, p, p, p, p, p]),
+    "dn_mlp_forward": (i32, [i32, i32, i32, p, i64, p, p, p, i64, i64, p, p, p, p, p, p, i64]
+                       + [p] * 10),
+    "dn_mlp_backward": (i32, [i32, i32, i32, p, i64, p, p, p, p, p, p, i64, p, p, p, p, p]),
+    "dn_mlp_adam_multi": (i32, [i32] + [p] * 11 + [f32, f32, f32, f32, p]),
+    "dn_mlp_partials_reduce": (i32, [i32, i32, p, p, p, p, p, p, p]),
     "dn_mlp_featurize": (i32, [i64, i32, p, p, f32, f32, p, i32, p, p, p, f32, p, p]),
     "dn_mlp_gather": (i32, [i32, i32, i32, p, p, p, p, p, p, p]),
     "dn_mlp_ensemble": (i32, [i64, i32, p, i64, p, f32, f32, f32, p, p, p]),

@@ -46,14 +46,33 @@ struct GemmArgs {
   const float* Zp; int64_t ldzp, szp; // EPI_DGELU: pre-activation multiplier input
 };
 
-// gelu(z) = z Phi(z) with Phi(z) = erfc(-z / sqrt(2)) / 2: erfc keeps full
-// relative accuracy for negative z, where 1 + erf(z / sqrt(2)) cancels.
-__device__ __forceinline__ float gelu(float z) {
-  return 0.5f * z * erfcf(-z * 0.70710678118654752f);
+// gelu(z) = z Phi(z), Phi(z) = erfc(-z / sqrt 2) / 2.  erfc(x), x >= 0, by
+// the Chebyshev fit of Press et al., Numerical Recipes, "erfcc":
+//   t = 1 / (1 + x/2),  erfc(x) = t exp(-x^2 + P(t)),
+// P of degree 9, fractional error below 1.2e-7 for every x, so Phi keeps
+// float32 relative accuracy in both tails (for z < 0 it is erfc(|z|/sqrt 2)/2
+// itself; for z > 0, 1 minus it).  About 20 VALU operations: the epilogues
+// that evaluate it on every activation cost as much as the matrix products
+// that produce them, so the evaluation count matters.
+__device__ __forceinline__ float phi_cdf(float z) {
+  const float x = fabsf(z) * 0.70710678118654752f;
+  const float t = frcp(fmaf(0.5f, x, 1.0f));
+  float p = 0.17087277f;
+  p = fmaf(p, t, -0.82215223f);
+  p = fmaf(p, t, 1.48851587f);
+  p = fmaf(p, t, -1.13520398f);
+  p = fmaf(p, t, 0.27886807f);
+  p = fmaf(p, t, -0.18628806f);
+  p = fmaf(p, t, 0.09678418f);
+  p = fmaf(p, t, 0.37409196f);
+  p = fmaf(p, t, 1.00002368f);
+  p = fmaf(p, t, -1.26551223f);
+  const float h = 0.5f * t * __expf(fmaf(-x, x, p));  // Phi(-|z|)
+  return z >= 0.0f ? 1.0f - h : h;
 }
+__device__ __forceinline__ float gelu(float z) { return z * phi_cdf(z); }
 __device__ __forceinline__ float gelu_d(float z) {
-  return 0.5f * erfcf(-z * 0.70710678118654752f) +
-         z * 0.39894228040143268f * __expf(-0.5f * z * z);
+  return phi_cdf(z) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
 }
 
 template <int EPI>
@@ -184,6 +203,323 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs G) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused forward of the whole network for a tile of rows of one member
+// (inference, and the forward of a training step).  The tile's activations
+// stay in LDS from layer to layer (128 x 512 BF16 = 130 KB with padding, or
+// 64 rows in half that); each wavefront computes 64 (128) output features of
+// all 128 (64) rows (4 x 2 or 2 x 4 blocks of 32 x 32, float32 accumulators
+// in registers) with the
+// weights read from L2 in MFMA fragment order ("packed": the 16 bytes lane l
+// of a 32x32x16 product needs are contiguous with its neighbours'), four
+// k-steps ahead.  Per layer: products over K, barrier, epilogue (bias, GELU,
+// BF16 into LDS over the consumed input; in training also the float32
+// pre-activation Z and the transposed BF16 activation H^T the backward
+// products read), barrier.  The output unit sums each row from LDS.
+// Packed weights: Wp[z][nb][ks][lane][8], element e of lane l in block
+// (nb, ks) = W[z][32 nb + (l & 31)][16 ks + 8 (l >> 5) + e].
+// ---------------------------------------------------------------------------
+constexpr int kFH = 512;         // hidden width
+constexpr int kFX = 128;         // padded input width
+constexpr int kFP = kFH + 8;     // LDS row pitch of a hidden layer (1040 B)
+constexpr int kFPX = kFX + 8;    // LDS row pitch of the input tile (272 B)
+
+struct FwdArgs {
+  int n;                          // rows per member, a multiple of the tile rows
+  const bf16* X; int64_t sx;      // [z][n][128]; sx = 0: one input for all members
+  const bf16* Wp[3]; int64_t sw[3];
+  const float* b[3];              // [z][512]
+  const bf16* w4; const float* b4;  // [z][512], [z]
+  float* y; int64_t sy;           // [z][n]
+  float* Z[3];                    // training: [z][n][512] float32 pre-activations
+  bf16* HT[3];                    // training: [z][512][n] BF16 activations
+  const float* t; float* dy;      // training: targets [z][n], dy = 2 (y - t) / n
+  float* part;                    // training: [z][tiles][5][512] per-tile partial sums
+};
+
+// Per-tile partial sums of the small gradients (fixed order; the tiles are
+// summed in order by partials_reduce_kernel): slot 0..2 db1..db3 (column
+// sums of dZ_l), 3 dw4 = H3^T dy, 4 db4 = sum dy (element 0).
+constexpr int kParts = 5;
+
+// Tile shape: RB row blocks and CB column blocks of 32 per wavefront, so the
+// workgroup has 32 RB rows and 16 / CB wavefronts (64 features x 128 rows
+// with RB = 4, CB = 2: one workgroup per CU; 128 features x 64 rows with
+// RB = 2, CB = 4: two per CU, whose epilogues and products can overlap).
+// Weights are prefetched PD = 8 / CB k-steps ahead.
+template <int RB, int CB>
+__device__ __forceinline__ void tile_products(f32x16 (&acc)[RB][CB], const bf16* sA, int pitch,
+                                              int KS, const bf16* wp, int lane) {
+  constexpr int PD = 8 / CB;
+#pragma unroll
+  for (int a = 0; a < RB; ++a)
+#pragma unroll
+    for (int b = 0; b < CB; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+  // column block cb of the caller's wavefront is packed block (its first) + cb
+  bf16x8 bq[PD][CB];
+#pragma unroll
+  for (int q = 0; q < PD; ++q)
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+      bq[q][cb] = *reinterpret_cast<const bf16x8*>(wp + ((int64_t)cb * KS + q) * 512);
+  const bf16* arow = sA + (lane & 31) * pitch + 8 * (lane >> 5);
+  for (int ks0 = 0; ks0 < KS; ks0 += PD) {
+#pragma unroll
+    for (int q = 0; q < PD; ++q) {
+      const int ks = ks0 + q;
+      bf16x8 b[CB];
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) b[cb] = bq[q][cb];
+      if (ks + PD < KS) {  // PD k-steps ahead
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+          bq[q][cb] = *reinterpret_cast<const bf16x8*>(wp + ((int64_t)cb * KS + ks + PD) * 512);
+      }
+      bf16x8 a[RB];
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+        a[rb] = *reinterpret_cast<const bf16x8*>(arow + rb * 32 * pitch + ks * 16);
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+          acc[rb][cb] =
+              __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
+    }
+  }
+}
+
+// Transposed store of an LDS tile [ROWS][512] (pitch kFP) to T[j][m0 + i]
+// (row length n), with the column sums over the tile's rows into psum[j]
+// (psum may be null).  Chunk c: feature j = c / (ROWS / 8), rows
+// 8 (c % (ROWS / 8)) .. + 7; the ROWS / 8 chunks of one feature sit on
+// consecutive lanes and are summed by a butterfly in fixed order.
+template <int ROWS, int NT>
+__device__ __forceinline__ void store_transposed(const bf16* sT, bf16* T, int64_t n,
+                                                 float* psum, int tid) {
+  constexpr int G = ROWS / 8;
+#pragma unroll 1
+  for (int q = 0; q < kFH * ROWS / 8 / NT; ++q) {
+    const int c = tid + NT * q;
+    const int j = c / G, i0 = (c % G) * 8;
+    bf16x8 v;
+    float sm = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      v[e] = sT[(i0 + e) * kFP + j];
+      sm += (float)v[e];
+    }
+    *reinterpret_cast<bf16x8*>(&T[(int64_t)j * n + i0]) = v;
+    if (psum) {
+#pragma unroll
+      for (int o = 1; o < G; o <<= 1) sm += __shfl_xor(sm, o);
+      if ((c % G) == 0) psum[j] = sm;
+    }
+  }
+}
+
+template <bool TRAIN, int RB, int CB>
+__global__ __launch_bounds__(1024 / CB, 2) void mlp_fwd_kernel(FwdArgs F) {
+  constexpr int ROWS = 32 * RB, NT = 1024 / CB;
+  __shared__ __align__(16) bf16 sH[ROWS * kFP];
+  __shared__ float sDy[TRAIN ? ROWS : 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int z = blockIdx.y;
+  const int64_t m0 = (int64_t)blockIdx.x * ROWS;
+  {
+    const bf16* X = F.X + z * F.sx + m0 * kFX;
+#pragma unroll
+    for (int q = 0; q < ROWS * kFX / 8 / NT; ++q) {
+      const int c = tid + NT * q;
+      const int r = c >> 4, k = (c & 15) * 8;
+      *reinterpret_cast<uint4*>(&sH[r * kFPX + k]) =
+          *reinterpret_cast<const uint4*>(&X[(int64_t)r * kFX + k]);
+    }
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (int l = 0; l < 3; ++l) {
+    const int KS = (l == 0 ? kFX : kFH) / 16;
+    const int pitch = l == 0 ? kFPX : kFP;
+    f32x16 acc[RB][CB];
+    tile_products<RB, CB>(acc, sH, pitch, KS,
+                          F.Wp[l] + z * F.sw[l] + (int64_t)(CB * wave) * KS * 512 + lane * 8,
+                          lane);
+    __syncthreads();  // every wavefront is done reading the layer input
+    const float* bias = F.b[l] + z * kFH;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        const int j = wave * 32 * CB + cb * 32 + (lane & 31);
+        const float bj = bias[j];
+        float* zrow = nullptr;
+        if (TRAIN) {
+          zrow = F.Z[l] + ((int64_t)z * F.n + m0 + rb * 32 + 4 * (lane >> 5)) * kFH + j;
+          // the block's store addresses are formed from here on, not hoisted
+          // out of the layer loop (where they would be held for all blocks)
+          asm volatile("" : "+v"(zrow));
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int i = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          const float v = acc[rb][cb][r] + bj;
+          if (TRAIN) zrow[(8 * (r >> 2) + (r & 3)) * kFH] = v;
+          sH[i * kFP + j] = (bf16)gelu(v);
+        }
+      }
+    __syncthreads();
+    if (TRAIN)
+      store_transposed<ROWS, NT>(sH, F.HT[l] + (int64_t)z * kFH * F.n + m0, F.n, nullptr,
+                                 tid);
+  }
+  // output unit: one wavefront per row at a time, 8 features per lane and a
+  // butterfly -- the summation order of dn_mlp_out, so y and dy are the
+  // layer products' bit for bit
+  {
+    const bf16x8 wv = *reinterpret_cast<const bf16x8*>(F.w4 + z * kFH + lane * 8);
+    for (int i = wave; i < ROWS; i += NT / 64) {
+      const bf16x8 hv = *reinterpret_cast<const bf16x8*>(sH + i * kFP + lane * 8);
+      float s = 0.0f;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) s += (float)hv[e] * (float)wv[e];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+      if (lane == 0) {
+        const float yv = s + F.b4[z];
+        F.y[z * F.sy + m0 + i] = yv;
+        if (TRAIN) {
+          const float d = 2.0f * (yv - F.t[z * F.sy + m0 + i]) / (float)F.n;
+          F.dy[z * F.sy + m0 + i] = d;
+          sDy[i] = d;
+        }
+      }
+    }
+  }
+  if (TRAIN) {
+    // partial dw4 = H3^T dy and db4 = sum dy over the tile's rows
+    __syncthreads();
+    float* part = F.part + ((int64_t)z * (F.n / ROWS) + blockIdx.x) * kParts * kFH;
+    for (int j = tid; j < kFH; j += NT) {
+      float s = 0.0f;
+      for (int i = 0; i < ROWS; ++i) s += (float)sH[i * kFP + j] * sDy[i];
+      part[3 * kFH + j] = s;
+    }
+    if (tid == 0) {
+      float s = 0.0f;
+      for (int i = 0; i < ROWS; ++i) s += sDy[i];
+      part[4 * kFH] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused backward data products for a tile of rows of one member:
+//   dZ3 = dy w4^T * gelu'(Z3),  dZ2 = (dZ3 W3) * gelu'(Z2),
+//   dZ1 = (dZ2 W2) * gelu'(Z1)
+// (BF16 results, as the layer products round them), each dZ held in LDS as
+// the next product's A operand, W_l^T read packed from L2; stores dZ_l^T
+// [z][512][n] for the weight-gradient products and the tile's column sums
+// of dZ_l (partial bias gradients).
+// ---------------------------------------------------------------------------
+struct BwdArgs {
+  int n;
+  const float* dy; int64_t sy;    // [z][n]
+  const bf16* w4;                 // [z][512]
+  const float* Z[3];              // [z][n][512]
+  const bf16* Wtp[2]; int64_t sw; // packed W2^T, W3^T, member stride
+  bf16* dZT[3];                   // [z][512][n]
+  float* part;                    // [z][tiles][5][512]
+};
+
+template <int RB, int CB>
+__global__ __launch_bounds__(1024 / CB, 2) void mlp_bwd_kernel(BwdArgs A) {
+  constexpr int ROWS = 32 * RB, NT = 1024 / CB;
+  __shared__ __align__(16) bf16 sD[ROWS * kFP];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int z = blockIdx.y;
+  const int64_t m0 = (int64_t)blockIdx.x * ROWS;
+  float* part = A.part + ((int64_t)z * (A.n / ROWS) + blockIdx.x) * kParts * kFH;
+  // dZ3: four consecutive features per thread and chunk
+  {
+    const float* Z3 = A.Z[2] + ((int64_t)z * A.n + m0) * kFH;
+    const bf16* w4 = A.w4 + z * kFH;
+#pragma unroll 4
+    for (int q = 0; q < ROWS * kFH / 4 / NT; ++q) {
+      const int c = tid + NT * q;
+      const int i = c >> 7, j = (c & 127) * 4;
+      const float4 zv = *reinterpret_cast<const float4*>(&Z3[(int64_t)i * kFH + j]);
+      const float d = A.dy[z * A.sy + m0 + i];
+      bf16* o = &sD[i * kFP + j];
+      o[0] = (bf16)(d * (float)w4[j] * gelu_d(zv.x));
+      o[1] = (bf16)(d * (float)w4[j + 1] * gelu_d(zv.y));
+      o[2] = (bf16)(d * (float)w4[j + 2] * gelu_d(zv.z));
+      o[3] = (bf16)(d * (float)w4[j + 3] * gelu_d(zv.w));
+    }
+  }
+  __syncthreads();
+  store_transposed<ROWS, NT>(sD, A.dZT[2] + (int64_t)z * kFH * A.n + m0, A.n, part + 2 * kFH,
+                             tid);
+#pragma unroll 1
+  for (int l = 2; l >= 1; --l) {  // dZ_l (index l-1) from dZ_{l+1} and W_{l+1}
+    f32x16 acc[RB][CB];
+    tile_products<RB, CB>(acc, sD, kFP, kFH / 16,
+                          A.Wtp[l - 1] + z * A.sw + (int64_t)(CB * wave) * (kFH / 16) * 512 +
+                              lane * 8,
+                          lane);
+    __syncthreads();  // every wavefront is done reading dZ_{l+1}
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) {
+        const int j = wave * 32 * CB + cb * 32 + (lane & 31);
+        const float* zrow =
+            A.Z[l - 1] + ((int64_t)z * A.n + m0 + rb * 32 + 4 * (lane >> 5)) * kFH + j;
+        asm volatile("" : "+v"(zrow));
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int i = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          sD[i * kFP + j] = (bf16)(acc[rb][cb][r] * gelu_d(zrow[(8 * (r >> 2) + (r & 3)) * kFH]));
+        }
+        // one block's Z loads in flight at a time (register pressure)
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    __syncthreads();
+    // (the next product only reads sD; its epilogue writes after a barrier)
+    store_transposed<ROWS, NT>(sD, A.dZT[l - 1] + (int64_t)z * kFH * A.n + m0, A.n,
+                               part + (l - 1) * kFH, tid);
+  }
+}
+
+// grad[z][s][j] = sum over tiles of part[z][tile][s][j], tiles in order; the
+// five sums go to b1, b2, b3, w4 (each [z][512]) and b4 ([z], element 0).
+__global__ __launch_bounds__(256) void partials_reduce_kernel(int tiles, const float* part,
+                                                              float* gb1, float* gb2,
+                                                              float* gb3, float* gw4,
+                                                              float* gb4) {
+  const int z = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;  // slot * 512 + j
+  if (e >= kParts * kFH) return;
+  const int slot = e / kFH, j = e % kFH;
+  if (slot == 4 && j > 0) return;
+  const float* p = part + (int64_t)z * tiles * kParts * kFH + e;
+  float s = 0.0f;
+  // loads in batches of 16 (independent), sums in tile order
+  int t = 0;
+  for (; t + 16 <= tiles; t += 16) {
+    float v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = p[(int64_t)(t + u) * kParts * kFH];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) s += v[u];
+  }
+  for (; t < tiles; ++t) s += p[(int64_t)t * kParts * kFH];
+  float* out = slot == 0 ? gb1 : slot == 1 ? gb2 : slot == 2 ? gb3 : slot == 3 ? gw4 : gb4;
+  out[z * (slot == 4 ? 1 : kFH) + j] = s;
+}
+
 // Output unit: y = H3 . w4 + b4 (one wavefront per sample).  With targets,
 // also r = y - t and dy = 2 r / B.
 __global__ __launch_bounds__(256) void mlp_out_kernel(int Bn, int Hd, const bf16* H3,
@@ -279,14 +615,13 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(int64_t n, int split
   out[t] = s;
 }
 
-// Adam on float32 masters [z][rows][cols]; BF16 copies [z][rows][cols] and
-// transposed [z][cols][rows] (either may be null).
-__global__ __launch_bounds__(256) void adam_kernel(int64_t n, int rows, int cols, float* p,
-                                                   const float* g, float* m, float* v,
-                                                   float lr_t, float b1, float b2, float eps,
-                                                   bf16* pb, bf16* pbt) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+// Adam on float32 masters [z][rows][cols]; BF16 copies [z][rows][cols],
+// transposed [z][cols][rows], and the packed MFMA fragment orders of W and
+// W^T (any may be null).
+__device__ __forceinline__ void adam_elem(int64_t t, int rows, int cols, float* p,
+                                          const float* g, float* m, float* v, float lr_t,
+                                          float b1, float b2, float eps, bf16* pb, bf16* pbt,
+                                          bf16* pbp, bf16* pbtp) {
   const float gi = g[t];
   const float mi = b1 * m[t] + (1.0f - b1) * gi;
   const float vi = b2 * v[t] + (1.0f - b2) * gi * gi;
@@ -298,7 +633,50 @@ __global__ __launch_bounds__(256) void adam_kernel(int64_t n, int rows, int cols
   const int64_t z = t / per, e = t % per, r = e / cols, c = e % cols;
   if (pb) pb[t] = (bf16)pi;
   if (pbt) pbt[z * per + c * rows + r] = (bf16)pi;
+  if (pbp) {  // MFMA fragment order of the fused forward (rows % 32 == 0, cols % 16 == 0)
+    const int64_t nb = r >> 5, ks = c >> 4, ln = ((c >> 3) & 1) * 32 + (r & 31);
+    pbp[z * per + ((nb * (cols >> 4) + ks) * 64 + ln) * 8 + (c & 7)] = (bf16)pi;
+  }
+  if (pbtp) {  // the same order for W^T [cols][rows] (cols % 32 == 0, rows % 16 == 0)
+    const int64_t nb = c >> 5, ks = r >> 4, ln = ((r >> 3) & 1) * 32 + (c & 31);
+    pbtp[z * per + ((nb * (rows >> 4) + ks) * 64 + ln) * 8 + (r & 7)] = (bf16)pi;
+  }
 }
+
+__global__ __launch_bounds__(256) void adam_kernel(int64_t n, int rows, int cols, float* p,
+                                                   const float* g, float* m, float* v,
+                                                   float lr_t, float b1, float b2, float eps,
+                                                   bf16* pb, bf16* pbt, bf16* pbp,
+                                                   bf16* pbtp) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  adam_elem(t, rows, cols, p, g, m, v, lr_t, b1, b2, eps, pb, pbt, pbp, pbtp);
+}
+
+// Every parameter tensor of a step in one launch: tensor k = blockIdx.y.
+constexpr int kAdamMax = 8;
+struct AdamTable {
+  int64_t n[kAdamMax];
+  int rows[kAdamMax], cols[kAdamMax];
+  float* p[kAdamMax];
+  const float* g[kAdamMax];
+  float* m[kAdamMax];
+  float* v[kAdamMax];
+  bf16* pb[kAdamMax];
+  bf16* pbt[kAdamMax];
+  bf16* pbp[kAdamMax];
+  bf16* pbtp[kAdamMax];
+};
+
+__global__ __launch_bounds__(256) void adam_multi_kernel(AdamTable T, float lr_t, float b1,
+                                                         float b2, float eps) {
+  const int k = blockIdx.y;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T.n[k]) return;
+  adam_elem(t, T.rows[k], T.cols[k], T.p[k], T.g[k], T.m[k], T.v[k], lr_t, b1, b2, eps,
+            T.pb[k], T.pbt[k], T.pbp[k], T.pbtp[k]);
+}
+
 
 // Features: x[0..100] = profile / max|profile|, x[101] = (d - dc) / ds, rest 0;
 // lmag = log(max|profile|).  With bounds, flag = 1 when any feature or lmag
@@ -341,11 +719,13 @@ __global__ __launch_bounds__(256) void gather_kernel(int Bn, int F, const int32_
   __shared__ __align__(16) bf16 tile[128 * 72];
   const int z = blockIdx.z, b0 = blockIdx.x * 64;
   const int32_t* id = idx + (int64_t)z * Bn;
-  for (int e = threadIdx.x; e < 64 * F; e += 256) {
-    const int bi = e / F, f = e % F;
-    const bf16 v = X[(int64_t)id[b0 + bi] * F + f];
-    X0[((int64_t)z * Bn + b0 + bi) * F + f] = v;
-    tile[f * 72 + bi] = v;
+  const int FC = F / 8;  // 16-byte chunks per row
+  for (int e = threadIdx.x; e < 64 * FC; e += 256) {
+    const int bi = e / FC, f = (e % FC) * 8;
+    const bf16x8 v = *reinterpret_cast<const bf16x8*>(&X[(int64_t)id[b0 + bi] * F + f]);
+    *reinterpret_cast<bf16x8*>(&X0[((int64_t)z * Bn + b0 + bi) * F + f]) = v;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) tile[(f + u) * 72 + bi] = v[u];
   }
   if (threadIdx.x < 64) t0[(int64_t)z * Bn + b0 + threadIdx.x] = t[id[b0 + threadIdx.x]];
   __syncthreads();
@@ -422,6 +802,85 @@ DN_EXPORT int dn_mlp_gemm(int epi, int nz, int M, int N, int K, int splits,
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
 
+// Fused forward (see mlp_fwd_kernel): tile = 128 or 64 rows per workgroup
+// (training: 64, the tile of dn_mlp_backward), n rows per member (a multiple
+// of tile), 128 input features, three hidden layers of 512, packed weights
+// Wp1..Wp3 with member strides sw1, sw23 (elements).  Inference: Z, HT, t,
+// dy, part null.  Training: Z[l] [z][n][512] float32, HT[l] [z][512][n]
+// BF16, t and dy [z][n] (stride sy), part [z][n/64][5][512] (slots 3, 4:
+// the tile's dw4 and db4 sums), all required.
+DN_EXPORT int dn_mlp_forward(int nz, int n, int tile, const void* X, int64_t sx,
+                             const void* Wp1, const void* Wp2, const void* Wp3, int64_t sw1,
+                             int64_t sw23, const float* b1, const float* b2, const float* b3,
+                             const void* w4, const float* b4, float* y, int64_t sy,
+                             float* Z1, float* Z2, float* Z3, void* HT1, void* HT2,
+                             void* HT3, const float* t, float* dy, float* part,
+                             hipStream_t stream) {
+  if (nz < 1 || (tile != 64 && tile != 128) || n < tile || n % tile || !X || !Wp1 || !Wp2 ||
+      !Wp3 || !b1 || !b2 || !b3 || !w4 || !b4 || !y || sy < n)
+    return DN_EINVAL;
+  const bool train = Z1 || Z2 || Z3 || HT1 || HT2 || HT3 || t || dy || part;
+  if (train && !(Z1 && Z2 && Z3 && HT1 && HT2 && HT3 && t && dy && part && tile == 64))
+    return DN_EINVAL;
+  if (nz > 65535 || n / tile > 0x7fffffff) return DN_ERANGE;
+  dn::FwdArgs F = {};
+  F.n = n; F.X = static_cast<const bf16*>(X); F.sx = sx;
+  F.Wp[0] = static_cast<const bf16*>(Wp1); F.Wp[1] = static_cast<const bf16*>(Wp2);
+  F.Wp[2] = static_cast<const bf16*>(Wp3);
+  F.sw[0] = sw1; F.sw[1] = sw23; F.sw[2] = sw23;
+  F.b[0] = b1; F.b[1] = b2; F.b[2] = b3;
+  F.w4 = static_cast<const bf16*>(w4); F.b4 = b4; F.y = y; F.sy = sy;
+  F.Z[0] = Z1; F.Z[1] = Z2; F.Z[2] = Z3;
+  F.HT[0] = static_cast<bf16*>(HT1); F.HT[1] = static_cast<bf16*>(HT2);
+  F.HT[2] = static_cast<bf16*>(HT3);
+  F.t = t; F.dy = dy; F.part = part;
+  const dim3 grid(n / tile, nz);
+  if (tile == 128) {
+    hipLaunchKernelGGL((dn::mlp_fwd_kernel<false, 4, 2>), grid, dim3(512), 0, stream, F);
+  } else {
+    if (train) hipLaunchKernelGGL((dn::mlp_fwd_kernel<true, 2, 4>), grid, dim3(256), 0, stream, F);
+    else hipLaunchKernelGGL((dn::mlp_fwd_kernel<false, 2, 4>), grid, dim3(256), 0, stream, F);
+  }
+  return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+}
+
+// Fused backward data products (see mlp_bwd_kernel), tile = 64 rows:
+// dy [z][n] (stride sy), w4 [z][512] BF16, Z1..Z3 [z][n][512], packed W2^T
+// and W3^T (member stride sw), dZ1T..dZ3T [z][512][n] BF16, part the
+// forward's partial buffer [z][n/tile][5][512] (slots 0..2 written here).
+DN_EXPORT int dn_mlp_backward(int nz, int n, int tile, const float* dy, int64_t sy,
+                              const void* w4, const float* Z1, const float* Z2,
+                              const float* Z3, const void* Wt2p, const void* Wt3p, int64_t sw,
+                              void* dZ1T, void* dZ2T, void* dZ3T, float* part,
+                              hipStream_t stream) {
+  if (nz < 1 || tile != 64 || n < tile || n % tile || !dy || sy < n || !w4 || !Z1 || !Z2 ||
+      !Z3 || !Wt2p || !Wt3p || !dZ1T || !dZ2T || !dZ3T || !part)
+    return DN_EINVAL;
+  if (nz > 65535) return DN_ERANGE;
+  dn::BwdArgs A = {};
+  A.n = n; A.dy = dy; A.sy = sy; A.w4 = static_cast<const bf16*>(w4);
+  A.Z[0] = Z1; A.Z[1] = Z2; A.Z[2] = Z3;
+  A.Wtp[0] = static_cast<const bf16*>(Wt2p); A.Wtp[1] = static_cast<const bf16*>(Wt3p);
+  A.sw = sw;
+  A.dZT[0] = static_cast<bf16*>(dZ1T); A.dZT[1] = static_cast<bf16*>(dZ2T);
+  A.dZT[2] = static_cast<bf16*>(dZ3T);
+  A.part = part;
+  hipLaunchKernelGGL((dn::mlp_bwd_kernel<2, 4>), dim3(n / tile, nz), dim3(256), 0, stream, A);
+  return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+}
+
+// Sums the per-tile partials [z][tiles][5][512] in tile order into the
+// gradients of b1, b2, b3, w4 ([z][512] each) and b4 ([z]).
+DN_EXPORT int dn_mlp_partials_reduce(int nz, int tiles, const float* part, float* gb1,
+                                     float* gb2, float* gb3, float* gw4, float* gb4,
+                                     hipStream_t stream) {
+  if (nz < 1 || tiles < 1 || !part || !gb1 || !gb2 || !gb3 || !gw4 || !gb4) return DN_EINVAL;
+  if (nz > 65535) return DN_ERANGE;
+  hipLaunchKernelGGL(dn::partials_reduce_kernel, dim3((dn::kParts * dn::kFH + 255) / 256, nz),
+                     dim3(256), 0, stream, tiles, part, gb1, gb2, gb3, gw4, gb4);
+  return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+}
+
 // y[z][b] = H3[z][b][:] . w4[z][:] + b4[z]; with t: dy = 2 (y - t) / Bn.
 DN_EXPORT int dn_mlp_out(int nz, int Bn, int Hd, const void* H3, int64_t sh, const void* w4,
                          int64_t sw, const float* b4, const float* t, int64_t st, float* y,
@@ -476,12 +935,46 @@ DN_EXPORT int dn_mlp_splitk_reduce(int nz, int64_t n, int splits, const float* P
 // bias-corrected step lr * sqrt(1 - b2^t) / (1 - b1^t).
 DN_EXPORT int dn_mlp_adam(int64_t n, int rows, int cols, float* p, const float* g, float* m,
                           float* v, float lr_t, float b1, float b2, float eps, void* pb,
-                          void* pbt, hipStream_t stream) {
-  if (n < 1 || rows < 1 || cols < 1 || n % ((int64_t)rows * cols) || !p || !g || !m || !v)
+                          void* pbt, void* pbp, void* pbtp, hipStream_t stream) {
+  if (n < 1 || rows < 1 || cols < 1 || n % ((int64_t)rows * cols) || !p || !g || !m || !v ||
+      (pbp && (rows % 32 || cols % 16)) || (pbtp && (cols % 32 || rows % 16)))
     return DN_EINVAL;
   hipLaunchKernelGGL(dn::adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
                      n, rows, cols, p, g, m, v, lr_t, b1, b2, eps, static_cast<bf16*>(pb),
-                     static_cast<bf16*>(pbt));
+                     static_cast<bf16*>(pbt), static_cast<bf16*>(pbp), static_cast<bf16*>(pbtp));
+  return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+}
+
+// Adam over `count` (<= 8) tensors in one launch; per tensor k the
+// arguments of dn_mlp_adam, passed as host arrays (n[k], rows[k], cols[k],
+// and the device addresses p[k], g[k], m[k], v[k], pb[k], pbt[k], pbp[k],
+// pbtp[k]; the copies may be 0).
+DN_EXPORT int dn_mlp_adam_multi(int count, const int64_t* n, const int* rows, const int* cols,
+                                const uint64_t* p, const uint64_t* g, const uint64_t* m,
+                                const uint64_t* v, const uint64_t* pb, const uint64_t* pbt,
+                                const uint64_t* pbp, const uint64_t* pbtp, float lr_t, float b1,
+                                float b2, float eps, hipStream_t stream) {
+  if (count < 1 || count > dn::kAdamMax || !n || !rows || !cols || !p || !g || !m || !v ||
+      !pb || !pbt || !pbp || !pbtp)
+    return DN_EINVAL;
+  dn::AdamTable T = {};
+  int64_t nmax = 0;
+  for (int k = 0; k < count; ++k) {
+    if (n[k] < 1 || rows[k] < 1 || cols[k] < 1 || n[k] % ((int64_t)rows[k] * cols[k]) ||
+        !p[k] || !g[k] || !m[k] || !v[k] || (pbp[k] && (rows[k] % 32 || cols[k] % 16)) ||
+        (pbtp[k] && (cols[k] % 32 || rows[k] % 16)))
+      return DN_EINVAL;
+    T.n[k] = n[k]; T.rows[k] = rows[k]; T.cols[k] = cols[k];
+    T.p[k] = reinterpret_cast<float*>(p[k]); T.g[k] = reinterpret_cast<const float*>(g[k]);
+    T.m[k] = reinterpret_cast<float*>(m[k]); T.v[k] = reinterpret_cast<float*>(v[k]);
+    T.pb[k] = reinterpret_cast<bf16*>(pb[k]); T.pbt[k] = reinterpret_cast<bf16*>(pbt[k]);
+    T.pbp[k] = reinterpret_cast<bf16*>(pbp[k]); T.pbtp[k] = reinterpret_cast<bf16*>(pbtp[k]);
+    nmax = n[k] > nmax ? n[k] : nmax;
+  }
+  const int64_t blocks = (nmax + 255) / 256;
+  if (blocks > 0x7fffffff) return DN_ERANGE;
+  hipLaunchKernelGGL(dn::adam_multi_kernel, dim3((unsigned)blocks, count), dim3(256), 0, stream,
+                     T, lr_t, b1, b2, eps);
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
 

@@ -14,7 +14,11 @@ current, the threshold of a profile scaled by c is the threshold divided by
 c, so the magnitude enters exactly through max|profile|.
 
 Members: 102(128) -> 512 -> 512 -> 512 -> 1, GELU, BF16 weights, float32
-accumulation, Adam on float32 masters; every kernel is in dn_mlp.hip.
+accumulation, Adam on float32 masters; every kernel is in dn_mlp.hip.  The
+forward (inference, and the forward of a training step) is one fused launch
+per 128-row tile and member (activations held in LDS, weights read in MFMA
+fragment order from a packed BF16 copy); the backward runs the layer
+products.
 Prediction falls back to bisection where a feature lies outside the range
 seen in training or where the ensemble's spread (max - min) / median exceeds
 ``spread`` (3 %).
@@ -71,7 +75,13 @@ class ThresholdSurrogate:
                    for k, v in self.params.items() if k[0] in "Ww"}
         self.bft = {k: torch.empty((nz, v.shape[2], v.shape[1]), dtype=_BF, device=dev)
                     for k, v in self.params.items() if k[0] == "W"}
+        # MFMA fragment order for the fused forward (W) and backward (W^T)
+        self.bfp = {k: torch.empty(v.shape, dtype=_BF, device=dev)
+                    for k, v in self.params.items() if k[0] == "W"}
+        self.bfpt = {k: torch.empty(v.shape, dtype=_BF, device=dev) for k in ("W2", "W3")
+                     for v in (self.params[k],)}
         self.t_adam = 0
+        self.tile = 64    # rows per workgroup of the fused inference forward
         self._refresh_bf16()
         self.ymean, self.ystd = 0.0, 1.0
         self.lo = self.hi = None
@@ -81,11 +91,22 @@ class ThresholdSurrogate:
     def _stream(self):
         return torch.cuda.current_stream(self.device).cuda_stream
 
+    @staticmethod
+    def pack(w):
+        """[nz][N][K] -> MFMA fragment order [nz][N/32][K/16][64 lanes][8]:
+        lane l of block (nb, ks) holds W[32 nb + l % 32][16 ks + 8 (l // 32) + e]."""
+        nz, N, K = w.shape
+        return (w.reshape(nz, N // 32, 32, K // 16, 2, 8).permute(0, 1, 3, 4, 2, 5)
+                .reshape(nz, N, K).contiguous())
+
     def _refresh_bf16(self):
         for k in self.bf:
             self.bf[k].copy_(self.params[k].to(_BF))
             if k in self.bft:
                 self.bft[k].copy_(self.params[k].transpose(1, 2).to(_BF))
+                self.bfp[k].copy_(self.pack(self.params[k].to(_BF)))
+                if k in self.bfpt:
+                    self.bfpt[k].copy_(self.pack(self.bft[k]))
 
     def _buf(self, name, shape, dtype):
         b = self._bufs.get(name)
@@ -120,13 +141,33 @@ class ThresholdSurrogate:
                                   _p(HT), ldht, sht, _p(Zp), ldzp, szp, self._stream())
         _lib.check(st, "mlp gemm")
 
-    # -- forward (inference) ---------------------------------------------------
+    # -- forward ----------------------------------------------------------------
+    def _fused_forward(self, X, sx, n, y, train=None, tile=None):
+        """One fused launch: X [n, 128] (sx = 0, shared) or [nz, n, 128]
+        (sx = n * 128); y [nz, n].  train = (Z1, Z2, Z3, HT1, HT2, HT3, t, dy,
+        part) with tile 64.  tile: rows per workgroup (128 or 64)."""
+        b, bf, bp = self.params, self.bf, self.bfp
+        tr = [_p(x) for x in train] if train is not None else [None] * 9
+        tile = tile or self.tile
+        st = self.lib.dn_mlp_forward(self.nz, n, tile, _p(X), sx, _p(bp["W1"]), _p(bp["W2"]),
+                                     _p(bp["W3"]), HID * F_PAD, HID * HID, _p(b["b1"]),
+                                     _p(b["b2"]), _p(b["b3"]), _p(bf["w4"]), _p(b["b4"]),
+                                     _p(y), n, *tr, self._stream())
+        _lib.check(st, "fused forward")
+
     def _forward_chunk(self, X, n):
         """Member predictions [nz, n] (standardised targets) for features X [n, 128]."""
+        y = self._buf("i_y", (self.nz, n), torch.float32)
+        self._fused_forward(X, 0, n, y)
+        return y
+
+    def _forward_chunk_layers(self, X, n):
+        """The same predictions through the layer products (dn_mlp_gemm +
+        dn_mlp_out): the unfused form of _forward_chunk."""
         nz = self.nz
         h1 = self._buf("i_h1", (nz, n, HID), _BF)
         h2 = self._buf("i_h2", (nz, n, HID), _BF)
-        y = self._buf("i_y", (nz, n), torch.float32)
+        y = self._buf("i_y2", (nz, n), torch.float32)
         b, bf = self.params, self.bf
         self._gemm(0, n, HID, F_PAD, X, F_PAD, 0, bf["W1"], F_PAD, HID * F_PAD, bias=b["b1"],
                    H=h1, ldh=HID, sh=n * HID)
@@ -191,9 +232,18 @@ class ThresholdSurrogate:
         self.X, self.t = X.contiguous(), t
         return int(X.shape[0])
 
-    def train_step(self, idx, B, lr, b1=0.9, b2=0.999, eps=1e-8, splits=8):
-        """One Adam step of every member on its minibatch idx[z] (int32 [nz, B])."""
+    def train_step(self, idx, B, lr, b1=0.9, b2=0.999, eps=1e-8, splits=8, fused=True):
+        """One Adam step of every member on its minibatch idx[z] (int32 [nz, B]).
+
+        fused: the forward and the backward data products run as one launch
+        each over 64-row tiles (dn_mlp_forward, dn_mlp_backward), with the
+        bias and output-weight gradients from per-tile partial sums; False
+        runs the same arithmetic as layer products (dn_mlp_gemm epilogues 0
+        and 1, dn_mlp_out, dn_mlp_out_bwd, dn_mlp_row_dot).  The weight
+        gradients are split-K products either way."""
         nz = self.nz
+        if fused and B % 64:
+            raise ValueError("the fused step needs a batch that is a multiple of 64")
         # weight-gradient products split the batch in multiples of the 64-wide K tile
         while splits > 1 and B % (64 * splits):
             splits //= 2
@@ -203,20 +253,69 @@ class ThresholdSurrogate:
         X0T = buf("x0t", (nz, F_PAD, B), _BF)
         t0 = buf("t0", (nz, B), torch.float32)
         Z = [buf(f"z{l}", (nz, B, HID), torch.float32) for l in (1, 2, 3)]
-        H = [buf(f"h{l}", (nz, B, HID), _BF) for l in (1, 2, 3)]
         HT = [buf(f"ht{l}", (nz, HID, B), _BF) for l in (1, 2, 3)]
-        dZ = buf("dz", (nz, B, HID), _BF)
-        dZT = buf("dzt", (nz, HID, B), _BF)
-        dZ2 = buf("dz2", (nz, B, HID), _BF)
-        dZ2T = buf("dz2t", (nz, HID, B), _BF)
+        dZT = [buf(f"dzt{l}", (nz, HID, B), _BF) for l in (1, 2, 3)]
         y = buf("y", (nz, B), torch.float32)
         dy = buf("dy", (nz, B), torch.float32)
-        part = buf("part", (nz * splits, HID, HID), torch.float32)
+        wpart = buf("wpart", (nz * splits, HID, HID), torch.float32)
         strm = self._stream()
         lib = self.lib
         _lib.check(lib.dn_mlp_gather(nz, B, F_PAD, _p(idx), _p(self.X), _p(self.t), _p(X0),
                                      _p(X0T), _p(t0), strm), "gather")
-        # forward
+        if fused:
+            tiles = B // 64
+            part = buf("part", (nz, tiles, 5, HID), torch.float32)
+            # forward: pre-activations, transposed activations, dy and the
+            # tiles' dw4 / db4 sums
+            self._fused_forward(X0, B * F_PAD, B, y, (*Z, *HT, t0, dy, part), tile=64)
+            # backward data products dZ3 -> dZ2 -> dZ1 and the tiles' bias sums
+            _lib.check(lib.dn_mlp_backward(nz, B, 64, _p(dy), B, _p(bf["w4"]), _p(Z[0]),
+                                           _p(Z[1]), _p(Z[2]), _p(self.bfpt["W2"]),
+                                           _p(self.bfpt["W3"]), HID * HID, _p(dZT[0]),
+                                           _p(dZT[1]), _p(dZT[2]), _p(part), strm), "backward")
+            _lib.check(lib.dn_mlp_partials_reduce(nz, tiles, _p(part), _p(G["b1"]), _p(G["b2"]),
+                                                  _p(G["b3"]), _p(G["w4"]), _p(G["b4"]), strm),
+                       "partials")
+        else:
+            self._layers_step(X0, t0, Z, HT, dZT, y, dy, B)
+        prevT = [X0T, HT[0], HT[1]]
+        kin = [F_PAD, HID, HID]
+        for l in (3, 2, 1):
+            K_in = kin[l - 1]
+            self._gemm(2, HID, K_in, B, dZT[l - 1], B, HID * B, prevT[l - 1], B, K_in * B,
+                       splits=splits, Z=wpart, ldz=K_in, sz=HID * K_in)
+            _lib.check(lib.dn_mlp_splitk_reduce(nz, HID * K_in, splits, _p(wpart),
+                                                _p(G[f"W{l}"]), strm), "reduce")
+        # Adam: every tensor in one launch
+        self.t_adam += 1
+        lr_t = lr * math.sqrt(1 - b2 ** self.t_adam) / (1 - b1 ** self.t_adam)
+        tab = self._adam_table()
+        _lib.check(lib.dn_mlp_adam_multi(len(P), *[a.ctypes.data for a in tab], lr_t, b1, b2,
+                                         eps, strm), "adam")
+        return y, dy
+
+    def _adam_table(self):
+        """Host arrays of dn_mlp_adam_multi (n, rows, cols, then the device
+        addresses of p, g, m, v and the four BF16 copies; 0 = none)."""
+        if getattr(self, "_tab", None) is None:
+            P = self.params
+            addr = lambda d, k: (d[k].data_ptr() if k in d else 0)
+            cols = [np.array([v.numel() for v in P.values()], np.int64),
+                    np.array([v.shape[1] for v in P.values()], np.int32),
+                    np.array([v.shape[2] for v in P.values()], np.int32)]
+            for d in (P, self.grads, self.m, self.v, self.bf, self.bft, self.bfp, self.bfpt):
+                cols.append(np.array([addr(d, k) for k in P], np.uint64))
+            self._tab = cols
+        return self._tab
+
+    def _layers_step(self, X0, t0, Z, HT, dZT, y, dy, B):
+        """Forward and backward data products of a training step as layer
+        products; fills Z, HT, y, dy, dZT and the bias / output gradients."""
+        nz, lib, strm = self.nz, self.lib, self._stream()
+        bf, bft, P, G = self.bf, self.bft, self.params, self.grads
+        buf = self._buf
+        H = [buf(f"h{l}", (nz, B, HID), _BF) for l in (1, 2, 3)]
+        dZ = [buf(f"dz{l}", (nz, B, HID), _BF) for l in (1, 2, 3)]
         ins = [(X0, F_PAD), (H[0], HID), (H[1], HID)]
         for l in range(3):
             A, K = ins[l]
@@ -225,39 +324,19 @@ class ThresholdSurrogate:
                        sh=B * HID, HT=HT[l], ldht=B, sht=HID * B)
         _lib.check(lib.dn_mlp_out(nz, B, HID, _p(H[2]), B * HID, _p(bf["w4"]), HID,
                                   _p(P["b4"]), _p(t0), B, _p(y), _p(dy), B, strm), "out")
-        # backward
         _lib.check(lib.dn_mlp_out_bwd(nz, B, HID, _p(dy), B, _p(bf["w4"]), HID, _p(Z[2]),
-                                      B * HID, _p(dZ), _p(dZT), B * HID, strm), "out bwd")
+                                      B * HID, _p(dZ[2]), _p(dZT[2]), B * HID, strm), "out bwd")
         _lib.check(lib.dn_mlp_row_dot(nz, HID, B, 1, _p(HT[2]), HID * B, _p(dy), B,
                                       _p(G["w4"]), HID, strm), "dw4")
         _lib.check(lib.dn_mlp_row_dot(nz, 1, B, 0, _p(dy), B, None, 0, _p(G["b4"]), 1, strm),
                    "db4")
-        prevT = [X0T, HT[0], HT[1]]
-        kin = [F_PAD, HID, HID]
-        cur, curT = dZ, dZT
         for l in (3, 2, 1):
-            K_in = kin[l - 1]
-            _lib.check(lib.dn_mlp_row_dot(nz, HID, B, 1, _p(curT), HID * B, None, 0,
+            _lib.check(lib.dn_mlp_row_dot(nz, HID, B, 1, _p(dZT[l - 1]), HID * B, None, 0,
                                           _p(G[f"b{l}"]), HID, strm), "db")
-            self._gemm(2, HID, K_in, B, curT, B, HID * B, prevT[l - 1], B, K_in * B,
-                       splits=splits, Z=part, ldz=K_in, sz=HID * K_in)
-            _lib.check(lib.dn_mlp_splitk_reduce(nz, HID * K_in, splits, _p(part),
-                                                _p(G[f"W{l}"]), strm), "reduce")
             if l > 1:
-                nxt, nxtT = (dZ2, dZ2T) if cur is dZ else (dZ, dZT)
-                self._gemm(1, B, HID, HID, cur, HID, B * HID, bft[f"W{l}"], HID, HID * HID,
-                           Zp=Z[l - 2], ldzp=HID, szp=B * HID, H=nxt, ldh=HID, sh=B * HID,
-                           HT=nxtT, ldht=B, sht=HID * B)
-                cur, curT = nxt, nxtT
-        # Adam
-        self.t_adam += 1
-        lr_t = lr * math.sqrt(1 - b2 ** self.t_adam) / (1 - b1 ** self.t_adam)
-        for k, p in P.items():
-            rows, cols = p.shape[1], p.shape[2]
-            _lib.check(lib.dn_mlp_adam(p.numel(), rows, cols, _p(p), _p(G[k]), _p(self.m[k]),
-                                       _p(self.v[k]), lr_t, b1, b2, eps,
-                                       _p(bf.get(k)), _p(bft.get(k)), strm), "adam")
-        return y, dy
+                self._gemm(1, B, HID, HID, dZ[l - 1], HID, B * HID, bft[f"W{l}"], HID,
+                           HID * HID, Zp=Z[l - 2], ldzp=HID, szp=B * HID, H=dZ[l - 2], ldh=HID,
+                           sh=B * HID, HT=dZT[l - 2], ldht=B, sht=HID * B)
 
     def fit(self, profile, diam, thr, epochs: int = 20, batch: int = 8192,
             lr: float = 2e-3, lr_min: float = 2e-5, seed: int = 1, log=None):

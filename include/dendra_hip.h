@@ -679,10 +679,52 @@ int dn_mlp_row_dot(int nz, int rows, int cols, int x_bf16, const void* X, int64_
 int dn_mlp_splitk_reduce(int nz, int64_t n, int splits, const float* P, float* out,
                          hipStream_t stream);
 /* One Adam step on float32 masters with bias-corrected step lr_t; refreshes
- * the BF16 copy pb and the transposed BF16 copy pbt (either may be NULL). */
+ * the BF16 copy pb, the transposed BF16 copy pbt, and the copies in MFMA
+ * fragment order of W (pbp, rows % 32 == 0, cols % 16 == 0; read by
+ * dn_mlp_forward) and of W^T (pbtp, cols % 32 == 0, rows % 16 == 0; read by
+ * dn_mlp_backward); any of the four may be NULL. */
 int dn_mlp_adam(int64_t n, int rows, int cols, float* p, const float* g, float* m, float* v,
-                float lr_t, float b1, float b2, float eps, void* pb, void* pbt,
-                hipStream_t stream);
+                float lr_t, float b1, float b2, float eps, void* pb, void* pbt, void* pbp,
+                void* pbtp, hipStream_t stream);
+/* dn_mlp_adam over `count` <= 8 tensors in one launch: per tensor k the
+ * arguments of dn_mlp_adam as host arrays (n, rows, cols; device addresses
+ * p, g, m, v and the copies pb, pbt, pbp, pbtp, 0 for none). */
+int dn_mlp_adam_multi(int count, const int64_t* n, const int* rows, const int* cols,
+                      const uint64_t* p, const uint64_t* g, const uint64_t* m,
+                      const uint64_t* v, const uint64_t* pb, const uint64_t* pbt,
+                      const uint64_t* pbp, const uint64_t* pbtp, float lr_t, float b1,
+                      float b2, float eps, hipStream_t stream);
+/* The whole forward in one launch per 128-row tile and member: x[128] ->
+ * 512 -> 512 -> 512 (GELU) -> 1, activations kept in LDS, weights Wp1..Wp3
+ * in MFMA fragment order ([z][N/32][K/16][64][8]: element e of lane l in
+ * block (nb, ks) is W[32 nb + l%32][16 ks + 8 (l/32) + e]; member strides
+ * sw1, sw23 in elements).  X [z][n][128] (sx = 0: one X for every member),
+ * n a multiple of 128; y [z][n] (stride sy).  Inference: Z*, HT*, t, dy
+ * NULL; tile = 128 or 64 rows per workgroup.  Training (tile 64): also Z_l
+ * [z][n][512] float32 pre-activations, HT_l [z][512][n] BF16 activations
+ * and dy = 2 (y - t) / n, as the layer products dn_mlp_gemm (epi 0) +
+ * dn_mlp_out produce them, and into part [z][n/64][5][512] the tile sums
+ * of dw4 = H3^T dy (slot 3) and db4 = sum dy (slot 4, element 0). */
+int dn_mlp_forward(int nz, int n, int tile, const void* X, int64_t sx, const void* Wp1,
+                   const void* Wp2, const void* Wp3, int64_t sw1, int64_t sw23,
+                   const float* b1, const float* b2, const float* b3, const void* w4,
+                   const float* b4, float* y, int64_t sy, float* Z1, float* Z2, float* Z3,
+                   void* HT1, void* HT2, void* HT3, const float* t, float* dy, float* part,
+                   hipStream_t stream);
+/* The backward data products of a training step in one launch per 64-row
+ * tile (tile must be 64): dZ3 = dy w4^T * gelu'(Z3), dZ2 = (dZ3 W3) *
+ * gelu'(Z2), dZ1 = (dZ2 W2) * gelu'(Z1) in BF16 as dn_mlp_out_bwd and
+ * dn_mlp_gemm (epi 1) produce them, each kept in LDS for the next product;
+ * W2^T, W3^T packed in MFMA fragment order (member stride sw).  Writes
+ * dZ_l^T [z][512][n] and the tile column sums of dZ_l into part slots 0..2. */
+int dn_mlp_backward(int nz, int n, int tile, const float* dy, int64_t sy, const void* w4,
+                    const float* Z1, const float* Z2, const float* Z3, const void* Wt2p,
+                    const void* Wt3p, int64_t sw, void* dZ1T, void* dZ2T, void* dZ3T,
+                    float* part, hipStream_t stream);
+/* Gradients of b1, b2, b3, w4 ([z][512]) and b4 ([z]) from the tile
+ * partials [z][tiles][5][512], summed over the tiles in order. */
+int dn_mlp_partials_reduce(int nz, int tiles, const float* part, float* gb1, float* gb2,
+                           float* gb3, float* gw4, float* gb4, hipStream_t stream);
 /* Features x[0..P-1] = profile / max|profile|, x[P] = (diam - dc) / ds, zero
  * padding to ldx; lmag = log max|profile|; with lo/hi (P+2 bounds, the last
  * for lmag) flag = 1 where a value is outside [lo - margin, hi + margin]. */

@@ -95,6 +95,83 @@ def _small_surrogate(n=2048, nz=2, seed=3):
     return sur, prof, diam
 
 
+@pytest.mark.parametrize("tile", [128, 64])
+def test_fused_forward_equals_layer_products(tile):
+    """dn_mlp_forward (one launch, activations in LDS, packed weights) against
+    the layer products dn_mlp_gemm + dn_mlp_out on the same inputs: the
+    pre-activations, BF16 activations, y and dy are the same sums in the same
+    order (bitwise equal)."""
+    from dendra_amd.surrogate import F_PAD, HID
+
+    sur, prof, diam = _small_surrogate(n=2048)
+    sur.tile = tile
+    BF_ = torch.bfloat16
+    # inference: one feature matrix for both members
+    X, _, _ = sur.featurize(prof, diam)
+    yf = sur._forward_chunk(X, 2048).clone()
+    yl = sur._forward_chunk_layers(X, 2048).clone()
+    assert torch.equal(yf, yl)
+    # training forward: per-member minibatches
+    nz, B = sur.nz, 384
+    idx = torch.stack([torch.randperm(2048, device="cuda")[:B] for _ in range(nz)])
+    X0 = sur.X[idx].contiguous()
+    t0 = sur.t[idx].contiguous()
+    Z = [torch.empty((nz, B, HID), device="cuda") for _ in range(3)]
+    HT = [torch.empty((nz, HID, B), dtype=BF_, device="cuda") for _ in range(3)]
+    y = torch.empty((nz, B), device="cuda")
+    dy = torch.empty((nz, B), device="cuda")
+    part = torch.empty((nz, B // 64, 5, HID), device="cuda")
+    sur._fused_forward(X0, B * F_PAD, B, y, (*Z, *HT, t0, dy, part), tile=64)
+    Zr = [torch.empty_like(z) for z in Z]
+    Hr = [torch.empty((nz, B, HID), dtype=BF_, device="cuda") for _ in range(3)]
+    HTr = [torch.empty_like(h) for h in HT]
+    ins = [(X0, F_PAD), (Hr[0], HID), (Hr[1], HID)]
+    for l in range(3):
+        A, K = ins[l]
+        sur._gemm(0, B, HID, K, A, K, B * K, sur.bf[f"W{l + 1}"], K, HID * K,
+                  bias=sur.params[f"b{l + 1}"], Z=Zr[l], ldz=HID, sz=B * HID, H=Hr[l],
+                  ldh=HID, sh=B * HID, HT=HTr[l], ldht=B, sht=HID * B)
+    yr = torch.empty_like(y)
+    dyr = torch.empty_like(dy)
+    from dendra_amd import _lib
+
+    p = _lib.ptr
+    _lib.check(sur.lib.dn_mlp_out(nz, B, HID, p(Hr[2]), B * HID, p(sur.bf["w4"]), HID,
+                                  p(sur.params["b4"]), p(t0), B, p(yr), p(dyr), B,
+                                  torch.cuda.current_stream().cuda_stream), "out")
+    for l in range(3):
+        assert torch.equal(Z[l], Zr[l]), l
+        assert torch.equal(HT[l], HTr[l]), l
+    assert torch.equal(y, yr) and torch.equal(dy, dyr)
+
+
+def test_fused_training_step_equals_layer_products():
+    """train_step(fused=True) (fused forward, fused backward data products,
+    tile partial sums) against train_step(fused=False) (layer products) on
+    the same minibatch: dy, the backward signals dZ_l^T and the weight
+    gradients are bitwise equal (same MFMA sums, same BF16 rounding); the
+    bias and output-weight gradients differ only in summation order."""
+    sur, _, _ = _small_surrogate()
+    B = 512
+    idx = torch.stack([torch.randperm(2048, device="cuda")[:B] for _ in range(2)]).to(
+        torch.int32).contiguous()
+    got = {}
+    for fused in (True, False):
+        sur.train_step(idx, B, lr=0.0, fused=fused)
+        got[fused] = ({k: v.clone() for k, v in sur.grads.items()},
+                      [sur._bufs[f"dzt{l}"].clone() for l in (1, 2, 3)],
+                      sur._bufs["dy"].clone())
+    (ga, da, ya), (gb, db, yb) = got[True], got[False]
+    for l in range(3):
+        assert torch.equal(da[l], db[l]), l
+    for k in ("W1", "W2", "W3"):
+        assert torch.equal(ga[k], gb[k]), k
+    for k in ("b1", "b2", "b3", "w4", "b4"):
+        err = _rel(ga[k].double().cpu().numpy(), gb[k].double().cpu().numpy())
+        assert err < 1e-5, (k, err)
+    assert torch.equal(ya, yb)
+
+
 def test_training_step_gradients_match_oracle():
     sur, _, _ = _small_surrogate()
     B = 256
@@ -134,6 +211,10 @@ def test_adam_step_matches_oracle():
                           mlp_ref.bf16(sur.params["W2"].double().cpu().numpy()))
     assert np.array_equal(sur.bft["W2"].double().cpu().numpy(),
                           mlp_ref.bf16(sur.params["W2"].double().cpu().numpy()).transpose(0, 2, 1))
+    for k in ("W1", "W2", "W3"):  # the packed copies the fused kernels read
+        assert torch.equal(sur.bfp[k], sur.pack(sur.bf[k])), k
+    for k in ("W2", "W3"):
+        assert torch.equal(sur.bfpt[k], sur.pack(sur.bft[k])), k
 
 
 def test_ensemble_prediction_matches_oracle():

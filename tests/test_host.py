@@ -325,7 +325,37 @@ def test_entry_points_reject_invalid_arguments_before_any_launch():
                               None) == -1
     assert lib.dn_mlp_gather(1, 64, 129, fake, fake, fake, fake, fake, fake, None) == -1
     assert lib.dn_mlp_adam(10, 3, 3, fake, fake, fake, fake, 1e-3, 0.9, 0.999, 1e-8, None,
-                           None, None) == -1
+                           None, None, None, None) == -1
+    # packed copies need rows % 32 == 0 and cols % 16 == 0 (W^T: the reverse)
+    assert lib.dn_mlp_adam(48 * 16, 48, 16, fake, fake, fake, fake, 1e-3, 0.9, 0.999, 1e-8,
+                           None, None, fake, None, None) == -1
+    assert lib.dn_mlp_adam(64 * 48, 64, 48, fake, fake, fake, fake, 1e-3, 0.9, 0.999, 1e-8,
+                           None, None, None, fake, None) == -1
+    # fused forward / backward: tile 64 or 128; training needs tile 64 and
+    # every training output; the backward takes tile 64 only
+    fw = [1, 256, 96, fake, 0, fake, fake, fake, 1, 1, fake, fake, fake, fake, fake, fake, 256]
+    assert lib.dn_mlp_forward(*fw, *[None] * 9, None) == -1
+    fw[2] = 128
+    assert lib.dn_mlp_forward(*fw, *[fake] * 9, None) == -1
+    fw[2] = 64
+    assert lib.dn_mlp_forward(*fw, *[fake] * 8, None, None) == -1
+    assert lib.dn_mlp_backward(1, 256, 128, fake, 256, fake, fake, fake, fake, fake, fake, 1,
+                               fake, fake, fake, fake, None) == -1
+    assert lib.dn_mlp_backward(1, 96, 64, fake, 96, fake, fake, fake, fake, fake, fake, 1,
+                               fake, fake, fake, fake, None) == -1
+    assert lib.dn_mlp_partials_reduce(1, 0, fake, fake, fake, fake, fake, fake, None) == -1
+    n = np.array([10], np.int64)
+    rc = np.array([2, 5], np.int32)
+    a = np.array([fake], np.uint64)
+    z = np.array([0], np.uint64)
+    assert lib.dn_mlp_adam_multi(9, n.ctypes.data, rc.ctypes.data, rc.ctypes.data,
+                                 *[a.ctypes.data] * 4, *[z.ctypes.data] * 4, 1e-3, 0.9, 0.999,
+                                 1e-8, None) == -1
+    # rows * cols must divide n
+    n7 = np.array([7], np.int64)
+    assert lib.dn_mlp_adam_multi(1, n7.ctypes.data, rc.ctypes.data, rc[1:].ctypes.data,
+                                 *[a.ctypes.data] * 4, *[z.ctypes.data] * 4, 1e-3, 0.9, 0.999,
+                                 1e-8, None) == -1
     assert lib.dn_mlp_ensemble(10, 9, fake, 10, fake, 0.0, 1.0, 0.03, fake, fake,
                                None) == -1
 
