@@ -636,6 +636,16 @@ def operator_bench(dev):
 
     # batched tridiagonal drop-ins (thomas_solve_cuda_t / pcr_solve_cuda_t):
     # 20 B per row (read a, b, c, d; write x)
+    # many very short systems: the LDS-staged lane-per-system sweep (auto)
+    B, K = 262144, 32
+    a, c = u((B, K - 1), -0.4, -0.1), u((B, K - 1), -0.4, -0.1)
+    b, d = u((B, K), 1.0, 2.0), u((B, K), -1.0, 1.0)
+    with torch.no_grad():
+        line = {"systems": B, "rows": K}
+        for meth in ("thomas", "pcr", "auto"):
+            ms = _time_op(lambda: solvers.solve_tridiagonal(a, b, c, d, meth))
+            line[meth] = _rate(ms, 20.0 * B * K)
+    res["tridiagonal_very_short"] = line
     for name, (B, K) in (("short", (65536, 201)), ("long", (2048, 4001))):
         a, c = u((B, K - 1), -0.4, -0.1), u((B, K - 1), -0.4, -0.1)
         b, d = u((B, K), 1.0, 2.0), u((B, K), -1.0, 1.0)

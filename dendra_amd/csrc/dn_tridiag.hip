@@ -8,10 +8,12 @@
 // ThomasSolve.backward (t_kernel_thomas.py:147-189) is the same call with
 // a and c exchanged.
 //
-// Two kernels:
-//  * thomas_lane: one lane per system, sequential Thomas sweep.  Best for
-//    many short systems.  Its elimination workspace lives in x (dp) and in a
-//    caller scratch of B*K values (cp).
+// Kernels:
+//  * thomas_lds: one lane per system for short systems (K <= 32 in float32,
+//    16 in float64), coefficients staged in LDS by coalesced loads, the
+//    sweep in registers.  Best for many short systems; no workspace.
+//  * thomas_lane: one lane per system for longer systems; its elimination
+//    workspace lives in x (dp) and in a caller scratch of B*K values (cp).
 //  * wave_hybrid: one 64-lane wavefront per system.  Each lane eliminates a
 //    contiguous chunk of nr rows (local Thomas with a spike column), the 64
 //    two-row interfaces are reduced to one row per lane by a cyclic-reduction
@@ -53,7 +55,121 @@ __global__ __launch_bounds__(64) void tridiag_wave_kernel(
   for (int64_t j = lane; j < K; j += 64) xr[j] = sd[j];
 }
 
-// Lane-per-system Thomas: restates the sweep of tridiag/pcr.py:125-183.
+// Lane-per-system Thomas for short systems (K <= KMAX) with the
+// coefficients staged in LDS: the 256 systems of a workgroup are contiguous
+// in every array, so each array is read into LDS with consecutive lanes on
+// consecutive elements (coalesced), each lane takes its system's row from
+// LDS (odd row pitch: consecutive lanes land on different banks) into
+// registers, sweeps it there (the sweep of tridiag/pcr.py:125-183: forward
+// elimination with c' and d' held in registers, then back-substitution),
+// and the solution leaves through LDS by coalesced stores.  No workspace.
+template <typename T, int KMAX>
+__global__ __launch_bounds__(256) void tridiag_thomas_lds_kernel(
+    const T* __restrict__ a, const T* __restrict__ b, const T* __restrict__ c,
+    const T* __restrict__ d, T* __restrict__ x, int64_t B, int K) {
+  __shared__ T st[256 * (KMAX + 1)];
+  const int tid = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * 256;
+  const int ns = (int)(B - r0 < 256 ? B - r0 : 256);  // systems in this workgroup
+  const int P = K | 1;                                // odd row pitch >= K
+  // coalesced copy of n values per system (first system r0) into LDS rows:
+  // element e = tid + 256 q is value i of system s, stepped without
+  // division; eight loads are in flight before their LDS stores
+  auto stage = [&](const T* src, int n) {
+    const int tot = ns * n;
+    const T* g = src + r0 * n;
+    const int ds = 256 / n, di = 256 % n;
+    int sy = tid / n, i = tid % n;
+    constexpr int U = 8;
+    for (int e0 = tid; e0 < tot; e0 += 256 * U) {
+      T v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + 256 * u;
+        v[u] = e < tot ? g[e] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (e0 + 256 * u < tot) st[sy * P + i] = v[u];
+        sy += ds;
+        i += di;
+        if (i >= n) {
+          i -= n;
+          ++sy;
+        }
+      }
+    }
+  };
+  T ra[KMAX], rb[KMAX], rc[KMAX], rd[KMAX];
+  const T* row = st + tid * P;
+  stage(b, K);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < KMAX; ++i) rb[i] = (tid < ns && i < K) ? row[i] : T(1);
+  __syncthreads();
+  if (K > 1) {
+    stage(a, K - 1);  // a[i-1] couples row i to row i-1
+    __syncthreads();
+#pragma unroll
+    for (int i = 1; i < KMAX; ++i) ra[i] = (tid < ns && i < K) ? row[i - 1] : T(0);
+    __syncthreads();
+    stage(c, K - 1);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < KMAX - 1; ++i) rc[i] = (tid < ns && i + 1 < K) ? row[i] : T(0);
+    __syncthreads();
+  } else {
+#pragma unroll
+    for (int i = 0; i < KMAX; ++i) ra[i] = rc[i] = T(0);
+  }
+  ra[0] = T(0);
+  rc[KMAX - 1] = T(0);
+  stage(d, K);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < KMAX; ++i) rd[i] = (tid < ns && i < K) ? row[i] : T(0);
+  // forward elimination: rc <- c', rd <- d'
+  {
+    const T den = rb[0];
+    rc[0] = rc[0] / den;
+    rd[0] = rd[0] / den;
+  }
+#pragma unroll
+  for (int i = 1; i < KMAX; ++i) {
+    if (i < K) {
+      const T den = rb[i] - ra[i] * rc[i - 1];
+      if (i < K - 1) rc[i] = rc[i] / den;
+      rd[i] = (rd[i] - ra[i] * rd[i - 1]) / den;
+    }
+  }
+  // back-substitution
+#pragma unroll
+  for (int i = KMAX - 2; i >= 0; --i)
+    if (i < K - 1) rd[i] = rd[i] - rc[i] * rd[i + 1];
+  __syncthreads();  // every lane has read its d row
+#pragma unroll
+  for (int i = 0; i < KMAX; ++i)
+    if (tid < ns && i < K) st[tid * P + i] = rd[i];
+  __syncthreads();
+  T* g = x + r0 * K;
+  const int ds = 256 / K, di = 256 % K;
+  int sy = tid / K, i = tid % K;
+  for (int e = tid; e < ns * K; e += 256) {
+    g[e] = st[sy * P + i];
+    sy += ds;
+    i += di;
+    if (i >= K) {
+      i -= K;
+      ++sy;
+    }
+  }
+}
+
+// Largest K of the LDS-staged lane kernel (registers: four rows of KMAX).
+template <typename T> constexpr int thomas_lds_kmax() { return sizeof(T) == 4 ? 32 : 16; }
+
+// Lane-per-system Thomas for longer systems (global workspace): restates the
+// sweep of tridiag/pcr.py:125-183.
 template <typename T>
 __global__ __launch_bounds__(256) void tridiag_thomas_lane_kernel(
     const T* __restrict__ a, const T* __restrict__ b, const T* __restrict__ c,
@@ -96,10 +212,24 @@ static int tridiag_launch(const T* a, const T* b, const T* c, const T* d, T* x,
   if (K > 1 && (!a || !c)) return DN_EINVAL;
   if (method == 0) method = (K <= 32 && B >= 4096) ? 1 : 2;
   if (method == 1) {
-    if (!scratch || scratch_elems < B * K) return DN_ERANGE;
-    int blocks = (int)((B + 255) / 256);
-    hipLaunchKernelGGL(tridiag_thomas_lane_kernel<T>, dim3(blocks), dim3(256), 0,
-                       stream, a, b, c, d, x, B, K, scratch);
+    const int64_t blocks = (B + 255) / 256;
+    if (blocks > 0x7fffffff) return DN_ERANGE;
+    if (K <= thomas_lds_kmax<T>()) {
+      if (K <= 8)
+        hipLaunchKernelGGL((tridiag_thomas_lds_kernel<T, 8>), dim3((unsigned)blocks),
+                           dim3(256), 0, stream, a, b, c, d, x, B, (int)K);
+      else if (K <= 16)
+        hipLaunchKernelGGL((tridiag_thomas_lds_kernel<T, 16>), dim3((unsigned)blocks),
+                           dim3(256), 0, stream, a, b, c, d, x, B, (int)K);
+      else
+        hipLaunchKernelGGL((tridiag_thomas_lds_kernel<T, thomas_lds_kmax<T>()>),
+                           dim3((unsigned)blocks), dim3(256), 0, stream, a, b, c, d, x, B,
+                           (int)K);
+    } else {
+      if (!scratch || scratch_elems < B * K) return DN_ERANGE;
+      hipLaunchKernelGGL(tridiag_thomas_lane_kernel<T>, dim3((unsigned)blocks), dim3(256), 0,
+                         stream, a, b, c, d, x, B, K, scratch);
+    }
   } else {
     int nr = (int)((K + 63) / 64);
     int64_t KP = 64 * (int64_t)nr;
@@ -120,7 +250,7 @@ static int tridiag_launch(const T* a, const T* b, const T* c, const T* d, T* x,
 DN_EXPORT int64_t dn_tridiag_scratch_elems(int64_t B, int64_t K, int method,
                                            int elem_bytes) {
   if (method == 0) method = (K <= 32 && B >= 4096) ? 1 : 2;
-  if (method == 1) return B * K;
+  if (method == 1) return K <= (elem_bytes == 4 ? 32 : 16) ? 0 : B * K;
   int64_t KP = 64 * ((K + 63) / 64);
   return (size_t)4 * KP * elem_bytes <= 64 * 1024 ? 0 : B * 4 * KP;
 }

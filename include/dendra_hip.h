@@ -34,7 +34,9 @@ extern "C" {
  * (dendra/models/integrators/triton/t_kernel_thomas.py:83-130,218-232) and
  * pcr_solve_cuda_t (triton/pcr_kernel_thomas.py:99-195,224-226).
  * a, c: (B, K-1); b, d, x: (B, K), row-major.  method: 0 auto, 1 Thomas
- * (lane per system), 2 wavefront hybrid PCR.  The adjoint solve of
+ * (lane per system; K <= 32 in float32 or 16 in float64 stages the
+ * coefficients in LDS and needs no scratch), 2 wavefront hybrid PCR.  The
+ * adjoint solve of
  * ThomasSolve.backward (t_kernel_thomas.py:147-189) is the same call with a
  * and c exchanged.  `scratch` holds dn_tridiag_scratch_elems() values (may
  * be NULL when that is 0). */

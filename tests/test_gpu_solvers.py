@@ -59,6 +59,27 @@ def test_tridiagonal_matches_oracle(orc, method, dtype):
         assert _backward_error(a, b, c, d, got) <= berr
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_lane_thomas_lds_many_short_systems(orc, dtype):
+    """The LDS-staged lane-per-system sweep (K <= 32 in float32, 16 in
+    float64): several workgroups of 256 systems and a partial last one, every
+    row count up to the limit (odd and even row pitches), and the automatic
+    choice for >= 4096 systems of <= 32 rows."""
+    from dendra_amd.solvers import solve_tridiagonal
+
+    rtol, atol, berr = TOL[dtype]
+    kmax = 32 if dtype == torch.float32 else 16
+    cases = [(785, K) for K in range(1, kmax + 1)] + [(4100, kmax)]
+    for seed, (B, K) in enumerate(cases):
+        a, b, c, d = _bands(100 + seed, B, K)
+        want = orc.thomas(a, b, c, d)
+        t = [torch.tensor(x, dtype=dtype, device="cuda") for x in (a, b, c, d)]
+        got = solve_tridiagonal(*t, method="thomas" if B < 4096 else "auto")
+        got = got.double().cpu().numpy()
+        np.testing.assert_allclose(got, want, rtol=rtol, atol=atol, err_msg=f"K={K} B={B}")
+        assert _backward_error(a[:64], b[:64], c[:64], d[:64], got[:64]) <= berr
+
+
 def test_thomas_known_answer():
     """3x3 system of tests/test_voltage_solvers.py:98-128."""
     from dendra_amd.solvers import thomas_solve

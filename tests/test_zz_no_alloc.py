@@ -40,7 +40,7 @@ def _no_alloc(lib, p, a, b, c, d, B, K, method):
     return x
 
 
-@pytest.mark.parametrize("method,shapes", [(1, [(65536, 32), (1000, 7)]),
+@pytest.mark.parametrize("method,shapes", [(1, [(16384, 64), (1000, 40)]),
                                            (2, [(60, 8000), (9, 5000)])])
 def test_no_alloc_solve_equals_scratch_solve(method, shapes):
     from dendra_amd import _lib
@@ -67,7 +67,7 @@ def test_no_alloc_refuses_oversize_and_runs_without_scratch():
 
     lib, p = _lib.load(), _lib.ptr
     cap = lib.dn_tridiag_noalloc_max_scratch_elems()
-    B, K = cap // 32 + 1, 32
+    B, K = cap // 64 + 1, 64
     assert lib.dn_tridiag_scratch_elems(B, K, 1, 4) > cap
     st = lib.dn_tridiag_solve_noalloc_f32(None, None, None, None, None, B, K, 1,
                                           torch.cuda.current_stream().cuda_stream)
@@ -77,5 +77,12 @@ def test_no_alloc_refuses_oversize_and_runs_without_scratch():
     assert lib.dn_tridiag_scratch_elems(300, 201, 2, 4) == 0
     got = _no_alloc(lib, p, a, b, c, d, 300, 201, 2)
     want = _with_scratch(lib, p, a, b, c, d, 300, 201, 2)
+    torch.cuda.synchronize()
+    assert torch.equal(got, want)
+    # lane-per-system sweeps of <= 32 rows stage in LDS: no workspace either
+    a, b, c, d = _system(rs, 70000, 32)
+    assert lib.dn_tridiag_scratch_elems(70000, 32, 1, 4) == 0
+    got = _no_alloc(lib, p, a, b, c, d, 70000, 32, 1)
+    want = _with_scratch(lib, p, a, b, c, d, 70000, 32, 1)
     torch.cuda.synchronize()
     assert torch.equal(got, want)
