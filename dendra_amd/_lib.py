@@ -84,10 +84,10 @@ u64 = C.c_uint64
 _SIGNATURES["dn_net_run_f32"] = (
     i32,
     [i32, i32] + [p] * 12 + [f32, p, p, p, p, i32, p, f32, u64, u32, f64]
-    + [p] * 6 + [i32] + [p] * 6 + [p, p, i64, p, i64, i64, i32, p],
+    + [p] * 7 + [i32] + [p] * 7 + [p, p, i64, p, i64, i64, i32, p],
 )
 _SIGNATURES["dn_net_deliver_tape_f32"] = (
-    i32, [i32, i32] + [p] * 6 + [i32] + [p] * 8 + [i64, p],
+    i32, [i32, i32] + [p] * 7 + [i32] + [p] * 9 + [i64, p],
 )
 
 # n_tasks, rcap, waves, task_nr, task_p, gl, grp, scale, sna, sk, act, n_knots,

@@ -14,12 +14,14 @@
 //   3. the populations advance: synapse decay (cnexp exact exponential),
 //      HH gates, currents including i_syn = (B - A)(v - e) (nA), solve.
 //
-// Delivery is O(active events): a spiking source marks its targets
-// (by-source CSR), and one wavefront per marked target scans that target's
-// contiguous incoming segment (connections are numbered target-major) in
-// connection order against the step's spike bitmask; one lane then adds the
-// hits sequentially, reproducing the reference's ordered float32 scatter_add
-// bit for bit.  All per-step state (cur slot, step index, list lengths) lives
+// Delivery is O(events): a spiking source walks its connections (by-source
+// CSR) and files each connection index at its target (a per-target hit list
+// of up to kHitMax entries, slots from a per-target counter); one thread per
+// hit target then adds its hits in connection order (the list sorted), which
+// is the order of the reference's float32 scatter_add, bit for bit.  A
+// target with more hits in one step than the list holds scans its incoming
+// segment (connections are numbered target-major) against the step's spike
+// bitmask instead.  All per-step state (cur slot, step index, list lengths) lives
 // in device memory, so a sequence of steps can be captured as a HIP graph.
 #include "dn_wave.hpp"
 
@@ -59,6 +61,7 @@ struct NetArgs {
   // connectivity
   const int32_t* out_ptr;  // by source, n_src + 1
   const int32_t* out_tgt;
+  const int32_t* out_e;    // connection index of each by-source entry
   const int32_t* in_ptr;   // by target (target-major numbering), n_cells + 1
   const int32_t* pre;      // per connection
   const int32_t* delay;    // steps, 1 <= delay < depth
@@ -68,7 +71,8 @@ struct NetArgs {
   // per-step scratch
   uint32_t* spk_bits;      // ceil(n_src / 32) words
   int32_t* spk_list;       // n_src
-  int32_t* tgt_flag;       // n_cells
+  int32_t* hit_cnt;        // n_cells: hits filed this step (zero between steps)
+  int32_t* hit_e;          // [n_cells][kHitMax] connection indices of the hits
   int32_t* tgt_list;       // n_cells
   int32_t* ctr;            // [0] cur, [1] step, [2] n_spikes, [3] n_active
   // optional records
@@ -136,8 +140,11 @@ __global__ __launch_bounds__(256) void net_deliver_detect(NetArgs a) {
   if (i < a.n_src) net_detect(a, i, step);
 }
 
-// Kb: every spiking source marks its targets; the first marker appends the
-// target to the active list.  One wavefront per spiking source.
+constexpr int kHitMax = 16;  // hits a target's list holds per step
+
+// Kb: every spiking source files its connections at their targets; the first
+// hit of a target appends it to the active list.  One wavefront per spiking
+// source.
 __global__ __launch_bounds__(256) void net_mark(NetArgs a) {
   const int lane = threadIdx.x & 63;
   const int wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -145,45 +152,71 @@ __global__ __launch_bounds__(256) void net_mark(NetArgs a) {
   const int ns = a.ctr[2];
   for (int k = wv; k < ns; k += n_wv) {
     const int src = a.spk_list[k];
-    for (int q = a.out_ptr[src] + lane; q < a.out_ptr[src + 1]; q += 64) {
-      const int t = a.out_tgt[q];
-      if (atomicExch(&a.tgt_flag[t], 1) == 0) a.tgt_list[atomicAdd(&a.ctr[3], 1)] = t;
+    const int q1 = a.out_ptr[src + 1];
+    // four entries per lane and pass: their loads, then their counter
+    // increments, are in flight together
+    for (int qb = a.out_ptr[src] + lane; qb < q1; qb += 4 * 64) {
+      int t[4], e[4], slot[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int q = qb + 64 * u;
+        t[u] = q < q1 ? a.out_tgt[q] : -1;
+        e[u] = q < q1 ? a.out_e[q] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) slot[u] = t[u] >= 0 ? atomicAdd(&a.hit_cnt[t[u]], 1) : -1;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (t[u] < 0) continue;
+        if (slot[u] < kHitMax) a.hit_e[(int64_t)t[u] * kHitMax + slot[u]] = e[u];
+        if (slot[u] == 0) a.tgt_list[atomicAdd(&a.ctr[3], 1)] = t[u];
+      }
     }
   }
 }
 
-// Kc: one wavefront per active target scans its incoming segment in
-// connection order and adds the hits sequentially (lane 0).
+// Kc: one thread per hit target adds its hits in connection order: the
+// filed connection indices taken smallest first, or, when more hits came
+// than the list holds, a scan of the target's incoming segment against the
+// step's spike bits.  The target's counter is cleared for the next step.
 __device__ __forceinline__ void net_enqueue_body(const NetArgs& a) {
-  const int lane = threadIdx.x & 63;
-  const int wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int n_wv = (gridDim.x * blockDim.x) >> 6;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nth = gridDim.x * blockDim.x;
   const int na = a.ctr[3];
   const int cur = a.ctr[0];
-  for (int k = wv; k < na; k += n_wv) {
+  for (int k = tid; k < na; k += nth) {
     const int t = a.tgt_list[k];
-    const int e0 = a.in_ptr[t], e1 = a.in_ptr[t + 1];
-    for (int base = e0; base < e1; base += 64) {
-      const int e = base + lane;
-      bool hit = false;
-      if (e < e1) {
-        const int s = a.pre[e];
-        hit = (a.spk_bits[s >> 5] >> (s & 31)) & 1u;
+    const int cnt = a.hit_cnt[t];
+    auto add = [&](int e) {
+      int slot = cur + a.delay[e];
+      if (slot >= a.depth) slot -= a.depth;
+      float* q = a.ring + (int64_t)slot * a.n_cells + t;
+      *q = *q + a.w[e];
+    };
+    if (cnt <= kHitMax) {
+      int e[kHitMax];
+#pragma unroll
+      for (int i = 0; i < kHitMax; ++i)
+        e[i] = i < cnt ? a.hit_e[(int64_t)t * kHitMax + i] : 0x7fffffff;
+      unsigned used = 0;
+      for (int o = 0; o < cnt; ++o) {
+        int best = 0x7fffffff, bi = 0;
+#pragma unroll
+        for (int i = 0; i < kHitMax; ++i)
+          if (!((used >> i) & 1u) && e[i] < best) {
+            best = e[i];
+            bi = i;
+          }
+        used |= 1u << bi;
+        add(best);
       }
-      uint64_t mask = __ballot(hit);
-      if (lane == 0) {
-        while (mask) {
-          const int b = __builtin_ctzll(mask);
-          mask &= mask - 1;
-          const int ee = base + b;
-          int slot = cur + a.delay[ee];
-          if (slot >= a.depth) slot -= a.depth;
-          float* q = a.ring + (int64_t)slot * a.n_cells + t;
-          *q = *q + a.w[ee];
-        }
+    } else {
+      for (int ee = a.in_ptr[t]; ee < a.in_ptr[t + 1]; ++ee) {
+        const int s = a.pre[ee];
+        if ((a.spk_bits[s >> 5] >> (s & 31)) & 1u) add(ee);
       }
     }
-    if (lane == 0) a.tgt_flag[t] = 0;
+    a.hit_cnt[t] = 0;
   }
 }
 
@@ -347,15 +380,15 @@ DN_EXPORT int dn_net_run_f32(
     const float* sna, const float* sk, const float* area, float dt, const float* hh,
     float* A, float* B, const float* syn, int n_src, uint8_t* latch, float theta,
     uint64_t seed, uint32_t bg_stream, double bg_p, const int32_t* out_ptr,
-    const int32_t* out_tgt, const int32_t* in_ptr, const int32_t* pre,
+    const int32_t* out_tgt, const int32_t* out_e, const int32_t* in_ptr, const int32_t* pre,
     const int32_t* delay, const float* w, int depth, float* ring, uint32_t* spk_bits,
-    int32_t* spk_list, int32_t* tgt_flag, int32_t* tgt_list, int32_t* ctr,
+    int32_t* spk_list, int32_t* hit_cnt, int32_t* hit_e, int32_t* tgt_list, int32_t* ctr,
     int32_t* ev_log, int32_t* ev_count, int64_t ev_cap, float* delivered,
     int64_t rec_steps, int64_t n_steps, int graph_steps, hipStream_t stream) {
   if (n_cells < 1 || n_point < 0 || n_point > n_cells || n_src < n_cells || !off || !soma ||
       !v || !m || !h || !n || !gL || !gRp || !scale || !area || !hh || !A || !B || !syn ||
-      !latch || !out_ptr || !out_tgt || !in_ptr || !pre || !delay || !w || depth < 2 ||
-      !ring || !spk_bits || !spk_list || !tgt_flag || !tgt_list || !ctr)
+      !latch || !out_ptr || !out_tgt || !out_e || !in_ptr || !pre || !delay || !w ||
+      depth < 2 || !ring || !spk_bits || !spk_list || !hit_cnt || !hit_e || !tgt_list || !ctr)
     return DN_EINVAL;
   if (graph_steps < 0 || (graph_steps > 0 && n_steps % graph_steps != 0)) return DN_EINVAL;
   dn::NetArgs a;
@@ -368,9 +401,9 @@ DN_EXPORT int dn_net_run_f32(
   a.e_syn = syn[3];
   a.n_src = n_src; a.latch = latch; a.theta = theta; a.seed = seed;
   a.bg_stream = bg_stream; a.bg_p = bg_p; a.out_ptr = out_ptr; a.out_tgt = out_tgt;
-  a.in_ptr = in_ptr; a.pre = pre; a.delay = delay; a.w = w; a.depth = depth;
-  a.ring = ring; a.spk_bits = spk_bits; a.spk_list = spk_list; a.tgt_flag = tgt_flag;
-  a.tgt_list = tgt_list; a.ctr = ctr; a.ev_log = ev_log; a.ev_count = ev_count;
+  a.out_e = out_e; a.in_ptr = in_ptr; a.pre = pre; a.delay = delay; a.w = w;
+  a.depth = depth; a.ring = ring; a.spk_bits = spk_bits; a.spk_list = spk_list;
+  a.hit_cnt = hit_cnt; a.hit_e = hit_e; a.tgt_list = tgt_list; a.ctr = ctr; a.ev_log = ev_log; a.ev_count = ev_count;
   a.ev_cap = ev_cap; a.delivered = delivered; a.rec_steps = rec_steps;
   a.tape = nullptr;
   const int n_point_cells = n_point, n_cable = n_cells - n_point;
@@ -410,20 +443,21 @@ DN_EXPORT int dn_net_run_f32(
 // numbered target-major (in_ptr) with the by-source CSR out_ptr/out_tgt.
 DN_EXPORT int dn_net_deliver_tape_f32(
     int n_targets, int n_src, const int32_t* out_ptr, const int32_t* out_tgt,
-    const int32_t* in_ptr, const int32_t* pre, const int32_t* delay, const float* w,
-    int depth, float* ring, uint32_t* spk_bits, int32_t* spk_list, int32_t* tgt_flag,
-    int32_t* tgt_list, int32_t* ctr, const uint8_t* tape, float* delivered,
-    int64_t n_steps, hipStream_t stream) {
-  if (n_targets < 1 || n_src < 1 || !out_ptr || !out_tgt || !in_ptr || !pre || !delay ||
-      !w || depth < 2 || !ring || !spk_bits || !spk_list || !tgt_flag || !tgt_list ||
-      !ctr || !tape || !delivered)
+    const int32_t* out_e, const int32_t* in_ptr, const int32_t* pre, const int32_t* delay,
+    const float* w, int depth, float* ring, uint32_t* spk_bits, int32_t* spk_list,
+    int32_t* hit_cnt, int32_t* hit_e, int32_t* tgt_list, int32_t* ctr, const uint8_t* tape,
+    float* delivered, int64_t n_steps, hipStream_t stream) {
+  if (n_targets < 1 || n_src < 1 || !out_ptr || !out_tgt || !out_e || !in_ptr || !pre ||
+      !delay || !w || depth < 2 || !ring || !spk_bits || !spk_list || !hit_cnt || !hit_e ||
+      !tgt_list || !ctr || !tape || !delivered)
     return DN_EINVAL;
   dn::NetArgs a = {};
   a.n_cells = n_targets;
   a.n_src = n_src;
-  a.out_ptr = out_ptr; a.out_tgt = out_tgt; a.in_ptr = in_ptr; a.pre = pre;
-  a.delay = delay; a.w = w; a.depth = depth; a.ring = ring; a.spk_bits = spk_bits;
-  a.spk_list = spk_list; a.tgt_flag = tgt_flag; a.tgt_list = tgt_list; a.ctr = ctr;
+  a.out_ptr = out_ptr; a.out_tgt = out_tgt; a.out_e = out_e; a.in_ptr = in_ptr;
+  a.pre = pre; a.delay = delay; a.w = w; a.depth = depth; a.ring = ring;
+  a.spk_bits = spk_bits; a.spk_list = spk_list; a.hit_cnt = hit_cnt; a.hit_e = hit_e;
+  a.tgt_list = tgt_list; a.ctr = ctr;
   a.delivered = delivered; a.rec_steps = n_steps; a.tape = tape;
   const int nblk = ((n_src > n_targets ? n_src : n_targets) + 255) / 256;
   for (int64_t s = 0; s < n_steps; ++s) {

@@ -64,7 +64,8 @@ class Connectivity:
         order = np.argsort(self.pre, kind="stable")
         out_ptr = np.zeros(self.n_src + 1, dtype=np.int64)
         out_ptr[1:] = np.cumsum(np.bincount(self.pre, minlength=self.n_src))
-        return in_ptr, out_ptr, self.post[order]
+        # by source: target and connection index of every connection
+        return in_ptr, out_ptr, self.post[order], order
 
 
 class _Delivery:
@@ -72,18 +73,22 @@ class _Delivery:
 
     def __init__(self, conn: Connectivity, device):
         i32 = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.int32, device=device)
-        in_ptr, out_ptr, out_tgt = conn.csr()
+        in_ptr, out_ptr, out_tgt, out_e = conn.csr()
         if conn.pre.size >= 2**31:
             raise ValueError("too many connections for int32 indices")
         self.conn = conn
         self.in_ptr, self.out_ptr, self.out_tgt = i32(in_ptr), i32(out_ptr), i32(out_tgt)
+        self.out_e = i32(out_e)
         self.pre, self.delay = i32(conn.pre), i32(conn.delay)
         self.w = torch.as_tensor(conn.w, dtype=torch.float32, device=device)
         self.depth = conn.depth
         self.ring = torch.zeros((self.depth, conn.n_tgt), dtype=torch.float32, device=device)
         self.spk_bits = torch.zeros((conn.n_src + 31) // 32, dtype=torch.int32, device=device)
         self.spk_list = torch.zeros(conn.n_src, dtype=torch.int32, device=device)
-        self.tgt_flag = torch.zeros(conn.n_tgt, dtype=torch.int32, device=device)
+        # per target: hits filed this step (zero between steps) and their
+        # connection indices (16 per target; more fall back to a scan)
+        self.hit_cnt = torch.zeros(conn.n_tgt, dtype=torch.int32, device=device)
+        self.hit_e = torch.zeros(conn.n_tgt * 16, dtype=torch.int32, device=device)
         self.tgt_list = torch.zeros(conn.n_tgt, dtype=torch.int32, device=device)
         self.ctr = torch.zeros(8, dtype=torch.int32, device=device)
 
@@ -100,10 +105,11 @@ def deliver_tape(conn: Connectivity, tape, device="cuda") -> np.ndarray:
     tape_t = torch.as_tensor(tape, device=device)
     out = torch.empty((n_steps, conn.n_tgt), dtype=torch.float32, device=device)
     st = lib.dn_net_deliver_tape_f32(
-        conn.n_tgt, conn.n_src, _lib.ptr(d.out_ptr), _lib.ptr(d.out_tgt),
+        conn.n_tgt, conn.n_src, _lib.ptr(d.out_ptr), _lib.ptr(d.out_tgt), _lib.ptr(d.out_e),
         _lib.ptr(d.in_ptr), _lib.ptr(d.pre), _lib.ptr(d.delay), _lib.ptr(d.w), d.depth,
-        _lib.ptr(d.ring), _lib.ptr(d.spk_bits), _lib.ptr(d.spk_list), _lib.ptr(d.tgt_flag),
-        _lib.ptr(d.tgt_list), _lib.ptr(d.ctr), _lib.ptr(tape_t), _lib.ptr(out), n_steps,
+        _lib.ptr(d.ring), _lib.ptr(d.spk_bits), _lib.ptr(d.spk_list), _lib.ptr(d.hit_cnt),
+        _lib.ptr(d.hit_e), _lib.ptr(d.tgt_list), _lib.ptr(d.ctr), _lib.ptr(tape_t),
+        _lib.ptr(out), n_steps,
         torch.cuda.current_stream(device).cuda_stream)
     _lib.check(st, "tape delivery")
     return out.cpu().numpy()
@@ -164,7 +170,7 @@ class Network:
         self.d.ring.zero_()
         self.d.ctr.zero_()
         self.d.spk_bits.zero_()
-        self.d.tgt_flag.zero_()
+        self.d.hit_cnt.zero_()
         self.t_step = 0
 
     def run(self, n_steps: int, record_events: bool = False, record_delivered: bool = False,
@@ -194,10 +200,11 @@ class Network:
                 _lib.ptr(self.area), self.dt, self.hh_arr.ctypes.data, _lib.ptr(self.A),
                 _lib.ptr(self.B), self.syn.ctypes.data, self.spec.conn.n_src,
                 _lib.ptr(self.latch), self.spec.theta, self.spec.seed, self.spec.bg_stream,
-                self.bg_p, _lib.ptr(d.out_ptr), _lib.ptr(d.out_tgt), _lib.ptr(d.in_ptr),
-                _lib.ptr(d.pre), _lib.ptr(d.delay), _lib.ptr(d.w), d.depth, _lib.ptr(d.ring),
-                _lib.ptr(d.spk_bits), _lib.ptr(d.spk_list), _lib.ptr(d.tgt_flag),
-                _lib.ptr(d.tgt_list), _lib.ptr(d.ctr), _lib.ptr(ev_log), _lib.ptr(ev_count),
+                self.bg_p, _lib.ptr(d.out_ptr), _lib.ptr(d.out_tgt), _lib.ptr(d.out_e),
+                _lib.ptr(d.in_ptr), _lib.ptr(d.pre), _lib.ptr(d.delay), _lib.ptr(d.w), d.depth,
+                _lib.ptr(d.ring), _lib.ptr(d.spk_bits), _lib.ptr(d.spk_list),
+                _lib.ptr(d.hit_cnt), _lib.ptr(d.hit_e), _lib.ptr(d.tgt_list), _lib.ptr(d.ctr),
+                _lib.ptr(ev_log), _lib.ptr(ev_count),
                 cap, _lib.ptr(delivered), n_steps if record_delivered else 0, n_steps,
                 graph_steps, self.stream.cuda_stream)
         torch.cuda.current_stream(self.device).wait_stream(self.stream)

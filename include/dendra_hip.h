@@ -514,20 +514,21 @@ int dn_hh_tree_step_f64(int n_trees, const int64_t* node_off, const int32_t* par
  * netcon_bitpack_kernel.cpp:32-149, called from netcon.py:3739-3883), and the
  * dense ring of NetCon.advance_non_diff_dense_* (netcon.py:5183-5205,
  * _event_delivery.py:30-199).  Connections are numbered target-major
- * (in_ptr, n_targets+1); out_ptr/out_tgt is the by-source CSR.  Per step the
- * ring row `ctr[0]` is copied to delivered[step][*] and cleared, then the
- * connections of the sources set in tape[step][*] add w into
- * ring[(cur + delay) % depth][post] in connection order (bitwise equal to the
- * ordered float32 scatter_add).  Scratch: spk_bits ceil(n_src/32) words,
- * spk_list n_src, tgt_flag/tgt_list n_targets (flags zero on entry),
- * ctr {cur, step, 0, 0}. */
+ * (in_ptr, n_targets+1); out_ptr/out_tgt/out_e is the by-source CSR (target
+ * and connection index of each entry).  Per step the ring row `ctr[0]` is
+ * copied to delivered[step][*] and cleared, then the connections of the
+ * sources set in tape[step][*] add w into ring[(cur + delay) % depth][post]
+ * in connection order (bitwise equal to the ordered float32 scatter_add).
+ * Scratch: spk_bits ceil(n_src/32) words, spk_list n_src, hit_cnt n_targets
+ * (zero on entry; zero again after every step), hit_e 16 * n_targets,
+ * tgt_list n_targets, ctr {cur, step, 0, 0}. */
 int dn_net_deliver_tape_f32(int n_targets, int n_src, const int32_t* out_ptr,
-                            const int32_t* out_tgt, const int32_t* in_ptr,
-                            const int32_t* pre, const int32_t* delay, const float* w,
-                            int depth, float* ring, uint32_t* spk_bits, int32_t* spk_list,
-                            int32_t* tgt_flag, int32_t* tgt_list, int32_t* ctr,
-                            const uint8_t* tape, float* delivered, int64_t n_steps,
-                            hipStream_t stream);
+                            const int32_t* out_tgt, const int32_t* out_e,
+                            const int32_t* in_ptr, const int32_t* pre, const int32_t* delay,
+                            const float* w, int depth, float* ring, uint32_t* spk_bits,
+                            int32_t* spk_list, int32_t* hit_cnt, int32_t* hit_e,
+                            int32_t* tgt_list, int32_t* ctr, const uint8_t* tape,
+                            float* delivered, int64_t n_steps, hipStream_t stream);
 
 /* ---- network step ------------------------------------------------------------
  * Replaces the Network.run hot loop (dendra/models/networks/net.py:4485-4545,
@@ -539,7 +540,8 @@ int dn_net_deliver_tape_f32(int n_targets, int n_src, const int32_t* out_ptr,
  * with p = bg_p per step (splitmix64 counter RNG of seed, bg_stream).
  * syn (host) = {factor, tau1_eff, tau2, e}.  Optional records: ev_log
  * (step, source) pairs up to ev_cap, delivered payloads for the first
- * rec_steps steps.  ctr: 8 int32 {cur, step, 0, ...}.  graph_steps > 0
+ * rec_steps steps.  Delivery arrays and scratch as dn_net_deliver_tape_f32.
+ * ctr: 8 int32 {cur, step, 0, ...}.  graph_steps > 0
  * captures that many steps as one HIP graph and replays it (stream must not
  * be the null stream). */
 int dn_net_run_f32(int n_cells, int n_point, const int64_t* off, const int32_t* soma,
@@ -548,10 +550,11 @@ int dn_net_run_f32(int n_cells, int n_point, const int64_t* off, const int32_t* 
                    const float* sk, const float* area, float dt, const float* hh,
                    float* A, float* B, const float* syn, int n_src, uint8_t* latch,
                    float theta, uint64_t seed, uint32_t bg_stream, double bg_p,
-                   const int32_t* out_ptr, const int32_t* out_tgt, const int32_t* in_ptr,
-                   const int32_t* pre, const int32_t* delay, const float* w, int depth,
-                   float* ring, uint32_t* spk_bits, int32_t* spk_list, int32_t* tgt_flag,
-                   int32_t* tgt_list, int32_t* ctr, int32_t* ev_log, int32_t* ev_count,
+                   const int32_t* out_ptr, const int32_t* out_tgt, const int32_t* out_e,
+                   const int32_t* in_ptr, const int32_t* pre, const int32_t* delay,
+                   const float* w, int depth, float* ring, uint32_t* spk_bits,
+                   int32_t* spk_list, int32_t* hit_cnt, int32_t* hit_e, int32_t* tgt_list,
+                   int32_t* ctr, int32_t* ev_log, int32_t* ev_count,
                    int64_t ev_cap, float* delivered, int64_t rec_steps, int64_t n_steps,
                    int graph_steps, hipStream_t stream);
 
