@@ -26,7 +26,10 @@
 //      section's interior at once.
 // With g > 0 and d >= 0 every self term and edge is a sum of positive
 // products (no cancellation on the diagonal); sums run in a fixed order, so
-// results are bitwise reproducible.
+// results are bitwise reproducible.  When they fit beside the section
+// tables, the system's d, g and rhs are staged in LDS by coalesced loads
+// (the sections then gather from LDS), x leaves through LDS by coalesced
+// stores, and the interior recovery coefficients stay in LDS.
 #include "dn_common.hpp"
 
 namespace dn {
@@ -43,7 +46,9 @@ template <typename T> struct ContractArgs {
   const int32_t *recv_ptr, *recv_sec, *src_ptr, *src;
 };
 
-template <typename T>
+// STAGE: 0 all node data in global memory, 1 d, g, rhs and x through LDS,
+// 2 also the interior recovery coefficients in LDS.
+template <typename T, int STAGE>
 __global__ __launch_bounds__(256) void tree_contract_kernel(ContractArgs<T> A) {
   extern __shared__ unsigned char dn_smem_raw[];
   const int ns = A.n_sec;
@@ -56,14 +61,30 @@ __global__ __launch_bounds__(256) void tree_contract_kernel(ContractArgs<T> A) {
     *CM = S + 5 * ns, *CF = S + 6 * ns, *CE = S + 7 * ns, *XS = S + 8 * ns;
   const int64_t K = A.K;
   const int64_t b = blockIdx.x;
+  const int tid = threadIdx.x, nt = blockDim.x;
   const T* d = A.d + b * K;
   const T* g = A.g + b * K;
   const T* rhs = A.rhs + b * K;
   T* x = A.x + b * K;
-  T* wa = A.work + b * 3 * K;
+  T* xo = x;  // where the solution is written (LDS when staged)
+  if constexpr (STAGE > 0) {
+    T* dl = S + 9 * ns;
+    T* gl = dl + K;
+    T* rl = gl + K;
+    for (int64_t i = tid; i < K; i += nt) {
+      dl[i] = d[i];
+      gl[i] = g[i];
+      rl[i] = rhs[i];
+    }
+    __syncthreads();
+    d = dl;
+    g = gl;
+    rhs = rl;
+    xo = dl;  // d is dead after the chain condensation
+  }
+  T* wa = STAGE == 2 ? S + 9 * ns + 3 * K : A.work + b * 3 * K;
   T* wb = wa + K;
   T* wc = wb + K;
-  const int tid = threadIdx.x, nt = blockDim.x;
 
   // 1. chain condensation, head first
   for (int s = tid; s < ns; s += nt) {
@@ -171,17 +192,25 @@ __global__ __launch_bounds__(256) void tree_contract_kernel(ContractArgs<T> A) {
     const int hd = A.sec_head[s], len = A.sec_len[s], ps = A.sec_parent[s];
     const T xp = ps >= 0 ? XS[ps] : T(0);
     T xv = XS[s];
-    x[A.perm[hd + len - 1]] = xv;
+    xo[A.perm[hd + len - 1]] = xv;
     for (int r = len - 2; r >= 0; --r) {
       xv = wa[hd + r] + wb[hd + r] * xp + wc[hd + r] * xv;
-      x[A.perm[hd + r]] = xv;
+      xo[A.perm[hd + r]] = xv;
     }
+  }
+  if constexpr (STAGE > 0) {
+    __syncthreads();
+    for (int64_t i = tid; i < K; i += nt) x[i] = xo[i];
   }
 }
 
 constexpr size_t kContractLdsMax = 160 * 1024;
 
 inline size_t contract_lds_bytes(int n_sec, size_t elem) { return 9 * (size_t)n_sec * elem; }
+// with d, g and rhs staged (stage 1), and the recovery coefficients (stage 2)
+inline size_t contract_stage_lds_bytes(int n_sec, int64_t K, size_t elem, int stage) {
+  return (9 * (size_t)n_sec + 3 * (size_t)stage * (size_t)K) * elem;
+}
 
 }  // namespace dn
 
@@ -214,8 +243,18 @@ static int contract_impl(const T* d, const T* g, const T* rhs, T* x, int64_t B, 
   A.elim_ptr = elim_ptr; A.elim_sec = elim_sec; A.elim_par = elim_par;
   A.elim_child = elim_child; A.recv_ptr = recv_ptr; A.recv_sec = recv_sec;
   A.src_ptr = src_ptr; A.src = src;
-  hipLaunchKernelGGL((dn::tree_contract_kernel<T>), dim3((unsigned)B), dim3(256), lds,
-                     stream, A);
+  // stage as much of the system in LDS as keeps two workgroups on a CU
+  const size_t lds2 = dn::contract_stage_lds_bytes(n_sec, K, sizeof(T), 2);
+  const size_t lds1 = dn::contract_stage_lds_bytes(n_sec, K, sizeof(T), 1);
+  if (lds2 <= dn::kContractLdsMax / 2)
+    hipLaunchKernelGGL((dn::tree_contract_kernel<T, 2>), dim3((unsigned)B), dim3(256), lds2,
+                       stream, A);
+  else if (lds1 <= dn::kContractLdsMax / 2)
+    hipLaunchKernelGGL((dn::tree_contract_kernel<T, 1>), dim3((unsigned)B), dim3(256), lds1,
+                       stream, A);
+  else
+    hipLaunchKernelGGL((dn::tree_contract_kernel<T, 0>), dim3((unsigned)B), dim3(256), lds,
+                       stream, A);
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
 
