@@ -19,12 +19,17 @@
 //    two-row interfaces are reduced to one row per lane by a cyclic-reduction
 //    step and solved by wavefront PCR through ds_bpermute, then every lane
 //    back-substitutes its chunk.  The system is staged in LDS (coalesced loads)
-//    when 4*64*nr values fit, else it runs in the caller's global scratch.
+//    when 4*64*nr values fit a CU's 160 KiB, else it runs in the caller's
+//    global scratch.
 #include "dn_wave.hpp"
 
 #include <mutex>
 
 namespace dn {
+
+// LDS of one CU (gfx950): a system of up to 10,240 float32 (5,120 float64)
+// rows stages whole in the workgroup's LDS
+constexpr size_t kTridiagLdsMax = 160 * 1024;
 
 template <typename T>
 __global__ __launch_bounds__(64) void tridiag_wave_kernel(
@@ -41,12 +46,29 @@ __global__ __launch_bounds__(64) void tridiag_wave_kernel(
   const T* br = b + r * K;
   const T* cr = c + r * (K - 1);
   const T* dr = d + r * K;
-  for (int64_t j = lane; j < KP; j += 64) {
-    bool in = j < K;
-    sa[j] = (in && j >= 1) ? ar[j - 1] : T(0);
-    sb[j] = in ? br[j] : T(1);
-    sc[j] = (j + 1 < K) ? cr[j] : T(0);
-    sd[j] = in ? dr[j] : T(0);
+  // eight rows per lane and pass: 32 loads in flight before their stores
+  constexpr int U = 8;
+  for (int64_t j0 = lane; j0 < KP; j0 += 64 * U) {
+    T va[U], vb[U], vc[U], vd[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t j = j0 + 64 * u;
+      const bool in = j < K;
+      va[u] = (in && j >= 1) ? ar[j - 1] : T(0);
+      vb[u] = in ? br[j] : T(1);
+      vc[u] = (j + 1 < K) ? cr[j] : T(0);
+      vd[u] = in ? dr[j] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t j = j0 + 64 * u;
+      if (j < KP) {
+        sa[j] = va[u];
+        sb[j] = vb[u];
+        sc[j] = vc[u];
+        sd[j] = vd[u];
+      }
+    }
   }
   __syncthreads();
   wave_hybrid_solve<T>(sa, sb, sc, sd, nr);
@@ -234,7 +256,7 @@ static int tridiag_launch(const T* a, const T* b, const T* c, const T* d, T* x,
     int nr = (int)((K + 63) / 64);
     int64_t KP = 64 * (int64_t)nr;
     size_t lds = (size_t)4 * KP * sizeof(T);
-    int use_lds = lds <= 64 * 1024;
+    int use_lds = lds <= kTridiagLdsMax;
     if (!use_lds && (!scratch || scratch_elems < B * 4 * KP)) return DN_ERANGE;
     if (B > 0x7fffffff) return DN_ERANGE;
     hipLaunchKernelGGL(tridiag_wave_kernel<T>, dim3((unsigned)B), dim3(64),
@@ -252,7 +274,7 @@ DN_EXPORT int64_t dn_tridiag_scratch_elems(int64_t B, int64_t K, int method,
   if (method == 0) method = (K <= 32 && B >= 4096) ? 1 : 2;
   if (method == 1) return K <= (elem_bytes == 4 ? 32 : 16) ? 0 : B * K;
   int64_t KP = 64 * ((K + 63) / 64);
-  return (size_t)4 * KP * elem_bytes <= 64 * 1024 ? 0 : B * 4 * KP;
+  return (size_t)4 * KP * elem_bytes <= dn::kTridiagLdsMax ? 0 : B * 4 * KP;
 }
 
 DN_EXPORT int dn_tridiag_solve_f32(const float* a, const float* b, const float* c,

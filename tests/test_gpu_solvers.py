@@ -13,8 +13,9 @@ import torch
 pytestmark = pytest.mark.gpu
 
 TOL = {torch.float32: (4e-4, 6e-5, 3e-5), torch.float64: (5e-11, 5e-12, 2e-12)}
+# K = 4100 stages in LDS, K = 12000 (beyond a CU's 160 KiB) runs in scratch
 CASES = [(101, 1, 1), (103, 3, 2), (107, 2, 7), (109, 4, 33), (113, 2, 65),
-         (5, 3, 64), (6, 2, 129), (7, 1, 832), (8, 2, 4100), (9, 8, 3)]
+         (5, 3, 64), (6, 2, 129), (7, 1, 832), (8, 2, 4100), (9, 8, 3), (10, 1, 12000)]
 
 
 def _bands(seed, B, K):

@@ -41,7 +41,7 @@ def _no_alloc(lib, p, a, b, c, d, B, K, method):
 
 
 @pytest.mark.parametrize("method,shapes", [(1, [(16384, 64), (1000, 40)]),
-                                           (2, [(60, 8000), (9, 5000)])])
+                                           (2, [(40, 12000), (9, 10300)])])
 def test_no_alloc_solve_equals_scratch_solve(method, shapes):
     from dendra_amd import _lib
 
