@@ -47,46 +47,92 @@ __device__ __forceinline__ T hist_gate(const void* hist, int64_t n_col, int64_t 
   return static_cast<const T*>(hist)[row * n_col + col];
 }
 
+// A workgroup of 256 threads owns kProjTargets targets, whose connections
+// are one contiguous range of the target-major CSR.  The range is taken in
+// chunks of kProjChunk: all threads gather a chunk's connections (edge,
+// column, tap offsets, gates, tap weights and w; kProjPer independent
+// connections per thread in flight) and leave each connection's value v in
+// LDS, then one lane per target adds its own values of the chunk in
+// ascending connection order.  The sums are those of the sequential
+// lane-per-target walk, bit for bit.
+constexpr int kProjTargets = 32;
+constexpr int kProjThreads = 256;
+constexpr int kProjPer = 8;
+constexpr int kProjChunk = kProjThreads * kProjPer;
+
 template <typename T, bool PACKED, int TAPS>
-__global__ __launch_bounds__(256) void history_project_kernel(
+__global__ __launch_bounds__(kProjThreads) void history_project_kernel(
     const void* __restrict__ hist, int64_t depth, int64_t n_col,
     const int64_t* __restrict__ slot_p, const int32_t* __restrict__ in_ptr,
     const int32_t* __restrict__ in_edge, const int32_t* __restrict__ col,
     const int64_t* __restrict__ mask, const int32_t* __restrict__ off,
     const T* __restrict__ w, const T* __restrict__ frac, T* __restrict__ out, int64_t width) {
 #pragma clang fp contract(off)
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= width) return;
+  __shared__ T vs[kProjChunk];
+  const int tid = threadIdx.x;
+  const int64_t t0 = (int64_t)blockIdx.x * kProjTargets;
+  const int nt = (int)(width - t0 < kProjTargets ? width - t0 : kProjTargets);
   const int slot = hist_slot(slot_p, depth);
-  T acc = T(0);
-  const int e0 = in_ptr[t], e1 = in_ptr[t + 1];
-#pragma unroll 4
-  for (int i = e0; i < e1; ++i) {
-    const int64_t e = in_edge[i];
-    const int32_t c = col[e];
-    T g[TAPS];
-#pragma unroll
-    for (int j = 0; j < TAPS; ++j)
-      g[j] = hist_gate<T, PACKED>(hist, n_col, hist_row(slot, off[e * TAPS + j], depth), c,
-                                  mask, e);
-    T s;
-    if (TAPS == 1) {
-      s = g[0];
-    } else if (TAPS == 2) {
-      const T a = frac[e];
-      const T lo = g[0] * (T(1) - a);
-      const T hi = g[1] * a;
-      s = lo + hi;
-    } else {
-      const T p0 = g[0] * frac[e * 3];
-      const T p1 = g[1] * frac[e * 3 + 1];
-      const T p2 = g[2] * frac[e * 3 + 2];
-      s = (p0 + p1) + p2;
-    }
-    const T v = w[e] * s;
-    acc = acc + v;
+  const int i0 = in_ptr[t0], i1 = in_ptr[t0 + nt];
+  int e0 = 0, e1 = 0;
+  if (tid < nt) {
+    e0 = in_ptr[t0 + tid];
+    e1 = in_ptr[t0 + tid + 1];
   }
-  out[t] = acc;
+  T acc = T(0);
+  for (int c0 = i0; c0 < i1; c0 += kProjChunk) {
+    const int n = i1 - c0 < kProjChunk ? i1 - c0 : kProjChunk;
+    // indices past the chunk repeat its last connection: every load is
+    // unconditional and in bounds, so all of them are in flight at once
+    int64_t e[kProjPer];
+#pragma unroll
+    for (int u = 0; u < kProjPer; ++u) {
+      const int k = tid + u * kProjThreads;
+      e[u] = (int64_t)in_edge[c0 + (k < n ? k : n - 1)];
+    }
+    int32_t c[kProjPer], o[kProjPer][TAPS];
+#pragma unroll
+    for (int u = 0; u < kProjPer; ++u) {
+      c[u] = col[e[u]];
+#pragma unroll
+      for (int j = 0; j < TAPS; ++j) o[u][j] = off[e[u] * TAPS + j];
+    }
+#pragma unroll
+    for (int u = 0; u < kProjPer; ++u) {
+      const int k = tid + u * kProjThreads;
+      {
+        T gv[TAPS];
+#pragma unroll
+        for (int j = 0; j < TAPS; ++j)
+          gv[j] = hist_gate<T, PACKED>(hist, n_col, hist_row(slot, o[u][j], depth), c[u], mask,
+                                       e[u]);
+        T s;
+        if (TAPS == 1) {
+          s = gv[0];
+        } else if (TAPS == 2) {
+          const T a = frac[e[u]];
+          const T lo = gv[0] * (T(1) - a);
+          const T hi = gv[1] * a;
+          s = lo + hi;
+        } else {
+          const T p0 = gv[0] * frac[e[u] * 3];
+          const T p1 = gv[1] * frac[e[u] * 3 + 1];
+          const T p2 = gv[2] * frac[e[u] * 3 + 2];
+          s = (p0 + p1) + p2;
+        }
+        // past the chunk: connection n - 1's own value into its own slot
+        vs[k < n ? k : n - 1] = w[e[u]] * s;
+      }
+    }
+    __syncthreads();
+    if (tid < nt) {
+      const int a = (e0 > c0 ? e0 : c0) - c0;
+      const int b = (e1 < c0 + n ? e1 : c0 + n) - c0;
+      for (int k = a; k < b; ++k) acc = acc + vs[k];
+    }
+    __syncthreads();
+  }
+  if (tid < nt) out[t0 + tid] = acc;
 }
 
 template <typename T, bool PACKED, int TAPS>
@@ -175,7 +221,8 @@ static void project_launch(int taps, const void* hist, int64_t depth, int64_t n_
                            const int32_t* col, const int64_t* mask, const int32_t* off,
                            const T* w, const T* frac, T* out, int64_t width,
                            hipStream_t stream) {
-  const dim3 g = grid_for(width), b(256);
+  const dim3 g((unsigned)((width + dn::kProjTargets - 1) / dn::kProjTargets)),
+      b(dn::kProjThreads);
   if (taps == 1)
     hipLaunchKernelGGL((dn::history_project_kernel<T, PACKED, 1>), g, b, 0, stream, hist,
                        depth, n_col, slot, in_ptr, in_edge, col, mask, off, w, frac, out,

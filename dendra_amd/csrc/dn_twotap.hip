@@ -31,6 +31,16 @@ __device__ __forceinline__ int ring_slot(const int64_t* slot, int depth) {
   return (int)(s < 0 ? s + depth : s);
 }
 
+// x where c holds, else +0, as a bit mask: the load of x stays unconditional
+// (a select lets the compiler put the load under a branch of its own and
+// wait for it there, one load at a time)
+__device__ __forceinline__ float keep_if(float x, bool c) {
+  return __uint_as_float(__float_as_uint(x) & (c ? ~0u : 0u));
+}
+__device__ __forceinline__ double keep_if(double x, bool c) {
+  return __longlong_as_double(__double_as_longlong(x) & (c ? ~0ll : 0ll));
+}
+
 __device__ __forceinline__ int64_t ring_row(int slot, int32_t base, int j, int depth) {
   return ((unsigned)slot + (unsigned)base + (unsigned)j) % (unsigned)depth;
 }
@@ -75,60 +85,249 @@ __global__ __launch_bounds__(256) void interp_delay_enqueue_kernel(
   }
 }
 
-// Tiled form: a workgroup of 64 lanes owns 64 target columns for all depth
-// rows.  The tile of the input ring is staged in LDS (coalesced), today's row
-// cleared, each lane applies its target's contributions in the same order as
-// above to the LDS cells, and the tile is written to the output ring.  The
-// copy the pure transition returns is thus fused, and the per-cell updates
-// are LDS read-modify-writes, so the connection loads can run ahead.
+// Tiled form: a workgroup of 256 threads owns kRingTileTargets target
+// columns for all depth rows.  The tile of the input ring is staged in LDS
+// (coalesced rows) with today's row cleared.  The tile's connections (one
+// contiguous range of the target-major CSR) are then taken in chunks of
+// kRingChunk: all 256 threads gather a chunk's w, g, f and base (rounds of
+// kRingPer independent loads per thread and array in flight) and leave each
+// connection's tap values and first row in LDS; then one lane per target
+// adds its own connections of the chunk, in ascending order, to its LDS
+// column.  Every ring cell therefore receives the same values in the same
+// order as the lane-per-target form above.  Two taps need all lower taps
+// before all upper taps: a tile whose connections fit in one chunk applies
+// both from one gather, a longer one makes one pass over its chunks per tap.
+// The tile is written back at the end, so the returned copy of the ring is
+// fused, and qin may equal qout (each workgroup reads its own columns whole
+// before it writes them).
+constexpr int kRingTileTargets = 16;
+constexpr int kRingThreads = 256;
+constexpr int kRingPer = 4;                             // loads in flight per thread
+constexpr int kRingBatch = kRingThreads * kRingPer;     // connections per gather round
+constexpr int kRingChunk = 4 * kRingBatch;              // connections staged in LDS
+
 template <typename T, int TAPS>
-__global__ __launch_bounds__(64) void interp_delay_tile_kernel(
-    const T* __restrict__ qin, T* __restrict__ qout, int depth, int64_t width,
-    const int64_t* __restrict__ slot_p, const int32_t* __restrict__ in_ptr,
-    const int32_t* __restrict__ in_edge, const T* __restrict__ w, const T* __restrict__ g,
-    const T* __restrict__ f, const int32_t* __restrict__ base) {
+struct RingStage {
+  T v[TAPS][kRingChunk];    // tap values of the chunk's connections
+  int32_t row[kRingChunk];  // first tap's ring row
+};
+
+// One connection's tap values and first row, packed so that the tile
+// kernel's random gather per connection is one aligned record (8 or 16 B for
+// float32, 16 or 32 B for float64) instead of four scattered words.
+template <typename T, int TAPS>
+struct alignas(sizeof(T) * TAPS + 4 <= 8 ? 8 : (sizeof(T) * TAPS + 4 <= 16 ? 16 : 32))
+    RingRec {
+  T v[TAPS];
+  int32_t row;
+};
+
+// The tap values of connection e, rounded as the reference rounds them: one
+// tap w g; two taps (w g)(1 - f) and (w g) f; three taps (w g) f_j.
+template <typename T, int TAPS>
+__device__ __forceinline__ void ring_tap_values(T wv, T gv, const T* fv, T* v) {
 #pragma clang fp contract(off)
-  extern __shared__ __align__(16) unsigned char dn_smem_raw[];
-  T* tile = reinterpret_cast<T*>(dn_smem_raw);
-  const int lane = threadIdx.x;
-  const int64_t t0 = (int64_t)blockIdx.x * 64;
-  const int64_t t = t0 + lane;
-  const bool live = t < width;
+  const T wg = wv * gv;
+  if (TAPS == 1) {
+    v[0] = wg;
+  } else if (TAPS == 2) {
+    v[0] = wg * (T(1) - fv[0]);
+    v[1] = wg * fv[0];
+  } else {
+#pragma unroll
+    for (int j = 0; j < TAPS; ++j) v[j] = wg * fv[j];
+  }
+}
+
+// Records in connection order (coalesced reads and writes).
+template <typename T, int TAPS>
+__global__ __launch_bounds__(256) void interp_delay_pack_kernel(
+    RingRec<T, TAPS>* __restrict__ rec, int64_t n_edge, int depth,
+    const int64_t* __restrict__ slot_p, const T* __restrict__ w, const T* __restrict__ g,
+    const T* __restrict__ f, const int32_t* __restrict__ base) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_edge) return;
   const int slot = ring_slot(slot_p, depth);
-  for (int r = 0; r < depth; ++r)
-    tile[r * 64 + lane] = (live && r != slot) ? qin[(int64_t)r * width + t] : T(0);
-  if (live) {
-    const int e0 = in_ptr[t], e1 = in_ptr[t + 1];
-    if (TAPS == 1) {
-#pragma unroll 4
-      for (int i = e0; i < e1; ++i) {
-        const int e = in_edge[i];
-        T* cell = tile + ring_row(slot, base[e], 0, depth) * 64 + lane;
-        *cell = *cell + w[e] * g[e];
-      }
-    } else if (TAPS == 2) {
-      for (int j = 0; j < 2; ++j) {
-#pragma unroll 4
-        for (int i = e0; i < e1; ++i) {
-          const int e = in_edge[i];
-          const T wg = w[e] * g[e];
-          const T tap = j == 0 ? T(1) - f[e] : f[e];
-          T* cell = tile + ring_row(slot, base[e], j, depth) * 64 + lane;
-          *cell = *cell + wg * tap;
-        }
-      }
-    } else {
-#pragma unroll 4
-      for (int i = e0; i < e1; ++i) {
-        const int e = in_edge[i];
-        const T wg = w[e] * g[e];
-        for (int j = 0; j < 3; ++j) {
-          T* cell = tile + ring_row(slot, base[e], j, depth) * 64 + lane;
-          *cell = *cell + wg * f[(int64_t)e * 3 + j];
-        }
+  T fv[3] = {T(0), T(0), T(0)};
+  if (TAPS == 2) fv[0] = f[e];
+  if (TAPS == 3)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) fv[j] = f[e * 3 + j];
+  RingRec<T, TAPS> r;
+  ring_tap_values<T, TAPS>(w[e], g[e], fv, r.v);
+  r.row = (int32_t)ring_row(slot, base[e], 0, depth);
+  rec[e] = r;
+}
+
+// Adds the tap values of the staged connections [a, b) to one LDS column in
+// order: connection k contributes v[j * kRingChunk + k] to row
+// rows[k] + shift + j (mod depth) for j < TS.  Items are taken GC
+// connections at a time, without branches: all their cells are read at
+// once, an item whose cell an earlier item of the group also hits continues
+// from that item's sum, and the cells are written back in item order (the
+// last write of a repeated cell carries all of the group's adds).  Each cell
+// therefore sees the sequential sums, with GC * TS reads in flight instead
+// of one.  Past b the group repeats connection b - 1 with the value -0,
+// which leaves every value unchanged bit for bit (x + -0 = x, also for
+// x = -0), so its writes store what the cell already holds.
+template <typename T, int TS, int GC, int TT>
+__device__ __forceinline__ void ring_apply(T* col, const int32_t* rows, const T* v, int a,
+                                           int b, int shift, int depth) {
+#pragma clang fp contract(off)
+  constexpr int G = GC * TS;
+  for (int k0 = a; k0 < b; k0 += GC) {
+    int r[G];
+    T x[G], add[G];
+#pragma unroll
+    for (int c = 0; c < GC; ++c) {
+      const bool live = k0 + c < b;
+      const int k = live ? k0 + c : b - 1;
+      const int r0 = rows[k] + shift;
+#pragma unroll
+      for (int j = 0; j < TS; ++j) {
+        const int u = c * TS + j;
+        int rr = r0 + j;  // < depth + 2: two wraps cover depth 1
+        rr = rr >= depth ? rr - depth : rr;
+        rr = rr >= depth ? rr - depth : rr;
+        r[u] = rr;
+        const T vk = v[j * kRingChunk + k];
+        add[u] = live ? vk : T(-0.0);
       }
     }
-    for (int r = 0; r < depth; ++r) qout[(int64_t)r * width + t] = tile[r * 64 + lane];
+#pragma unroll
+    for (int u = 0; u < G; ++u) x[u] = col[r[u] * TT];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      T base = x[u];
+#pragma unroll
+      for (int q = 0; q < u; ++q) base = r[q] == r[u] ? x[q] : base;
+      x[u] = base + add[u];
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) col[r[u] * TT] = x[u];
+  }
+}
+
+template <typename T, int TAPS, bool REC>
+__global__ __launch_bounds__(kRingThreads) void interp_delay_tile_kernel(
+    const T* qin, T* qout, int depth, int64_t width, const int64_t* __restrict__ slot_p,
+    const int32_t* __restrict__ in_ptr, const int32_t* __restrict__ in_edge,
+    const T* __restrict__ w, const T* __restrict__ g, const T* __restrict__ f,
+    const int32_t* __restrict__ base, const RingRec<T, TAPS>* __restrict__ rec) {
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned char dn_smem_raw[];
+  RingStage<T, TAPS>& st = *reinterpret_cast<RingStage<T, TAPS>*>(dn_smem_raw);
+  T* tile = reinterpret_cast<T*>(dn_smem_raw + sizeof(RingStage<T, TAPS>));
+  constexpr int TT = kRingTileTargets;
+  constexpr int RSTEP = kRingThreads / TT;  // rows staged per pass of the workgroup
+  const int tid = threadIdx.x;
+  const int64_t t0 = (int64_t)blockIdx.x * TT;
+  const int nt = (int)(width - t0 < TT ? width - t0 : TT);
+  const int slot = ring_slot(slot_p, depth);
+  {
+    // loads clamped into the ring (no branches, so all eight are in flight)
+    const int c = tid % TT;
+    const bool live = c < nt;
+    const T* src = qin + t0 + (live ? c : 0);
+    for (int r0 = tid / TT; r0 < depth; r0 += 8 * RSTEP) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int r = r0 + u * RSTEP < depth ? r0 + u * RSTEP : depth - 1;
+        v[u] = keep_if(src[(int64_t)r * width], live && r != slot);
+      }
+      // rows past the ring repeat row depth - 1 with its own value
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int r = r0 + u * RSTEP < depth ? r0 + u * RSTEP : depth - 1;
+        tile[r * TT + c] = v[u];
+      }
+    }
+  }
+  const int i0 = in_ptr[t0], i1 = in_ptr[t0 + nt];
+  int e0 = 0, e1 = 0;
+  if (tid < nt) {
+    e0 = in_ptr[t0 + tid];
+    e1 = in_ptr[t0 + tid + 1];
+  }
+  const int passes = (TAPS == 2 && i1 - i0 > kRingChunk) ? 2 : 1;
+  for (int pass = 0; pass < passes; ++pass) {
+    for (int c0 = i0; c0 < i1; c0 += kRingChunk) {
+      const int n = i1 - c0 < kRingChunk ? i1 - c0 : kRingChunk;
+      for (int b0 = 0; b0 < n; b0 += kRingBatch) {
+        // indices past the chunk repeat its last connection: every load is
+        // unconditional and in bounds, so all of them are in flight at once
+        int e[kRingPer];
+#pragma unroll
+        for (int u = 0; u < kRingPer; ++u) {
+          const int k = b0 + tid + u * kRingThreads;
+          e[u] = in_edge[c0 + (k < n ? k : n - 1)];
+        }
+        T val[kRingPer][TAPS];
+        int32_t row[kRingPer];
+        if (REC) {
+          RingRec<T, TAPS> rv[kRingPer];
+#pragma unroll
+          for (int u = 0; u < kRingPer; ++u) rv[u] = rec[e[u]];
+#pragma unroll
+          for (int u = 0; u < kRingPer; ++u) {
+#pragma unroll
+            for (int j = 0; j < TAPS; ++j) val[u][j] = rv[u].v[j];
+            row[u] = rv[u].row;
+          }
+        } else {
+          T wv[kRingPer], gv[kRingPer], fv[kRingPer][3];
+          int32_t bv[kRingPer];
+#pragma unroll
+          for (int u = 0; u < kRingPer; ++u) {
+            wv[u] = w[e[u]];
+            gv[u] = g[e[u]];
+            bv[u] = base[e[u]];
+            if (TAPS == 2) fv[u][0] = f[e[u]];
+            if (TAPS == 3)
+#pragma unroll
+              for (int j = 0; j < 3; ++j) fv[u][j] = f[(int64_t)e[u] * 3 + j];
+          }
+#pragma unroll
+          for (int u = 0; u < kRingPer; ++u) {
+            ring_tap_values<T, TAPS>(wv[u], gv[u], fv[u], val[u]);
+            row[u] = (int32_t)ring_row(slot, bv[u], 0, depth);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kRingPer; ++u) {
+          // past the chunk: connection n - 1's own values into its own slot
+          const int k = b0 + tid + u * kRingThreads < n ? b0 + tid + u * kRingThreads : n - 1;
+          if (TAPS == 2 && passes == 2) {
+            st.v[0][k] = pass == 0 ? val[u][0] : val[u][TAPS - 1];
+          } else {
+#pragma unroll
+            for (int j = 0; j < TAPS; ++j) st.v[j][k] = val[u][j];
+          }
+          st.row[k] = row[u];
+        }
+      }
+      __syncthreads();
+      if (tid < nt) {
+        const int a = (e0 > c0 ? e0 : c0) - c0;
+        const int b = (e1 < c0 + n ? e1 : c0 + n) - c0;
+        T* col = tile + tid;
+        if (TAPS == 3) {
+          ring_apply<T, 3, 3, TT>(col, st.row, &st.v[0][0], a, b, 0, depth);
+        } else if (TAPS == 2 && passes == 1) {
+          ring_apply<T, 1, 8, TT>(col, st.row, st.v[0], a, b, 0, depth);
+          ring_apply<T, 1, 8, TT>(col, st.row, st.v[TAPS - 1], a, b, 1, depth);
+        } else {
+          ring_apply<T, 1, 8, TT>(col, st.row, st.v[0], a, b, pass, depth);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  {
+    const int c = tid % TT;
+    if (c < nt)
+      for (int r = tid / TT; r < depth; r += RSTEP) qout[(int64_t)r * width + t0 + c] = tile[r * TT + c];
   }
 }
 
@@ -168,25 +367,80 @@ __global__ __launch_bounds__(256) void interp_delay_vjp_kernel(
 
 }  // namespace dn
 
+// LDS of one ring-tile workgroup: the chunk stage (48-64 KiB for float32,
+// 80-112 KiB for float64) and depth rows of the tile's columns; deeper rings
+// take the global-memory form.
+constexpr size_t kRingLdsMax = 160 * 1024;
+
+template <typename T, int TAPS>
+static void ring_tiles_launch(const T* src, T* q, int64_t depth, int64_t width,
+                              const int64_t* slot, const int32_t* in_ptr,
+                              const int32_t* in_edge, const T* w, const T* g, const T* f,
+                              const int32_t* base, const void* rec, size_t lds,
+                              hipStream_t stream) {
+  const dim3 tiles((unsigned)((width + dn::kRingTileTargets - 1) / dn::kRingTileTargets));
+  const dim3 block(dn::kRingThreads);
+  const auto* r = static_cast<const dn::RingRec<T, TAPS>*>(rec);
+  if (rec)
+    hipLaunchKernelGGL((dn::interp_delay_tile_kernel<T, TAPS, true>), tiles, block, lds,
+                       stream, src, q, (int)depth, width, slot, in_ptr, in_edge, w, g, f,
+                       base, r);
+  else
+    hipLaunchKernelGGL((dn::interp_delay_tile_kernel<T, TAPS, false>), tiles, block, lds,
+                       stream, src, q, (int)depth, width, slot, in_ptr, in_edge, w, g, f,
+                       base, r);
+}
+
+template <typename T, int TAPS>
+static void ring_pack_launch(void* rec, int64_t n_edge, int64_t depth, const int64_t* slot,
+                             const T* w, const T* g, const T* f, const int32_t* base,
+                             hipStream_t stream) {
+  hipLaunchKernelGGL((dn::interp_delay_pack_kernel<T, TAPS>),
+                     dim3((unsigned)((n_edge + 255) / 256)), dim3(256), 0, stream,
+                     static_cast<dn::RingRec<T, TAPS>*>(rec), n_edge, (int)depth, slot, w, g, f,
+                     base);
+}
+
+template <typename T>
+static size_t ring_rec_bytes(int taps) {
+  return taps == 1 ? sizeof(dn::RingRec<T, 1>)
+                   : taps == 2 ? sizeof(dn::RingRec<T, 2>) : sizeof(dn::RingRec<T, 3>);
+}
+
+// n_edge >= 0 with work: the records are packed first (work holds n_edge of
+// them) and the tile kernel gathers one record per connection.
 template <typename T>
 static int interp_enqueue_impl(const T* qin, T* q, int64_t depth, int64_t width,
                                const int64_t* slot, int taps, const int32_t* in_ptr,
                                const int32_t* in_edge, const T* w, const T* g, const T* f,
-                               const int32_t* base, hipStream_t stream) {
+                               const int32_t* base, int64_t n_edge, void* work,
+                               int64_t work_bytes, hipStream_t stream) {
   if (!q || !slot || !in_ptr || depth < 1 || depth > (1 << 30) || width < 1) return DN_EINVAL;
   if (taps < 1 || taps > 3 || width > (int64_t)INT32_MAX * 256) return DN_EINVAL;
-  const size_t lds = (size_t)depth * 64 * sizeof(T);
-  if (qin && qin != q && lds <= 64 * 1024) {
-    const dim3 tiles((unsigned)((width + 63) / 64));
+  if (work && (n_edge < 0 || n_edge > (int64_t)INT32_MAX ||
+               work_bytes < n_edge * (int64_t)ring_rec_bytes<T>(taps)))
+    return DN_EINVAL;
+  const size_t stage = taps == 1   ? sizeof(dn::RingStage<T, 1>)
+                       : taps == 2 ? sizeof(dn::RingStage<T, 2>)
+                                   : sizeof(dn::RingStage<T, 3>);
+  const size_t lds = stage + (size_t)depth * dn::kRingTileTargets * sizeof(T);
+  if (lds <= kRingLdsMax) {
+    const T* src = qin ? qin : q;
+    const void* rec = (work && n_edge > 0) ? work : nullptr;
+    if (rec) {
+      if (taps == 1) ring_pack_launch<T, 1>(work, n_edge, depth, slot, w, g, f, base, stream);
+      else if (taps == 2) ring_pack_launch<T, 2>(work, n_edge, depth, slot, w, g, f, base, stream);
+      else ring_pack_launch<T, 3>(work, n_edge, depth, slot, w, g, f, base, stream);
+    }
     if (taps == 1)
-      hipLaunchKernelGGL((dn::interp_delay_tile_kernel<T, 1>), tiles, dim3(64), lds, stream,
-                         qin, q, (int)depth, width, slot, in_ptr, in_edge, w, g, f, base);
+      ring_tiles_launch<T, 1>(src, q, depth, width, slot, in_ptr, in_edge, w, g, f, base, rec,
+                              lds, stream);
     else if (taps == 2)
-      hipLaunchKernelGGL((dn::interp_delay_tile_kernel<T, 2>), tiles, dim3(64), lds, stream,
-                         qin, q, (int)depth, width, slot, in_ptr, in_edge, w, g, f, base);
+      ring_tiles_launch<T, 2>(src, q, depth, width, slot, in_ptr, in_edge, w, g, f, base, rec,
+                              lds, stream);
     else
-      hipLaunchKernelGGL((dn::interp_delay_tile_kernel<T, 3>), tiles, dim3(64), lds, stream,
-                         qin, q, (int)depth, width, slot, in_ptr, in_edge, w, g, f, base);
+      ring_tiles_launch<T, 3>(src, q, depth, width, slot, in_ptr, in_edge, w, g, f, base, rec,
+                              lds, stream);
     return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
   }
   if (qin && qin != q &&
@@ -237,7 +491,7 @@ DN_EXPORT int dn_interp_delay_enqueue_f32(const float* qin, float* q, int64_t de
                                           const float* w, const float* g, const float* f,
                                           const int32_t* base, hipStream_t stream) {
   return interp_enqueue_impl<float>(qin, q, depth, width, slot, taps, in_ptr, in_edge, w, g,
-                                    f, base, stream);
+                                    f, base, -1, nullptr, 0, stream);
 }
 DN_EXPORT int dn_interp_delay_enqueue_f64(const double* qin, double* q, int64_t depth, int64_t width,
                                           const int64_t* slot, int taps,
@@ -245,7 +499,31 @@ DN_EXPORT int dn_interp_delay_enqueue_f64(const double* qin, double* q, int64_t 
                                           const double* w, const double* g, const double* f,
                                           const int32_t* base, hipStream_t stream) {
   return interp_enqueue_impl<double>(qin, q, depth, width, slot, taps, in_ptr, in_edge, w,
-                                     g, f, base, stream);
+                                     g, f, base, -1, nullptr, 0, stream);
+}
+DN_EXPORT int64_t dn_interp_delay_work_bytes(int64_t n_edge, int taps, int f64) {
+  if (n_edge < 0 || taps < 1 || taps > 3) return -1;
+  return n_edge * (int64_t)(f64 ? ring_rec_bytes<double>(taps) : ring_rec_bytes<float>(taps));
+}
+DN_EXPORT int dn_interp_delay_enqueue_ws_f32(const float* qin, float* q, int64_t depth,
+                                             int64_t width, const int64_t* slot, int taps,
+                                             const int32_t* in_ptr, const int32_t* in_edge,
+                                             const float* w, const float* g, const float* f,
+                                             const int32_t* base, int64_t n_edge, void* work,
+                                             int64_t work_bytes, hipStream_t stream) {
+  if (!work) return DN_EINVAL;
+  return interp_enqueue_impl<float>(qin, q, depth, width, slot, taps, in_ptr, in_edge, w, g,
+                                    f, base, n_edge, work, work_bytes, stream);
+}
+DN_EXPORT int dn_interp_delay_enqueue_ws_f64(const double* qin, double* q, int64_t depth,
+                                             int64_t width, const int64_t* slot, int taps,
+                                             const int32_t* in_ptr, const int32_t* in_edge,
+                                             const double* w, const double* g, const double* f,
+                                             const int32_t* base, int64_t n_edge, void* work,
+                                             int64_t work_bytes, hipStream_t stream) {
+  if (!work) return DN_EINVAL;
+  return interp_enqueue_impl<double>(qin, q, depth, width, slot, taps, in_ptr, in_edge, w,
+                                     g, f, base, n_edge, work, work_bytes, stream);
 }
 DN_EXPORT int dn_interp_delay_vjp_f32(const float* G, int64_t depth, int64_t width,
                                       const int64_t* slot, int taps, int64_t n_edge,

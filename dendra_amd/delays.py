@@ -108,10 +108,13 @@ class _InterpEnqueue(torch.autograd.Function):
         out = torch.empty_like(qin)
         w, g = weights.contiguous(), gates.contiguous()
         f = None if fraction is None else fraction.contiguous()  # None: one tap
-        p = _lib.ptr
-        st = _fn("dn_interp_delay_enqueue", out)(p(qin), p(out), depth, width, p(slot), taps,
-                                                 p(in_ptr), p(in_edge), p(w), p(g), p(f),
-                                                 p(base), _lib.stream_handle(out.device))
+        p, lib, n = _lib.ptr, _lib.load(), w.numel()
+        nbytes = lib.dn_interp_delay_work_bytes(n, taps, int(out.dtype == torch.float64))
+        work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=out.device)
+        st = _fn("dn_interp_delay_enqueue_ws", out)(p(qin), p(out), depth, width, p(slot), taps,
+                                                    p(in_ptr), p(in_edge), p(w), p(g), p(f),
+                                                    p(base), n, p(work), nbytes,
+                                                    _lib.stream_handle(out.device))
         _lib.check(st, "interpolated-delay enqueue")
         ctx.save_for_backward(slot, w, g, f, base, post)
         ctx.shape, ctx.taps = (depth, width), taps

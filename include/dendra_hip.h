@@ -170,7 +170,13 @@ int dn_bt3_tree_f64(const double* main, const double* edge, const double* rhs,
  * target's connections in ascending order; w, g, base (n_edge) and f
  * (n_edge, or 3 n_edge) are per connection; base is the first tap's row
  * offset modulo depth.  dn_interp_delay_vjp_* writes the per-connection
- * cotangents dw, dg, df for a ring cotangent G. */
+ * cotangents dw, dg, df for a ring cotangent G.
+ * dn_interp_delay_enqueue_ws_*: the same transition with a workspace of
+ * dn_interp_delay_work_bytes(n_edge, taps, f64) bytes (n_edge = number of
+ * connections): each connection's tap values and first row are packed into
+ * one record (connection order, coalesced) before the target-major pass, so
+ * that pass gathers one record per connection instead of w, g, f and base
+ * separately; results are identical bit for bit. */
 int dn_interp_delay_enqueue_f32(const float* qin, float* q, int64_t depth, int64_t width,
                                 const int64_t* slot, int taps, const int32_t* in_ptr,
                                 const int32_t* in_edge, const float* w, const float* g,
@@ -179,6 +185,17 @@ int dn_interp_delay_enqueue_f64(const double* qin, double* q, int64_t depth, int
                                 const int64_t* slot, int taps, const int32_t* in_ptr,
                                 const int32_t* in_edge, const double* w, const double* g,
                                 const double* f, const int32_t* base, hipStream_t stream);
+int64_t dn_interp_delay_work_bytes(int64_t n_edge, int taps, int f64);
+int dn_interp_delay_enqueue_ws_f32(const float* qin, float* q, int64_t depth, int64_t width,
+                                   const int64_t* slot, int taps, const int32_t* in_ptr,
+                                   const int32_t* in_edge, const float* w, const float* g,
+                                   const float* f, const int32_t* base, int64_t n_edge,
+                                   void* work, int64_t work_bytes, hipStream_t stream);
+int dn_interp_delay_enqueue_ws_f64(const double* qin, double* q, int64_t depth, int64_t width,
+                                   const int64_t* slot, int taps, const int32_t* in_ptr,
+                                   const int32_t* in_edge, const double* w, const double* g,
+                                   const double* f, const int32_t* base, int64_t n_edge,
+                                   void* work, int64_t work_bytes, hipStream_t stream);
 int dn_interp_delay_vjp_f32(const float* G, int64_t depth, int64_t width,
                             const int64_t* slot, int taps, int64_t n_edge,
                             const int32_t* post, const float* w, const float* g,

@@ -43,13 +43,22 @@ def case(rng, dtype, taps, packed, depth=201, n_src=700, n=20000, width=300):
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("taps", [1, 2, 3])
 @pytest.mark.parametrize("packed", [False, True])
-def test_projection_is_bitwise_the_oracle(dtype, taps, packed):
+@pytest.mark.parametrize("shape", [(20000, 300, "uniform"), (40000, 45, "uniform"),
+                                   (12000, 70, "skewed"), (100, 300, "uniform")])
+def test_projection_is_bitwise_the_oracle(dtype, taps, packed, shape):
+    """Fan-ins of 66 and 890 connections per target (a tile's connections
+    span one to many staged chunks), one target holding half of all
+    connections, mostly empty targets, widths off the tile size."""
     from dendra_amd.history import source_history_project
     from dendra_amd.delays import delay_interpolation
 
-    rng = np.random.default_rng(3 * taps + packed)
-    w, src, post, delays, history, meta = case(rng, dtype, taps, packed)
-    depth, width, slot, dt = history.shape[0], 300, 150, 0.025
+    n, width, kind = shape
+    rng = np.random.default_rng(3 * taps + packed + n)
+    w, src, post, delays, history, meta = case(rng, dtype, taps, packed, n=n, width=width)
+    if kind == "skewed":
+        post[: n // 2] = 5
+        rng.shuffle(post)
+    depth, slot, dt = history.shape[0], 150, 0.025
     tdt = torch.float32 if dtype == np.float32 else torch.float64
     kw = {k: dev(v) for k, v in meta.items()}
     if taps == 1:

@@ -179,6 +179,136 @@ def test_three_tap_gradients_match_autograd():
         torch.testing.assert_close(a.cpu(), b, rtol=1e-11, atol=1e-13)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("taps", [2, 3])
+@pytest.mark.parametrize("shape", [(201, 45, 40000, "uniform"), (201, 70, 12000, "skewed"),
+                                   (3000, 10, 600, "uniform"), (2, 40, 3000, "uniform")])
+def test_long_fan_in_and_deep_rings_are_bitwise_the_oracle(dtype, taps, shape):
+    """Targets with hundreds of connections (the tile's connections span many
+    staged chunks, so two taps take one pass per tap), one target holding
+    half of all connections, widths that are not a multiple of the tile, and
+    a ring too deep for the LDS tile (the global-memory form)."""
+    from dendra_amd.delays import dense_interpolated_delay_transition
+
+    depth, width, n, kind = shape
+    rng = np.random.default_rng(depth + width + taps)
+    q, w, g, delays, post = case(rng, dtype, depth=depth, width=width, n=n)
+    if kind == "skewed":
+        post[: n // 2] = 3
+        rng.shuffle(post)
+    dt, slot = 0.025, depth - 2
+    dev = lambda a, t=None: torch.as_tensor(a, device="cuda", dtype=t)
+    kw = dict(post_idx=dev(post, torch.int64), depth=depth, width=width, taps=taps)
+    if taps == 2:
+        lower, frac = twotap_ref.two_tap_split(delays, dt, dtype)
+        tr = dense_interpolated_delay_transition(
+            dev(q), dev([slot], torch.int64), dev([0], torch.int64), dev(w), dev(g),
+            dev(delays.astype(dtype)), dt, sigma=0.0, **kw)
+        want = twotap_ref.enqueue(q, slot, w, g, frac, lower, post)
+    else:
+        offsets, fractions = twotap_ref.three_tap_split(delays, dt, 0.6, dtype)
+        tr = dense_interpolated_delay_transition(
+            dev(q), dev([slot], torch.int64), dev([0], torch.int64), dev(w), dev(g),
+            dev(delays.astype(dtype)), dt, sigma=0.6,
+            delay_taps=(dev(offsets, torch.int64), dev(fractions)), **kw)
+        want = twotap_ref.enqueue3(q, slot, w, g, fractions, offsets, post)
+    assert np.array_equal(tr.queue.cpu().numpy().view(np.uint8), want.view(np.uint8))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("taps", [1, 2, 3])
+def test_workspace_and_direct_entries_agree_with_the_oracle(dtype, taps):
+    """dn_interp_delay_enqueue_ws_* (records packed first) and
+    dn_interp_delay_enqueue_* (four gathers per connection) give the oracle's
+    ring bit for bit; one target holds half of the 12,000 connections, so its
+    tile spans several staged chunks."""
+    from dendra_amd import _lib
+    from dendra_amd.delays import delay_plan
+
+    rng = np.random.default_rng(40 + taps)
+    depth, width, n, slot = 201, 70, 12000, 199
+    q, w, g, _, post = case(rng, dtype, depth=depth, width=width, n=n)
+    post[: n // 2] = 3
+    rng.shuffle(post)
+    if taps == 3:
+        lower = rng.integers(1, 190, n)
+        offsets = np.stack([lower - 1, lower, lower + 1], -1)
+        f = rng.dirichlet([2.0, 3.0, 2.0], n).astype(dtype)
+        base = (offsets[:, 0] % depth).astype(np.int32)
+        want = twotap_ref.enqueue3(q, slot, w, g, f, offsets, post)
+    else:
+        lower = rng.integers(0, 200, n)
+        base = (lower % depth).astype(np.int32)
+        f = rng.uniform(0, 1, n).astype(dtype)
+        if taps == 2:
+            want = twotap_ref.enqueue(q, slot, w, g, f, lower, post)
+        else:
+            want = np.array(q, copy=True)
+            want[slot] = 0
+            for e in np.argsort(post, kind="stable"):
+                r = (slot + lower[e]) % depth
+                want[r, post[e]] = dtype(want[r, post[e]] + dtype(w[e]) * dtype(g[e]))
+    dev = lambda a, t=None: torch.as_tensor(a, device="cuda", dtype=t)
+    in_ptr, in_edge, _ = delay_plan(dev(post, torch.int64), width)
+    tw, tg, tf, tb, ts, src = dev(w), dev(g), dev(f), dev(base), dev([slot], torch.int64), dev(q)
+    lib, p, sfx = _lib.load(), _lib.ptr, "f32" if dtype == np.float32 else "f64"
+    stream = _lib.stream_handle(src.device)
+    fp = None if taps == 1 else p(tf)
+    direct = torch.empty_like(src)
+    assert getattr(lib, f"dn_interp_delay_enqueue_{sfx}")(
+        p(src), p(direct), depth, width, p(ts), taps, p(in_ptr), p(in_edge), p(tw), p(tg), fp,
+        p(tb), stream) == 0
+    nbytes = lib.dn_interp_delay_work_bytes(n, taps, int(dtype == np.float64))
+    work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    packed = torch.empty_like(src)
+    fn = getattr(lib, f"dn_interp_delay_enqueue_ws_{sfx}")
+    assert fn(p(src), p(packed), depth, width, p(ts), taps, p(in_ptr), p(in_edge), p(tw), p(tg),
+              fp, p(tb), n, p(work), nbytes, stream) == 0
+    assert fn(p(src), p(packed), depth, width, p(ts), taps, p(in_ptr), p(in_edge), p(tw), p(tg),
+              fp, p(tb), n, p(work), nbytes - 1, stream) == -1  # too small: DN_EINVAL
+    torch.cuda.synchronize()
+    for got in (direct, packed):
+        assert np.array_equal(got.cpu().numpy().view(np.uint8), want.view(np.uint8))
+
+
+@pytest.mark.parametrize("qin_null", [True, False])
+def test_in_place_enqueue_through_the_c_abi(qin_null):
+    """qin NULL or equal to q: the ring is transformed in place (one tap,
+    fixed delays), equal to the out-of-place result."""
+    from dendra_amd import _lib
+    from dendra_amd.delays import delay_plan
+
+    rng = np.random.default_rng(12)
+    depth, width, n = 201, 300, 30000
+    q, w, g, _, post = case(rng, np.float32, depth=depth, width=width, n=n)
+    base = rng.integers(0, depth, n).astype(np.int32)
+    dev = lambda a, t=None: torch.as_tensor(a, device="cuda", dtype=t)
+    post_t = dev(post, torch.int64)
+    in_ptr, in_edge, _ = delay_plan(post_t, width)
+    tw, tg, tb, slot = dev(w), dev(g), dev(base), dev([17], torch.int64)
+    lib, p = _lib.load(), _lib.ptr
+    ref = torch.empty((depth, width), device="cuda")
+    src = dev(q)
+    st = lib.dn_interp_delay_enqueue_f32(p(src), p(ref), depth, width, p(slot), 1, p(in_ptr),
+                                         p(in_edge), p(tw), p(tg), None, p(tb),
+                                         _lib.stream_handle(src.device))
+    assert st == 0
+    inplace = dev(q)
+    st = lib.dn_interp_delay_enqueue_f32(None if qin_null else p(inplace), p(inplace), depth,
+                                         width, p(slot), 1, p(in_ptr), p(in_edge), p(tw),
+                                         p(tg), None, p(tb), _lib.stream_handle(src.device))
+    assert st == 0
+    torch.cuda.synchronize()
+    assert torch.equal(inplace, ref)
+    assert torch.equal(src, dev(q))
+    want = np.array(q, copy=True)
+    want[17] = 0
+    for e in np.argsort(post, kind="stable"):
+        r = (17 + base[e]) % depth
+        want[r, post[e]] = np.float32(want[r, post[e]] + np.float32(w[e]) * np.float32(g[e]))
+    assert np.array_equal(ref.cpu().numpy().view(np.uint8), want.view(np.uint8))
+
+
 def test_delay_plan_follows_each_post_idx_tensor():
     """A freed post_idx whose storage the allocator hands to a new tensor of
     the same size must not lend its plan to it."""
