@@ -135,6 +135,7 @@ for _sfx in ("f32", "f64"):
     _SIGNATURES[f"dn_bt3_tree_{_sfx}"] = (i32, [p] * 6 + [i64, p, p, p, i64, i64, p, i64, p])
 _SIGNATURES["dn_tree_contract_max_sections"] = (i32, [i32])
 _SIGNATURES["dn_interp_delay_work_bytes"] = (i64, [i64, i32, i32])
+_SIGNATURES["dn_history_work_bytes"] = (i64, [i64, i32, i32, i32])
 for _sfx in ("f32", "f64"):
     _SIGNATURES[f"dn_interp_delay_enqueue_{_sfx}"] = (i32, [p, p, i64, i64, p, i32] + [p] * 6 + [p])
     _SIGNATURES[f"dn_interp_delay_enqueue_ws_{_sfx}"] = (
@@ -143,6 +144,8 @@ for _sfx in ("f32", "f64"):
         i32, [p, i64, i64, p, i32, i64] + [p] * 8 + [p])
     _SIGNATURES[f"dn_history_project_{_sfx}"] = (
         i32, [p, i32, i64, i64, p, i32] + [p] * 8 + [i64, p])
+    _SIGNATURES[f"dn_history_project_ws_{_sfx}"] = (
+        i32, [p, i32, i64, i64, p, i32] + [p] * 8 + [i64, i64, p, i64, p])
     _SIGNATURES[f"dn_history_project_vjp_{_sfx}"] = (
         i32, [p, p, i32, i64, i64, p, i32, i64] + [p] * 9 + [p])
     _SIGNATURES[f"dn_history_record_{_sfx}"] = (i32, [p, i64, i64, p, p, p])

@@ -49,24 +49,98 @@ __device__ __forceinline__ T hist_gate(const void* hist, int64_t n_col, int64_t 
 
 // A workgroup of 256 threads owns kProjTargets targets, whose connections
 // are one contiguous range of the target-major CSR.  The range is taken in
-// chunks of kProjChunk: all threads gather a chunk's connections (edge,
-// column, tap offsets, gates, tap weights and w; kProjPer independent
-// connections per thread in flight) and leave each connection's value v in
-// LDS, then one lane per target adds its own values of the chunk in
-// ascending connection order.  The sums are those of the sequential
-// lane-per-target walk, bit for bit.
+// chunks of kProjChunk: all threads gather a chunk's connections and leave
+// each connection's value v in LDS (kProjPer connections per thread, every
+// load unconditional so that all are in flight), then one lane per target
+// adds its own values of the chunk in ascending connection order.  The sums
+// are those of the sequential lane-per-target walk, bit for bit.
 constexpr int kProjTargets = 32;
 constexpr int kProjThreads = 256;
 constexpr int kProjPer = 8;
 constexpr int kProjChunk = kProjThreads * kProjPer;
 
+// One connection's history reads packed for one step: its column, the rows
+// (slot - off_j) mod depth of its taps, the tap coefficients (two taps:
+// 1 - a and a, rounded as the reference rounds them; three taps: f_j), w,
+// and for a packed history its bit mask.  Written in connection order
+// (coalesced), so the target-major pass gathers one record and the taps'
+// history words per connection instead of five scattered arrays.
+template <typename T, int TAPS>
+struct alignas(16) ProjRec {
+  int32_t col;
+  int32_t row[TAPS];
+  T c[TAPS];
+  T w;
+};
+template <typename T, int TAPS>
+struct alignas(16) ProjRecMask {
+  int64_t mask;
+  int32_t col;
+  int32_t row[TAPS];
+  T c[TAPS];
+  T w;
+};
 template <typename T, bool PACKED, int TAPS>
+using ProjRecT = typename std::conditional<PACKED, ProjRecMask<T, TAPS>, ProjRec<T, TAPS>>::type;
+
+template <typename T, bool PACKED, int TAPS>
+__global__ __launch_bounds__(256) void history_pack_rec_kernel(
+    ProjRecT<T, PACKED, TAPS>* __restrict__ rec, int64_t n_conn, int64_t depth,
+    const int64_t* __restrict__ slot_p, const int32_t* __restrict__ col,
+    const int64_t* __restrict__ mask, const int32_t* __restrict__ off,
+    const T* __restrict__ w, const T* __restrict__ frac) {
+#pragma clang fp contract(off)
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_conn) return;
+  const int slot = hist_slot(slot_p, depth);
+  ProjRecT<T, PACKED, TAPS> r;
+  if constexpr (PACKED) r.mask = mask[e];
+  r.col = col[e];
+#pragma unroll
+  for (int j = 0; j < TAPS; ++j) r.row[j] = (int32_t)hist_row(slot, off[e * TAPS + j], depth);
+  if (TAPS == 1) {
+    r.c[0] = T(1);
+  } else if (TAPS == 2) {
+    const T a = frac[e];
+    r.c[0] = T(1) - a;
+    r.c[TAPS - 1] = a;
+  } else {
+#pragma unroll
+    for (int j = 0; j < TAPS; ++j) r.c[j] = frac[e * 3 + j];
+  }
+  r.w = w[e];
+  rec[e] = r;
+}
+
+// v = w s with s = g0 (one tap), g0 c0 + g1 c1 (two), (g0 c0 + g1 c1) + g2 c2
+// (three), every product and sum rounded.
+template <typename T, int TAPS>
+__device__ __forceinline__ T proj_value(const T* gv, const T* c, T w) {
+#pragma clang fp contract(off)
+  T s;
+  if (TAPS == 1) {
+    s = gv[0];
+  } else if (TAPS == 2) {
+    const T lo = gv[0] * c[0];
+    const T hi = gv[TAPS - 1] * c[TAPS - 1];
+    s = lo + hi;
+  } else {
+    const T p0 = gv[0] * c[0];
+    const T p1 = gv[1] * c[1];
+    const T p2 = gv[TAPS - 1] * c[TAPS - 1];
+    s = (p0 + p1) + p2;
+  }
+  return w * s;
+}
+
+template <typename T, bool PACKED, int TAPS, bool REC>
 __global__ __launch_bounds__(kProjThreads) void history_project_kernel(
     const void* __restrict__ hist, int64_t depth, int64_t n_col,
     const int64_t* __restrict__ slot_p, const int32_t* __restrict__ in_ptr,
     const int32_t* __restrict__ in_edge, const int32_t* __restrict__ col,
     const int64_t* __restrict__ mask, const int32_t* __restrict__ off,
-    const T* __restrict__ w, const T* __restrict__ frac, T* __restrict__ out, int64_t width) {
+    const T* __restrict__ w, const T* __restrict__ frac,
+    const ProjRecT<T, PACKED, TAPS>* __restrict__ rec, T* __restrict__ out, int64_t width) {
 #pragma clang fp contract(off)
   __shared__ T vs[kProjChunk];
   const int tid = threadIdx.x;
@@ -83,46 +157,67 @@ __global__ __launch_bounds__(kProjThreads) void history_project_kernel(
   for (int c0 = i0; c0 < i1; c0 += kProjChunk) {
     const int n = i1 - c0 < kProjChunk ? i1 - c0 : kProjChunk;
     // indices past the chunk repeat its last connection: every load is
-    // unconditional and in bounds, so all of them are in flight at once
+    // unconditional and in bounds
     int64_t e[kProjPer];
 #pragma unroll
     for (int u = 0; u < kProjPer; ++u) {
       const int k = tid + u * kProjThreads;
       e[u] = (int64_t)in_edge[c0 + (k < n ? k : n - 1)];
     }
-    int32_t c[kProjPer], o[kProjPer][TAPS];
+    int32_t cu[kProjPer], ru[kProjPer][TAPS];
+    T cf[kProjPer][TAPS], wu[kProjPer];
+    int64_t mu[kProjPer];
+    if (REC) {
+      ProjRecT<T, PACKED, TAPS> rv[kProjPer];
 #pragma unroll
-    for (int u = 0; u < kProjPer; ++u) {
-      c[u] = col[e[u]];
+      for (int u = 0; u < kProjPer; ++u) rv[u] = rec[e[u]];
 #pragma unroll
-      for (int j = 0; j < TAPS; ++j) o[u][j] = off[e[u] * TAPS + j];
+      for (int u = 0; u < kProjPer; ++u) {
+        cu[u] = rv[u].col;
+        wu[u] = rv[u].w;
+        if constexpr (PACKED) mu[u] = rv[u].mask;
+#pragma unroll
+        for (int j = 0; j < TAPS; ++j) {
+          ru[u][j] = rv[u].row[j];
+          cf[u][j] = rv[u].c[j];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kProjPer; ++u) {
+        cu[u] = col[e[u]];
+        wu[u] = w[e[u]];
+        if (PACKED) mu[u] = mask[e[u]];
+#pragma unroll
+        for (int j = 0; j < TAPS; ++j)
+          ru[u][j] = (int32_t)hist_row(slot, off[e[u] * TAPS + j], depth);
+        if (TAPS == 1) {
+          cf[u][0] = T(1);
+        } else if (TAPS == 2) {
+          const T a = frac[e[u]];
+          cf[u][0] = T(1) - a;
+          cf[u][TAPS - 1] = a;
+        } else {
+#pragma unroll
+          for (int j = 0; j < TAPS; ++j) cf[u][j] = frac[e[u] * 3 + j];
+        }
+      }
     }
 #pragma unroll
     for (int u = 0; u < kProjPer; ++u) {
       const int k = tid + u * kProjThreads;
-      {
-        T gv[TAPS];
+      T gv[TAPS];
 #pragma unroll
-        for (int j = 0; j < TAPS; ++j)
-          gv[j] = hist_gate<T, PACKED>(hist, n_col, hist_row(slot, o[u][j], depth), c[u], mask,
-                                       e[u]);
-        T s;
-        if (TAPS == 1) {
-          s = gv[0];
-        } else if (TAPS == 2) {
-          const T a = frac[e[u]];
-          const T lo = gv[0] * (T(1) - a);
-          const T hi = gv[1] * a;
-          s = lo + hi;
+      for (int j = 0; j < TAPS; ++j) {
+        if constexpr (PACKED) {
+          const int64_t word = static_cast<const int64_t*>(hist)[ru[u][j] * n_col + cu[u]];
+          gv[j] = (word & mu[u]) != 0 ? T(1) : T(0);
         } else {
-          const T p0 = gv[0] * frac[e[u] * 3];
-          const T p1 = gv[1] * frac[e[u] * 3 + 1];
-          const T p2 = gv[2] * frac[e[u] * 3 + 2];
-          s = (p0 + p1) + p2;
+          gv[j] = static_cast<const T*>(hist)[ru[u][j] * n_col + cu[u]];
         }
-        // past the chunk: connection n - 1's own value into its own slot
-        vs[k < n ? k : n - 1] = w[e[u]] * s;
       }
+      // past the chunk: connection n - 1's own value into its own slot
+      vs[k < n ? k : n - 1] = proj_value<T, TAPS>(gv, cf[u], wu[u]);
     }
     __syncthreads();
     if (tid < nt) {
@@ -215,45 +310,84 @@ __global__ __launch_bounds__(256) void history_pack_kernel(
 
 static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
+template <typename T, bool PACKED, int TAPS>
+static void project_launch_taps(const void* hist, int64_t depth, int64_t n_col,
+                                const int64_t* slot, const int32_t* in_ptr,
+                                const int32_t* in_edge, const int32_t* col, const int64_t* mask,
+                                const int32_t* off, const T* w, const T* frac, T* out,
+                                int64_t width, int64_t n_conn, void* work, hipStream_t stream) {
+  const dim3 g((unsigned)((width + dn::kProjTargets - 1) / dn::kProjTargets)),
+      b(dn::kProjThreads);
+  using Rec = dn::ProjRecT<T, PACKED, TAPS>;
+  Rec* rec = static_cast<Rec*>(work);
+  if (rec) {
+    hipLaunchKernelGGL((dn::history_pack_rec_kernel<T, PACKED, TAPS>), grid_for(n_conn),
+                       dim3(256), 0, stream, rec, n_conn, depth, slot, col, mask, off, w, frac);
+    hipLaunchKernelGGL((dn::history_project_kernel<T, PACKED, TAPS, true>), g, b, 0, stream,
+                       hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off, w, frac, rec,
+                       out, width);
+  } else {
+    hipLaunchKernelGGL((dn::history_project_kernel<T, PACKED, TAPS, false>), g, b, 0, stream,
+                       hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off, w, frac, rec,
+                       out, width);
+  }
+}
+
 template <typename T, bool PACKED>
 static void project_launch(int taps, const void* hist, int64_t depth, int64_t n_col,
                            const int64_t* slot, const int32_t* in_ptr, const int32_t* in_edge,
                            const int32_t* col, const int64_t* mask, const int32_t* off,
-                           const T* w, const T* frac, T* out, int64_t width,
-                           hipStream_t stream) {
-  const dim3 g((unsigned)((width + dn::kProjTargets - 1) / dn::kProjTargets)),
-      b(dn::kProjThreads);
+                           const T* w, const T* frac, T* out, int64_t width, int64_t n_conn,
+                           void* work, hipStream_t stream) {
   if (taps == 1)
-    hipLaunchKernelGGL((dn::history_project_kernel<T, PACKED, 1>), g, b, 0, stream, hist,
-                       depth, n_col, slot, in_ptr, in_edge, col, mask, off, w, frac, out,
-                       width);
+    project_launch_taps<T, PACKED, 1>(hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
+                                      w, frac, out, width, n_conn, work, stream);
   else if (taps == 2)
-    hipLaunchKernelGGL((dn::history_project_kernel<T, PACKED, 2>), g, b, 0, stream, hist,
-                       depth, n_col, slot, in_ptr, in_edge, col, mask, off, w, frac, out,
-                       width);
+    project_launch_taps<T, PACKED, 2>(hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
+                                      w, frac, out, width, n_conn, work, stream);
   else
-    hipLaunchKernelGGL((dn::history_project_kernel<T, PACKED, 3>), g, b, 0, stream, hist,
-                       depth, n_col, slot, in_ptr, in_edge, col, mask, off, w, frac, out,
-                       width);
+    project_launch_taps<T, PACKED, 3>(hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
+                                      w, frac, out, width, n_conn, work, stream);
 }
 
+static int64_t project_rec_bytes(int taps, int packed, int f64) {
+  auto sz = [&](auto tag) -> int64_t {
+    using T = decltype(tag);
+    if (packed)
+      return taps == 1   ? sizeof(dn::ProjRecMask<T, 1>)
+             : taps == 2 ? sizeof(dn::ProjRecMask<T, 2>)
+                         : sizeof(dn::ProjRecMask<T, 3>);
+    return taps == 1   ? sizeof(dn::ProjRec<T, 1>)
+           : taps == 2 ? sizeof(dn::ProjRec<T, 2>)
+                       : sizeof(dn::ProjRec<T, 3>);
+  };
+  return f64 ? sz(double(0)) : sz(float(0));
+}
+
+// work non-null: n_conn records are packed first (connection order) and the
+// target-major pass gathers one record per connection.
 template <typename T>
 static int project_impl(const void* hist, int packed, int64_t depth, int64_t n_col,
                         const int64_t* slot, int taps, const int32_t* in_ptr,
                         const int32_t* in_edge, const int32_t* col, const int64_t* mask,
                         const int32_t* off, const T* w, const T* frac, T* out, int64_t width,
-                        hipStream_t stream) {
+                        int64_t n_conn, void* work, int64_t work_bytes, hipStream_t stream) {
   if (!slot || !in_ptr || !out || depth < 1 || n_col < 0 || width < 0) return DN_EINVAL;
   if (taps < 1 || taps > 3 || width > (int64_t)INT32_MAX * 256) return DN_EINVAL;
+  if (depth > INT32_MAX) return DN_EINVAL;
+  if (work && (n_conn < 0 || n_conn > (int64_t)INT32_MAX * 256 ||
+               work_bytes < n_conn * project_rec_bytes(taps, packed, sizeof(T) == 8)))
+    return DN_EINVAL;
   if (width == 0) return DN_OK;
   if (!hist || !in_edge || !col || !off || !w || (taps > 1 && !frac) || (packed && !mask))
     return DN_EINVAL;
+  void* rec = (work && n_conn > 0) ? work : nullptr;
   if (packed)
     project_launch<T, true>(taps, hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                            w, frac, out, width, stream);
+                            w, frac, out, width, n_conn, rec, stream);
   else
     project_launch<T, false>(taps, hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                             w, frac, out, width, stream);
+                             w, frac, out, width, n_conn, rec, stream);
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
 
@@ -315,7 +449,17 @@ static int record_impl(T* hist, int64_t depth, int64_t n_col, const int64_t* slo
       const int64_t* mask, const int32_t* off, const T* w, const T* frac, T* out,            \
       int64_t width, hipStream_t stream) {                                                   \
     return project_impl<T>(hist, packed, depth, n_col, slot, taps, in_ptr, in_edge, col,     \
-                           mask, off, w, frac, out, width, stream);                          \
+                           mask, off, w, frac, out, width, -1, nullptr, 0, stream);          \
+  }                                                                                          \
+  DN_EXPORT int dn_history_project_ws_##SFX(                                                 \
+      const void* hist, int packed, int64_t depth, int64_t n_col, const int64_t* slot,       \
+      int taps, const int32_t* in_ptr, const int32_t* in_edge, const int32_t* col,           \
+      const int64_t* mask, const int32_t* off, const T* w, const T* frac, T* out,            \
+      int64_t width, int64_t n_conn, void* work, int64_t work_bytes, hipStream_t stream) {   \
+    if (!work) return DN_EINVAL;                                                             \
+    return project_impl<T>(hist, packed, depth, n_col, slot, taps, in_ptr, in_edge, col,     \
+                           mask, off, w, frac, out, width, n_conn, work, work_bytes,         \
+                           stream);                                                          \
   }                                                                                          \
   DN_EXPORT int dn_history_project_vjp_##SFX(                                                \
       const T* grad_out, const void* hist, int packed, int64_t depth, int64_t n_col,         \
@@ -330,6 +474,11 @@ static int record_impl(T* hist, int64_t depth, int64_t n_col, const int64_t* slo
                                         hipStream_t stream) {                                \
     return record_impl<T>(hist, depth, n_col, slot, gates, stream);                          \
   }
+
+DN_EXPORT int64_t dn_history_work_bytes(int64_t n_conn, int taps, int packed, int f64) {
+  if (n_conn < 0 || taps < 1 || taps > 3) return -1;
+  return n_conn * project_rec_bytes(taps, packed != 0, f64 != 0);
+}
 
 DN_HISTORY_EXPORTS(float, f32)
 DN_HISTORY_EXPORTS(double, f64)

@@ -156,11 +156,14 @@ class _Project(torch.autograd.Function):
         packed = history.dtype == torch.int64
         depth, n_col = map(int, history.shape)
         out = torch.empty(width, dtype=weights.dtype, device=weights.device)
-        p = _lib.ptr
-        fn = getattr(_lib.load(), f"dn_history_project_{_suffix(weights.dtype)}")
+        p, lib, n = _lib.ptr, _lib.load(), weights.numel()
+        f64 = int(weights.dtype == torch.float64)
+        nbytes = lib.dn_history_work_bytes(n, taps, int(packed), f64)
+        work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=weights.device)
+        fn = getattr(lib, f"dn_history_project_ws_{_suffix(weights.dtype)}")
         st = fn(p(history), int(packed), depth, n_col, p(slot), taps, p(in_ptr), p(in_edge),
                 p(col), p(mask) if packed else None, p(off), p(weights),
-                p(frac) if taps > 1 else None, p(out), width,
+                p(frac) if taps > 1 else None, p(out), width, n, p(work), nbytes,
                 _lib.stream_handle(weights.device))
         _lib.check(st, "source-history projection")
         ctx.save_for_backward(history, weights, frac, slot, post, col, mask, off)

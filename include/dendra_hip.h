@@ -218,6 +218,11 @@ int dn_interp_delay_vjp_f64(const double* G, int64_t depth, int64_t width,
  *     with upper weight frac[e], 3 Gaussian with frac[e][3]) and
  *     out[t] = sum over the target-major CSR (in_ptr, in_edge) of
  *     w[e] * mixture, accumulated from 0 in ascending connection order.
+ *   dn_history_project_ws_*: the same projection with a workspace of
+ *     dn_history_work_bytes(n_conn, taps, packed, f64) bytes (n_conn = number
+ *     of connections): each connection's column, tap rows, tap weights, w
+ *     (and mask) are packed into one record (connection order, coalesced)
+ *     before the target-major pass; results are identical bit for bit.
  *   dn_history_project_vjp_*: per connection dw, dfrac (taps 2, 3) and, for
  *     a floating history, dhist += (atomic; zero it first, may be null).
  *   dn_history_record_*: row slot mod depth := gates (in place).
@@ -234,6 +239,19 @@ int dn_history_project_f64(const void* hist, int packed, int64_t depth, int64_t 
                            const int32_t* in_edge, const int32_t* col, const int64_t* mask,
                            const int32_t* off, const double* w, const double* frac,
                            double* out, int64_t width, hipStream_t stream);
+int64_t dn_history_work_bytes(int64_t n_conn, int taps, int packed, int f64);
+int dn_history_project_ws_f32(const void* hist, int packed, int64_t depth, int64_t n_col,
+                              const int64_t* slot, int taps, const int32_t* in_ptr,
+                              const int32_t* in_edge, const int32_t* col, const int64_t* mask,
+                              const int32_t* off, const float* w, const float* frac,
+                              float* out, int64_t width, int64_t n_conn, void* work,
+                              int64_t work_bytes, hipStream_t stream);
+int dn_history_project_ws_f64(const void* hist, int packed, int64_t depth, int64_t n_col,
+                              const int64_t* slot, int taps, const int32_t* in_ptr,
+                              const int32_t* in_edge, const int32_t* col, const int64_t* mask,
+                              const int32_t* off, const double* w, const double* frac,
+                              double* out, int64_t width, int64_t n_conn, void* work,
+                              int64_t work_bytes, hipStream_t stream);
 int dn_history_project_vjp_f32(const float* grad_out, const void* hist, int packed,
                                int64_t depth, int64_t n_col, const int64_t* slot, int taps,
                                int64_t n_conn, const int32_t* post, const int32_t* col,

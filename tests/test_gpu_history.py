@@ -83,6 +83,53 @@ def test_projection_is_bitwise_the_oracle(dtype, taps, packed, shape):
     assert np.array_equal(g.view(np.uint8), want.view(np.uint8))
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("taps", [1, 2, 3])
+@pytest.mark.parametrize("packed", [False, True])
+def test_workspace_and_direct_entries_agree_with_the_oracle(dtype, taps, packed):
+    """dn_history_project_ws_* (records packed first) and
+    dn_history_project_* (five gathers per connection) give the oracle's
+    payload bit for bit; one target holds half of the 12,000 connections."""
+    from dendra_amd import _lib
+    from dendra_amd.delays import delay_plan
+
+    rng = np.random.default_rng(50 + 2 * taps + packed)
+    n, width, slot = 12000, 70, 123
+    w, src, post, _, history, meta = case(rng, dtype, taps, packed, n=n, width=width)
+    post[: n // 2] = 5
+    rng.shuffle(post)
+    depth, n_col = history.shape
+    off = rng.integers(0, depth, (n, taps)).astype(np.int32)
+    frac = None
+    if taps == 2:
+        frac = rng.uniform(0, 1, n).astype(dtype)
+    elif taps == 3:
+        frac = rng.dirichlet([2.0, 3.0, 2.0], n).astype(dtype)
+    col = (meta["word_idx"] if packed else meta["source_pos"]).astype(np.int32)
+    mask = meta.get("bit_mask")
+    want = history_ref.project(history, slot, w, post, width, col=col, offsets=off, frac=frac,
+                               mask=mask, packed=packed)
+    in_ptr, in_edge, _ = delay_plan(dev(post, torch.int64), width)
+    th, tw, toff, tcol, ts = dev(history), dev(w), dev(off), dev(col), dev([slot], torch.int64)
+    tf = None if frac is None else dev(frac)
+    tm = None if mask is None else dev(mask, torch.int64)
+    lib, p, sfx = _lib.load(), _lib.ptr, "f32" if dtype == np.float32 else "f64"
+    stream = _lib.stream_handle(th.device)
+    args = (p(th), int(packed), depth, n_col, p(ts), taps, p(in_ptr), p(in_edge), p(tcol),
+            p(tm), p(toff), p(tw), p(tf))
+    direct = torch.empty(width, dtype=tw.dtype, device="cuda")
+    assert getattr(lib, f"dn_history_project_{sfx}")(*args, p(direct), width, stream) == 0
+    nbytes = lib.dn_history_work_bytes(n, taps, int(packed), int(dtype == np.float64))
+    work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    packed_out = torch.empty_like(direct)
+    fn = getattr(lib, f"dn_history_project_ws_{sfx}")
+    assert fn(*args, p(packed_out), width, n, p(work), nbytes, stream) == 0
+    assert fn(*args, p(packed_out), width, n, p(work), nbytes - 1, stream) == -1  # DN_EINVAL
+    torch.cuda.synchronize()
+    for got in (direct, packed_out):
+        assert np.array_equal(got.cpu().numpy().view(np.uint8), want.view(np.uint8))
+
+
 @pytest.mark.parametrize("packed", [False, True])
 @pytest.mark.parametrize("taps", [None, 2, 3])
 def test_reference_fixture_values_and_gradients(packed, taps):
