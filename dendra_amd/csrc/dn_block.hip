@@ -75,14 +75,28 @@ __global__ __launch_bounds__(256) void bt3_thomas_kernel(const T* __restrict__ l
   for (int c = 0; c < 3; ++c) d[c] = D[c];
   for (int c = 0; c < 9; ++c) w(0, c) = inv[c];
   for (int c = 0; c < 3; ++c) w(0, 9 + c) = d[c];
+  // step k's coupling, block and right-hand side are fetched during step
+  // k - 1 (index clamped into the system: the loads are unconditional)
+  T nl[3], nu[3], nm[9], nd[3];
+  auto fetch = [&](int64_t k) {
+    for (int c = 0; c < 3; ++c) nl[c] = L[(k - 1) * 3 + c];
+    for (int c = 0; c < 3; ++c) nu[c] = U[(k - 1) * 3 + c];
+    for (int c = 0; c < 9; ++c) nm[c] = M[k * 9 + c];
+    for (int c = 0; c < 3; ++c) nd[c] = D[k * 3 + c];
+  };
+  if (K > 1) fetch(1);
   for (int64_t k = 1; k < K; ++k) {
-    const T l[3] = {L[(k - 1) * 3], L[(k - 1) * 3 + 1], L[(k - 1) * 3 + 2]};
-    const T u[3] = {U[(k - 1) * 3], U[(k - 1) * 3 + 1], U[(k - 1) * 3 + 2]};
+    T l[3], u[3], m[9], dk[3];
+    for (int c = 0; c < 3; ++c) l[c] = nl[c];
+    for (int c = 0; c < 3; ++c) u[c] = nu[c];
+    for (int c = 0; c < 9; ++c) m[c] = nm[c];
+    for (int c = 0; c < 3; ++c) dk[c] = nd[c];
+    fetch(k + 1 < K ? k + 1 : K - 1);
     T wd[3];
     mv3(inv, d, wd);
     for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) a[r * 3 + c] = M[k * 9 + r * 3 + c] - l[r] * inv[r * 3 + c] * u[c];
-    for (int r = 0; r < 3; ++r) d[r] = D[k * 3 + r] - l[r] * wd[r];
+      for (int c = 0; c < 3; ++c) a[r * 3 + c] = m[r * 3 + c] - l[r] * inv[r * 3 + c] * u[c];
+    for (int r = 0; r < 3; ++r) d[r] = dk[r] - l[r] * wd[r];
     inv3(a, inv);
     for (int c = 0; c < 9; ++c) w(k, c) = inv[c];
     for (int c = 0; c < 3; ++c) w(k, 9 + c) = d[c];
@@ -90,10 +104,18 @@ __global__ __launch_bounds__(256) void bt3_thomas_kernel(const T* __restrict__ l
   T xn[3];
   mv3(inv, d, xn);
   for (int c = 0; c < 3; ++c) X[(K - 1) * 3 + c] = xn[c];
+  T niv[9], nw[3], nU[3];
+  auto fetch_back = [&](int64_t k) {
+    for (int c = 0; c < 9; ++c) niv[c] = w(k, c);
+    for (int c = 0; c < 3; ++c) nw[c] = w(k, 9 + c);
+    for (int c = 0; c < 3; ++c) nU[c] = U[k * 3 + c];
+  };
+  if (K > 1) fetch_back(K - 2);
   for (int64_t k = K - 2; k >= 0; --k) {
     T t[3], iv[9];
-    for (int c = 0; c < 9; ++c) iv[c] = w(k, c);
-    for (int c = 0; c < 3; ++c) t[c] = w(k, 9 + c) - U[k * 3 + c] * xn[c];
+    for (int c = 0; c < 9; ++c) iv[c] = niv[c];
+    for (int c = 0; c < 3; ++c) t[c] = nw[c] - nU[c] * xn[c];
+    fetch_back(k > 0 ? k - 1 : 0);
     mv3(iv, t, xn);
     for (int c = 0; c < 3; ++c) X[k * 3 + c] = xn[c];
   }
@@ -192,9 +214,23 @@ __global__ __launch_bounds__(256) void bt3_spd_kernel(
     for (int c = 0; c < 9; ++c) C[c] = lk[c];
     for (int c = 0; c < 3; ++c) d[c] = D[c];
     for (int c = 0; c < 3; ++c) w(0, c) = d[c];
+    // step k's inputs are fetched during step k - 1 (clamped index:
+    // unconditional loads)
+    T nl[3], nu[3], nm[9], nd[3];
+    auto fetch = [&](int64_t k) {
+      for (int c = 0; c < 3; ++c) nl[c] = L[(k - 1) * 3 + c];
+      for (int c = 0; c < 3; ++c) nu[c] = U[(k - 1) * 3 + c];
+      for (int c = 0; c < 9; ++c) nm[c] = M[k * 9 + c];
+      for (int c = 0; c < 3; ++c) nd[c] = D[k * 3 + c];
+    };
+    if (K > 1) fetch(1);
     for (int64_t k = 1; k < K; ++k) {
-      const T l[3] = {L[(k - 1) * 3], L[(k - 1) * 3 + 1], L[(k - 1) * 3 + 2]};
-      const T u[3] = {U[(k - 1) * 3], U[(k - 1) * 3 + 1], U[(k - 1) * 3 + 2]};
+      T l[3], u[3], m[9], dk[3];
+      for (int c = 0; c < 3; ++c) l[c] = nl[c];
+      for (int c = 0; c < 3; ++c) u[c] = nu[c];
+      for (int c = 0; c < 9; ++c) m[c] = nm[c];
+      for (int c = 0; c < 3; ++c) dk[c] = nd[c];
+      fetch(k + 1 < K ? k + 1 : K - 1);
       // columns of M'^-1_{k-1} diag(u), and M'^-1_{k-1} d'_{k-1}
       T col[3][3], wd[3];
       for (int j = 0; j < 3; ++j) {
@@ -203,8 +239,8 @@ __global__ __launch_bounds__(256) void bt3_spd_kernel(
       }
       chol_solve3(lk, d, wd);
       for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) a[r * 3 + c] = M[k * 9 + r * 3 + c] - l[r] * col[c][r];
-      for (int r = 0; r < 3; ++r) d[r] = D[k * 3 + r] - l[r] * wd[r];
+        for (int c = 0; c < 3; ++c) a[r * 3 + c] = m[r * 3 + c] - l[r] * col[c][r];
+      for (int r = 0; r < 3; ++r) d[r] = dk[r] - l[r] * wd[r];
       bad += !chol3_shifted(a, lk, eps_rel, eps_abs);
       for (int c = 0; c < 9; ++c) C[k * 9 + c] = lk[c];
       for (int c = 0; c < 3; ++c) w(k, c) = d[c];
@@ -213,21 +249,41 @@ __global__ __launch_bounds__(256) void bt3_spd_kernel(
     for (int c = 0; c < 9; ++c) lk[c] = C[c];
     for (int c = 0; c < 3; ++c) d[c] = D[c];
     for (int c = 0; c < 3; ++c) w(0, c) = d[c];
+    T nl[3], nd[3], nc[9];
+    auto fetch = [&](int64_t k) {
+      for (int c = 0; c < 3; ++c) nl[c] = L[(k - 1) * 3 + c];
+      for (int c = 0; c < 3; ++c) nd[c] = D[k * 3 + c];
+      for (int c = 0; c < 9; ++c) nc[c] = C[k * 9 + c];
+    };
+    if (K > 1) fetch(1);
     for (int64_t k = 1; k < K; ++k) {
+      T l[3], dk[3], ck[9];
+      for (int c = 0; c < 3; ++c) l[c] = nl[c];
+      for (int c = 0; c < 3; ++c) dk[c] = nd[c];
+      for (int c = 0; c < 9; ++c) ck[c] = nc[c];
+      fetch(k + 1 < K ? k + 1 : K - 1);
       T wd[3];
       chol_solve3(lk, d, wd);
-      for (int r = 0; r < 3; ++r) d[r] = D[k * 3 + r] - L[(k - 1) * 3 + r] * wd[r];
-      for (int c = 0; c < 9; ++c) lk[c] = C[k * 9 + c];
+      for (int r = 0; r < 3; ++r) d[r] = dk[r] - l[r] * wd[r];
+      for (int c = 0; c < 9; ++c) lk[c] = ck[c];
       for (int c = 0; c < 3; ++c) w(k, c) = d[c];
     }
   }
   T xn[3];
   chol_solve3(lk, d, xn);
   for (int c = 0; c < 3; ++c) X[(K - 1) * 3 + c] = xn[c];
+  T nc[9], nw[3], nU[3];
+  auto fetch_back = [&](int64_t k) {
+    for (int c = 0; c < 9; ++c) nc[c] = C[k * 9 + c];
+    for (int c = 0; c < 3; ++c) nw[c] = w(k, c);
+    for (int c = 0; c < 3; ++c) nU[c] = U[k * 3 + c];
+  };
+  if (K > 1) fetch_back(K - 2);
   for (int64_t k = K - 2; k >= 0; --k) {
     T t[3];
-    for (int c = 0; c < 9; ++c) lk[c] = C[k * 9 + c];
-    for (int c = 0; c < 3; ++c) t[c] = w(k, c) - U[k * 3 + c] * xn[c];
+    for (int c = 0; c < 9; ++c) lk[c] = nc[c];
+    for (int c = 0; c < 3; ++c) t[c] = nw[c] - nU[c] * xn[c];
+    fetch_back(k > 0 ? k - 1 : 0);
     chol_solve3(lk, t, xn);
     for (int c = 0; c < 3; ++c) X[k * 3 + c] = xn[c];
   }
@@ -617,9 +673,12 @@ __global__ __launch_bounds__(256) void bt3_tree_layer_kernel(
 
 }  // namespace dn
 
+// one wavefront per workgroup for the lane-per-system sweeps: a few thousand
+// systems then spread over as many compute units as they have wavefronts
+constexpr int kSweepThreads = 64;
 template <typename T>
 static unsigned blocks_for(int64_t B) {
-  return (unsigned)((B + 255) / 256);
+  return (unsigned)((B + kSweepThreads - 1) / kSweepThreads);
 }
 
 DN_EXPORT int64_t dn_block_work_elems(int kind, int64_t B, int64_t K) {
@@ -634,7 +693,7 @@ static int bt3_thomas_impl(const T* lower, const T* main, const T* upper, const 
   if (!main || !rhs || !x || !work || B < 1 || K < 1) return DN_EINVAL;
   if (K > 1 && (!lower || !upper)) return DN_EINVAL;
   if (work_elems < 12 * B * K || B > (int64_t)INT32_MAX * 256) return DN_EINVAL;
-  hipLaunchKernelGGL((dn::bt3_thomas_kernel<T>), dim3(blocks_for<T>(B)), dim3(256), 0,
+  hipLaunchKernelGGL((dn::bt3_thomas_kernel<T>), dim3(blocks_for<T>(B)), dim3(kSweepThreads), 0,
                      stream, lower, main, upper, rhs, x, work, B, K);
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
@@ -648,7 +707,7 @@ static int bt3_spd_impl(const T* lower, const T* main, const T* upper, const T* 
   if (factor && !main) return DN_EINVAL;
   if (K > 1 && (!lower || !upper)) return DN_EINVAL;
   if (work_elems < 3 * B * K || B > (int64_t)INT32_MAX * 256) return DN_EINVAL;
-  hipLaunchKernelGGL((dn::bt3_spd_kernel<T>), dim3(blocks_for<T>(B)), dim3(256), 0, stream,
+  hipLaunchKernelGGL((dn::bt3_spd_kernel<T>), dim3(blocks_for<T>(B)), dim3(kSweepThreads), 0, stream,
                      lower, main, upper, rhs, x, chol, info, work, B, K, eps_rel, eps_abs,
                      factor);
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
@@ -697,7 +756,7 @@ static int bt3_tree_impl(const T* main, const T* edge, const T* rhs, const int64
                          child_ptr, child_idx, x, work, B, K, topo_lds, gmem);
     }
   } else {
-    hipLaunchKernelGGL((dn::bt3_tree_kernel<T>), dim3(blocks_for<T>(B)), dim3(256), 0,
+    hipLaunchKernelGGL((dn::bt3_tree_kernel<T>), dim3(blocks_for<T>(B)), dim3(kSweepThreads), 0,
                        stream, main, edge, rhs, parent, order, x, work, B, K, gmem);
   }
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;

@@ -96,6 +96,24 @@ def delay_plan(post_idx: torch.Tensor, width: int):
     return plan
 
 
+# Index tensors derived from a caller's tensor (tap rows reduced modulo the
+# ring depth, as int32) are cached the same way: per tensor object, checked
+# against its version and size and the derivation's own key, so a step whose
+# delays did not change reuses them instead of recomputing.
+_derived = WeakIdKeyDictionary()
+
+
+def derived_index(t: torch.Tensor, key, make):
+    """make(t) cached per (t, t._version, t.numel(), key)."""
+    stamp = (t._version, t.numel(), t.device, key)
+    hit = _derived.get(t)
+    if hit is not None and hit[0] == stamp:
+        return hit[1]
+    out = make(t)
+    _derived[t] = (stamp, out)
+    return out
+
+
 def _fn(name, like):
     return getattr(_lib.load(), f"{name}_{'f64' if like.dtype == torch.float64 else 'f32'}")
 
@@ -159,8 +177,10 @@ def dense_interpolated_delay_transition(queue, slot, global_step, weights, conne
     if delay_taps is None:
         delay_taps = delay_interpolation(delays, dt, dtype=queue.dtype, taps=taps, sigma=sigma)
     offsets, fraction = delay_taps
-    first = offsets.reshape(-1) if taps == 2 else offsets.reshape(-1, 3)[:, 0]
-    base = first.remainder(int(depth)).to(torch.int32).contiguous()
+    base = derived_index(
+        offsets, ("first-tap rows", int(depth), taps),
+        lambda o: (o.reshape(-1) if taps == 2 else o.reshape(-1, 3)[:, 0])
+        .remainder(int(depth)).to(torch.int32).contiguous())
     fraction = fraction.reshape(-1).to(queue.dtype)
     in_ptr, in_edge, post = delay_plan(post_idx, int(width))
     slot_t = slot.reshape(1).to(torch.int64)
@@ -308,7 +328,9 @@ def dense_fixed_delay_transition(queue, slot, global_step, weights, connection_g
     if delay.numel() != n:
         raise ValueError("delay_steps must hold one delay per connection (or one for all)")
     in_ptr, in_edge, post = delay_plan(post_idx, int(width))
-    base = delay.to(torch.int64).remainder(int(depth)).to(torch.int32).contiguous()
+    base = derived_index(delay, ("fixed rows", int(depth), n),
+                         lambda d: d.to(torch.int64).expand(n).remainder(int(depth))
+                         .to(torch.int32).contiguous())
     slot_t = slot.reshape(1).to(torch.int64)
     out = _InterpEnqueue.apply(queue, slot_t, weights.reshape(-1),
                                connection_gates.reshape(-1), None, base, in_ptr, in_edge,

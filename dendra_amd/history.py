@@ -35,7 +35,7 @@ import torch
 from torch.utils.weak import WeakIdKeyDictionary
 
 from . import _lib
-from .delays import delay_interpolation, delay_plan
+from .delays import delay_interpolation, delay_plan, derived_index
 
 _FLOAT = (torch.float32, torch.float64)
 _checked = WeakIdKeyDictionary()
@@ -224,20 +224,22 @@ def source_history_project(history, slot, weights, *, post_idx, depth, width, so
             raise ValueError("taps must be 2 or 3")
         offsets, interp = delay_taps
         if taps == 2:
-            offsets = offsets.reshape(n)
-            offs = torch.stack((offsets, offsets + 1), dim=-1)
+            tap_offsets = lambda o: torch.stack((o.reshape(n), o.reshape(n) + 1), dim=-1)
             frac = interp.reshape(n).to(weights.dtype)
         else:
-            offs = offsets.reshape(n, 3)
+            tap_offsets = lambda o: o.reshape(n, 3)
             frac = interp.reshape(n, 3).to(weights.dtype)
         n_taps = taps
     else:
         if delay_steps is None:
             raise ValueError("source_history_project needs delay_steps, delays or delay_taps")
-        offs = delay_steps.reshape(n, 1)
+        offsets = delay_steps
+        tap_offsets = lambda o: o.reshape(n, 1)
         frac = weights.new_empty(0)
         n_taps = 1
-    off = offs.remainder(depth).to(torch.int32).contiguous()
+    off = derived_index(offsets, ("history tap rows", depth, n_taps, n),
+                        lambda o: tap_offsets(o).remainder(depth).to(torch.int32)
+                        .contiguous())
     n_col = int(history.shape[1])
     if packed:
         col = _int32_index(word_idx, n_col, "word_idx")

@@ -328,3 +328,26 @@ def test_delay_plan_follows_each_post_idx_tensor():
         want = twotap_ref.enqueue(q, 3, w, g, frac, lower, post)
         assert np.array_equal(tr.queue.cpu().numpy().view(np.uint8), want.view(np.uint8))
         del p_t, tr
+
+
+def test_cached_tap_rows_follow_in_place_edits_of_the_offsets():
+    """The ring's first-tap rows are derived once per offsets tensor; an
+    in-place edit of that tensor (a new version) must be seen by the next
+    step."""
+    from dendra_amd.delays import dense_interpolated_delay_transition
+
+    rng = np.random.default_rng(13)
+    q, w, g, delays, post = case(rng, np.float32, depth=31, width=40, n=300)
+    dev = lambda a, t=None: torch.as_tensor(a, device="cuda", dtype=t)
+    lower = rng.integers(1, 20, 300)
+    frac = rng.uniform(0, 1, 300).astype(np.float32)
+    off_t, frac_t = dev(lower, torch.int64), dev(frac)
+    for edit in range(3):
+        tr = dense_interpolated_delay_transition(
+            dev(q), dev([5], torch.int64), dev([0], torch.int64), dev(w), dev(g),
+            dev(delays.astype(np.float32)), 0.025, post_idx=dev(post, torch.int64), depth=31,
+            width=40, taps=2, sigma=0.0, delay_taps=(off_t, frac_t))
+        want = twotap_ref.enqueue(q, 5, w, g, frac, off_t.cpu().numpy(), post)
+        assert np.array_equal(tr.queue.cpu().numpy().view(np.uint8), want.view(np.uint8))
+        off_t.add_(3 + edit)
+
