@@ -22,6 +22,16 @@
 
 namespace dn {
 
+// x where c holds, else +0, as a bit mask: the load of x stays unconditional
+// (a select lets the compiler put the load under a branch of its own and
+// wait for it there, one load at a time)
+__device__ __forceinline__ float keep_if(float x, bool c) {
+  return __uint_as_float(__float_as_uint(x) & (c ? ~0u : 0u));
+}
+__device__ __forceinline__ double keep_if(double x, bool c) {
+  return __longlong_as_double(__double_as_longlong(x) & (c ? ~0ll : 0ll));
+}
+
 // Launch-contract status word (caller-owned device int32, nullable).  A
 // persistent kernel that meets work outside what its launch promised sets
 // one of these bits and leaves that work unadvanced.

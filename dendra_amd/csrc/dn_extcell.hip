@@ -229,6 +229,91 @@ __device__ __forceinline__ void xc_assemble(const XcArgs<T>& a, int64_t k, int64
   }
 }
 
+// Inputs of one compartment's step in the two-lane form: voltages of the
+// compartment and its neighbours, the couplings, gates and coefficients.
+// Loaded one compartment ahead (every index clamped into the cable, so all
+// loads are unconditional and in flight together) while the current one is
+// assembled and eliminated.
+template <typename T>
+struct XpIn {
+  T vk[3], vp[3], vn[3], lp[3], ln[3], mai[9], cr[3], cn[3];
+  T m, h, n, sna, sk, area, iamp, xg0, xg1, lead;
+};
+
+template <typename T>
+__device__ __forceinline__ void xp_load(const XcArgs<T>& a, int64_t k, int64_t b, bool up,
+                                        XpIn<T>& in) {
+  const int64_t B = a.B, K = a.K;  // K >= 2
+  const int64_t kp = k > 0 ? k - 1 : 0, kn = k + 1 < K ? k + 1 : K - 1;
+  const int64_t en = k + 1 < K ? k : K - 2;
+  // coupling applied from the left at step k (t > 0): down diag(l_{k-1}),
+  // up diag(u_k); the one the next block multiplies from the right: down
+  // diag(u_k), up diag(l_{k-1}) (absent at the cable's ends)
+  const T* left = up ? a.upper : a.lower;
+  const T* right = up ? a.lower : a.upper;
+  const int64_t el = up ? (k < K - 1 ? k : K - 2) : kp;
+  const int64_t er = up ? k - 1 : k;
+  const bool has_r = er >= 0 && er + 1 < K;
+  const int64_t erc = er < 0 ? 0 : (er > K - 2 ? K - 2 : er);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    in.vk[c] = a.vc[(k * 3 + c) * B + b];
+    in.vp[c] = a.vc[(kp * 3 + c) * B + b];
+    in.vn[c] = a.vc[(kn * 3 + c) * B + b];
+    in.lp[c] = keep_if(a.lower[(kp * 3 + c) * B + b], k > 0);
+    in.ln[c] = keep_if(a.lower[(en * 3 + c) * B + b], k + 1 < K);
+    in.cr[c] = left[(el * 3 + c) * B + b];
+    in.cn[c] = keep_if(right[(erc * 3 + c) * B + b], has_r);
+  }
+  const int64_t i = k * B + b;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) in.mai[c] = a.maind[(k * 9 + c) * B + b];
+  in.m = a.m[i];
+  in.h = a.h[i];
+  in.n = a.n[i];
+  in.area = a.area[i];
+  in.xg0 = a.xg[(k * 2) * B + b];
+  in.xg1 = a.xg[(k * 2 + 1) * B + b];
+  in.sna = a.sna ? a.sna[i] : T(1);
+  in.sk = a.sk ? a.sk[i] : T(1);
+  in.iamp = a.iamp ? a.iamp[i] : T(0);
+  in.lead = a.lead ? a.lead[i] : T(0);
+}
+
+// The block row of compartment k (membrane step, then the assembled 3x3
+// block and right-hand side), as xc_assemble but from prefetched inputs.
+template <typename T>
+__device__ __forceinline__ void xp_assemble(const XcArgs<T>& a, int64_t k, int64_t b,
+                                            const HHParams<T>& hp, T bath_s, T intra_s,
+                                            XpIn<T>& in, T* A, T* R) {
+  const int64_t i = k * a.B + b;
+  const T v = in.vk[0] - in.vk[1];
+  T G, J;
+  hh_advance<T>(v, a.dt, hp, in.sna, in.sk, in.m, in.h, in.n, G, J);
+  a.m[i] = in.m;
+  a.h[i] = in.h;
+  a.n[i] = in.n;
+  const T g = G * in.area;
+  const T d = (J - G * v) * in.area + in.iamp * intra_s;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) A[c] = in.mai[c];
+  A[0] += g;
+  A[1] -= g;
+  A[3] -= g;
+  A[4] += g;
+  const T shell = in.xg0 * (in.vk[1] - in.vk[2]);
+  const T bathf = in.xg1 * (in.lead * bath_s - in.vk[2]);
+  R[0] = d;
+  R[1] = -d - shell;
+  R[2] = shell + bathf;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T f = -in.ln[c] * (in.vn[c] - in.vk[c]);
+    const T fprev = -in.lp[c] * (in.vk[c] - in.vp[c]);
+    R[c] += f - fprev;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void extcell_pair_kernel(XcArgs<T> a) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -244,25 +329,22 @@ __global__ __launch_bounds__(256) void extcell_pair_kernel(XcArgs<T> a) {
     const T intra_s = (a.iamp && a.iwave) ? a.iwave[s] : T(0);
     T Ainv[9], Rp[3], cprev[3];  // previous block's inverse, rhs, coupling pair
     const int64_t n_own = up ? K - sp : sp;
+    XpIn<T> cur, nxt;
+    xp_load(a, up ? K - 1 : 0, b, up, cur);
     for (int64_t t = 0; t < n_own; ++t) {
       const int64_t k = up ? K - 1 - t : t;
+      const int64_t tn = t + 1 < n_own ? t + 1 : t;
+      xp_load(a, up ? K - 1 - tn : tn, b, up, nxt);
       T A[9], R[3];
-      xc_assemble(a, k, b, hp, bath_s, intra_s, A, R);
+      xp_assemble(a, k, b, hp, bath_s, intra_s, cur, A, R);
       if (t > 0) {
         // down: couplings (k, k-1) = diag(l_{k-1}), (k-1, k) = diag(u_{k-1});
         // up:   couplings (k, k+1) = diag(u_k),    (k+1, k) = diag(l_k)
-        T cr[3], cc[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const int64_t e = up ? k : k - 1;
-          cr[c] = up ? a.upper[(e * 3 + c) * B + b] : a.lower[(e * 3 + c) * B + b];
-          cc[c] = cprev[c];
-        }
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
 #pragma unroll
-          for (int c = 0; c < 3; ++c) A[r * 3 + c] -= cr[r] * Ainv[r * 3 + c] * cc[c];
-          R[r] -= cr[r] * (Ainv[r * 3] * Rp[0] + Ainv[r * 3 + 1] * Rp[1] + Ainv[r * 3 + 2] * Rp[2]);
+          for (int c = 0; c < 3; ++c) A[r * 3 + c] -= cur.cr[r] * Ainv[r * 3 + c] * cprev[c];
+          R[r] -= cur.cr[r] * (Ainv[r * 3] * Rp[0] + Ainv[r * 3 + 1] * Rp[1] + Ainv[r * 3 + 2] * Rp[2]);
         }
       }
       xc_inv3(A, Ainv);
@@ -272,13 +354,9 @@ __global__ __launch_bounds__(256) void extcell_pair_kernel(XcArgs<T> a) {
       for (int c = 0; c < 3; ++c) {
         W[(k * 12 + 9 + c) * B + b] = R[c];
         Rp[c] = R[c];
-        // the coupling the next eliminated block multiplies from the right:
-        // down: diag(u_k) of edge (k, k+1); up: diag(l_{k-1}) of edge (k-1, k)
-        const int64_t e = up ? k - 1 : k;
-        cprev[c] = (e >= 0 && e + 1 < K)
-                       ? (up ? a.lower[(e * 3 + c) * B + b] : a.upper[(e * 3 + c) * B + b])
-                       : T(0);
+        cprev[c] = cur.cn[c];
       }
+      cur = nxt;
     }
     // interface: exchange (Ainv, R') of compartments s-1 (down) and s (up)
     T Ao[9], Ro[3];
@@ -324,28 +402,45 @@ __global__ __launch_bounds__(256) void extcell_pair_kernel(XcArgs<T> a) {
     T xn[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) xn[c] = up ? xu[c] : xd[c];
+    // the next (outer) compartment's inverse, eliminated right-hand side,
+    // coupling and voltage are fetched a compartment ahead (clamped index)
+    const T* outer = up ? a.lower : a.upper;
+    T nAi[9], nR[3], nC[3], nV[3];
+    auto fetch_back = [&](int64_t t) {
+      const int64_t k = up ? K - 1 - t : t;
+      const int64_t e = up ? (k > 0 ? k - 1 : 0) : (k < K - 1 ? k : K - 2);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) nAi[c] = W[(k * 12 + c) * B + b];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        nR[c] = W[(k * 12 + 9 + c) * B + b];
+        nC[c] = outer[(e * 3 + c) * B + b];
+        nV[c] = a.vc[(k * 3 + c) * B + b];
+      }
+    };
+    fetch_back(n_own - 1);
     for (int64_t t = n_own - 1; t >= 0; --t) {
       const int64_t k = up ? K - 1 - t : t;
-      T x[3];
+      T Ai[9], r[3], x[3], vk[3];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) Ai[c] = nAi[c];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        r[c] = nR[c] - nC[c] * xn[c];
+        vk[c] = nV[c];
+      }
+      fetch_back(t > 0 ? t - 1 : 0);
       if (t == n_own - 1) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) x[c] = xn[c];
       } else {
-        T Ai[9], r[3];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) Ai[c] = W[(k * 12 + c) * B + b];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const T cpl = up ? a.lower[((k - 1) * 3 + c) * B + b] : a.upper[(k * 3 + c) * B + b];
-          r[c] = W[(k * 12 + 9 + c) * B + b] - cpl * xn[c];
-        }
 #pragma unroll
         for (int c = 0; c < 3; ++c)
           x[c] = Ai[c * 3] * r[0] + Ai[c * 3 + 1] * r[1] + Ai[c * 3 + 2] * r[2];
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        a.vc[(k * 3 + c) * B + b] += x[c];
+        a.vc[(k * 3 + c) * B + b] = vk[c] + x[c];
         xn[c] = x[c];
       }
     }

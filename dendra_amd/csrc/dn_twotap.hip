@@ -31,16 +31,6 @@ __device__ __forceinline__ int ring_slot(const int64_t* slot, int depth) {
   return (int)(s < 0 ? s + depth : s);
 }
 
-// x where c holds, else +0, as a bit mask: the load of x stays unconditional
-// (a select lets the compiler put the load under a branch of its own and
-// wait for it there, one load at a time)
-__device__ __forceinline__ float keep_if(float x, bool c) {
-  return __uint_as_float(__float_as_uint(x) & (c ? ~0u : 0u));
-}
-__device__ __forceinline__ double keep_if(double x, bool c) {
-  return __longlong_as_double(__double_as_longlong(x) & (c ? ~0ll : 0ll));
-}
-
 __device__ __forceinline__ int64_t ring_row(int slot, int32_t base, int j, int depth) {
   return ((unsigned)slot + (unsigned)base + (unsigned)j) % (unsigned)depth;
 }
