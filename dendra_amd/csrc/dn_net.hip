@@ -165,11 +165,17 @@ __global__ __launch_bounds__(256) void net_mark(NetArgs a) {
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) slot[u] = t[u] >= 0 ? atomicAdd(&a.hit_cnt[t[u]], 1) : -1;
+      // the entries' first hits take one run of the target list per lane
+      // (one counter increment, not one per entry)
+      int nf = 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) nf += (t[u] >= 0 && slot[u] == 0);
+      int at = nf ? atomicAdd(&a.ctr[3], nf) : 0;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         if (t[u] < 0) continue;
         if (slot[u] < kHitMax) a.hit_e[(int64_t)t[u] * kHitMax + slot[u]] = e[u];
-        if (slot[u] == 0) a.tgt_list[atomicAdd(&a.ctr[3], 1)] = t[u];
+        if (slot[u] == 0) a.tgt_list[at++] = t[u];
       }
     }
   }
@@ -186,7 +192,19 @@ __device__ __forceinline__ void net_enqueue_body(const NetArgs& a) {
   const int cur = a.ctr[0];
   for (int k = tid; k < na; k += nth) {
     const int t = a.tgt_list[k];
+    // the count and the whole hit row are loaded together (the row read
+    // whole and masked by the count, so no load waits on the count)
     const int cnt = a.hit_cnt[t];
+    const int4* row = reinterpret_cast<const int4*>(a.hit_e + (int64_t)t * kHitMax);
+    int4 rv[kHitMax / 4];
+#pragma unroll
+    for (int i = 0; i < kHitMax / 4; ++i) rv[i] = row[i];
+    // keeps the row's loads beside the count's (else they sink into the
+    // branch that uses them and start only after the count arrives)
+    asm volatile("" ::"v"(cnt), "v"(rv[0].x), "v"(rv[0].y), "v"(rv[0].z), "v"(rv[0].w),
+                 "v"(rv[1].x), "v"(rv[1].y), "v"(rv[1].z), "v"(rv[1].w), "v"(rv[2].x),
+                 "v"(rv[2].y), "v"(rv[2].z), "v"(rv[2].w), "v"(rv[3].x), "v"(rv[3].y),
+                 "v"(rv[3].z), "v"(rv[3].w));
     auto add = [&](int e) {
       int slot = cur + a.delay[e];
       if (slot >= a.depth) slot -= a.depth;
@@ -196,8 +214,17 @@ __device__ __forceinline__ void net_enqueue_body(const NetArgs& a) {
     if (cnt <= kHitMax) {
       int e[kHitMax];
 #pragma unroll
-      for (int i = 0; i < kHitMax; ++i)
-        e[i] = i < cnt ? a.hit_e[(int64_t)t * kHitMax + i] : 0x7fffffff;
+      for (int i = 0; i < kHitMax / 4; ++i) {
+        e[4 * i] = rv[i].x;
+        e[4 * i + 1] = rv[i].y;
+        e[4 * i + 2] = rv[i].z;
+        e[4 * i + 3] = rv[i].w;
+      }
+#pragma unroll
+      for (int i = 0; i < kHitMax; ++i) {
+        const int m = -(int)(i < cnt);  // all ones for a filed hit
+        e[i] = (e[i] & m) | (0x7fffffff & ~m);
+      }
       unsigned used = 0;
       for (int o = 0; o < cnt; ++o) {
         int best = 0x7fffffff, bi = 0;
@@ -390,6 +417,7 @@ DN_EXPORT int dn_net_run_f32(
       !latch || !out_ptr || !out_tgt || !out_e || !in_ptr || !pre || !delay || !w ||
       depth < 2 || !ring || !spk_bits || !spk_list || !hit_cnt || !hit_e || !tgt_list || !ctr)
     return DN_EINVAL;
+  if (reinterpret_cast<uintptr_t>(hit_e) % 16) return DN_EINVAL;  // rows read as int4
   if (graph_steps < 0 || (graph_steps > 0 && n_steps % graph_steps != 0)) return DN_EINVAL;
   dn::NetArgs a;
   a.n_cells = n_cells; a.off = off; a.soma = soma; a.v = v; a.m = m; a.h = h; a.n = n;
@@ -451,6 +479,7 @@ DN_EXPORT int dn_net_deliver_tape_f32(
       !delay || !w || depth < 2 || !ring || !spk_bits || !spk_list || !hit_cnt || !hit_e ||
       !tgt_list || !ctr || !tape || !delivered)
     return DN_EINVAL;
+  if (reinterpret_cast<uintptr_t>(hit_e) % 16) return DN_EINVAL;  // rows read as int4
   dn::NetArgs a = {};
   a.n_cells = n_targets;
   a.n_src = n_src;

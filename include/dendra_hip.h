@@ -555,7 +555,8 @@ int dn_hh_tree_step_f64(int n_trees, const int64_t* node_off, const int32_t* par
  * sources set in tape[step][*] add w into ring[(cur + delay) % depth][post]
  * in connection order (bitwise equal to the ordered float32 scatter_add).
  * Scratch: spk_bits ceil(n_src/32) words, spk_list n_src, hit_cnt n_targets
- * (zero on entry; zero again after every step), hit_e 16 * n_targets,
+ * (zero on entry; zero again after every step), hit_e 16 * n_targets
+ * (16-byte aligned),
  * tgt_list n_targets, ctr {cur, step, 0, 0}. */
 int dn_net_deliver_tape_f32(int n_targets, int n_src, const int32_t* out_ptr,
                             const int32_t* out_tgt, const int32_t* out_e,
