@@ -83,8 +83,18 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs G) {
   static_assert(2 * kStage >= kTN * kTS, "transpose tile must fit");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
-  const int z = blockIdx.z / G.splits, split = blockIdx.z % G.splits;
+  // Workgroups are dispatched to the 8 XCDs round-robin by linear id, and
+  // each XCD has its own L2.  Tiles are renumbered so that each XCD takes one
+  // contiguous run of them: the tiles of one (member, split) product, which
+  // share their A rows and B columns, then meet in the same L2 instead of
+  // being fetched once per XCD.
+  const int nbx = gridDim.x, nby = gridDim.y;
+  const int total = nbx * nby * gridDim.z;
+  const int lin = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+  const int tile = total % 8 == 0 ? (lin % 8) * (total / 8) + lin / 8 : lin;
+  const int bx = tile % nbx, by = (tile / nbx) % nby, bz = tile / (nbx * nby);
+  const int m0 = by * kTM, n0 = bx * kTN;
+  const int z = bz / G.splits, split = bz % G.splits;
   const int kspan = G.K / G.splits;
   const int kbeg = split * kspan;
   const int ntile = kspan / kTK;
@@ -177,7 +187,7 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs G) {
         const int64_t m = m0 + i, n = n0 + j;
         float v = acc[a][b][r];
         if (EPI == EPI_F32) {
-          G.Z[(int64_t)blockIdx.z * G.sz + m * G.ldz + n] = v;
+          G.Z[(int64_t)bz * G.sz + m * G.ldz + n] = v;
           continue;
         }
         if (EPI == EPI_GELU) {
@@ -629,16 +639,18 @@ __device__ __forceinline__ void adam_elem(int64_t t, int rows, int cols, float* 
   v[t] = vi;
   const float pi = p[t] - lr_t * mi / (sqrtf(vi) + eps);
   p[t] = pi;
-  const int64_t per = (int64_t)rows * cols;
-  const int64_t z = t / per, e = t % per, r = e / cols, c = e % cols;
+  // 32-bit index arithmetic (the entry points reject n >= 2^31)
+  const uint32_t per = (uint32_t)rows * (uint32_t)cols, tu = (uint32_t)t;
+  const uint32_t z = tu / per, e = tu - z * per, r = e / (uint32_t)cols,
+                 c = e - r * (uint32_t)cols;
   if (pb) pb[t] = (bf16)pi;
   if (pbt) pbt[z * per + c * rows + r] = (bf16)pi;
   if (pbp) {  // MFMA fragment order of the fused forward (rows % 32 == 0, cols % 16 == 0)
-    const int64_t nb = r >> 5, ks = c >> 4, ln = ((c >> 3) & 1) * 32 + (r & 31);
+    const uint32_t nb = r >> 5, ks = c >> 4, ln = ((c >> 3) & 1) * 32 + (r & 31);
     pbp[z * per + ((nb * (cols >> 4) + ks) * 64 + ln) * 8 + (c & 7)] = (bf16)pi;
   }
   if (pbtp) {  // the same order for W^T [cols][rows] (cols % 32 == 0, rows % 16 == 0)
-    const int64_t nb = c >> 5, ks = r >> 4, ln = ((r >> 3) & 1) * 32 + (c & 31);
+    const uint32_t nb = c >> 5, ks = r >> 4, ln = ((r >> 3) & 1) * 32 + (c & 31);
     pbtp[z * per + ((nb * (rows >> 4) + ks) * 64 + ln) * 8 + (r & 7)] = (bf16)pi;
   }
 }
@@ -939,6 +951,7 @@ DN_EXPORT int dn_mlp_adam(int64_t n, int rows, int cols, float* p, const float* 
   if (n < 1 || rows < 1 || cols < 1 || n % ((int64_t)rows * cols) || !p || !g || !m || !v ||
       (pbp && (rows % 32 || cols % 16)) || (pbtp && (cols % 32 || rows % 16)))
     return DN_EINVAL;
+  if (n > INT32_MAX) return DN_ERANGE;  // 32-bit element indices
   hipLaunchKernelGGL(dn::adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
                      n, rows, cols, p, g, m, v, lr_t, b1, b2, eps, static_cast<bf16*>(pb),
                      static_cast<bf16*>(pbt), static_cast<bf16*>(pbp), static_cast<bf16*>(pbtp));
@@ -964,6 +977,7 @@ DN_EXPORT int dn_mlp_adam_multi(int count, const int64_t* n, const int* rows, co
         !p[k] || !g[k] || !m[k] || !v[k] || (pbp[k] && (rows[k] % 32 || cols[k] % 16)) ||
         (pbtp[k] && (cols[k] % 32 || rows[k] % 16)))
       return DN_EINVAL;
+    if (n[k] > INT32_MAX) return DN_ERANGE;  // 32-bit element indices
     T.n[k] = n[k]; T.rows[k] = rows[k]; T.cols[k] = cols[k];
     T.p[k] = reinterpret_cast<float*>(p[k]); T.g[k] = reinterpret_cast<const float*>(g[k]);
     T.m[k] = reinterpret_cast<float*>(m[k]); T.v[k] = reinterpret_cast<float*>(v[k]);
