@@ -134,6 +134,12 @@ for _sfx in ("f32", "f64"):
     _SIGNATURES[f"dn_bt3_spd_{_sfx}"] = (i32, [p] * 7 + [i64, i64, p, i64, i32, p])
     _SIGNATURES[f"dn_bt3_tree_{_sfx}"] = (i32, [p] * 6 + [i64, p, p, p, i64, i64, p, i64, p])
 _SIGNATURES["dn_tree_contract_max_sections"] = (i32, [i32])
+_SIGNATURES["dn_bt3_contract_max_sections"] = (i32, [i32, i64])
+_SIGNATURES["dn_bt3_contract_work_elems"] = (i64, [i64, i64])
+_SIGNATURES["dn_extcell_tree_contract_work_elems"] = (i64, [i64, i64])
+for _sfx in ("f32", "f64"):
+    _SIGNATURES[f"dn_bt3_tree_contract_{_sfx}"] = (
+        i32, [p] * 5 + [i64, i64, p, i32] + [p] * 5 + [i32] + [p] * 8 + [p, i64, p])
 _SIGNATURES["dn_interp_delay_work_bytes"] = (i64, [i64, i32, i32])
 _SIGNATURES["dn_history_work_bytes"] = (i64, [i64, i32, i32, i32])
 for _sfx in ("f32", "f64"):
@@ -156,6 +162,9 @@ _SIGNATURES["dn_extcell_tree_work_elems"] = (i64, [i64, i64])
 for _sfx, _ft in (("f32", f32), ("f64", f64)):
     _SIGNATURES[f"dn_extcell_tree_run_{_sfx}"] = (
         i32, [i64, i64] + [p] * 8 + [i64] + [p] * 10 + [_ft, i64] + [p] * 5 + [i64, p, i64, p])
+    _SIGNATURES[f"dn_extcell_tree_run_contract_{_sfx}"] = (
+        i32, [i64, i64] + [p] * 8 + [i32] + [p] * 5 + [i32] + [p] * 8 + [p] * 8
+        + [_ft, i64] + [p] * 5 + [i64, p, i64, p])
 for _sfx, _ft in (("f32", f32), ("f64", f64)):
     _SIGNATURES[f"dn_extcell_run_{_sfx}"] = (
         i32, [i64, i64] + [p] * 13 + [_ft, i64] + [p] * 5 + [i64, p, i64, p])

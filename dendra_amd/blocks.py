@@ -224,7 +224,65 @@ def _depth_levels(parent):
     return res
 
 
+# Contraction plans per parent tensor object (version-checked, weakly held):
+# the block tree solve then runs as chain condensation plus rake/compress
+# rounds (its dependent chain is the longest section plus twice the rounds,
+# not the tree's depth); None when the parents do not form one tree.
+_bc_plans = WeakIdKeyDictionary()
+
+
+def _contraction_plan(parent):
+    from .contraction import TreeContractionPlan
+
+    stamp = (parent._version, parent.numel(), parent.device)
+    hit = _bc_plans.get(parent)
+    if hit is not None and hit[0] == stamp:
+        return hit[1]
+    try:
+        plan = TreeContractionPlan(parent.detach().to("cpu", torch.int64), 16)
+    except ValueError:
+        plan = None
+    _bc_plans[parent] = (stamp, plan)
+    return plan
+
+
+def _contract_cap(elem_bytes: int, K: int) -> int:
+    return int(_lib.load().dn_bt3_contract_max_sections(int(elem_bytes), int(K)))
+
+
+def contraction_tables(plan, device):
+    """The contraction plan's int32 tables in the block entry points' order."""
+    from .contraction import plan_tables
+
+    return plan_tables(plan, device)
+
+
+def _tree_contract_raw(main, edge, rhs, plan):
+    main, edge, rhs = (t.contiguous() for t in (main, edge, rhs))
+    (perm, sec_head, sec_len, sec_child, sec_nchild, sec_parent,
+     ptr, sec, par, child, rptr, recv, sptr, src) = contraction_tables(plan, rhs.device)
+    B, K = map(int, rhs.shape[:2])
+    x = torch.empty_like(rhs, memory_format=torch.contiguous_format)
+    n = int(_lib.load().dn_bt3_contract_work_elems(B, K))
+    work = torch.empty(max(n, 1), dtype=rhs.dtype, device=rhs.device)
+    p = _lib.ptr
+    st = _fn("dn_bt3_tree_contract", rhs)(
+        p(main), p(edge), p(rhs), None, p(x), B, K, p(perm), plan.n_sections, p(sec_head),
+        p(sec_len), p(sec_child), p(sec_nchild), p(sec_parent), plan.n_rounds, p(ptr), p(sec),
+        p(par), p(child), p(rptr), p(recv), p(sptr), p(src), p(work), n,
+        _lib.stream_handle(rhs.device))
+    _lib.check(st, "block tree contraction solve")
+    return x
+
+
 def _tree_raw(main, edge, rhs, parent, order, layer_ptr):
+    plan = _contraction_plan(parent)
+    if plan is not None and plan.n_sections <= _contract_cap(rhs.element_size(), plan.size):
+        return _tree_contract_raw(main, edge, rhs, plan)
+    return _tree_layers_raw(main, edge, rhs, parent, order, layer_ptr)
+
+
+def _tree_layers_raw(main, edge, rhs, parent, order, layer_ptr):
     main, edge, rhs = (t.contiguous() for t in (main, edge, rhs))
     child_ptr, child_idx = _children_csr(parent)
     # whole depth levels let the kernel pass the children's terms through

@@ -671,6 +671,285 @@ __global__ __launch_bounds__(256) void bt3_tree_layer_kernel(
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// block tree solve by contraction (the 3x3-block form of the scalar tree
+// contraction): one workgroup per system over a host plan of sections
+// (chains of at most S+1 nodes, head at the top, tail where the child
+// sections attach) and rake/compress rounds over the section tree.
+//   A_ii = main_i (+ membrane) + diag(e_i) + sum_children diag(e_c),
+//   A_ip = A_pi = -diag(e_i)   (main blocks need not be symmetric, so the
+//   two couplings U = A_{j,p}, L = A_{p,j} of every eliminated pair are
+//   carried separately).
+// 1. each section eliminates its nodes but the tail, head first; the tail
+//    keeps a self block, a right-hand side and couplings to the parent
+//    section's tail, which receives PM, PF; interior recovery
+//    x_q = a_q + B_q x_ptail + C_q x_{q+1} goes to the workspace
+//    ([B][K][21], by section position);
+// 2. rounds: a rake folds a leaf section into its parent, a compress joins
+//    its parent and child (new couplings CU, CL); survivors pull their
+//    eliminated neighbours' terms in a fixed order;
+// 3. back-substitution: the root, the rounds in reverse, every interior.
+// A path of n sections takes about log2(n) rounds, so the dependent chain
+// is the longest section plus twice the rounds, not the tree's depth.
+// ---------------------------------------------------------------------------
+template <typename T> struct BcArgs {
+  const T *main, *edge, *rhs, *gmem;
+  T* x;
+  T* work;  // interior recovery terms, [B][K][21] by section position
+  int64_t K;
+  int n_sec, n_rounds;
+  const int32_t *perm, *sec_head, *sec_len, *sec_child, *sec_nchild, *sec_parent;
+  const int32_t *elim_ptr, *elim_sec, *elim_par, *elim_child;
+  const int32_t *recv_ptr, *recv_sec, *src_ptr, *src;
+};
+constexpr int kBcSlots = 45;  // LDS values per section
+constexpr int kBcThreads = 128;
+
+template <typename T>
+__device__ __forceinline__ void mm3(const T* a, const T* b, T* c) {
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k)
+      c[r * 3 + k] = a[r * 3] * b[k] + a[r * 3 + 1] * b[3 + k] + a[r * 3 + 2] * b[6 + k];
+}
+
+// node block of position q: main (+ membrane) + diag(own edge)
+template <typename T>
+__device__ __forceinline__ void bc_node(const BcArgs<T>& A, const T* M, const T* E,
+                                        const T* G, int k, T* S, T* e) {
+  for (int c = 0; c < 9; ++c) S[c] = M[(int64_t)k * 9 + c];
+  if (G) add_membrane(S, G[k]);
+  for (int c = 0; c < 3; ++c) {
+    e[c] = E[(int64_t)k * 3 + c];
+    S[c * 4] += e[c];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBcThreads) void bt3_contract_kernel(BcArgs<T> A) {
+  extern __shared__ __align__(16) unsigned char dn_smem_raw[];
+  const int ns = A.n_sec;
+  T* L0 = reinterpret_cast<T*>(dn_smem_raw);
+  // per section: self block SM, right-hand side SF, couplings SU = A_{j,p}
+  // and SL = A_{p,j} to the parent section's tail, what it hands its parent
+  // (PM, PF), its tail's solution XS.  A section eliminated in a round keeps
+  // its recovery terms x_j = RF + RM x_p + RE x_c in SF, SU, SL (no longer
+  // read once it is eliminated), so RF/RM/RE alias them.
+  T *SM = L0, *SF = SM + 9 * ns, *SU = SF + 3 * ns, *SL = SU + 9 * ns, *PM = SL + 9 * ns,
+    *PF = PM + 9 * ns, *XS = PF + 3 * ns;
+  T *RF = SF, *RM = SU, *RE = SL;
+  const int64_t K = A.K, b = blockIdx.x;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const T* M = A.main + b * K * 9;
+  const T* E = A.edge + b * K * 3;
+  const T* R = A.rhs + b * K * 3;
+  const T* G = A.gmem ? A.gmem + b * K : nullptr;
+  T* X = A.x + b * K * 3;
+  // interior recovery terms by section position, 21 contiguous values per
+  // node (each lane's writes and reads of a node fall in one or two lines;
+  // component-major layouts across the lanes measured slower)
+  T* W = A.work + b * K * 21;
+
+  // 1. chain condensation, head first
+  for (int s = tid; s < ns; s += nt) {
+    const int hd = A.sec_head[s], len = A.sec_len[s];
+    T S[9], e[3], F[3], U[9], Lc[9], dS[9], dF[3];
+    int k = A.perm[hd];
+    bc_node(A, M, E, G, k, S, e);
+    for (int c = 0; c < 3; ++c) F[c] = R[(int64_t)k * 3 + c];
+    for (int c = 0; c < 9; ++c) U[c] = Lc[c] = dS[c] = T(0);
+    for (int c = 0; c < 3; ++c) {
+      U[c * 4] = Lc[c * 4] = -e[c];
+      dF[c] = T(0);
+    }
+    // the next node's block, edge and right-hand side are fetched a node
+    // ahead (index clamped into the section: the loads are unconditional)
+    const int last = hd + len - 1;
+    T pS[9], pe[3], pr[3];
+    auto fetch = [&](int q) {
+      const int n = A.perm[q];
+      bc_node(A, M, E, G, n, pS, pe);
+      for (int c = 0; c < 3; ++c) pr[c] = R[(int64_t)n * 3 + c];
+    };
+    fetch(hd + 1 < last ? hd + 1 : last);
+    for (int q = hd; q < last; ++q) {
+      T Sn[9], en[3], rn[3];
+      for (int c = 0; c < 9; ++c) Sn[c] = pS[c];
+      for (int c = 0; c < 3; ++c) {
+        en[c] = pe[c];
+        rn[c] = pr[c];
+      }
+      fetch(q + 2 < last ? q + 2 : last);      for (int c = 0; c < 3; ++c) S[c * 4] += en[c];  // the chain child's edge
+      T Si[9], a[3], SiU[9], LS[9], t[9];
+      inv3(S, Si);
+      mv3(Si, F, a);
+      mm3(Si, U, SiU);
+      mm3(Lc, Si, LS);
+      // recovery x_q = a + B x_ptail + C x_n with B = -Si U, C = Si diag(en)
+      T* w = W + (int64_t)q * 21;
+      for (int c = 0; c < 3; ++c) w[c] = a[c];
+      for (int c = 0; c < 9; ++c) w[3 + c] = -SiU[c];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) w[12 + r * 3 + c] = Si[r * 3 + c] * en[c];
+      // the parent tail: dS -= L Si U, dF -= L Si F; new A_{pt,n} = L Si diag(en)
+      mm3(LS, U, t);
+      for (int c = 0; c < 9; ++c) dS[c] -= t[c];
+      for (int r = 0; r < 3; ++r)
+        dF[r] -= LS[r * 3] * F[0] + LS[r * 3 + 1] * F[1] + LS[r * 3 + 2] * F[2];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) Lc[r * 3 + c] = LS[r * 3 + c] * en[c];
+      // the next node: S_n -= diag(en) Si diag(en), F_n += en (Si F),
+      // U_n = diag(en) Si U
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          Sn[r * 3 + c] -= en[r] * Si[r * 3 + c] * en[c];
+          U[r * 3 + c] = en[r] * SiU[r * 3 + c];
+        }
+      for (int c = 0; c < 9; ++c) S[c] = Sn[c];
+      for (int c = 0; c < 3; ++c) F[c] = rn[c] + en[c] * a[c];
+    }
+    // the tail's children are the heads of the child sections
+    const int c0 = A.sec_child[s], nc = A.sec_nchild[s];
+    for (int c = c0; c < c0 + nc; ++c) {
+      const int h = A.perm[A.sec_head[c]];
+      for (int r = 0; r < 3; ++r) S[r * 4] += E[(int64_t)h * 3 + r];
+    }
+    for (int c = 0; c < 9; ++c) {
+      SM[s * 9 + c] = S[c];
+      SU[s * 9 + c] = U[c];
+      SL[s * 9 + c] = Lc[c];
+      PM[s * 9 + c] = dS[c];
+    }
+    for (int c = 0; c < 3; ++c) {
+      SF[s * 3 + c] = F[c];
+      PF[s * 3 + c] = dF[c];
+    }
+  }
+  __syncthreads();
+  for (int s = tid; s < ns; s += nt) {  // the child sections' heads act on the tail
+    const int c0 = A.sec_child[s], nc = A.sec_nchild[s];
+    for (int c = c0; c < c0 + nc; ++c) {
+      for (int v = 0; v < 9; ++v) SM[s * 9 + v] += PM[c * 9 + v];
+      for (int v = 0; v < 3; ++v) SF[s * 3 + v] += PF[c * 3 + v];
+    }
+  }
+  __syncthreads();
+
+  // 2. contraction rounds over the section tree.  A compressed section's
+  // thread also updates its child directly: the child survives the round,
+  // only this thread touches it before the barrier, and the order of the
+  // additions to it is fixed by the plan (this one first, then the pulls).
+  for (int r = 0; r < A.n_rounds; ++r) {
+    for (int i = A.elim_ptr[r] + tid; i < A.elim_ptr[r + 1]; i += nt) {
+      const int j = A.elim_sec[i], c = A.elim_child[i];
+      T Si[9], LS[9], t[9], f[3], Uj[9], Fj[3];
+      inv3(SM + j * 9, Si);
+      mm3(SL + j * 9, Si, LS);
+      for (int v = 0; v < 9; ++v) Uj[v] = SU[j * 9 + v];
+      for (int v = 0; v < 3; ++v) Fj[v] = SF[j * 3 + v];
+      mm3(LS, Uj, t);
+      for (int v = 0; v < 9; ++v) PM[j * 9 + v] = -t[v];
+      mv3(LS, Fj, f);
+      for (int v = 0; v < 3; ++v) PF[j * 3 + v] = -f[v];
+      T re[9];
+      if (c < 0) {  // rake
+        for (int v = 0; v < 9; ++v) re[v] = T(0);
+      } else {  // compress: A_{c,j} = U_c, A_{j,c} = L_c
+        T US[9], Lc[9], Uc[9];
+        for (int v = 0; v < 9; ++v) {
+          Lc[v] = SL[c * 9 + v];
+          Uc[v] = SU[c * 9 + v];
+        }
+        mm3(Si, Lc, t);
+        for (int v = 0; v < 9; ++v) re[v] = -t[v];
+        mm3(Uc, Si, US);
+        mm3(US, Lc, t);
+        for (int v = 0; v < 9; ++v) SM[c * 9 + v] -= t[v];
+        mv3(US, Fj, f);
+        for (int v = 0; v < 3; ++v) SF[c * 3 + v] -= f[v];
+        mm3(US, Uj, t);
+        for (int v = 0; v < 9; ++v) SU[c * 9 + v] = -t[v];  // new A_{c,p}
+        mm3(LS, Lc, t);
+        for (int v = 0; v < 9; ++v) SL[c * 9 + v] = -t[v];  // new A_{p,c}
+      }
+      // recovery terms into the eliminated section's own slots
+      mv3(Si, Fj, f);
+      mm3(Si, Uj, t);
+      for (int v = 0; v < 3; ++v) RF[j * 3 + v] = f[v];
+      for (int v = 0; v < 9; ++v) {
+        RM[j * 9 + v] = -t[v];
+        RE[j * 9 + v] = re[v];
+      }
+    }
+    __syncthreads();
+    for (int i = A.recv_ptr[r] + tid; i < A.recv_ptr[r + 1]; i += nt) {
+      const int q = A.recv_sec[i];
+      T m[9], f[3];
+      for (int v = 0; v < 9; ++v) m[v] = SM[q * 9 + v];
+      for (int v = 0; v < 3; ++v) f[v] = SF[q * 3 + v];
+      for (int k = A.src_ptr[i]; k < A.src_ptr[i + 1]; ++k) {
+        const int sj = A.src[k];
+        if (sj >= 0) {  // an eliminated child (a compressed parent updated q already)
+          for (int v = 0; v < 9; ++v) m[v] += PM[sj * 9 + v];
+          for (int v = 0; v < 3; ++v) f[v] += PF[sj * 3 + v];
+        }
+      }
+      for (int v = 0; v < 9; ++v) SM[q * 9 + v] = m[v];
+      for (int v = 0; v < 3; ++v) SF[q * 3 + v] = f[v];
+    }
+    __syncthreads();
+  }
+
+  // 3. back-substitution: root (section 0), rounds in reverse, interiors
+  if (tid == 0) {
+    T Si[9], xr[3];
+    inv3(SM, Si);
+    mv3(Si, SF, xr);
+    for (int v = 0; v < 3; ++v) XS[v] = xr[v];
+  }
+  __syncthreads();
+  for (int r = A.n_rounds - 1; r >= 0; --r) {
+    for (int i = A.elim_ptr[r] + tid; i < A.elim_ptr[r + 1]; i += nt) {
+      const int j = A.elim_sec[i], p = A.elim_par[i], c = A.elim_child[i];
+      T xj[3], t[3];
+      mv3(RM + j * 9, XS + p * 3, t);
+      for (int v = 0; v < 3; ++v) xj[v] = RF[j * 3 + v] + t[v];
+      if (c >= 0) {
+        mv3(RE + j * 9, XS + c * 3, t);
+        for (int v = 0; v < 3; ++v) xj[v] += t[v];
+      }
+      for (int v = 0; v < 3; ++v) XS[j * 3 + v] = xj[v];
+    }
+    __syncthreads();
+  }
+  for (int s = tid; s < ns; s += nt) {
+    const int hd = A.sec_head[s], len = A.sec_len[s], ps = A.sec_parent[s];
+    T xp[3], xv[3];
+    for (int v = 0; v < 3; ++v) {
+      xp[v] = ps >= 0 ? XS[ps * 3 + v] : T(0);
+      xv[v] = XS[s * 3 + v];
+    }
+    int k = A.perm[hd + len - 1];
+    for (int v = 0; v < 3; ++v) X[(int64_t)k * 3 + v] = xv[v];
+    // each interior's recovery terms are fetched a node ahead (clamped)
+    T pw[21];
+    auto fetch_w = [&](int q) {
+      for (int v = 0; v < 21; ++v) pw[v] = W[(int64_t)q * 21 + v];
+    };
+    if (len > 1) fetch_w(hd + len - 2);
+    for (int q = hd + len - 2; q >= hd; --q) {
+      T w[21], t1[3], t2[3];
+      for (int v = 0; v < 21; ++v) w[v] = pw[v];
+      fetch_w(q > hd ? q - 1 : hd);
+      mv3(w + 3, xp, t1);
+      mv3(w + 12, xv, t2);
+      for (int v = 0; v < 3; ++v) xv[v] = w[v] + t1[v] + t2[v];
+      k = A.perm[q];
+      for (int v = 0; v < 3; ++v) X[(int64_t)k * 3 + v] = xv[v];
+    }
+  }
+}
+
 }  // namespace dn
 
 // one wavefront per workgroup for the lane-per-system sweeps: a few thousand
@@ -680,6 +959,54 @@ template <typename T>
 static unsigned blocks_for(int64_t B) {
   return (unsigned)((B + kSweepThreads - 1) / kSweepThreads);
 }
+
+// LDS of the block contraction: kBcSlots values per section.
+constexpr size_t kBcLdsMax = 160 * 1024;
+
+// most sections one system may have (K does not bound the LDS)
+DN_EXPORT int dn_bt3_contract_max_sections(int elem_bytes, int64_t K) {
+  if ((elem_bytes != 4 && elem_bytes != 8) || K < 0) return -1;
+  return (int)(kBcLdsMax / ((size_t)dn::kBcSlots * elem_bytes));
+}
+
+DN_EXPORT int64_t dn_bt3_contract_work_elems(int64_t B, int64_t K) {
+  if (B < 0 || K < 0) return -1;
+  return 21 * B * K;
+}
+
+template <typename T>
+static int bt3_contract_impl(const dn::BcArgs<T>& a, int64_t B, int64_t work_elems,
+                             hipStream_t stream) {
+  if (!a.main || !a.edge || !a.rhs || !a.x || !a.work || B < 1 || a.K < 1 || a.n_sec < 1 ||
+      a.n_rounds < 0 || !a.perm || !a.sec_head || !a.sec_len || !a.sec_child ||
+      !a.sec_nchild || !a.sec_parent || !a.elim_ptr || !a.elim_sec || !a.elim_par ||
+      !a.elim_child || !a.recv_ptr || !a.recv_sec || !a.src_ptr || !a.src)
+    return DN_EINVAL;
+  if (work_elems < 21 * B * a.K || B > 0x7fffffff || a.K > 0x7fffffff) return DN_EINVAL;
+  const size_t lds = (size_t)dn::kBcSlots * a.n_sec * sizeof(T);
+  if (lds > kBcLdsMax) return DN_ERANGE;
+  hipLaunchKernelGGL((dn::bt3_contract_kernel<T>), dim3((unsigned)B), dim3(dn::kBcThreads),
+                     lds, stream, a);
+  return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
+}
+
+#define DN_BT3_CONTRACT_EXPORT(T, SFX)                                                       \
+  DN_EXPORT int dn_bt3_tree_contract_##SFX(                                                  \
+      const T* main, const T* edge, const T* rhs, const T* gmem, T* x, int64_t B, int64_t K, \
+      const int32_t* perm, int n_sec, const int32_t* sec_head, const int32_t* sec_len,       \
+      const int32_t* sec_child, const int32_t* sec_nchild, const int32_t* sec_parent,        \
+      int n_rounds, const int32_t* elim_ptr, const int32_t* elim_sec,                        \
+      const int32_t* elim_par, const int32_t* elim_child, const int32_t* recv_ptr,           \
+      const int32_t* recv_sec, const int32_t* src_ptr, const int32_t* src, T* work,          \
+      int64_t work_elems, hipStream_t stream) {                                              \
+    dn::BcArgs<T> a{main,     edge,     rhs,      gmem,       x,          work,              \
+                    K,        n_sec,    n_rounds, perm,       sec_head,   sec_len,           \
+                    sec_child, sec_nchild, sec_parent, elim_ptr, elim_sec, elim_par,          \
+                    elim_child, recv_ptr, recv_sec, src_ptr, src};                           \
+    return bt3_contract_impl<T>(a, B, work_elems, stream);                                   \
+  }
+DN_BT3_CONTRACT_EXPORT(float, f32)
+DN_BT3_CONTRACT_EXPORT(double, f64)
 
 DN_EXPORT int64_t dn_block_work_elems(int kind, int64_t B, int64_t K) {
   if (B < 0 || K < 0) return -1;
@@ -892,13 +1219,17 @@ static int extcell_tree_impl(const dn::XtArgs<T>& a0, const int32_t* parent32,
                              const int64_t* parent, const int64_t* order,
                              const int64_t* layer_ptr, int64_t n_layers, int64_t n_steps,
                              T* rec, int64_t rec_every, T* work, int64_t work_elems,
-                             hipStream_t stream) {
+                             hipStream_t stream, const dn::BcArgs<T>* plan = nullptr) {
   const int64_t B = a0.B, K = a0.K;
   if (B < 1 || K < 1 || n_steps < 0 || B * K > (int64_t)INT32_MAX * 256) return DN_EINVAL;
   if (!a0.main || !a0.edge || !a0.area || !a0.xg || !a0.hh || !a0.vc || !a0.m || !a0.h ||
-      !a0.n || !parent32 || !a0.child_ptr || !parent || !order || !work)
+      !a0.n || !parent32 || !a0.child_ptr || !work)
     return DN_EINVAL;
-  if (work_elems < 27 * B * K || (rec && rec_every <= 0)) return DN_EINVAL;
+  if (!plan && (!parent || !order)) return DN_EINVAL;
+  // D (membrane g; 9 B K reserved), R, X, then the solve's workspace: 12 B K
+  // for the layer form, 21 B K for the contraction
+  const int64_t solve_elems = (plan ? 21 : 12) * B * K;
+  if (work_elems < 15 * B * K + solve_elems || (rec && rec_every <= 0)) return DN_EINVAL;
   dn::XtArgs<T> a = a0;
   a.parent = parent32;
   a.D = work;
@@ -909,9 +1240,22 @@ static int extcell_tree_impl(const dn::XtArgs<T>& a0, const int32_t* parent32,
   for (int64_t s = 0; s < n_steps; ++s) {
     hipLaunchKernelGGL((dn::extcell_tree_assemble_kernel<T>), nodes, dim3(256), 0, stream, a,
                        s);
-    const int st = bt3_tree_impl<T>(a.main, a.edge, a.R, parent, order, layer_ptr, n_layers,
-                                    a.child_ptr, a.child_idx, X, B, K, solve_work, 12 * B * K,
-                                    stream, a.D);
+    int st;
+    if (plan) {
+      dn::BcArgs<T> c = *plan;
+      c.main = a.main;
+      c.edge = a.edge;
+      c.rhs = a.R;
+      c.gmem = a.D;
+      c.x = X;
+      c.work = solve_work;
+      c.K = K;
+      st = bt3_contract_impl<T>(c, B, solve_elems, stream);
+    } else {
+      st = bt3_tree_impl<T>(a.main, a.edge, a.R, parent, order, layer_ptr, n_layers,
+                            a.child_ptr, a.child_idx, X, B, K, solve_work, 12 * B * K, stream,
+                            a.D);
+    }
     if (st != DN_OK) return st;
     T* r = (rec && (s + 1) % rec_every == 0) ? rec + ((s + 1) / rec_every - 1) * B * K : nullptr;
     hipLaunchKernelGGL((dn::extcell_tree_update_kernel<T>), nodes, dim3(256), 0, stream, a.vc,
@@ -922,6 +1266,9 @@ static int extcell_tree_impl(const dn::XtArgs<T>& a0, const int32_t* parent32,
 
 DN_EXPORT int64_t dn_extcell_tree_work_elems(int64_t B, int64_t K) {
   return (B < 0 || K < 0) ? -1 : 27 * B * K;
+}
+DN_EXPORT int64_t dn_extcell_tree_contract_work_elems(int64_t B, int64_t K) {
+  return (B < 0 || K < 0) ? -1 : 36 * B * K;
 }
 
 #define DN_EXTCELL_TREE_EXPORT(T, SFX)                                                       \
@@ -938,6 +1285,27 @@ DN_EXPORT int64_t dn_extcell_tree_work_elems(int64_t B, int64_t K) {
                     B, K, dt};                                                               \
     return extcell_tree_impl<T>(a, parent32, parent, order, layer_ptr, n_layers, n_steps,    \
                                 rec, rec_every, work, work_elems, stream);                   \
+  }                                                                                          \
+  DN_EXPORT int dn_extcell_tree_run_contract_##SFX(                                          \
+      int64_t B, int64_t K, const T* main, const T* edge, const T* area, const T* xg,        \
+      const int32_t* parent32, const int32_t* child_ptr, const int32_t* child_idx,           \
+      const int32_t* perm, int n_sec, const int32_t* sec_head, const int32_t* sec_len,       \
+      const int32_t* sec_child, const int32_t* sec_nchild, const int32_t* sec_parent,        \
+      int n_rounds, const int32_t* elim_ptr, const int32_t* elim_sec,                        \
+      const int32_t* elim_par, const int32_t* elim_child, const int32_t* recv_ptr,           \
+      const int32_t* recv_sec, const int32_t* src_ptr, const int32_t* src, const T* sna,     \
+      const T* sk, const T* hh, const T* lead, const T* amp, const T* wave, const T* iamp,   \
+      const T* i_wave, T dt, int64_t n_steps, T* vc, T* m, T* h, T* n, T* rec,               \
+      int64_t rec_every, T* work, int64_t work_elems, hipStream_t stream) {                  \
+    dn::XtArgs<T> a{main, edge, area, xg,      sna, sk,      hh, lead, amp, wave, iamp,     \
+                    i_wave, nullptr, child_ptr, child_idx, vc, m, h, n, nullptr, nullptr,  \
+                    B, K, dt};                                                               \
+    dn::BcArgs<T> plan{nullptr,   nullptr,    nullptr,    nullptr,  nullptr,  nullptr,       \
+                       K,         n_sec,      n_rounds,   perm,     sec_head, sec_len,       \
+                       sec_child, sec_nchild, sec_parent, elim_ptr, elim_sec, elim_par,      \
+                       elim_child, recv_ptr,  recv_sec,   src_ptr,  src};                    \
+    return extcell_tree_impl<T>(a, parent32, nullptr, nullptr, nullptr, 0, n_steps, rec,     \
+                                rec_every, work, work_elems, stream, &plan);                 \
   }
 
 DN_EXTCELL_TREE_EXPORT(float, f32)

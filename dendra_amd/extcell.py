@@ -324,8 +324,9 @@ class ExtCellTrees:
 
     def __init__(self, spec: ExtCellTreeSpec, dt: float, hh: HH | None = None,
                  dtype=torch.float32, device="cuda", threads: int = 16,
-                 workspace: dict | None = None):
-        from .blocks import _children_csr
+                 workspace: dict | None = None, solver: str = "auto"):
+        from .blocks import _children_csr, _contract_cap, contraction_tables
+        from .contraction import TreeContractionPlan
         from .topology import build_dhs_layers, build_morphology
 
         if dtype not in _DT:
@@ -355,12 +356,30 @@ class ExtCellTrees:
         self._order = dev(order, torch.int64)
         self._layers = dev(layers, torch.int64)
         self._child_ptr, self._child_idx = _children_csr(self._parent)
+        # the block tree solve: by contraction (sections, then rake/compress
+        # rounds) when the plan fits one workgroup's LDS, else over the
+        # depth levels
+        if solver not in ("auto", "layers", "contraction"):
+            raise ValueError("solver must be 'auto', 'layers' or 'contraction'")
+        self._plan = None
+        if solver != "layers":
+            plan = TreeContractionPlan(np.asarray(par, dtype=np.int64), 16)
+            elem = torch.tensor([], dtype=dtype).element_size()
+            fits = plan.n_sections <= _contract_cap(elem, K)
+            if fits:
+                self._plan = plan
+                self._tables = contraction_tables(plan, self.device)
+            elif solver == "contraction":
+                raise ValueError("the contraction plan does not fit one workgroup")
+        self.solver = "contraction" if self._plan is not None else "layers"
         opt = lambda a: None if a is None else dev(a)
         self._sna, self._sk, self._lead, self._iamp = (opt(spec.sna), opt(spec.sk),
                                                        opt(spec.lead), opt(spec.iamp))
         self._amp = opt(spec.ve_amp)
         self._hh = torch.as_tensor(self.hh.array(), dtype=dtype, device=self.device)
-        n = int(_lib.load().dn_extcell_tree_work_elems(B, K))
+        lib = _lib.load()
+        n = int(lib.dn_extcell_tree_work_elems(B, K) if self._plan is None else
+                lib.dn_extcell_tree_contract_work_elems(B, K))
         self._work = torch.empty(n, dtype=dtype, device=self.device)
         self.reset()
 
@@ -398,15 +417,26 @@ class ExtCellTrees:
             rec = torch.empty(n_steps // record_every, self.B, self.K, dtype=self.dtype,
                               device=self.device)
         p = _lib.ptr
-        fn = lib.dn_extcell_tree_run_f64 if self.dtype == torch.float64 else \
-            lib.dn_extcell_tree_run_f32
-        st = fn(self.B, self.K, p(self._main), p(self._edge), p(self._area), p(self._xg),
+        sfx = "f64" if self.dtype == torch.float64 else "f32"
+        handle = _lib.stream_handle(self.device) if stream is None else stream.cuda_stream
+        tail = (p(self._sna), p(self._sk), p(self._hh), p(self._lead), p(self._amp), p(wv),
+                p(self._iamp), p(iw), self.dt, int(n_steps), p(self.vc), p(self.m),
+                p(self.h), p(self.n), p(rec), int(record_every), p(self._work),
+                self._work.numel(), handle)
+        if self._plan is not None:
+            (perm, sec_head, sec_len, sec_child, sec_nchild, sec_parent,
+             ptr, sec, par, child, rptr, recv, sptr, src) = self._tables
+            st = getattr(lib, f"dn_extcell_tree_run_contract_{sfx}")(
+                self.B, self.K, p(self._main), p(self._edge), p(self._area), p(self._xg),
+                p(self._parent32), p(self._child_ptr), p(self._child_idx), p(perm),
+                self._plan.n_sections, p(sec_head), p(sec_len), p(sec_child), p(sec_nchild),
+                p(sec_parent), self._plan.n_rounds, p(ptr), p(sec), p(par), p(child), p(rptr),
+                p(recv), p(sptr), p(src), *tail)
+        else:
+            st = getattr(lib, f"dn_extcell_tree_run_{sfx}")(
+                self.B, self.K, p(self._main), p(self._edge), p(self._area), p(self._xg),
                 p(self._parent32), p(self._parent), p(self._order), p(self._layers),
-                self._layers.numel() - 1, p(self._child_ptr), p(self._child_idx), p(self._sna),
-                p(self._sk), p(self._hh), p(self._lead), p(self._amp), p(wv), p(self._iamp),
-                p(iw), self.dt, int(n_steps), p(self.vc), p(self.m), p(self.h), p(self.n),
-                p(rec), int(record_every), p(self._work), self._work.numel(),
-                _lib.stream_handle(self.device) if stream is None else stream.cuda_stream)
+                self._layers.numel() - 1, p(self._child_ptr), p(self._child_idx), *tail)
         _lib.check(st, "ExtCell tree run")
         self.t_step += n_steps
         return rec

@@ -152,6 +152,40 @@ int dn_bt3_tree_f64(const double* main, const double* edge, const double* rhs,
                     double* x, int64_t B, int64_t K, double* work, int64_t work_elems,
                     hipStream_t stream);
 
+/* dn_bt3_tree_contract_*: the same block tree system (gmem, nullable: a
+ * membrane conductance g per node added to its main block as
+ * g [[1,-1,0],[-1,1,0],[0,0,0]]) solved by contraction over a section plan,
+ * the 3x3-block form of dn_tree_contract_* (same tables: perm, n_sec
+ * sections sec_head/sec_len/sec_child/sec_nchild/sec_parent, n_rounds
+ * rake/compress rounds: elim_ptr .. src; see dendra_amd.contraction):
+ * each section eliminates its nodes head first, then the section tree is
+ * contracted, then everything is recovered.  Main blocks need not be
+ * symmetric.  One workgroup per system; DN_ERANGE when n_sec exceeds
+ * dn_bt3_contract_max_sections(element bytes, K); work holds
+ * dn_bt3_contract_work_elems(B, K) elements. */
+int dn_bt3_contract_max_sections(int elem_bytes, int64_t K);
+int64_t dn_bt3_contract_work_elems(int64_t B, int64_t K);
+int dn_bt3_tree_contract_f32(const float* main, const float* edge, const float* rhs,
+                             const float* gmem, float* x, int64_t B, int64_t K,
+                             const int32_t* perm, int n_sec, const int32_t* sec_head,
+                             const int32_t* sec_len, const int32_t* sec_child,
+                             const int32_t* sec_nchild, const int32_t* sec_parent, int n_rounds,
+                             const int32_t* elim_ptr, const int32_t* elim_sec,
+                             const int32_t* elim_par, const int32_t* elim_child,
+                             const int32_t* recv_ptr, const int32_t* recv_sec,
+                             const int32_t* src_ptr, const int32_t* src, float* work,
+                             int64_t work_elems, hipStream_t stream);
+int dn_bt3_tree_contract_f64(const double* main, const double* edge, const double* rhs,
+                             const double* gmem, double* x, int64_t B, int64_t K,
+                             const int32_t* perm, int n_sec, const int32_t* sec_head,
+                             const int32_t* sec_len, const int32_t* sec_child,
+                             const int32_t* sec_nchild, const int32_t* sec_parent, int n_rounds,
+                             const int32_t* elim_ptr, const int32_t* elim_sec,
+                             const int32_t* elim_par, const int32_t* elim_child,
+                             const int32_t* recv_ptr, const int32_t* recv_sec,
+                             const int32_t* src_ptr, const int32_t* src, double* work,
+                             int64_t work_elems, hipStream_t stream);
+
 /* ---- NetCon delivery with fractional (interpolated) delays ----------------
  * Replaces the ordered two-tap enqueue (dendra/models/networks/
  * _two_tap_enqueue_cpu.cpp:26-152, dendra_native::ordered_two_tap_enqueue)
@@ -367,8 +401,13 @@ int dn_extcell_run_f64(int64_t B, int64_t K, const double* maind, const double* 
  * (K + 1) / child_idx the children CSR (ascending child index); optional
  * sna/sk/lead/iamp (B,K), amp (B), wave/i_wave (n_steps); state vc (B,K,3),
  * m/h/n (B,K) in place; rec (optional) vi - ve0 every rec_every steps,
- * (n_rec,B,K).  work: dn_extcell_tree_work_elems(B, K) elements. */
+ * (n_rec,B,K).  work: dn_extcell_tree_work_elems(B, K) elements.
+ * dn_extcell_tree_run_contract_*: the same run with the block tree solve by
+ * contraction over a section plan (the tables of dn_bt3_tree_contract_*) in
+ * place of parent/order/layer_ptr; its work holds
+ * dn_extcell_tree_contract_work_elems(B, K) elements. */
 int64_t dn_extcell_tree_work_elems(int64_t B, int64_t K);
+int64_t dn_extcell_tree_contract_work_elems(int64_t B, int64_t K);
 int dn_extcell_tree_run_f32(int64_t B, int64_t K, const float* main, const float* edge,
                             const float* area, const float* xg, const int32_t* parent32,
                             const int64_t* parent, const int64_t* order,
@@ -389,6 +428,29 @@ int dn_extcell_tree_run_f64(int64_t B, int64_t K, const double* main, const doub
                             const double* i_wave, double dt, int64_t n_steps, double* vc, double* m,
                             double* h, double* n, double* rec, int64_t rec_every, double* work,
                             int64_t work_elems, hipStream_t stream);
+int dn_extcell_tree_run_contract_f32(
+    int64_t B, int64_t K, const float* main, const float* edge, const float* area,
+    const float* xg, const int32_t* parent32, const int32_t* child_ptr, const int32_t* child_idx,
+    const int32_t* perm, int n_sec, const int32_t* sec_head, const int32_t* sec_len,
+    const int32_t* sec_child, const int32_t* sec_nchild, const int32_t* sec_parent, int n_rounds,
+    const int32_t* elim_ptr, const int32_t* elim_sec, const int32_t* elim_par,
+    const int32_t* elim_child, const int32_t* recv_ptr, const int32_t* recv_sec,
+    const int32_t* src_ptr, const int32_t* src, const float* sna, const float* sk,
+    const float* hh, const float* lead, const float* amp, const float* wave, const float* iamp,
+    const float* i_wave, float dt, int64_t n_steps, float* vc, float* m, float* h, float* n,
+    float* rec, int64_t rec_every, float* work, int64_t work_elems, hipStream_t stream);
+int dn_extcell_tree_run_contract_f64(
+    int64_t B, int64_t K, const double* main, const double* edge, const double* area,
+    const double* xg, const int32_t* parent32, const int32_t* child_ptr,
+    const int32_t* child_idx, const int32_t* perm, int n_sec, const int32_t* sec_head,
+    const int32_t* sec_len, const int32_t* sec_child, const int32_t* sec_nchild,
+    const int32_t* sec_parent, int n_rounds, const int32_t* elim_ptr, const int32_t* elim_sec,
+    const int32_t* elim_par, const int32_t* elim_child, const int32_t* recv_ptr,
+    const int32_t* recv_sec, const int32_t* src_ptr, const int32_t* src, const double* sna,
+    const double* sk, const double* hh, const double* lead, const double* amp,
+    const double* wave, const double* iamp, const double* i_wave, double dt, int64_t n_steps,
+    double* vc, double* m, double* h, double* n, double* rec, int64_t rec_every, double* work,
+    int64_t work_elems, hipStream_t stream);
 
 /* ---- fused HH + backward Euler, unbranched cables -------------------------
  * Replaces the per-step transition of _bwd_euler_ub

@@ -99,3 +99,30 @@ def test_extcell_tree_batch_equals_single_runs_and_oracle(dtype, B):
         got = rec.reshape(rec.shape[0], B, -1)[:, b]
         assert np.abs(got - orec.reshape(orec.shape[0], -1)).max() <= 1e-8
 
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_extcell_tree_contraction_equals_the_layer_solve(dtype):
+    """The block tree solve by contraction (sections, rake/compress rounds)
+    and over whole depth levels give the same ExtCell tree run on a
+    1000-node random binary tree (102 sections of up to 17 nodes, 168 depth
+    levels), to rounding."""
+    from dendra_amd.extcell import ExtCellTrees
+    from dendra_amd.model import HH
+    from dendra_amd.workloads import random_binary_tree
+
+    parents = [int(p) for p in random_binary_tree(1000, 1, 7)[0]]
+    rng = np.random.default_rng(77)
+    ts, _ = tree_spec(rng, 6, parents)
+    steps, dt = 120, 0.005
+    wave = np.zeros(steps)
+    wave[10:60] = 1.0
+    hh = HH(celsius=6.3)
+    recs = []
+    for solver in ("contraction", "layers"):
+        trees = ExtCellTrees(ts, dt, hh, dtype, "cuda", solver=solver)
+        assert trees.solver == solver
+        recs.append(trees.run(steps, wave=wave, record_every=10).double().cpu().numpy())
+    scale = np.abs(recs[1]).max()
+    tol = 1e-10 if dtype == torch.float64 else 2e-5
+    assert np.abs(recs[0] - recs[1]).max() <= tol * scale
+
