@@ -157,6 +157,7 @@ for _sfx in ("f32", "f64"):
     _SIGNATURES[f"dn_history_record_{_sfx}"] = (i32, [p, i64, i64, p, p, p])
 _SIGNATURES["dn_history_record_packed"] = (i32, [p, i64, i64, p, i64, p, p, p, p])
 _SIGNATURES["dn_calendar_work_bytes"] = (i64, [i64, i64, i32])
+_SIGNATURES["dn_calendar_pop_work_bytes"] = (i64, [i64, i64, i32])
 _SIGNATURES["dn_extcell_work_elems"] = (i64, [i64, i64])
 _SIGNATURES["dn_extcell_tree_work_elems"] = (i64, [i64, i64])
 for _sfx, _ft in (("f32", f32), ("f64", f64)):

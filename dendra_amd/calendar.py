@@ -64,9 +64,13 @@ def _kind(dtype):
     return _KIND[dtype]
 
 
-def _work(n, cap, dtype, device):
+def _work(n, cap, dtype, device, width=None):
+    """Workspace of an append (n candidates) or, with `width`, of a pop that
+    adds the due values through per-index hit lists."""
     kind, _ = _kind(dtype)
-    nbytes = int(_lib.load().dn_calendar_work_bytes(int(n), int(cap), kind))
+    lib = _lib.load()
+    nbytes = int(lib.dn_calendar_work_bytes(int(n), int(cap), kind) if width is None else
+                 lib.dn_calendar_pop_work_bytes(int(cap), int(width), kind))
     if nbytes < 0:
         raise ValueError("pending calendar too large for the GPU kernels")
     return torch.empty(nbytes, dtype=torch.uint8, device=device), nbytes
@@ -148,9 +152,10 @@ class _Pop(torch.autograd.Function):
     def forward(ctx, values, base, state, slot, width, scratch):
         cap = values.numel()
         dev = values.device
-        n_slots = torch.zeros(cap, dtype=torch.long, device=dev)
-        n_indices = torch.zeros(cap, dtype=torch.long, device=dev)
-        n_values = torch.zeros(cap, dtype=values.dtype, device=dev)
+        # the kernel zeroes the entries past the kept ones
+        n_slots = torch.empty(cap, dtype=torch.long, device=dev)
+        n_indices = torch.empty(cap, dtype=torch.long, device=dev)
+        n_values = torch.empty(cap, dtype=values.dtype, device=dev)
         size = torch.empty_like(state.size)
         if scratch is not None:
             payload = scratch
@@ -160,7 +165,7 @@ class _Pop(torch.autograd.Function):
             payload = torch.zeros(width, dtype=values.dtype, device=dev)
         keys = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
         dest = torch.empty(max(cap, 1), dtype=torch.long, device=dev)
-        work, nbytes = _work(0, cap, values.dtype, dev)
+        work, nbytes = _work(0, cap, values.dtype, dev, width=width)
         p = _lib.ptr
         fn = getattr(_lib.load(), f"dn_calendar_pop_{_kind(values.dtype)[1]}")
         st = fn(p(state.slots), p(state.indices), p(values), cap, p(state.size), p(slot),

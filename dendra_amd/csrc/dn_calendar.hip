@@ -164,9 +164,21 @@ __global__ __launch_bounds__(kCalThreads) void cal_pop_compact_kernel(
     const uint8_t* __restrict__ keep, int64_t cap, const int64_t* __restrict__ tile_off,
     const int64_t* __restrict__ slots, const int64_t* __restrict__ indices,
     const T* __restrict__ values, int64_t* __restrict__ n_slots,
-    int64_t* __restrict__ n_indices, T* __restrict__ n_values, int64_t* __restrict__ dest) {
+    int64_t* __restrict__ n_indices, T* __restrict__ n_values, int64_t* __restrict__ dest,
+    const int64_t* __restrict__ new_size) {
   __shared__ int64_t buf[kCalThreads];
   const int64_t base = (int64_t)blockIdx.x * kCalTile + threadIdx.x * kCalItems;
+  // the output positions past the kept entries are zeroed here, so the
+  // caller need not clear n_* first
+  const int64_t kept = *new_size;
+  for (int k = 0; k < kCalItems; ++k) {
+    const int64_t q = base + k;
+    if (q < cap && q >= kept) {
+      n_slots[q] = 0;
+      n_indices[q] = 0;
+      n_values[q] = T(0);
+    }
+  }
   int64_t c = 0;
   for (int k = 0; k < kCalItems; ++k)
     if (base + k < cap && keep[base + k]) ++c;
@@ -198,6 +210,88 @@ __global__ __launch_bounds__(256) void cal_reduce_kernel(const int32_t* __restri
   T acc = payload[k];
   for (int64_t j = i; j < cap && key[j] == k; ++j) acc = acc + val[j];
   payload[k] = acc;
+}
+
+// pop without a sort (the workspace of dn_calendar_pop_work_bytes): each due
+// entry counts itself at its index and files its position in the index's hit
+// row (up to kCalHit); the first hit lists the index.  Then one thread per
+// listed index adds the index's values in position order (selection over
+// its short row; a scan of the keys past kCalHit hits), so every payload
+// entry sees the sequential scatter_add's sums, and clears the count.
+constexpr int kCalHit = 16;
+
+__global__ __launch_bounds__(256) void cal_hit_clear_kernel(int32_t* __restrict__ hcnt,
+                                                            int64_t width,
+                                                            int32_t* __restrict__ tcount) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = i; t < width; t += n) hcnt[t] = 0;
+  if (i == 0) *tcount = 0;
+}
+
+__global__ __launch_bounds__(kCalThreads) void cal_mark_kernel(
+    const int32_t* __restrict__ key, int64_t cap, int64_t width, int32_t* __restrict__ hcnt,
+    int32_t* __restrict__ hpos, int32_t* __restrict__ tlist, int32_t* __restrict__ tcount) {
+  const int64_t base = (int64_t)blockIdx.x * kCalTile + threadIdx.x * kCalItems;
+#pragma unroll
+  for (int k = 0; k < kCalItems; ++k) {
+    const int64_t p = base + k;
+    if (p >= cap) break;
+    const int32_t t = key[p];
+    if (t >= width) continue;
+    const int s = atomicAdd(&hcnt[t], 1);
+    if (s < kCalHit) hpos[(int64_t)t * kCalHit + s] = (int32_t)p;
+    if (s == 0) tlist[atomicAdd(tcount, 1)] = t;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void cal_ordered_add_kernel(
+    const int32_t* __restrict__ key, const T* __restrict__ val, int64_t cap,
+    int32_t* __restrict__ hcnt, const int32_t* __restrict__ hpos,
+    const int32_t* __restrict__ tlist, const int32_t* __restrict__ tcount,
+    T* __restrict__ payload) {
+  const int n = *tcount;
+  const int nth = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += nth) {
+    const int t = tlist[i];
+    const int cnt = hcnt[t];
+    T acc = payload[t];
+    if (cnt <= kCalHit) {
+      const int4* row = reinterpret_cast<const int4*>(hpos + (int64_t)t * kCalHit);
+      int e[kCalHit];
+#pragma unroll
+      for (int q = 0; q < kCalHit / 4; ++q) {
+        const int4 v = row[q];
+        e[4 * q] = v.x;
+        e[4 * q + 1] = v.y;
+        e[4 * q + 2] = v.z;
+        e[4 * q + 3] = v.w;
+      }
+#pragma unroll
+      for (int q = 0; q < kCalHit; ++q) {
+        const int m = -(int)(q < cnt);  // all ones for a filed hit
+        e[q] = (e[q] & m) | (0x7fffffff & ~m);
+      }
+      unsigned used = 0;
+      for (int o = 0; o < cnt; ++o) {
+        int best = 0x7fffffff, bi = 0;
+#pragma unroll
+        for (int q = 0; q < kCalHit; ++q)
+          if (!((used >> q) & 1u) && e[q] < best) {
+            best = e[q];
+            bi = q;
+          }
+        used |= 1u << bi;
+        acc = acc + val[best];
+      }
+    } else {
+      for (int64_t p = 0; p < cap; ++p)
+        if (key[p] == t) acc = acc + val[p];
+    }
+    payload[t] = acc;
+    hcnt[t] = 0;
+  }
 }
 
 }  // namespace dn
@@ -273,6 +367,13 @@ CalWork carve(void* work, int64_t n, int64_t cap) {
 
 constexpr int64_t kCalMax = (int64_t)1 << 30;
 
+// the pop's hit lists after the sort workspace: counts, list count, rows of
+// kCalHit positions, the index list
+int64_t cal_hit_bytes(int64_t width) {
+  const int64_t w = width > 0 ? width : 1;
+  return 2 * align_up(4 * w) + align_up(4) + align_up(4 * w * dn::kCalHit);
+}
+
 template <typename T>
 int append_impl(int64_t* slots, int64_t* indices, T* values, int64_t cap,
                 const int64_t* old_size, const uint8_t* old_ovf, int64_t* new_size,
@@ -323,8 +424,26 @@ int pop_impl(const int64_t* slots, const int64_t* indices, const T* values, int6
   if (tiles == 0) return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
   hipLaunchKernelGGL((dn::cal_pop_compact_kernel<T>), dim3((unsigned)tiles),
                      dim3(dn::kCalThreads), 0, stream, w.flags, cap, w.tile_off, slots, indices,
-                     values, n_slots, n_indices, n_values, keep_dest);
-  if (width > 0) {
+                     values, n_slots, n_indices, n_values, keep_dest, new_size);
+  if (width > 0 && work_bytes >= cal_work_bytes<T>(0, cap) + cal_hit_bytes(width)) {
+    char* h = static_cast<char*>(work) + cal_work_bytes<T>(0, cap);
+    int32_t* hcnt = reinterpret_cast<int32_t*>(h);
+    h += align_up(4 * width);
+    int32_t* tlist = reinterpret_cast<int32_t*>(h);
+    h += align_up(4 * width);
+    int32_t* tcount = reinterpret_cast<int32_t*>(h);
+    h += align_up(4);
+    int32_t* hpos = reinterpret_cast<int32_t*>(h);
+    const unsigned clear_blocks = (unsigned)((width + 255) / 256 < 1024 ? (width + 255) / 256 : 1024);
+    hipLaunchKernelGGL(dn::cal_hit_clear_kernel, dim3(clear_blocks), dim3(256), 0, stream, hcnt,
+                       width, tcount);
+    hipLaunchKernelGGL(dn::cal_mark_kernel, dim3((unsigned)tiles), dim3(dn::kCalThreads), 0,
+                       stream, key, cap, width, hcnt, hpos, tlist, tcount);
+    const int64_t lim = width < cap ? width : cap;
+    const unsigned add_blocks = (unsigned)((lim + 255) / 256 < 1024 ? (lim + 255) / 256 : 1024);
+    hipLaunchKernelGGL((dn::cal_ordered_add_kernel<T>), dim3(add_blocks), dim3(256), 0, stream,
+                       key, values, cap, hcnt, hpos, tlist, tcount, payload);
+  } else if (width > 0) {
     // keys are non-negative (index or width): sorted as unsigned
     const uint32_t* ukey = reinterpret_cast<const uint32_t*>(key);
     uint32_t* usorted = reinterpret_cast<uint32_t*>(w.key_sorted);
@@ -346,6 +465,16 @@ int pop_impl(const int64_t* slots, const int64_t* indices, const T* values, int6
 }
 
 }  // namespace
+
+DN_EXPORT int64_t dn_calendar_pop_work_bytes(int64_t cap, int64_t width, int kind) {
+  if (cap < 0 || cap > kCalMax || width < 0 || width >= INT32_MAX) return -1;
+  switch (kind) {
+    case 0: return cal_work_bytes<float>(0, cap) + cal_hit_bytes(width);
+    case 1: return cal_work_bytes<double>(0, cap) + cal_hit_bytes(width);
+    case 2: return cal_work_bytes<int32_t>(0, cap) + cal_hit_bytes(width);
+    default: return -1;
+  }
+}
 
 DN_EXPORT int64_t dn_calendar_work_bytes(int64_t n, int64_t cap, int kind) {
   if (n < 0 || cap < 0 || n > kCalMax || cap > kCalMax) return -1;

@@ -320,13 +320,19 @@ int dn_history_record_packed(int64_t* hist, int64_t depth, int64_t n_words,
  *   dn_calendar_pop_*: entries p < size with slots[p] == *slot are due;
  *     payload[indices[p]] += values[p] in position order (payload holds its
  *     initial value; indices outside [0, width) are dropped); the other valid
- *     entries go, in order, to the front of n_* (zeroed by the caller);
+ *     entries go, in order, to the front of n_*, whose remaining entries are
+ *     zeroed;
  *     new_size = their count; keys_out (may be null) gets index for due
  *     entries and width otherwise, keep_dest (may be null) each entry's new
  *     position or -1.
  * work: dn_calendar_work_bytes(n, cap, kind) bytes (kind 0 f32, 1 f64,
- * 2 i32; n = candidates, 0 for a pop). */
+ * 2 i32; n = candidates, 0 for a pop).  A pop given
+ * dn_calendar_pop_work_bytes(cap, width, kind) bytes adds the due values
+ * through per-index hit lists (each due entry files its position at its
+ * index; one thread per hit index adds them in position order) instead of
+ * a stable radix sort of all cap keys; the payload is the same bit for bit. */
 int64_t dn_calendar_work_bytes(int64_t n, int64_t cap, int kind);
+int64_t dn_calendar_pop_work_bytes(int64_t cap, int64_t width, int kind);
 int dn_calendar_append_f32(int64_t* slots, int64_t* indices, float* values, int64_t cap,
                           const int64_t* old_size, const uint8_t* old_ovf,
                           int64_t* new_size, uint8_t* new_ovf, int64_t n,

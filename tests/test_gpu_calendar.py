@@ -69,6 +69,57 @@ def test_random_append_pop_sequence_is_bitwise_the_oracle(dtype):
     assert bool(later.overflow.item()) and int(later.size.item()) == size
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.int32])
+@pytest.mark.parametrize("width", [3, 1000])
+def test_pop_hit_lists_and_sort_agree_with_the_oracle(dtype, width):
+    """The pop's two ordered reductions through the C ABI: per-index hit
+    lists (the workspace of dn_calendar_pop_work_bytes; indices with more
+    than 16 due entries take the key scan) and the stable radix sort (the
+    smaller workspace of dn_calendar_work_bytes) give the oracle's payload
+    and calendar bit for bit; width 3 puts hundreds of due entries on each
+    index, width 1000 about one."""
+    from dendra_amd import _lib
+
+    rng = np.random.default_rng(width + {np.float32: 0, np.float64: 1, np.int32: 2}[dtype])
+    cap, size, slot = 5000, 4200, 7
+    slots = np.zeros(cap, np.int64)
+    slots[:size] = rng.integers(0, 12, size)
+    idx = np.zeros(cap, np.int64)
+    idx[:size] = rng.integers(0, width, size)
+    if dtype == np.int32:
+        vals = rng.integers(-5, 6, cap).astype(dtype)
+    else:
+        vals = (rng.standard_normal(cap) * 10.0 ** rng.integers(-3, 4, cap)).astype(dtype)
+    base = (rng.integers(-3, 4, width) if dtype == np.int32 else
+            rng.standard_normal(width)).astype(dtype)
+    ref = {"slots": slots.copy(), "indices": idx.copy(), "values": vals.copy(), "size": size,
+           "overflow": False}
+    r, want = calendar_ref.pop(ref, slot, width, base=base)
+    kind, sfx = {np.float32: (0, "f32"), np.float64: (1, "f64"), np.int32: (2, "i32")}[dtype]
+    lib, p = _lib.load(), _lib.ptr
+    fn = getattr(lib, f"dn_calendar_pop_{sfx}")
+    for nbytes in (lib.dn_calendar_pop_work_bytes(cap, width, kind),
+                   lib.dn_calendar_work_bytes(0, cap, kind)):
+        ts, ti, tv = dev(slots), dev(idx), dev(vals)
+        tsize, tslot = dev([size], torch.int64), dev([slot], torch.int64)
+        # outputs start dirty: the kernel zeroes the entries past the kept ones
+        ns, ni = torch.full((cap,), 7, dtype=torch.long, device="cuda"), torch.full(
+            (cap,), 7, dtype=torch.long, device="cuda")
+        nv = torch.full((cap,), 7, dtype=tv.dtype, device="cuda")
+        nsize = torch.empty(1, dtype=torch.long, device="cuda")
+        payload = dev(base)
+        work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        st = fn(p(ts), p(ti), p(tv), cap, p(tsize), p(tslot), width, p(ns), p(ni), p(nv),
+                p(nsize), p(payload), None, None, p(work), nbytes, _lib.stream_handle(tv.device))
+        assert st == 0
+        torch.cuda.synchronize()
+        assert np.array_equal(payload.cpu().numpy().view(np.uint8), want.view(np.uint8))
+        assert int(nsize.item()) == r["size"]
+        assert np.array_equal(nv.cpu().numpy().view(np.uint8), r["values"].view(np.uint8))
+        assert np.array_equal(ni.cpu().numpy(), r["indices"])
+        assert np.array_equal(ns.cpu().numpy(), r["slots"])
+
+
 def test_reference_overflow_mask_and_ordering_cases():
     from dendra_amd.calendar import (pending_calendar_append, pending_calendar_empty,
                                      pending_calendar_pop, pending_calendar_from_legacy)
