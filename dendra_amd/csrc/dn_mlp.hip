@@ -376,7 +376,8 @@ __global__ __launch_bounds__(1024 / CB, 2) void mlp_fwd_kernel(FwdArgs F) {
         for (int r = 0; r < 16; ++r) {
           const int i = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
           const float v = acc[rb][cb][r] + bj;
-          if (TRAIN) zrow[(8 * (r >> 2) + (r & 3)) * kFH] = v;
+          // training keeps gelu'(Z), the only use the backward makes of Z
+          if (TRAIN) zrow[(8 * (r >> 2) + (r & 3)) * kFH] = gelu_d(v);
           sH[i * kFP + j] = (bf16)gelu(v);
         }
       }
@@ -438,7 +439,7 @@ struct BwdArgs {
   int n;
   const float* dy; int64_t sy;    // [z][n]
   const bf16* w4;                 // [z][512]
-  const float* Z[3];              // [z][n][512]
+  const float* Z[3];              // [z][n][512] gelu'(Z_l) from the fused forward
   const bf16* Wtp[2]; int64_t sw; // packed W2^T, W3^T, member stride
   bf16* dZT[3];                   // [z][512][n]
   float* part;                    // [z][tiles][5][512]
@@ -463,10 +464,10 @@ __global__ __launch_bounds__(1024 / CB, 2) void mlp_bwd_kernel(BwdArgs A) {
       const float4 zv = *reinterpret_cast<const float4*>(&Z3[(int64_t)i * kFH + j]);
       const float d = A.dy[z * A.sy + m0 + i];
       bf16* o = &sD[i * kFP + j];
-      o[0] = (bf16)(d * (float)w4[j] * gelu_d(zv.x));
-      o[1] = (bf16)(d * (float)w4[j + 1] * gelu_d(zv.y));
-      o[2] = (bf16)(d * (float)w4[j + 2] * gelu_d(zv.z));
-      o[3] = (bf16)(d * (float)w4[j + 3] * gelu_d(zv.w));
+      o[0] = (bf16)(d * (float)w4[j] * zv.x);
+      o[1] = (bf16)(d * (float)w4[j + 1] * zv.y);
+      o[2] = (bf16)(d * (float)w4[j + 2] * zv.z);
+      o[3] = (bf16)(d * (float)w4[j + 3] * zv.w);
     }
   }
   __syncthreads();
@@ -491,7 +492,7 @@ __global__ __launch_bounds__(1024 / CB, 2) void mlp_bwd_kernel(BwdArgs A) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int i = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-          sD[i * kFP + j] = (bf16)(acc[rb][cb][r] * gelu_d(zrow[(8 * (r >> 2) + (r & 3)) * kFH]));
+          sD[i * kFP + j] = (bf16)(acc[rb][cb][r] * zrow[(8 * (r >> 2) + (r & 3)) * kFH]);
         }
         // one block's Z loads in flight at a time (register pressure)
         __builtin_amdgcn_sched_barrier(0);

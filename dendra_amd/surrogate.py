@@ -265,8 +265,8 @@ class ThresholdSurrogate:
         if fused:
             tiles = B // 64
             part = buf("part", (nz, tiles, 5, HID), torch.float32)
-            # forward: pre-activations, transposed activations, dy and the
-            # tiles' dw4 / db4 sums
+            # forward: gelu'(pre-activations), transposed activations, dy and
+            # the tiles' dw4 / db4 sums
             self._fused_forward(X0, B * F_PAD, B, y, (*Z, *HT, t0, dy, part), tile=64)
             # backward data products dZ3 -> dZ2 -> dZ1 and the tiles' bias sums
             _lib.check(lib.dn_mlp_backward(nz, B, 64, _p(dy), B, _p(bf["w4"]), _p(Z[0]),

@@ -809,9 +809,10 @@ int dn_mlp_adam_multi(int count, const int64_t* n, const int* rows, const int* c
  * block (nb, ks) is W[32 nb + l%32][16 ks + 8 (l/32) + e]; member strides
  * sw1, sw23 in elements).  X [z][n][128] (sx = 0: one X for every member),
  * n a multiple of 128; y [z][n] (stride sy).  Inference: Z*, HT*, t, dy
- * NULL; tile = 128 or 64 rows per workgroup.  Training (tile 64): also Z_l
- * [z][n][512] float32 pre-activations, HT_l [z][512][n] BF16 activations
- * and dy = 2 (y - t) / n, as the layer products dn_mlp_gemm (epi 0) +
+ * NULL; tile = 128 or 64 rows per workgroup.  Training (tile 64): also, in
+ * Z1..Z3 [z][n][512] float32, gelu'(Z_l) of the pre-activations (the only
+ * use the backward makes of them), HT_l [z][512][n] BF16 activations and
+ * dy = 2 (y - t) / n, as the layer products dn_mlp_gemm (epi 0) +
  * dn_mlp_out produce them, and into part [z][n/64][5][512] the tile sums
  * of dw4 = H3^T dy (slot 3) and db4 = sum dy (slot 4, element 0). */
 int dn_mlp_forward(int nz, int n, int tile, const void* X, int64_t sx, const void* Wp1,
@@ -821,7 +822,8 @@ int dn_mlp_forward(int nz, int n, int tile, const void* X, int64_t sx, const voi
                    void* HT1, void* HT2, void* HT3, const float* t, float* dy, float* part,
                    hipStream_t stream);
 /* The backward data products of a training step in one launch per 64-row
- * tile (tile must be 64): dZ3 = dy w4^T * gelu'(Z3), dZ2 = (dZ3 W3) *
+ * tile (tile must be 64), from the gelu'(Z_l) that dn_mlp_forward leaves in
+ * Z1..Z3: dZ3 = dy w4^T * gelu'(Z3), dZ2 = (dZ3 W3) *
  * gelu'(Z2), dZ1 = (dZ2 W2) * gelu'(Z1) in BF16 as dn_mlp_out_bwd and
  * dn_mlp_gemm (epi 1) produce them, each kept in LDS for the next product;
  * W2^T, W3^T packed in MFMA fragment order (member stride sw).  Writes
