@@ -83,6 +83,7 @@ class ThresholdSurrogate:
         self.t_adam = 0
         self.tile = 64    # rows per workgroup of the fused inference forward
         self._refresh_bf16()
+        self._plain_stale = False  # bf[W*] / bft behind the parameters
         self.ymean, self.ystd = 0.0, 1.0
         self.lo = self.hi = None
         self._bufs = {}
@@ -161,9 +162,21 @@ class ThresholdSurrogate:
         self._fused_forward(X, 0, n, y)
         return y
 
+    def _plain_copies(self):
+        """The plain and transposed BF16 weights the layer products read; a
+        fused training step's Adam leaves them behind (it writes only the
+        packed copies the fused kernels read) and they are converted here,
+        with the same rounding, when a layer product next needs them."""
+        if self._plain_stale:
+            for k in self.bft:
+                self.bf[k].copy_(self.params[k].to(_BF))
+                self.bft[k].copy_(self.params[k].transpose(1, 2).to(_BF))
+            self._plain_stale = False
+
     def _forward_chunk_layers(self, X, n):
         """The same predictions through the layer products (dn_mlp_gemm +
         dn_mlp_out): the unfused form of _forward_chunk."""
+        self._plain_copies()
         nz = self.nz
         h1 = self._buf("i_h1", (nz, n, HID), _BF)
         h2 = self._buf("i_h2", (nz, n, HID), _BF)
@@ -289,28 +302,37 @@ class ThresholdSurrogate:
         # Adam: every tensor in one launch
         self.t_adam += 1
         lr_t = lr * math.sqrt(1 - b2 ** self.t_adam) / (1 - b1 ** self.t_adam)
-        tab = self._adam_table()
+        tab = self._adam_table(fused)
         _lib.check(lib.dn_mlp_adam_multi(len(P), *[a.ctypes.data for a in tab], lr_t, b1, b2,
                                          eps, strm), "adam")
+        self._plain_stale = self._plain_stale or fused
         return y, dy
 
-    def _adam_table(self):
+    def _adam_table(self, fused=False):
         """Host arrays of dn_mlp_adam_multi (n, rows, cols, then the device
-        addresses of p, g, m, v and the four BF16 copies; 0 = none)."""
-        if getattr(self, "_tab", None) is None:
+        addresses of p, g, m, v and the four BF16 copies; 0 = none).  fused:
+        only the copies the fused kernels read (packed W, packed W^T, plain
+        w4); the plain and transposed matrices are then refreshed lazily."""
+        tabs = getattr(self, "_tabs", None)
+        if tabs is None:
+            tabs = self._tabs = {}
+        if fused not in tabs:
             P = self.params
             addr = lambda d, k: (d[k].data_ptr() if k in d else 0)
+            plain = {k: v for k, v in self.bf.items() if not fused or k not in self.bft}
             cols = [np.array([v.numel() for v in P.values()], np.int64),
                     np.array([v.shape[1] for v in P.values()], np.int32),
                     np.array([v.shape[2] for v in P.values()], np.int32)]
-            for d in (P, self.grads, self.m, self.v, self.bf, self.bft, self.bfp, self.bfpt):
+            for d in (P, self.grads, self.m, self.v, plain, {} if fused else self.bft,
+                      self.bfp, self.bfpt):
                 cols.append(np.array([addr(d, k) for k in P], np.uint64))
-            self._tab = cols
-        return self._tab
+            tabs[fused] = cols
+        return tabs[fused]
 
     def _layers_step(self, X0, t0, Z, HT, dZT, y, dy, B):
         """Forward and backward data products of a training step as layer
         products; fills Z, HT, y, dy, dZT and the bias / output gradients."""
+        self._plain_copies()
         nz, lib, strm = self.nz, self.lib, self._stream()
         bf, bft, P, G = self.bf, self.bft, self.params, self.grads
         buf = self._buf

@@ -210,12 +210,27 @@ def test_adam_step_matches_oracle():
         g = sur.grads[k].double().cpu().numpy()
         want, _, _ = mlp_ref.adam(p0[k], g, np.zeros_like(g), np.zeros_like(g), 1, 1e-3)
         assert np.allclose(v.double().cpu().numpy(), want, rtol=1e-5, atol=1e-7), k
-    # BF16 copies follow the masters
+    # the fused step's Adam writes the packed copies the fused kernels read
+    # (and the plain w4); the plain and transposed matrices follow lazily,
+    # with the same rounding, when a layer product needs them
+    bf16 = lambda k: sur.params[k].to(torch.bfloat16)
+    for k in ("W1", "W2", "W3"):
+        assert torch.equal(sur.bfp[k], sur.pack(bf16(k))), k
+    for k in ("W2", "W3"):
+        assert torch.equal(sur.bfpt[k], sur.pack(bf16(k).transpose(1, 2).contiguous())), k
+    assert torch.equal(sur.bf["w4"], bf16("w4"))
+    assert sur._plain_stale
+    sur._plain_copies()
     assert np.array_equal(sur.bf["W2"].double().cpu().numpy(),
                           mlp_ref.bf16(sur.params["W2"].double().cpu().numpy()))
     assert np.array_equal(sur.bft["W2"].double().cpu().numpy(),
                           mlp_ref.bf16(sur.params["W2"].double().cpu().numpy()).transpose(0, 2, 1))
-    for k in ("W1", "W2", "W3"):  # the packed copies the fused kernels read
+    # a layer-product step's Adam writes every copy itself
+    sur.train_step(idx, B, lr=1e-3, fused=False)
+    assert not sur._plain_stale
+    for k in ("W1", "W2", "W3"):
+        assert torch.equal(sur.bf[k], bf16(k)), k
+        assert torch.equal(sur.bft[k], bf16(k).transpose(1, 2).contiguous()), k
         assert torch.equal(sur.bfp[k], sur.pack(sur.bf[k])), k
     for k in ("W2", "W3"):
         assert torch.equal(sur.bfpt[k], sur.pack(sur.bft[k])), k
