@@ -149,52 +149,39 @@ __global__ __launch_bounds__(256) void interp_delay_pack_kernel(
   rec[e] = r;
 }
 
-// Adds the tap values of the staged connections [a, b) to one LDS column in
-// order: connection k contributes v[j * kRingChunk + k] to row
-// rows[k] + shift + j (mod depth) for j < TS.  Items are taken GC
-// connections at a time, without branches: all their cells are read at
-// once, an item whose cell an earlier item of the group also hits continues
-// from that item's sum, and the cells are written back in item order (the
-// last write of a repeated cell carries all of the group's adds).  Each cell
-// therefore sees the sequential sums, with GC * TS reads in flight instead
-// of one.  Past b the group repeats connection b - 1 with the value -0,
-// which leaves every value unchanged bit for bit (x + -0 = x, also for
-// x = -0), so its writes store what the cell already holds.
-template <typename T, int TS, int GC, int TT>
-__device__ __forceinline__ void ring_apply(T* col, const int32_t* rows, const T* v, int a,
-                                           int b, int shift, int depth) {
+// One wavefront adds one target's staged tap values to its LDS column in
+// the reference's order: items m in [TS a, TS b) are connection k = m / TS,
+// tap j = m % TS, value v[j * kRingChunk + k] into row rows[k] + shift + j
+// (mod depth).  The items go 64 at a time, one per lane; within a window the
+// lanes whose row no earlier pending lane shares add in one round (the
+// smallest lane of each row wins an LDS atomic min on that row's claim
+// word, adds, and frees the word), then the next round, and so on, so every
+// cell receives its items in item order -- bitwise the sequential walk --
+// and a window costs as many rounds as its most repeated row has items.
+// claim: depth words of this wavefront, all 0xffffffff between calls.
+template <typename T, int TS, int TT>
+__device__ __forceinline__ void ring_apply_wave(T* col, const int32_t* rows, const T* v, int a,
+                                                int b, int shift, int depth,
+                                                uint32_t* claim, int ln) {
 #pragma clang fp contract(off)
-  constexpr int G = GC * TS;
-  for (int k0 = a; k0 < b; k0 += GC) {
-    int r[G];
-    T x[G], add[G];
-#pragma unroll
-    for (int c = 0; c < GC; ++c) {
-      const bool live = k0 + c < b;
-      const int k = live ? k0 + c : b - 1;
-      const int r0 = rows[k] + shift;
-#pragma unroll
-      for (int j = 0; j < TS; ++j) {
-        const int u = c * TS + j;
-        int rr = r0 + j;  // < depth + 2: two wraps cover depth 1
-        rr = rr >= depth ? rr - depth : rr;
-        rr = rr >= depth ? rr - depth : rr;
-        r[u] = rr;
-        const T vk = v[j * kRingChunk + k];
-        add[u] = live ? vk : T(-0.0);
+  for (int m0 = TS * a; m0 < TS * b; m0 += 64) {
+    const int m = m0 + ln;
+    bool pending = m < TS * b;
+    const int mm = pending ? m : TS * b - 1;
+    const int k = TS == 1 ? mm : mm / TS;
+    const int j = TS == 1 ? 0 : mm - k * TS;
+    int r = rows[k] + shift + j;  // < depth + 2: two wraps cover depth 1
+    r = r >= depth ? r - depth : r;
+    r = r >= depth ? r - depth : r;
+    const T add = v[j * kRingChunk + k];
+    while (__any(pending)) {
+      if (pending) atomicMin(&claim[r], (uint32_t)ln);
+      if (pending && claim[r] == (uint32_t)ln) {
+        col[r * TT] = col[r * TT] + add;
+        claim[r] = 0xffffffffu;
+        pending = false;
       }
     }
-#pragma unroll
-    for (int u = 0; u < G; ++u) x[u] = col[r[u] * TT];
-#pragma unroll
-    for (int u = 0; u < G; ++u) {
-      T base = x[u];
-#pragma unroll
-      for (int q = 0; q < u; ++q) base = r[q] == r[u] ? x[q] : base;
-      x[u] = base + add[u];
-    }
-#pragma unroll
-    for (int u = 0; u < G; ++u) col[r[u] * TT] = x[u];
   }
 }
 
@@ -235,11 +222,10 @@ __global__ __launch_bounds__(kRingThreads) void interp_delay_tile_kernel(
     }
   }
   const int i0 = in_ptr[t0], i1 = in_ptr[t0 + nt];
-  int e0 = 0, e1 = 0;
-  if (tid < nt) {
-    e0 = in_ptr[t0 + tid];
-    e1 = in_ptr[t0 + tid + 1];
-  }
+  const int lane = tid & 63;
+  // each wavefront's claim words (one per ring row), free = 0xffffffff
+  uint32_t* claim = reinterpret_cast<uint32_t*>(tile + depth * TT) + (tid >> 6) * depth;
+  for (int r = lane; r < depth; r += 64) claim[r] = 0xffffffffu;
   const int passes = (TAPS == 2 && i1 - i0 > kRingChunk) ? 2 : 1;
   for (int pass = 0; pass < passes; ++pass) {
     for (int c0 = i0; c0 < i1; c0 += kRingChunk) {
@@ -298,17 +284,21 @@ __global__ __launch_bounds__(kRingThreads) void interp_delay_tile_kernel(
         }
       }
       __syncthreads();
-      if (tid < nt) {
-        const int a = (e0 > c0 ? e0 : c0) - c0;
-        const int b = (e1 < c0 + n ? e1 : c0 + n) - c0;
-        T* col = tile + tid;
+      // one wavefront per target (targets wave, wave + 4, ...), its lanes
+      // over the target's items
+      for (int ti = tid >> 6; ti < nt; ti += kRingThreads / 64) {
+        const int ea = in_ptr[t0 + ti], eb = in_ptr[t0 + ti + 1];
+        const int a = (ea > c0 ? ea : c0) - c0;
+        const int b = (eb < c0 + n ? eb : c0 + n) - c0;
+        if (a >= b) continue;
+        T* col = tile + ti;
         if (TAPS == 3) {
-          ring_apply<T, 3, 3, TT>(col, st.row, &st.v[0][0], a, b, 0, depth);
+          ring_apply_wave<T, 3, TT>(col, st.row, &st.v[0][0], a, b, 0, depth, claim, lane);
         } else if (TAPS == 2 && passes == 1) {
-          ring_apply<T, 1, 8, TT>(col, st.row, st.v[0], a, b, 0, depth);
-          ring_apply<T, 1, 8, TT>(col, st.row, st.v[TAPS - 1], a, b, 1, depth);
+          ring_apply_wave<T, 1, TT>(col, st.row, st.v[0], a, b, 0, depth, claim, lane);
+          ring_apply_wave<T, 1, TT>(col, st.row, st.v[TAPS - 1], a, b, 1, depth, claim, lane);
         } else {
-          ring_apply<T, 1, 8, TT>(col, st.row, st.v[0], a, b, pass, depth);
+          ring_apply_wave<T, 1, TT>(col, st.row, st.v[0], a, b, pass, depth, claim, lane);
         }
       }
       __syncthreads();
@@ -413,7 +403,8 @@ static int interp_enqueue_impl(const T* qin, T* q, int64_t depth, int64_t width,
   const size_t stage = taps == 1   ? sizeof(dn::RingStage<T, 1>)
                        : taps == 2 ? sizeof(dn::RingStage<T, 2>)
                                    : sizeof(dn::RingStage<T, 3>);
-  const size_t lds = stage + (size_t)depth * dn::kRingTileTargets * sizeof(T);
+  const size_t lds = stage + (size_t)depth * dn::kRingTileTargets * sizeof(T) +
+                     (size_t)depth * 4 * (dn::kRingThreads / 64);
   if (lds <= kRingLdsMax) {
     const T* src = qin ? qin : q;
     const void* rec = (work && n_edge > 0) ? work : nullptr;
