@@ -364,4 +364,16 @@ __device__ __forceinline__ T pcr_lanes(T A, T B, T C, T D) {
   return D * frcp(B);
 }
 
+// pos[perm[p]] = p for p < n: the inverse of a permutation of [0, n), such
+// as each connection's position in a target-major CSR (perm = in_edge).
+// Entries of perm outside [0, n) are skipped.
+template <int = 0>
+__global__ __launch_bounds__(256) void inverse_permutation_kernel(
+    int32_t* __restrict__ pos, const int32_t* __restrict__ perm, int64_t n) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int32_t e = perm[p];
+  if ((uint32_t)e < (uint64_t)n) pos[e] = (int32_t)p;
+}
+
 }  // namespace dn

@@ -62,9 +62,10 @@ constexpr int kProjChunk = kProjThreads * kProjPer;
 // One connection's history reads packed for one step: its column, the rows
 // (slot - off_j) mod depth of its taps, the tap coefficients (two taps:
 // 1 - a and a, rounded as the reference rounds them; three taps: f_j), w,
-// and for a packed history its bit mask.  Written in connection order
-// (coalesced), so the target-major pass gathers one record and the taps'
-// history words per connection instead of five scattered arrays.
+// and for a packed history its bit mask.  Read in connection order
+// (coalesced) and written at the connection's position in the target-major
+// CSR, so the target-major pass reads the records in order and gathers only
+// the taps' history words, instead of five scattered arrays.
 template <typename T, int TAPS>
 struct alignas(16) ProjRec {
   int32_t col;
@@ -88,10 +89,11 @@ __global__ __launch_bounds__(256) void history_pack_rec_kernel(
     ProjRecT<T, PACKED, TAPS>* __restrict__ rec, int64_t n_conn, int64_t depth,
     const int64_t* __restrict__ slot_p, const int32_t* __restrict__ col,
     const int64_t* __restrict__ mask, const int32_t* __restrict__ off,
-    const T* __restrict__ w, const T* __restrict__ frac) {
+    const T* __restrict__ w, const T* __restrict__ frac, const int32_t* __restrict__ pos) {
 #pragma clang fp contract(off)
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_conn) return;
+  const int32_t d = pos[e];
   const int slot = hist_slot(slot_p, depth);
   ProjRecT<T, PACKED, TAPS> r;
   if constexpr (PACKED) r.mask = mask[e];
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(256) void history_pack_rec_kernel(
     for (int j = 0; j < TAPS; ++j) r.c[j] = frac[e * 3 + j];
   }
   r.w = w[e];
-  rec[e] = r;
+  if ((uint32_t)d < (uint64_t)n_conn) rec[d] = r;
 }
 
 // v = w s with s = g0 (one tap), g0 c0 + g1 c1 (two), (g0 c0 + g1 c1) + g2 c2
@@ -162,7 +164,8 @@ __global__ __launch_bounds__(kProjThreads) void history_project_kernel(
 #pragma unroll
     for (int u = 0; u < kProjPer; ++u) {
       const int k = tid + u * kProjThreads;
-      e[u] = (int64_t)in_edge[c0 + (k < n ? k : n - 1)];
+      e[u] = c0 + (k < n ? k : n - 1);
+      if (!REC) e[u] = in_edge[e[u]];
     }
     int32_t cu[kProjPer], ru[kProjPer][TAPS];
     T cf[kProjPer][TAPS], wu[kProjPer];
@@ -315,14 +318,16 @@ static void project_launch_taps(const void* hist, int64_t depth, int64_t n_col,
                                 const int64_t* slot, const int32_t* in_ptr,
                                 const int32_t* in_edge, const int32_t* col, const int64_t* mask,
                                 const int32_t* off, const T* w, const T* frac, T* out,
-                                int64_t width, int64_t n_conn, void* work, hipStream_t stream) {
+                                int64_t width, int64_t n_conn, void* work,
+                                const int32_t* pos, hipStream_t stream) {
   const dim3 g((unsigned)((width + dn::kProjTargets - 1) / dn::kProjTargets)),
       b(dn::kProjThreads);
   using Rec = dn::ProjRecT<T, PACKED, TAPS>;
   Rec* rec = static_cast<Rec*>(work);
   if (rec) {
     hipLaunchKernelGGL((dn::history_pack_rec_kernel<T, PACKED, TAPS>), grid_for(n_conn),
-                       dim3(256), 0, stream, rec, n_conn, depth, slot, col, mask, off, w, frac);
+                       dim3(256), 0, stream, rec, n_conn, depth, slot, col, mask, off, w, frac,
+                       pos);
     hipLaunchKernelGGL((dn::history_project_kernel<T, PACKED, TAPS, true>), g, b, 0, stream,
                        hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off, w, frac, rec,
                        out, width);
@@ -338,16 +343,16 @@ static void project_launch(int taps, const void* hist, int64_t depth, int64_t n_
                            const int64_t* slot, const int32_t* in_ptr, const int32_t* in_edge,
                            const int32_t* col, const int64_t* mask, const int32_t* off,
                            const T* w, const T* frac, T* out, int64_t width, int64_t n_conn,
-                           void* work, hipStream_t stream) {
+                           void* work, const int32_t* pos, hipStream_t stream) {
   if (taps == 1)
     project_launch_taps<T, PACKED, 1>(hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                                      w, frac, out, width, n_conn, work, stream);
+                                      w, frac, out, width, n_conn, work, pos, stream);
   else if (taps == 2)
     project_launch_taps<T, PACKED, 2>(hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                                      w, frac, out, width, n_conn, work, stream);
+                                      w, frac, out, width, n_conn, work, pos, stream);
   else
     project_launch_taps<T, PACKED, 3>(hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                                      w, frac, out, width, n_conn, work, stream);
+                                      w, frac, out, width, n_conn, work, pos, stream);
 }
 
 static int64_t project_rec_bytes(int taps, int packed, int f64) {
@@ -364,30 +369,40 @@ static int64_t project_rec_bytes(int taps, int packed, int f64) {
   return f64 ? sz(double(0)) : sz(float(0));
 }
 
-// work non-null: n_conn records are packed first (connection order) and the
-// target-major pass gathers one record per connection.
+// work non-null: n_conn records are packed first, each at its connection's
+// position in the target-major CSR (work holds the records, then n_conn int32
+// positions, computed there when pos is null), and the target-major pass
+// reads them in order.  in_edge must then be a permutation of [0, n_conn).
 template <typename T>
 static int project_impl(const void* hist, int packed, int64_t depth, int64_t n_col,
                         const int64_t* slot, int taps, const int32_t* in_ptr,
-                        const int32_t* in_edge, const int32_t* col, const int64_t* mask,
-                        const int32_t* off, const T* w, const T* frac, T* out, int64_t width,
-                        int64_t n_conn, void* work, int64_t work_bytes, hipStream_t stream) {
+                        const int32_t* in_edge, const int32_t* pos, const int32_t* col,
+                        const int64_t* mask, const int32_t* off, const T* w, const T* frac,
+                        T* out, int64_t width, int64_t n_conn, void* work, int64_t work_bytes,
+                        hipStream_t stream) {
   if (!slot || !in_ptr || !out || depth < 1 || n_col < 0 || width < 0) return DN_EINVAL;
   if (taps < 1 || taps > 3 || width > (int64_t)INT32_MAX * 256) return DN_EINVAL;
   if (depth > INT32_MAX) return DN_EINVAL;
-  if (work && (n_conn < 0 || n_conn > (int64_t)INT32_MAX * 256 ||
-               work_bytes < n_conn * project_rec_bytes(taps, packed, sizeof(T) == 8)))
+  if (work && (n_conn < 0 || n_conn > (int64_t)INT32_MAX ||
+               work_bytes < n_conn * (project_rec_bytes(taps, packed, sizeof(T) == 8) + 4)))
     return DN_EINVAL;
   if (width == 0) return DN_OK;
   if (!hist || !in_edge || !col || !off || !w || (taps > 1 && !frac) || (packed && !mask))
     return DN_EINVAL;
   void* rec = (work && n_conn > 0) ? work : nullptr;
+  if (rec && !pos) {
+    int32_t* own = reinterpret_cast<int32_t*>(
+        static_cast<char*>(work) + n_conn * project_rec_bytes(taps, packed, sizeof(T) == 8));
+    hipLaunchKernelGGL(dn::inverse_permutation_kernel<>, grid_for(n_conn), dim3(256), 0,
+                       stream, own, in_edge, n_conn);
+    pos = own;
+  }
   if (packed)
     project_launch<T, true>(taps, hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                            w, frac, out, width, n_conn, rec, stream);
+                            w, frac, out, width, n_conn, rec, pos, stream);
   else
     project_launch<T, false>(taps, hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                             w, frac, out, width, n_conn, rec, stream);
+                             w, frac, out, width, n_conn, rec, pos, stream);
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
 
@@ -448,17 +463,18 @@ static int record_impl(T* hist, int64_t depth, int64_t n_col, const int64_t* slo
       int taps, const int32_t* in_ptr, const int32_t* in_edge, const int32_t* col,           \
       const int64_t* mask, const int32_t* off, const T* w, const T* frac, T* out,            \
       int64_t width, hipStream_t stream) {                                                   \
-    return project_impl<T>(hist, packed, depth, n_col, slot, taps, in_ptr, in_edge, col,     \
-                           mask, off, w, frac, out, width, -1, nullptr, 0, stream);          \
+    return project_impl<T>(hist, packed, depth, n_col, slot, taps, in_ptr, in_edge, nullptr, \
+                           col, mask, off, w, frac, out, width, -1, nullptr, 0, stream);     \
   }                                                                                          \
   DN_EXPORT int dn_history_project_ws_##SFX(                                                 \
       const void* hist, int packed, int64_t depth, int64_t n_col, const int64_t* slot,       \
-      int taps, const int32_t* in_ptr, const int32_t* in_edge, const int32_t* col,           \
-      const int64_t* mask, const int32_t* off, const T* w, const T* frac, T* out,            \
-      int64_t width, int64_t n_conn, void* work, int64_t work_bytes, hipStream_t stream) {   \
+      int taps, const int32_t* in_ptr, const int32_t* in_edge, const int32_t* pos,           \
+      const int32_t* col, const int64_t* mask, const int32_t* off, const T* w,               \
+      const T* frac, T* out, int64_t width, int64_t n_conn, void* work, int64_t work_bytes,  \
+      hipStream_t stream) {                                                                  \
     if (!work) return DN_EINVAL;                                                             \
-    return project_impl<T>(hist, packed, depth, n_col, slot, taps, in_ptr, in_edge, col,     \
-                           mask, off, w, frac, out, width, n_conn, work, work_bytes,         \
+    return project_impl<T>(hist, packed, depth, n_col, slot, taps, in_ptr, in_edge, pos,     \
+                           col, mask, off, w, frac, out, width, n_conn, work, work_bytes,    \
                            stream);                                                          \
   }                                                                                          \
   DN_EXPORT int dn_history_project_vjp_##SFX(                                                \
@@ -477,7 +493,7 @@ static int record_impl(T* hist, int64_t depth, int64_t n_col, const int64_t* slo
 
 DN_EXPORT int64_t dn_history_work_bytes(int64_t n_conn, int taps, int packed, int f64) {
   if (n_conn < 0 || taps < 1 || taps > 3) return -1;
-  return n_conn * project_rec_bytes(taps, packed != 0, f64 != 0);
+  return n_conn * (project_rec_bytes(taps, packed != 0, f64 != 0) + 4);
 }
 
 DN_HISTORY_EXPORTS(float, f32)

@@ -129,14 +129,17 @@ __device__ __forceinline__ void ring_tap_values(T wv, T gv, const T* fv, T* v) {
   }
 }
 
-// Records in connection order (coalesced reads and writes).
+// Records read in connection order (coalesced) and written at the
+// connection's CSR position pos[e], so the tile pass reads them in order.
 template <typename T, int TAPS>
 __global__ __launch_bounds__(256) void interp_delay_pack_kernel(
     RingRec<T, TAPS>* __restrict__ rec, int64_t n_edge, int depth,
     const int64_t* __restrict__ slot_p, const T* __restrict__ w, const T* __restrict__ g,
-    const T* __restrict__ f, const int32_t* __restrict__ base) {
+    const T* __restrict__ f, const int32_t* __restrict__ base,
+    const int32_t* __restrict__ pos) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_edge) return;
+  const int32_t d = pos[e];
   const int slot = ring_slot(slot_p, depth);
   T fv[3] = {T(0), T(0), T(0)};
   if (TAPS == 2) fv[0] = f[e];
@@ -146,7 +149,7 @@ __global__ __launch_bounds__(256) void interp_delay_pack_kernel(
   RingRec<T, TAPS> r;
   ring_tap_values<T, TAPS>(w[e], g[e], fv, r.v);
   r.row = (int32_t)ring_row(slot, base[e], 0, depth);
-  rec[e] = r;
+  if ((uint32_t)d < (uint64_t)n_edge) rec[d] = r;
 }
 
 // One wavefront adds one target's staged tap values to its LDS column in
@@ -237,7 +240,8 @@ __global__ __launch_bounds__(kRingThreads) void interp_delay_tile_kernel(
 #pragma unroll
         for (int u = 0; u < kRingPer; ++u) {
           const int k = b0 + tid + u * kRingThreads;
-          e[u] = in_edge[c0 + (k < n ? k : n - 1)];
+          e[u] = c0 + (k < n ? k : n - 1);
+          if (!REC) e[u] = in_edge[e[u]];
         }
         T val[kRingPer][TAPS];
         int32_t row[kRingPer];
@@ -374,11 +378,11 @@ static void ring_tiles_launch(const T* src, T* q, int64_t depth, int64_t width,
 template <typename T, int TAPS>
 static void ring_pack_launch(void* rec, int64_t n_edge, int64_t depth, const int64_t* slot,
                              const T* w, const T* g, const T* f, const int32_t* base,
-                             hipStream_t stream) {
+                             const int32_t* pos, hipStream_t stream) {
   hipLaunchKernelGGL((dn::interp_delay_pack_kernel<T, TAPS>),
                      dim3((unsigned)((n_edge + 255) / 256)), dim3(256), 0, stream,
                      static_cast<dn::RingRec<T, TAPS>*>(rec), n_edge, (int)depth, slot, w, g, f,
-                     base);
+                     base, pos);
 }
 
 template <typename T>
@@ -387,18 +391,21 @@ static size_t ring_rec_bytes(int taps) {
                    : taps == 2 ? sizeof(dn::RingRec<T, 2>) : sizeof(dn::RingRec<T, 3>);
 }
 
-// n_edge >= 0 with work: the records are packed first (work holds n_edge of
-// them) and the tile kernel gathers one record per connection.
+// n_edge >= 0 with work: the records are packed first, each at its
+// connection's position in the target-major CSR (work holds n_edge records,
+// then n_edge int32 positions, computed there when pos is null), and the
+// tile kernel reads them in order.  in_edge must be a permutation of
+// [0, n_edge) (in_ptr[width] == n_edge).
 template <typename T>
 static int interp_enqueue_impl(const T* qin, T* q, int64_t depth, int64_t width,
                                const int64_t* slot, int taps, const int32_t* in_ptr,
-                               const int32_t* in_edge, const T* w, const T* g, const T* f,
-                               const int32_t* base, int64_t n_edge, void* work,
-                               int64_t work_bytes, hipStream_t stream) {
+                               const int32_t* in_edge, const int32_t* pos, const T* w,
+                               const T* g, const T* f, const int32_t* base, int64_t n_edge,
+                               void* work, int64_t work_bytes, hipStream_t stream) {
   if (!q || !slot || !in_ptr || depth < 1 || depth > (1 << 30) || width < 1) return DN_EINVAL;
   if (taps < 1 || taps > 3 || width > (int64_t)INT32_MAX * 256) return DN_EINVAL;
   if (work && (n_edge < 0 || n_edge > (int64_t)INT32_MAX ||
-               work_bytes < n_edge * (int64_t)ring_rec_bytes<T>(taps)))
+               work_bytes < n_edge * ((int64_t)ring_rec_bytes<T>(taps) + 4)))
     return DN_EINVAL;
   const size_t stage = taps == 1   ? sizeof(dn::RingStage<T, 1>)
                        : taps == 2 ? sizeof(dn::RingStage<T, 2>)
@@ -409,9 +416,16 @@ static int interp_enqueue_impl(const T* qin, T* q, int64_t depth, int64_t width,
     const T* src = qin ? qin : q;
     const void* rec = (work && n_edge > 0) ? work : nullptr;
     if (rec) {
-      if (taps == 1) ring_pack_launch<T, 1>(work, n_edge, depth, slot, w, g, f, base, stream);
-      else if (taps == 2) ring_pack_launch<T, 2>(work, n_edge, depth, slot, w, g, f, base, stream);
-      else ring_pack_launch<T, 3>(work, n_edge, depth, slot, w, g, f, base, stream);
+      if (!pos) {
+        int32_t* own = reinterpret_cast<int32_t*>(static_cast<char*>(work) +
+                                                  n_edge * (int64_t)ring_rec_bytes<T>(taps));
+        hipLaunchKernelGGL(dn::inverse_permutation_kernel<>, dim3((unsigned)((n_edge + 255) / 256)),
+                           dim3(256), 0, stream, own, in_edge, n_edge);
+        pos = own;
+      }
+      if (taps == 1) ring_pack_launch<T, 1>(work, n_edge, depth, slot, w, g, f, base, pos, stream);
+      else if (taps == 2) ring_pack_launch<T, 2>(work, n_edge, depth, slot, w, g, f, base, pos, stream);
+      else ring_pack_launch<T, 3>(work, n_edge, depth, slot, w, g, f, base, pos, stream);
     }
     if (taps == 1)
       ring_tiles_launch<T, 1>(src, q, depth, width, slot, in_ptr, in_edge, w, g, f, base, rec,
@@ -471,40 +485,40 @@ DN_EXPORT int dn_interp_delay_enqueue_f32(const float* qin, float* q, int64_t de
                                           const int32_t* in_ptr, const int32_t* in_edge,
                                           const float* w, const float* g, const float* f,
                                           const int32_t* base, hipStream_t stream) {
-  return interp_enqueue_impl<float>(qin, q, depth, width, slot, taps, in_ptr, in_edge, w, g,
-                                    f, base, -1, nullptr, 0, stream);
+  return interp_enqueue_impl<float>(qin, q, depth, width, slot, taps, in_ptr, in_edge, nullptr,
+                                    w, g, f, base, -1, nullptr, 0, stream);
 }
 DN_EXPORT int dn_interp_delay_enqueue_f64(const double* qin, double* q, int64_t depth, int64_t width,
                                           const int64_t* slot, int taps,
                                           const int32_t* in_ptr, const int32_t* in_edge,
                                           const double* w, const double* g, const double* f,
                                           const int32_t* base, hipStream_t stream) {
-  return interp_enqueue_impl<double>(qin, q, depth, width, slot, taps, in_ptr, in_edge, w,
-                                     g, f, base, -1, nullptr, 0, stream);
+  return interp_enqueue_impl<double>(qin, q, depth, width, slot, taps, in_ptr, in_edge,
+                                     nullptr, w, g, f, base, -1, nullptr, 0, stream);
 }
 DN_EXPORT int64_t dn_interp_delay_work_bytes(int64_t n_edge, int taps, int f64) {
   if (n_edge < 0 || taps < 1 || taps > 3) return -1;
-  return n_edge * (int64_t)(f64 ? ring_rec_bytes<double>(taps) : ring_rec_bytes<float>(taps));
+  return n_edge * (int64_t)((f64 ? ring_rec_bytes<double>(taps) : ring_rec_bytes<float>(taps)) + 4);
 }
 DN_EXPORT int dn_interp_delay_enqueue_ws_f32(const float* qin, float* q, int64_t depth,
                                              int64_t width, const int64_t* slot, int taps,
                                              const int32_t* in_ptr, const int32_t* in_edge,
-                                             const float* w, const float* g, const float* f,
-                                             const int32_t* base, int64_t n_edge, void* work,
-                                             int64_t work_bytes, hipStream_t stream) {
+                                             const int32_t* pos, const float* w, const float* g,
+                                             const float* f, const int32_t* base, int64_t n_edge,
+                                             void* work, int64_t work_bytes, hipStream_t stream) {
   if (!work) return DN_EINVAL;
-  return interp_enqueue_impl<float>(qin, q, depth, width, slot, taps, in_ptr, in_edge, w, g,
-                                    f, base, n_edge, work, work_bytes, stream);
+  return interp_enqueue_impl<float>(qin, q, depth, width, slot, taps, in_ptr, in_edge, pos, w,
+                                    g, f, base, n_edge, work, work_bytes, stream);
 }
 DN_EXPORT int dn_interp_delay_enqueue_ws_f64(const double* qin, double* q, int64_t depth,
                                              int64_t width, const int64_t* slot, int taps,
                                              const int32_t* in_ptr, const int32_t* in_edge,
-                                             const double* w, const double* g, const double* f,
-                                             const int32_t* base, int64_t n_edge, void* work,
-                                             int64_t work_bytes, hipStream_t stream) {
+                                             const int32_t* pos, const double* w, const double* g,
+                                             const double* f, const int32_t* base, int64_t n_edge,
+                                             void* work, int64_t work_bytes, hipStream_t stream) {
   if (!work) return DN_EINVAL;
-  return interp_enqueue_impl<double>(qin, q, depth, width, slot, taps, in_ptr, in_edge, w,
-                                     g, f, base, n_edge, work, work_bytes, stream);
+  return interp_enqueue_impl<double>(qin, q, depth, width, slot, taps, in_ptr, in_edge, pos,
+                                     w, g, f, base, n_edge, work, work_bytes, stream);
 }
 DN_EXPORT int dn_interp_delay_vjp_f32(const float* G, int64_t depth, int64_t width,
                                       const int64_t* slot, int taps, int64_t n_edge,

@@ -114,6 +114,18 @@ def derived_index(t: torch.Tensor, key, make):
     return out
 
 
+def csr_positions(in_edge: torch.Tensor) -> torch.Tensor:
+    """pos[in_edge[p]] = p (int32): each connection's position in the
+    target-major CSR, cached with the plan; the record-packing operators
+    write each connection's record there so that their target-major passes
+    read the records in order."""
+    def make(t):
+        pos = torch.empty_like(t)
+        pos[t.long()] = torch.arange(t.numel(), dtype=torch.int32, device=t.device)
+        return pos
+    return derived_index(in_edge, "csr positions", make)
+
+
 def _fn(name, like):
     return getattr(_lib.load(), f"{name}_{'f64' if like.dtype == torch.float64 else 'f32'}")
 
@@ -129,9 +141,10 @@ class _InterpEnqueue(torch.autograd.Function):
         p, lib, n = _lib.ptr, _lib.load(), w.numel()
         nbytes = lib.dn_interp_delay_work_bytes(n, taps, int(out.dtype == torch.float64))
         work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=out.device)
+        pos = csr_positions(in_edge) if n else in_edge
         st = _fn("dn_interp_delay_enqueue_ws", out)(p(qin), p(out), depth, width, p(slot), taps,
-                                                    p(in_ptr), p(in_edge), p(w), p(g), p(f),
-                                                    p(base), n, p(work), nbytes,
+                                                    p(in_ptr), p(in_edge), p(pos), p(w), p(g),
+                                                    p(f), p(base), n, p(work), nbytes,
                                                     _lib.stream_handle(out.device))
         _lib.check(st, "interpolated-delay enqueue")
         ctx.save_for_backward(slot, w, g, f, base, post)

@@ -35,7 +35,7 @@ import torch
 from torch.utils.weak import WeakIdKeyDictionary
 
 from . import _lib
-from .delays import delay_interpolation, delay_plan, derived_index
+from .delays import csr_positions, delay_interpolation, delay_plan, derived_index
 
 _FLOAT = (torch.float32, torch.float64)
 _checked = WeakIdKeyDictionary()
@@ -161,8 +161,9 @@ class _Project(torch.autograd.Function):
         nbytes = lib.dn_history_work_bytes(n, taps, int(packed), f64)
         work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=weights.device)
         fn = getattr(lib, f"dn_history_project_ws_{_suffix(weights.dtype)}")
+        pos = csr_positions(in_edge) if n else in_edge
         st = fn(p(history), int(packed), depth, n_col, p(slot), taps, p(in_ptr), p(in_edge),
-                p(col), p(mask) if packed else None, p(off), p(weights),
+                p(pos), p(col), p(mask) if packed else None, p(off), p(weights),
                 p(frac) if taps > 1 else None, p(out), width, n, p(work), nbytes,
                 _lib.stream_handle(weights.device))
         _lib.check(st, "source-history projection")

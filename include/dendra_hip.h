@@ -207,10 +207,14 @@ int dn_bt3_tree_contract_f64(const double* main, const double* edge, const doubl
  * cotangents dw, dg, df for a ring cotangent G.
  * dn_interp_delay_enqueue_ws_*: the same transition with a workspace of
  * dn_interp_delay_work_bytes(n_edge, taps, f64) bytes (n_edge = number of
- * connections): each connection's tap values and first row are packed into
- * one record (connection order, coalesced) before the target-major pass, so
- * that pass gathers one record per connection instead of w, g, f and base
- * separately; results are identical bit for bit. */
+ * connections; in_edge must then be a permutation of [0, n_edge), i.e.
+ * in_ptr[width] == n_edge): each connection's tap values and first row are
+ * packed into one record, read in connection order and written at the
+ * connection's CSR position pos[e] (pos[in_edge[p]] = p; pass it once
+ * computed, e.g. cached with the CSR, or NULL to have it computed in the
+ * workspace each call), so the target-major pass reads the records in order
+ * instead of gathering w, g, f and base; results are identical bit for
+ * bit. */
 int dn_interp_delay_enqueue_f32(const float* qin, float* q, int64_t depth, int64_t width,
                                 const int64_t* slot, int taps, const int32_t* in_ptr,
                                 const int32_t* in_edge, const float* w, const float* g,
@@ -222,12 +226,14 @@ int dn_interp_delay_enqueue_f64(const double* qin, double* q, int64_t depth, int
 int64_t dn_interp_delay_work_bytes(int64_t n_edge, int taps, int f64);
 int dn_interp_delay_enqueue_ws_f32(const float* qin, float* q, int64_t depth, int64_t width,
                                    const int64_t* slot, int taps, const int32_t* in_ptr,
-                                   const int32_t* in_edge, const float* w, const float* g,
+                                   const int32_t* in_edge, const int32_t* pos, const float* w,
+                                   const float* g,
                                    const float* f, const int32_t* base, int64_t n_edge,
                                    void* work, int64_t work_bytes, hipStream_t stream);
 int dn_interp_delay_enqueue_ws_f64(const double* qin, double* q, int64_t depth, int64_t width,
                                    const int64_t* slot, int taps, const int32_t* in_ptr,
-                                   const int32_t* in_edge, const double* w, const double* g,
+                                   const int32_t* in_edge, const int32_t* pos, const double* w,
+                                   const double* g,
                                    const double* f, const int32_t* base, int64_t n_edge,
                                    void* work, int64_t work_bytes, hipStream_t stream);
 int dn_interp_delay_vjp_f32(const float* G, int64_t depth, int64_t width,
@@ -254,9 +260,12 @@ int dn_interp_delay_vjp_f64(const double* G, int64_t depth, int64_t width,
  *     w[e] * mixture, accumulated from 0 in ascending connection order.
  *   dn_history_project_ws_*: the same projection with a workspace of
  *     dn_history_work_bytes(n_conn, taps, packed, f64) bytes (n_conn = number
- *     of connections): each connection's column, tap rows, tap weights, w
- *     (and mask) are packed into one record (connection order, coalesced)
- *     before the target-major pass; results are identical bit for bit.
+ *     of connections; in_edge a permutation of [0, n_conn)): each
+ *     connection's column, tap rows, tap weights, w (and mask) are packed
+ *     into one record written at the connection's CSR position pos[e]
+ *     (pos[in_edge[p]] = p, passed in or NULL to compute it in the
+ *     workspace), so the target-major pass reads the records in order;
+ *     results are identical bit for bit.
  *   dn_history_project_vjp_*: per connection dw, dfrac (taps 2, 3) and, for
  *     a floating history, dhist += (atomic; zero it first, may be null).
  *   dn_history_record_*: row slot mod depth := gates (in place).
@@ -276,13 +285,15 @@ int dn_history_project_f64(const void* hist, int packed, int64_t depth, int64_t 
 int64_t dn_history_work_bytes(int64_t n_conn, int taps, int packed, int f64);
 int dn_history_project_ws_f32(const void* hist, int packed, int64_t depth, int64_t n_col,
                               const int64_t* slot, int taps, const int32_t* in_ptr,
-                              const int32_t* in_edge, const int32_t* col, const int64_t* mask,
+                              const int32_t* in_edge, const int32_t* pos, const int32_t* col,
+                              const int64_t* mask,
                               const int32_t* off, const float* w, const float* frac,
                               float* out, int64_t width, int64_t n_conn, void* work,
                               int64_t work_bytes, hipStream_t stream);
 int dn_history_project_ws_f64(const void* hist, int packed, int64_t depth, int64_t n_col,
                               const int64_t* slot, int taps, const int32_t* in_ptr,
-                              const int32_t* in_edge, const int32_t* col, const int64_t* mask,
+                              const int32_t* in_edge, const int32_t* pos, const int32_t* col,
+                              const int64_t* mask,
                               const int32_t* off, const double* w, const double* frac,
                               double* out, int64_t width, int64_t n_conn, void* work,
                               int64_t work_bytes, hipStream_t stream);

@@ -87,11 +87,12 @@ def test_projection_is_bitwise_the_oracle(dtype, taps, packed, shape):
 @pytest.mark.parametrize("taps", [1, 2, 3])
 @pytest.mark.parametrize("packed", [False, True])
 def test_workspace_and_direct_entries_agree_with_the_oracle(dtype, taps, packed):
-    """dn_history_project_ws_* (records packed first) and
+    """dn_history_project_ws_* (records packed at their CSR positions, the
+    positions passed in or computed in the workspace) and
     dn_history_project_* (five gathers per connection) give the oracle's
     payload bit for bit; one target holds half of the 12,000 connections."""
     from dendra_amd import _lib
-    from dendra_amd.delays import delay_plan
+    from dendra_amd.delays import csr_positions, delay_plan
 
     rng = np.random.default_rng(50 + 2 * taps + packed)
     n, width, slot = 12000, 70, 123
@@ -121,12 +122,15 @@ def test_workspace_and_direct_entries_agree_with_the_oracle(dtype, taps, packed)
     assert getattr(lib, f"dn_history_project_{sfx}")(*args, p(direct), width, stream) == 0
     nbytes = lib.dn_history_work_bytes(n, taps, int(packed), int(dtype == np.float64))
     work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
-    packed_out = torch.empty_like(direct)
+    packed_out, own = torch.empty_like(direct), torch.empty_like(direct)
     fn = getattr(lib, f"dn_history_project_ws_{sfx}")
-    assert fn(*args, p(packed_out), width, n, p(work), nbytes, stream) == 0
-    assert fn(*args, p(packed_out), width, n, p(work), nbytes - 1, stream) == -1  # DN_EINVAL
+    ws = lambda pos: args[:8] + (pos,) + args[8:]
+    pos = p(csr_positions(in_edge))
+    assert fn(*ws(pos), p(packed_out), width, n, p(work), nbytes, stream) == 0
+    assert fn(*ws(None), p(own), width, n, p(work), nbytes, stream) == 0
+    assert fn(*ws(pos), p(packed_out), width, n, p(work), nbytes - 1, stream) == -1  # DN_EINVAL
     torch.cuda.synchronize()
-    for got in (direct, packed_out):
+    for got in (direct, packed_out, own):
         assert np.array_equal(got.cpu().numpy().view(np.uint8), want.view(np.uint8))
 
 

@@ -218,12 +218,13 @@ def test_long_fan_in_and_deep_rings_are_bitwise_the_oracle(dtype, taps, shape):
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("taps", [1, 2, 3])
 def test_workspace_and_direct_entries_agree_with_the_oracle(dtype, taps):
-    """dn_interp_delay_enqueue_ws_* (records packed first) and
+    """dn_interp_delay_enqueue_ws_* (records packed at their CSR positions,
+    the positions passed in or computed in the workspace) and
     dn_interp_delay_enqueue_* (four gathers per connection) give the oracle's
     ring bit for bit; one target holds half of the 12,000 connections, so its
     tile spans several staged chunks."""
     from dendra_amd import _lib
-    from dendra_amd.delays import delay_plan
+    from dendra_amd.delays import csr_positions, delay_plan
 
     rng = np.random.default_rng(40 + taps)
     depth, width, n, slot = 201, 70, 12000, 199
@@ -260,14 +261,18 @@ def test_workspace_and_direct_entries_agree_with_the_oracle(dtype, taps):
         p(tb), stream) == 0
     nbytes = lib.dn_interp_delay_work_bytes(n, taps, int(dtype == np.float64))
     work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
-    packed = torch.empty_like(src)
+    packed, own = torch.empty_like(src), torch.empty_like(src)
     fn = getattr(lib, f"dn_interp_delay_enqueue_ws_{sfx}")
-    assert fn(p(src), p(packed), depth, width, p(ts), taps, p(in_ptr), p(in_edge), p(tw), p(tg),
-              fp, p(tb), n, p(work), nbytes, stream) == 0
-    assert fn(p(src), p(packed), depth, width, p(ts), taps, p(in_ptr), p(in_edge), p(tw), p(tg),
-              fp, p(tb), n, p(work), nbytes - 1, stream) == -1  # too small: DN_EINVAL
+    pos = csr_positions(in_edge)
+    assert np.array_equal(pos.cpu().numpy()[in_edge.cpu().numpy()], np.arange(n))
+    assert fn(p(src), p(packed), depth, width, p(ts), taps, p(in_ptr), p(in_edge), p(pos), p(tw),
+              p(tg), fp, p(tb), n, p(work), nbytes, stream) == 0
+    assert fn(p(src), p(own), depth, width, p(ts), taps, p(in_ptr), p(in_edge), None, p(tw),
+              p(tg), fp, p(tb), n, p(work), nbytes, stream) == 0
+    assert fn(p(src), p(packed), depth, width, p(ts), taps, p(in_ptr), p(in_edge), p(pos), p(tw),
+              p(tg), fp, p(tb), n, p(work), nbytes - 1, stream) == -1  # too small: DN_EINVAL
     torch.cuda.synchronize()
-    for got in (direct, packed):
+    for got in (direct, packed, own):
         assert np.array_equal(got.cpu().numpy().view(np.uint8), want.view(np.uint8))
 
 
