@@ -141,7 +141,7 @@ for _sfx in ("f32", "f64"):
     _SIGNATURES[f"dn_bt3_tree_contract_{_sfx}"] = (
         i32, [p] * 5 + [i64, i64, p, i32] + [p] * 5 + [i32] + [p] * 8 + [p, i64, p])
 _SIGNATURES["dn_interp_delay_work_bytes"] = (i64, [i64, i32, i32])
-_SIGNATURES["dn_history_work_bytes"] = (i64, [i64, i32, i32, i32])
+_SIGNATURES["dn_history_work_bytes"] = (i64, [i64, i32, i32, i32, i64, i64])
 for _sfx in ("f32", "f64"):
     _SIGNATURES[f"dn_interp_delay_enqueue_{_sfx}"] = (i32, [p, p, i64, i64, p, i32] + [p] * 6 + [p])
     _SIGNATURES[f"dn_interp_delay_enqueue_ws_{_sfx}"] = (

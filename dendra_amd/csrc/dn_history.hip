@@ -135,9 +135,32 @@ __device__ __forceinline__ T proj_value(const T* gv, const T* c, T w) {
   return w * s;
 }
 
+// Column-major copy of a floating history: ht[c * depth + r] = hist[r * n_col + c],
+// so the 2-3 adjacent tap rows a connection reads share one cache line
+// (64 x 64 tiles through LDS, both sides coalesced).
+template <typename T>
+__global__ __launch_bounds__(256) void history_transpose_kernel(const T* __restrict__ hist,
+                                                                T* __restrict__ ht,
+                                                                int64_t depth, int64_t n_col) {
+  __shared__ T tile[64][65];
+  const int64_t c0 = (int64_t)blockIdx.x * 64, r0 = (int64_t)blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int i = ty; i < 64; i += 4) {
+    const int64_t r = r0 + i, c = c0 + tx;
+    if (r < depth && c < n_col) tile[i][tx] = hist[r * n_col + c];
+  }
+  __syncthreads();
+  for (int i = ty; i < 64; i += 4) {
+    const int64_t c = c0 + i, r = r0 + tx;
+    if (r < depth && c < n_col) ht[c * depth + r] = tile[tx][i];
+  }
+}
+
+// hist element (row, col) at row * rs + col * cs: rs = n_col, cs = 1 for the
+// ring as given, rs = 1, cs = depth for its column-major copy.
 template <typename T, bool PACKED, int TAPS, bool REC>
 __global__ __launch_bounds__(kProjThreads) void history_project_kernel(
-    const void* __restrict__ hist, int64_t depth, int64_t n_col,
+    const void* __restrict__ hist, int64_t depth, int64_t rs, int64_t cs,
     const int64_t* __restrict__ slot_p, const int32_t* __restrict__ in_ptr,
     const int32_t* __restrict__ in_edge, const int32_t* __restrict__ col,
     const int64_t* __restrict__ mask, const int32_t* __restrict__ off,
@@ -213,10 +236,10 @@ __global__ __launch_bounds__(kProjThreads) void history_project_kernel(
 #pragma unroll
       for (int j = 0; j < TAPS; ++j) {
         if constexpr (PACKED) {
-          const int64_t word = static_cast<const int64_t*>(hist)[ru[u][j] * n_col + cu[u]];
+          const int64_t word = static_cast<const int64_t*>(hist)[ru[u][j] * rs + cu[u] * cs];
           gv[j] = (word & mu[u]) != 0 ? T(1) : T(0);
         } else {
-          gv[j] = static_cast<const T*>(hist)[ru[u][j] * n_col + cu[u]];
+          gv[j] = static_cast<const T*>(hist)[ru[u][j] * rs + cu[u] * cs];
         }
       }
       // past the chunk: connection n - 1's own value into its own slot
@@ -319,7 +342,7 @@ static void project_launch_taps(const void* hist, int64_t depth, int64_t n_col,
                                 const int32_t* in_edge, const int32_t* col, const int64_t* mask,
                                 const int32_t* off, const T* w, const T* frac, T* out,
                                 int64_t width, int64_t n_conn, void* work,
-                                const int32_t* pos, hipStream_t stream) {
+                                const int32_t* pos, const void* ht, hipStream_t stream) {
   const dim3 g((unsigned)((width + dn::kProjTargets - 1) / dn::kProjTargets)),
       b(dn::kProjThreads);
   using Rec = dn::ProjRecT<T, PACKED, TAPS>;
@@ -328,13 +351,16 @@ static void project_launch_taps(const void* hist, int64_t depth, int64_t n_col,
     hipLaunchKernelGGL((dn::history_pack_rec_kernel<T, PACKED, TAPS>), grid_for(n_conn),
                        dim3(256), 0, stream, rec, n_conn, depth, slot, col, mask, off, w, frac,
                        pos);
+    // the column-major copy when there is one
+    const void* src = ht ? ht : hist;
+    const int64_t rs = ht ? 1 : n_col, cs = ht ? depth : 1;
     hipLaunchKernelGGL((dn::history_project_kernel<T, PACKED, TAPS, true>), g, b, 0, stream,
-                       hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off, w, frac, rec,
+                       src, depth, rs, cs, slot, in_ptr, in_edge, col, mask, off, w, frac, rec,
                        out, width);
   } else {
     hipLaunchKernelGGL((dn::history_project_kernel<T, PACKED, TAPS, false>), g, b, 0, stream,
-                       hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off, w, frac, rec,
-                       out, width);
+                       hist, depth, n_col, (int64_t)1, slot, in_ptr, in_edge, col, mask, off, w,
+                       frac, rec, out, width);
   }
 }
 
@@ -343,16 +369,17 @@ static void project_launch(int taps, const void* hist, int64_t depth, int64_t n_
                            const int64_t* slot, const int32_t* in_ptr, const int32_t* in_edge,
                            const int32_t* col, const int64_t* mask, const int32_t* off,
                            const T* w, const T* frac, T* out, int64_t width, int64_t n_conn,
-                           void* work, const int32_t* pos, hipStream_t stream) {
+                           void* work, const int32_t* pos, const void* ht,
+                           hipStream_t stream) {
   if (taps == 1)
     project_launch_taps<T, PACKED, 1>(hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                                      w, frac, out, width, n_conn, work, pos, stream);
+                                      w, frac, out, width, n_conn, work, pos, ht, stream);
   else if (taps == 2)
     project_launch_taps<T, PACKED, 2>(hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                                      w, frac, out, width, n_conn, work, pos, stream);
+                                      w, frac, out, width, n_conn, work, pos, ht, stream);
   else
     project_launch_taps<T, PACKED, 3>(hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                                      w, frac, out, width, n_conn, work, pos, stream);
+                                      w, frac, out, width, n_conn, work, pos, ht, stream);
 }
 
 static int64_t project_rec_bytes(int taps, int packed, int f64) {
@@ -369,10 +396,21 @@ static int64_t project_rec_bytes(int taps, int packed, int f64) {
   return f64 ? sz(double(0)) : sz(float(0));
 }
 
+// Workspace layout: n_conn records, n_conn int32 CSR positions, then (from a
+// 256-byte boundary) the column-major copy of a floating history.
+static int64_t project_copy_offset(int64_t n_conn, int taps, int packed, int f64) {
+  return (n_conn * (project_rec_bytes(taps, packed, f64) + 4) + 255) / 256 * 256;
+}
+static bool project_copy_fits(int64_t depth, int64_t n_col) {
+  return (depth + 63) / 64 <= 65535 && (n_col + 63) / 64 <= (int64_t)INT32_MAX;
+}
+
 // work non-null: n_conn records are packed first, each at its connection's
 // position in the target-major CSR (work holds the records, then n_conn int32
 // positions, computed there when pos is null), and the target-major pass
 // reads them in order.  in_edge must then be a permutation of [0, n_conn).
+// A floating history is first copied column-major into the workspace when it
+// has room for it, so that each connection's tap rows share a cache line.
 template <typename T>
 static int project_impl(const void* hist, int packed, int64_t depth, int64_t n_col,
                         const int64_t* slot, int taps, const int32_t* in_ptr,
@@ -384,7 +422,7 @@ static int project_impl(const void* hist, int packed, int64_t depth, int64_t n_c
   if (taps < 1 || taps > 3 || width > (int64_t)INT32_MAX * 256) return DN_EINVAL;
   if (depth > INT32_MAX) return DN_EINVAL;
   if (work && (n_conn < 0 || n_conn > (int64_t)INT32_MAX ||
-               work_bytes < n_conn * (project_rec_bytes(taps, packed, sizeof(T) == 8) + 4)))
+               work_bytes < project_copy_offset(n_conn, taps, packed, sizeof(T) == 8)))
     return DN_EINVAL;
   if (width == 0) return DN_OK;
   if (!hist || !in_edge || !col || !off || !w || (taps > 1 && !frac) || (packed && !mask))
@@ -397,12 +435,23 @@ static int project_impl(const void* hist, int packed, int64_t depth, int64_t n_c
                        stream, own, in_edge, n_conn);
     pos = own;
   }
+  const void* ht = nullptr;
+  if (rec && !packed && project_copy_fits(depth, n_col)) {
+    const int64_t at = project_copy_offset(n_conn, taps, packed, sizeof(T) == 8);
+    if (work_bytes >= at + depth * n_col * (int64_t)sizeof(T) && n_col > 0) {
+      T* copy = reinterpret_cast<T*>(static_cast<char*>(work) + at);
+      hipLaunchKernelGGL(dn::history_transpose_kernel<T>,
+                         dim3((unsigned)((n_col + 63) / 64), (unsigned)((depth + 63) / 64)),
+                         dim3(256), 0, stream, static_cast<const T*>(hist), copy, depth, n_col);
+      ht = copy;
+    }
+  }
   if (packed)
     project_launch<T, true>(taps, hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                            w, frac, out, width, n_conn, rec, pos, stream);
+                            w, frac, out, width, n_conn, rec, pos, ht, stream);
   else
     project_launch<T, false>(taps, hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                             w, frac, out, width, n_conn, rec, pos, stream);
+                             w, frac, out, width, n_conn, rec, pos, ht, stream);
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
 
@@ -491,9 +540,12 @@ static int record_impl(T* hist, int64_t depth, int64_t n_col, const int64_t* slo
     return record_impl<T>(hist, depth, n_col, slot, gates, stream);                          \
   }
 
-DN_EXPORT int64_t dn_history_work_bytes(int64_t n_conn, int taps, int packed, int f64) {
-  if (n_conn < 0 || taps < 1 || taps > 3) return -1;
-  return n_conn * (project_rec_bytes(taps, packed != 0, f64 != 0) + 4);
+DN_EXPORT int64_t dn_history_work_bytes(int64_t n_conn, int taps, int packed, int f64,
+                                        int64_t depth, int64_t n_col) {
+  if (n_conn < 0 || taps < 1 || taps > 3 || depth < 1 || n_col < 0) return -1;
+  const int64_t at = project_copy_offset(n_conn, taps, packed != 0, f64 != 0);
+  if (packed || !project_copy_fits(depth, n_col)) return at;
+  return at + depth * n_col * (f64 ? 8 : 4);
 }
 
 DN_HISTORY_EXPORTS(float, f32)

@@ -158,7 +158,7 @@ class _Project(torch.autograd.Function):
         out = torch.empty(width, dtype=weights.dtype, device=weights.device)
         p, lib, n = _lib.ptr, _lib.load(), weights.numel()
         f64 = int(weights.dtype == torch.float64)
-        nbytes = lib.dn_history_work_bytes(n, taps, int(packed), f64)
+        nbytes = lib.dn_history_work_bytes(n, taps, int(packed), f64, depth, n_col)
         work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=weights.device)
         fn = getattr(lib, f"dn_history_project_ws_{_suffix(weights.dtype)}")
         pos = csr_positions(in_edge) if n else in_edge

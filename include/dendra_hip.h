@@ -259,13 +259,17 @@ int dn_interp_delay_vjp_f64(const double* G, int64_t depth, int64_t width,
  *     out[t] = sum over the target-major CSR (in_ptr, in_edge) of
  *     w[e] * mixture, accumulated from 0 in ascending connection order.
  *   dn_history_project_ws_*: the same projection with a workspace of
- *     dn_history_work_bytes(n_conn, taps, packed, f64) bytes (n_conn = number
- *     of connections; in_edge a permutation of [0, n_conn)): each
+ *     dn_history_work_bytes(n_conn, taps, packed, f64, depth, n_col) bytes
+ *     (n_conn = number of connections; in_edge a permutation of
+ *     [0, n_conn); n_col = 0 gives the size without the copy below): each
  *     connection's column, tap rows, tap weights, w (and mask) are packed
  *     into one record written at the connection's CSR position pos[e]
  *     (pos[in_edge[p]] = p, passed in or NULL to compute it in the
- *     workspace), so the target-major pass reads the records in order;
- *     results are identical bit for bit.
+ *     workspace), so the target-major pass reads the records in order; a
+ *     floating history is first copied column-major into the workspace
+ *     (when work_bytes leaves room for it, as the size returned does), so
+ *     a connection's adjacent tap rows share a cache line; results are
+ *     identical bit for bit.
  *   dn_history_project_vjp_*: per connection dw, dfrac (taps 2, 3) and, for
  *     a floating history, dhist += (atomic; zero it first, may be null).
  *   dn_history_record_*: row slot mod depth := gates (in place).
@@ -282,7 +286,8 @@ int dn_history_project_f64(const void* hist, int packed, int64_t depth, int64_t 
                            const int32_t* in_edge, const int32_t* col, const int64_t* mask,
                            const int32_t* off, const double* w, const double* frac,
                            double* out, int64_t width, hipStream_t stream);
-int64_t dn_history_work_bytes(int64_t n_conn, int taps, int packed, int f64);
+int64_t dn_history_work_bytes(int64_t n_conn, int taps, int packed, int f64, int64_t depth,
+                              int64_t n_col);
 int dn_history_project_ws_f32(const void* hist, int packed, int64_t depth, int64_t n_col,
                               const int64_t* slot, int taps, const int32_t* in_ptr,
                               const int32_t* in_edge, const int32_t* pos, const int32_t* col,

@@ -120,17 +120,23 @@ def test_workspace_and_direct_entries_agree_with_the_oracle(dtype, taps, packed)
             p(tm), p(toff), p(tw), p(tf))
     direct = torch.empty(width, dtype=tw.dtype, device="cuda")
     assert getattr(lib, f"dn_history_project_{sfx}")(*args, p(direct), width, stream) == 0
-    nbytes = lib.dn_history_work_bytes(n, taps, int(packed), int(dtype == np.float64))
+    nbytes = lib.dn_history_work_bytes(n, taps, int(packed), int(dtype == np.float64), depth,
+                                       n_col)
     work = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
     packed_out, own = torch.empty_like(direct), torch.empty_like(direct)
     fn = getattr(lib, f"dn_history_project_ws_{sfx}")
     ws = lambda pos: args[:8] + (pos,) + args[8:]
     pos = p(csr_positions(in_edge))
+    # without room for the column-major copy of a floating history
+    small = lib.dn_history_work_bytes(n, taps, int(packed), int(dtype == np.float64), depth, 0)
+    assert small == nbytes if packed else small < nbytes
+    no_copy = torch.empty_like(direct)
     assert fn(*ws(pos), p(packed_out), width, n, p(work), nbytes, stream) == 0
     assert fn(*ws(None), p(own), width, n, p(work), nbytes, stream) == 0
-    assert fn(*ws(pos), p(packed_out), width, n, p(work), nbytes - 1, stream) == -1  # DN_EINVAL
+    assert fn(*ws(pos), p(no_copy), width, n, p(work), small, stream) == 0
+    assert fn(*ws(pos), p(packed_out), width, n, p(work), small - 1, stream) == -1  # DN_EINVAL
     torch.cuda.synchronize()
-    for got in (direct, packed_out, own):
+    for got in (direct, packed_out, own, no_copy):
         assert np.array_equal(got.cpu().numpy().view(np.uint8), want.view(np.uint8))
 
 
