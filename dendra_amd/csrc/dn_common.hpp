@@ -364,6 +364,31 @@ __device__ __forceinline__ T pcr_lanes(T A, T B, T C, T D) {
   return D * frcp(B);
 }
 
+// Zeroes a record before its fields are set, so its padding is defined.
+template <typename R>
+__device__ __forceinline__ void zero_record(R& r) {
+  __builtin_memset(&r, 0, sizeof(R));
+}
+
+// Stores a record with every byte of its slot, padding included, in 16-byte
+// (or 8-byte) vector stores: a scattered store that leaves bytes of a cache
+// sector unwritten makes the memory side merge it with the old contents.
+template <typename R>
+__device__ __forceinline__ void store_whole(R* dst, const R& r) {
+  static_assert(sizeof(R) % 8 == 0, "records are whole 8-byte words");
+  if constexpr (sizeof(R) % 16 == 0) {
+    uint4 q[sizeof(R) / 16];
+    __builtin_memcpy(q, &r, sizeof(R));
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(R) / 16; ++i) reinterpret_cast<uint4*>(dst)[i] = q[i];
+  } else {
+    uint2 q[sizeof(R) / 8];
+    __builtin_memcpy(q, &r, sizeof(R));
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(R) / 8; ++i) reinterpret_cast<uint2*>(dst)[i] = q[i];
+  }
+}
+
 // pos[perm[p]] = p for p < n: the inverse of a permutation of [0, n), such
 // as each connection's position in a target-major CSR (perm = in_edge).
 // Entries of perm outside [0, n) are skipped.

@@ -147,9 +147,10 @@ __global__ __launch_bounds__(256) void interp_delay_pack_kernel(
 #pragma unroll
     for (int j = 0; j < 3; ++j) fv[j] = f[e * 3 + j];
   RingRec<T, TAPS> r;
+  zero_record(r);
   ring_tap_values<T, TAPS>(w[e], g[e], fv, r.v);
   r.row = (int32_t)ring_row(slot, base[e], 0, depth);
-  if ((uint32_t)d < (uint64_t)n_edge) rec[d] = r;
+  if ((uint32_t)d < (uint64_t)n_edge) store_whole(rec + d, r);
 }
 
 // One wavefront adds one target's staged tap values to its LDS column in
