@@ -151,7 +151,7 @@ for _sfx in ("f32", "f64"):
     _SIGNATURES[f"dn_history_project_{_sfx}"] = (
         i32, [p, i32, i64, i64, p, i32] + [p] * 8 + [i64, p])
     _SIGNATURES[f"dn_history_project_ws_{_sfx}"] = (
-        i32, [p, i32, i64, i64, p, i32] + [p] * 9 + [i64, i64, p, i64, p])
+        i32, [p, i32, i64, i64, p, i32] + [p] * 9 + [i64, i64, p, i64, i32, p])
     _SIGNATURES[f"dn_history_project_vjp_{_sfx}"] = (
         i32, [p, p, i32, i64, i64, p, i32, i64] + [p] * 9 + [p])
     _SIGNATURES[f"dn_history_record_{_sfx}"] = (i32, [p, i64, i64, p, p, p])

@@ -59,17 +59,19 @@ constexpr int kProjThreads = 256;
 constexpr int kProjPer = 8;
 constexpr int kProjChunk = kProjThreads * kProjPer;
 
-// One connection's history reads packed for one step: its column, the rows
-// (slot - off_j) mod depth of its taps, the tap coefficients (two taps:
-// 1 - a and a, rounded as the reference rounds them; three taps: f_j), w,
-// and for a packed history its bit mask.  Read in connection order
-// (coalesced) and written at the connection's position in the target-major
-// CSR, so the target-major pass reads the records in order and gathers only
-// the taps' history words, instead of five scattered arrays.
+// One connection's history reads packed into one record: its column, the
+// tap offsets off_j (the rows (slot - off_j) mod depth follow from the
+// slot, so the records stay valid while the slot advances), the tap
+// coefficients (two taps: 1 - a and a, rounded as the reference rounds
+// them; three taps: f_j), w, and for a packed history its bit mask.  Read in
+// connection order (coalesced) and written at the connection's position in
+// the target-major CSR, so the target-major pass reads the records in order
+// and gathers only the taps' history words, instead of five scattered
+// arrays.
 template <typename T, int TAPS>
 struct alignas(16) ProjRec {
   int32_t col;
-  int32_t row[TAPS];
+  int32_t off[TAPS];
   T c[TAPS];
   T w;
 };
@@ -77,7 +79,7 @@ template <typename T, int TAPS>
 struct alignas(16) ProjRecMask {
   int64_t mask;
   int32_t col;
-  int32_t row[TAPS];
+  int32_t off[TAPS];
   T c[TAPS];
   T w;
 };
@@ -86,21 +88,19 @@ using ProjRecT = typename std::conditional<PACKED, ProjRecMask<T, TAPS>, ProjRec
 
 template <typename T, bool PACKED, int TAPS>
 __global__ __launch_bounds__(256) void history_pack_rec_kernel(
-    ProjRecT<T, PACKED, TAPS>* __restrict__ rec, int64_t n_conn, int64_t depth,
-    const int64_t* __restrict__ slot_p, const int32_t* __restrict__ col,
+    ProjRecT<T, PACKED, TAPS>* __restrict__ rec, int64_t n_conn, const int32_t* __restrict__ col,
     const int64_t* __restrict__ mask, const int32_t* __restrict__ off,
     const T* __restrict__ w, const T* __restrict__ frac, const int32_t* __restrict__ pos) {
 #pragma clang fp contract(off)
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_conn) return;
   const int32_t d = pos[e];
-  const int slot = hist_slot(slot_p, depth);
   ProjRecT<T, PACKED, TAPS> r;
   zero_record(r);
   if constexpr (PACKED) r.mask = mask[e];
   r.col = col[e];
 #pragma unroll
-  for (int j = 0; j < TAPS; ++j) r.row[j] = (int32_t)hist_row(slot, off[e * TAPS + j], depth);
+  for (int j = 0; j < TAPS; ++j) r.off[j] = off[e * TAPS + j];
   if (TAPS == 1) {
     r.c[0] = T(1);
   } else if (TAPS == 2) {
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kProjThreads) void history_project_kernel(
         if constexpr (PACKED) mu[u] = rv[u].mask;
 #pragma unroll
         for (int j = 0; j < TAPS; ++j) {
-          ru[u][j] = rv[u].row[j];
+          ru[u][j] = (int32_t)hist_row(slot, rv[u].off[j], depth);
           cf[u][j] = rv[u].c[j];
         }
       }
@@ -354,15 +354,16 @@ static void project_launch_taps(const void* hist, int64_t depth, int64_t n_col,
                                 const int32_t* in_edge, const int32_t* col, const int64_t* mask,
                                 const int32_t* off, const T* w, const T* frac, T* out,
                                 int64_t width, int64_t n_conn, void* work,
-                                const int32_t* pos, const void* ht, hipStream_t stream) {
+                                const int32_t* pos, const void* ht, bool reuse,
+                                hipStream_t stream) {
   const dim3 g((unsigned)((width + dn::kProjTargets - 1) / dn::kProjTargets)),
       b(dn::kProjThreads);
   using Rec = dn::ProjRecT<T, PACKED, TAPS>;
   Rec* rec = static_cast<Rec*>(work);
   if (rec) {
-    hipLaunchKernelGGL((dn::history_pack_rec_kernel<T, PACKED, TAPS>), grid_for(n_conn),
-                       dim3(256), 0, stream, rec, n_conn, depth, slot, col, mask, off, w, frac,
-                       pos);
+    if (!reuse)
+      hipLaunchKernelGGL((dn::history_pack_rec_kernel<T, PACKED, TAPS>), grid_for(n_conn),
+                         dim3(256), 0, stream, rec, n_conn, col, mask, off, w, frac, pos);
     // the column-major copy when there is one
     const void* src = ht ? ht : hist;
     const int64_t rs = ht ? 1 : n_col, cs = ht ? depth : 1;
@@ -381,17 +382,17 @@ static void project_launch(int taps, const void* hist, int64_t depth, int64_t n_
                            const int64_t* slot, const int32_t* in_ptr, const int32_t* in_edge,
                            const int32_t* col, const int64_t* mask, const int32_t* off,
                            const T* w, const T* frac, T* out, int64_t width, int64_t n_conn,
-                           void* work, const int32_t* pos, const void* ht,
+                           void* work, const int32_t* pos, const void* ht, bool reuse,
                            hipStream_t stream) {
   if (taps == 1)
     project_launch_taps<T, PACKED, 1>(hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                                      w, frac, out, width, n_conn, work, pos, ht, stream);
+                                      w, frac, out, width, n_conn, work, pos, ht, reuse, stream);
   else if (taps == 2)
     project_launch_taps<T, PACKED, 2>(hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                                      w, frac, out, width, n_conn, work, pos, ht, stream);
+                                      w, frac, out, width, n_conn, work, pos, ht, reuse, stream);
   else
     project_launch_taps<T, PACKED, 3>(hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                                      w, frac, out, width, n_conn, work, pos, ht, stream);
+                                      w, frac, out, width, n_conn, work, pos, ht, reuse, stream);
 }
 
 static int64_t project_rec_bytes(int taps, int packed, int f64) {
@@ -423,13 +424,16 @@ static bool project_copy_fits(int64_t depth, int64_t n_col) {
 // reads them in order.  in_edge must then be a permutation of [0, n_conn).
 // A floating history is first copied column-major into the workspace when it
 // has room for it, so that each connection's tap rows share a cache line.
+// reuse != 0: work already holds the records of an earlier call with the same
+// in_edge, pos, col, mask, off, w and frac (hist and slot may differ), and
+// they are read as they are.
 template <typename T>
 static int project_impl(const void* hist, int packed, int64_t depth, int64_t n_col,
                         const int64_t* slot, int taps, const int32_t* in_ptr,
                         const int32_t* in_edge, const int32_t* pos, const int32_t* col,
                         const int64_t* mask, const int32_t* off, const T* w, const T* frac,
                         T* out, int64_t width, int64_t n_conn, void* work, int64_t work_bytes,
-                        hipStream_t stream) {
+                        int reuse, hipStream_t stream) {
   if (!slot || !in_ptr || !out || depth < 1 || n_col < 0 || width < 0) return DN_EINVAL;
   if (taps < 1 || taps > 3 || width > (int64_t)INT32_MAX * 256) return DN_EINVAL;
   if (depth > INT32_MAX) return DN_EINVAL;
@@ -440,7 +444,7 @@ static int project_impl(const void* hist, int packed, int64_t depth, int64_t n_c
   if (!hist || !in_edge || !col || !off || !w || (taps > 1 && !frac) || (packed && !mask))
     return DN_EINVAL;
   void* rec = (work && n_conn > 0) ? work : nullptr;
-  if (rec && !pos) {
+  if (rec && !pos && !reuse) {
     int32_t* own = reinterpret_cast<int32_t*>(
         static_cast<char*>(work) + n_conn * project_rec_bytes(taps, packed, sizeof(T) == 8));
     hipLaunchKernelGGL(dn::inverse_permutation_kernel<>, grid_for(n_conn), dim3(256), 0,
@@ -460,10 +464,10 @@ static int project_impl(const void* hist, int packed, int64_t depth, int64_t n_c
   }
   if (packed)
     project_launch<T, true>(taps, hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                            w, frac, out, width, n_conn, rec, pos, ht, stream);
+                            w, frac, out, width, n_conn, rec, pos, ht, reuse != 0, stream);
   else
     project_launch<T, false>(taps, hist, depth, n_col, slot, in_ptr, in_edge, col, mask, off,
-                             w, frac, out, width, n_conn, rec, pos, ht, stream);
+                             w, frac, out, width, n_conn, rec, pos, ht, reuse != 0, stream);
   return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
 }
 
@@ -525,18 +529,18 @@ static int record_impl(T* hist, int64_t depth, int64_t n_col, const int64_t* slo
       const int64_t* mask, const int32_t* off, const T* w, const T* frac, T* out,            \
       int64_t width, hipStream_t stream) {                                                   \
     return project_impl<T>(hist, packed, depth, n_col, slot, taps, in_ptr, in_edge, nullptr, \
-                           col, mask, off, w, frac, out, width, -1, nullptr, 0, stream);     \
+                           col, mask, off, w, frac, out, width, -1, nullptr, 0, 0, stream);  \
   }                                                                                          \
   DN_EXPORT int dn_history_project_ws_##SFX(                                                 \
       const void* hist, int packed, int64_t depth, int64_t n_col, const int64_t* slot,       \
       int taps, const int32_t* in_ptr, const int32_t* in_edge, const int32_t* pos,           \
       const int32_t* col, const int64_t* mask, const int32_t* off, const T* w,               \
       const T* frac, T* out, int64_t width, int64_t n_conn, void* work, int64_t work_bytes,  \
-      hipStream_t stream) {                                                                  \
+      int reuse, hipStream_t stream) {                                                       \
     if (!work) return DN_EINVAL;                                                             \
     return project_impl<T>(hist, packed, depth, n_col, slot, taps, in_ptr, in_edge, pos,     \
                            col, mask, off, w, frac, out, width, n_conn, work, work_bytes,    \
-                           stream);                                                          \
+                           reuse, stream);                                                   \
   }                                                                                          \
   DN_EXPORT int dn_history_project_vjp_##SFX(                                                \
       const T* grad_out, const void* hist, int packed, int64_t depth, int64_t n_col,         \

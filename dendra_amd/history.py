@@ -149,9 +149,30 @@ def read_packed_history(history, rows, word_idx, bit_mask, *, dtype):
 
 # -- projection --------------------------------------------------------------
 
+# Per weights tensor: the projection's workspace and the tensors its packed
+# records were built from.  The records hold tap offsets, not rows, so they
+# stay valid while the slot advances and the history changes; the steps of a
+# simulation with fixed weights and delays then only read them.  A change of
+# any source (version, identity), of the sizes or of the stream rebuilds
+# them.  The entry keeps its sources alive, so identity checks are sound.
+_records = WeakIdKeyDictionary()
+
+
+def _record_workspace(owner, sources, key, nbytes, device):
+    stream = torch.cuda.current_stream(device).cuda_stream
+    stamp = (owner._version, key, int(nbytes), stream, tuple(t._version for t in sources))
+    hit = _records.get(owner)
+    if hit is not None and hit[0] == stamp and len(hit[1]) == len(sources) and \
+            all(a is b for a, b in zip(hit[1], sources)):
+        return hit[2], True
+    work = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+    _records[owner] = (stamp, tuple(sources), work)
+    return work, False
+
+
 class _Project(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, history, weights, frac, slot, plan, col, mask, off, taps, width):
+    def forward(ctx, history, weights, frac, slot, plan, col, mask, off, taps, width, cache):
         in_ptr, in_edge, post = plan
         packed = history.dtype == torch.int64
         depth, n_col = map(int, history.shape)
@@ -159,12 +180,14 @@ class _Project(torch.autograd.Function):
         p, lib, n = _lib.ptr, _lib.load(), weights.numel()
         f64 = int(weights.dtype == torch.float64)
         nbytes = lib.dn_history_work_bytes(n, taps, int(packed), f64, depth, n_col)
-        work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=weights.device)
-        fn = getattr(lib, f"dn_history_project_ws_{_suffix(weights.dtype)}")
         pos = csr_positions(in_edge) if n else in_edge
+        owner, sources, key = cache
+        work, reuse = _record_workspace(owner, sources + (in_edge, pos), key, nbytes,
+                                        weights.device)
+        fn = getattr(lib, f"dn_history_project_ws_{_suffix(weights.dtype)}")
         st = fn(p(history), int(packed), depth, n_col, p(slot), taps, p(in_ptr), p(in_edge),
                 p(pos), p(col), p(mask) if packed else None, p(off), p(weights),
-                p(frac) if taps > 1 else None, p(out), width, n, p(work), nbytes,
+                p(frac) if taps > 1 else None, p(out), width, n, p(work), nbytes, int(reuse),
                 _lib.stream_handle(weights.device))
         _lib.check(st, "source-history projection")
         ctx.save_for_backward(history, weights, frac, slot, post, col, mask, off)
@@ -189,7 +212,7 @@ class _Project(torch.autograd.Function):
                 p(frac) if taps > 1 else None, p(dw), p(dfrac) if taps > 1 else None,
                 p(dhist) if need_h else None, _lib.stream_handle(weights.device))
         _lib.check(st, "source-history projection cotangent")
-        return dhist, dw, dfrac, None, None, None, None, None, None, None
+        return dhist, dw, dfrac, None, None, None, None, None, None, None, None
 
 
 def source_history_project(history, slot, weights, *, post_idx, depth, width, source_pos=None,
@@ -249,9 +272,14 @@ def source_history_project(history, slot, weights, *, post_idx, depth, width, so
         col = _int32_index(source_pos, n_col, "source_pos")
         mask = col  # unused placeholder tensor
     plan = delay_plan(post_idx, width)
+    # the records' sources, by the caller's tensors where the operands are
+    # fresh views or conversions of them (the weights are the cache's key)
+    sources = tuple(t for t in (off, col, bit_mask if packed else None,
+                                delay_taps[1] if delay_taps is not None else None)
+                    if t is not None)
     out = _Project.apply(history.contiguous(), weights.reshape(n).contiguous(),
-                         frac.contiguous(), _slot_tensor(slot, weights.device), plan, col,
-                         mask, off, n_taps, width)
+                         frac.contiguous(), _slot_tensor(slot, weights.device), plan, col, mask,
+                         off, n_taps, width, (weights, sources, (n_taps, packed, n)))
     if _scratch is None:
         return out
     return _scratch.copy_(out)

@@ -269,7 +269,11 @@ int dn_interp_delay_vjp_f64(const double* G, int64_t depth, int64_t width,
  *     floating history is first copied column-major into the workspace
  *     (when work_bytes leaves room for it, as the size returned does), so
  *     a connection's adjacent tap rows share a cache line; results are
- *     identical bit for bit.
+ *     identical bit for bit.  The records hold tap offsets, not rows: with
+ *     reuse != 0 the call reads the records an earlier call left in work
+ *     (same in_edge, pos, col, mask, off, w, frac and sizes; hist and slot
+ *     may differ), as the steps of a simulation with fixed weights and
+ *     delays do; reuse = 0 packs them.
  *   dn_history_project_vjp_*: per connection dw, dfrac (taps 2, 3) and, for
  *     a floating history, dhist += (atomic; zero it first, may be null).
  *   dn_history_record_*: row slot mod depth := gates (in place).
@@ -294,14 +298,16 @@ int dn_history_project_ws_f32(const void* hist, int packed, int64_t depth, int64
                               const int64_t* mask,
                               const int32_t* off, const float* w, const float* frac,
                               float* out, int64_t width, int64_t n_conn, void* work,
-                              int64_t work_bytes, hipStream_t stream);
+                              int64_t work_bytes, int reuse,
+                              hipStream_t stream);
 int dn_history_project_ws_f64(const void* hist, int packed, int64_t depth, int64_t n_col,
                               const int64_t* slot, int taps, const int32_t* in_ptr,
                               const int32_t* in_edge, const int32_t* pos, const int32_t* col,
                               const int64_t* mask,
                               const int32_t* off, const double* w, const double* frac,
                               double* out, int64_t width, int64_t n_conn, void* work,
-                              int64_t work_bytes, hipStream_t stream);
+                              int64_t work_bytes, int reuse,
+                              hipStream_t stream);
 int dn_history_project_vjp_f32(const float* grad_out, const void* hist, int packed,
                                int64_t depth, int64_t n_col, const int64_t* slot, int taps,
                                int64_t n_conn, const int32_t* post, const int32_t* col,

@@ -83,6 +83,55 @@ def test_projection_is_bitwise_the_oracle(dtype, taps, packed, shape):
     assert np.array_equal(g.view(np.uint8), want.view(np.uint8))
 
 
+@pytest.mark.parametrize("taps", [1, 2, 3])
+@pytest.mark.parametrize("packed", [False, True])
+def test_projection_steps_reuse_records_until_a_source_changes(taps, packed):
+    """Steps with the same weights and delays reuse the packed records (the
+    slot advances, the history changes); an in-place weight update, new
+    delays or a new plan rebuild them.  Every step is bitwise the oracle."""
+    from dendra_amd import history as hmod
+    from dendra_amd.delays import delay_interpolation
+
+    rng = np.random.default_rng(90 + 2 * taps + packed)
+    n, width, dt = 15000, 120, 0.025
+    w, src, post, delays, history, meta = case(rng, np.float32, taps, packed, n=n, width=width)
+    depth = history.shape[0]
+    kw = {k: dev(v) for k, v in meta.items()}
+    col = meta["word_idx"] if packed else meta["source_pos"]
+    tw, tpost = dev(w), dev(post)
+
+    def delay_args(d):
+        if taps == 1:
+            steps = np.maximum(np.rint(d / dt), 1).astype(np.int64)
+            return {"delay_steps": dev(steps)}, steps[:, None], None
+        off_t, frac_t = delay_interpolation(dev(d), dt, dtype=torch.float32, taps=taps,
+                                            sigma=0.5)
+        off = off_t.cpu().numpy()
+        offsets = np.stack([off, off + 1], -1) if taps == 2 else off
+        return {"delay_taps": (off_t, frac_t), "taps": taps}, offsets, frac_t.cpu().numpy()
+
+    dargs, offsets, frac = delay_args(delays)
+    hits = []
+    for step, slot in enumerate((150, 151, 152, 153)):
+        if step == 2:  # an optimizer step on the weights, in place
+            tw.mul_(0.5)
+            w = w * np.float32(0.5)
+        if step == 3:  # new delays
+            delays = rng.uniform(0.05, 4.9, n)
+            dargs, offsets, frac = delay_args(delays)
+        h = history if step == 0 else (
+            rng.integers(-(2 ** 62), 2 ** 62, history.shape, dtype=np.int64) if packed
+            else rng.uniform(-1, 1, history.shape).astype(np.float32))
+        before = hmod._records.get(tw)
+        got = hmod.source_history_project(dev(h), dev([slot]), tw, post_idx=tpost, depth=depth,
+                                          width=width, **dargs, **kw)
+        hits.append(before is not None and hmod._records.get(tw)[2] is before[2])
+        want = history_ref.project(h, slot, w, post, width, col=col, offsets=offsets,
+                                   frac=frac, mask=meta.get("bit_mask"), packed=packed)
+        assert np.array_equal(got.cpu().numpy().view(np.uint8), want.view(np.uint8)), step
+    assert hits == [False, True, False, False]
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("taps", [1, 2, 3])
 @pytest.mark.parametrize("packed", [False, True])
@@ -131,13 +180,29 @@ def test_workspace_and_direct_entries_agree_with_the_oracle(dtype, taps, packed)
     small = lib.dn_history_work_bytes(n, taps, int(packed), int(dtype == np.float64), depth, 0)
     assert small == nbytes if packed else small < nbytes
     no_copy = torch.empty_like(direct)
-    assert fn(*ws(pos), p(packed_out), width, n, p(work), nbytes, stream) == 0
-    assert fn(*ws(None), p(own), width, n, p(work), nbytes, stream) == 0
-    assert fn(*ws(pos), p(no_copy), width, n, p(work), small, stream) == 0
-    assert fn(*ws(pos), p(packed_out), width, n, p(work), small - 1, stream) == -1  # DN_EINVAL
+    assert fn(*ws(pos), p(packed_out), width, n, p(work), nbytes, 0, stream) == 0
+    assert fn(*ws(None), p(own), width, n, p(work), nbytes, 0, stream) == 0
+    assert fn(*ws(pos), p(no_copy), width, n, p(work), small, 0, stream) == 0
+    assert fn(*ws(pos), p(packed_out), width, n, p(work), small - 1, 0, stream) == -1  # EINVAL
+    # reuse = 1: the records of the last call serve a later step (another
+    # slot, another history) with the same connections
+    rng2 = np.random.default_rng(7)
+    if packed:
+        history2 = rng2.integers(-(2 ** 62), 2 ** 62, history.shape, dtype=np.int64)
+    else:
+        history2 = rng2.uniform(0, 1, history.shape).astype(dtype)
+    slot2 = 57
+    want2 = history_ref.project(history2, slot2, w, post, width, col=col, offsets=off,
+                                frac=frac, mask=mask, packed=packed)
+    th2, ts2 = dev(history2), dev([slot2], torch.int64)
+    args2 = (p(th2),) + args[1:4] + (p(ts2),) + args[5:]
+    again = torch.empty_like(direct)
+    assert fn(*(args2[:8] + (pos,) + args2[8:]), p(again), width, n, p(work), nbytes, 1,
+              stream) == 0
     torch.cuda.synchronize()
     for got in (direct, packed_out, own, no_copy):
         assert np.array_equal(got.cpu().numpy().view(np.uint8), want.view(np.uint8))
+    assert np.array_equal(again.cpu().numpy().view(np.uint8), want2.view(np.uint8))
 
 
 @pytest.mark.parametrize("packed", [False, True])
