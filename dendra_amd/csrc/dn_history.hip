@@ -166,7 +166,8 @@ __global__ __launch_bounds__(kProjThreads) void history_project_kernel(
     const int32_t* __restrict__ in_edge, const int32_t* __restrict__ col,
     const int64_t* __restrict__ mask, const int32_t* __restrict__ off,
     const T* __restrict__ w, const T* __restrict__ frac,
-    const ProjRecT<T, PACKED, TAPS>* __restrict__ rec, T* __restrict__ out, int64_t width) {
+    const ProjRecT<T, PACKED, TAPS>* __restrict__ rec, T* __restrict__ out, int64_t width,
+    int64_t n_rec) {
 #pragma clang fp contract(off)
   __shared__ T vs[kProjChunk];
   const int tid = threadIdx.x;
@@ -189,7 +190,9 @@ __global__ __launch_bounds__(kProjThreads) void history_project_kernel(
     for (int u = 0; u < kProjPer; ++u) {
       const int k = tid + u * kProjThreads;
       e[u] = c0 + (k < n ? k : n - 1);
-      if (!REC) e[u] = in_edge[e[u]];
+      // records: positions past the n_rec records stay in bounds
+      if (REC) e[u] = e[u] < n_rec ? e[u] : n_rec - 1;
+      else e[u] = in_edge[e[u]];
     }
     int32_t cu[kProjPer], ru[kProjPer][TAPS];
     T cf[kProjPer][TAPS], wu[kProjPer];
@@ -369,11 +372,11 @@ static void project_launch_taps(const void* hist, int64_t depth, int64_t n_col,
     const int64_t rs = ht ? 1 : n_col, cs = ht ? depth : 1;
     hipLaunchKernelGGL((dn::history_project_kernel<T, PACKED, TAPS, true>), g, b, 0, stream,
                        src, depth, rs, cs, slot, in_ptr, in_edge, col, mask, off, w, frac, rec,
-                       out, width);
+                       out, width, n_conn);
   } else {
     hipLaunchKernelGGL((dn::history_project_kernel<T, PACKED, TAPS, false>), g, b, 0, stream,
                        hist, depth, n_col, (int64_t)1, slot, in_ptr, in_edge, col, mask, off, w,
-                       frac, rec, out, width);
+                       frac, rec, out, width, n_conn);
   }
 }
 

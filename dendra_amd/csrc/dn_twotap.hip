@@ -194,7 +194,7 @@ __global__ __launch_bounds__(kRingThreads) void interp_delay_tile_kernel(
     const T* qin, T* qout, int depth, int64_t width, const int64_t* __restrict__ slot_p,
     const int32_t* __restrict__ in_ptr, const int32_t* __restrict__ in_edge,
     const T* __restrict__ w, const T* __restrict__ g, const T* __restrict__ f,
-    const int32_t* __restrict__ base, const RingRec<T, TAPS>* __restrict__ rec) {
+    const int32_t* __restrict__ base, const RingRec<T, TAPS>* __restrict__ rec, int n_rec) {
 #pragma clang fp contract(off)
   extern __shared__ __align__(16) unsigned char dn_smem_raw[];
   RingStage<T, TAPS>& st = *reinterpret_cast<RingStage<T, TAPS>*>(dn_smem_raw);
@@ -242,7 +242,10 @@ __global__ __launch_bounds__(kRingThreads) void interp_delay_tile_kernel(
         for (int u = 0; u < kRingPer; ++u) {
           const int k = b0 + tid + u * kRingThreads;
           e[u] = c0 + (k < n ? k : n - 1);
-          if (!REC) e[u] = in_edge[e[u]];
+          // records: CSR positions past the n_rec records (a CSR that does
+          // not cover exactly the records' connections) stay in bounds
+          if (REC) e[u] = e[u] < n_rec ? e[u] : n_rec - 1;
+          else e[u] = in_edge[e[u]];
         }
         T val[kRingPer][TAPS];
         int32_t row[kRingPer];
@@ -361,19 +364,19 @@ template <typename T, int TAPS>
 static void ring_tiles_launch(const T* src, T* q, int64_t depth, int64_t width,
                               const int64_t* slot, const int32_t* in_ptr,
                               const int32_t* in_edge, const T* w, const T* g, const T* f,
-                              const int32_t* base, const void* rec, size_t lds,
-                              hipStream_t stream) {
+                              const int32_t* base, const void* rec, int64_t n_rec,
+                              size_t lds, hipStream_t stream) {
   const dim3 tiles((unsigned)((width + dn::kRingTileTargets - 1) / dn::kRingTileTargets));
   const dim3 block(dn::kRingThreads);
   const auto* r = static_cast<const dn::RingRec<T, TAPS>*>(rec);
   if (rec)
     hipLaunchKernelGGL((dn::interp_delay_tile_kernel<T, TAPS, true>), tiles, block, lds,
                        stream, src, q, (int)depth, width, slot, in_ptr, in_edge, w, g, f,
-                       base, r);
+                       base, r, (int)n_rec);
   else
     hipLaunchKernelGGL((dn::interp_delay_tile_kernel<T, TAPS, false>), tiles, block, lds,
                        stream, src, q, (int)depth, width, slot, in_ptr, in_edge, w, g, f,
-                       base, r);
+                       base, r, (int)n_rec);
 }
 
 template <typename T, int TAPS>
@@ -430,13 +433,13 @@ static int interp_enqueue_impl(const T* qin, T* q, int64_t depth, int64_t width,
     }
     if (taps == 1)
       ring_tiles_launch<T, 1>(src, q, depth, width, slot, in_ptr, in_edge, w, g, f, base, rec,
-                              lds, stream);
+                              n_edge, lds, stream);
     else if (taps == 2)
       ring_tiles_launch<T, 2>(src, q, depth, width, slot, in_ptr, in_edge, w, g, f, base, rec,
-                              lds, stream);
+                              n_edge, lds, stream);
     else
       ring_tiles_launch<T, 3>(src, q, depth, width, slot, in_ptr, in_edge, w, g, f, base, rec,
-                              lds, stream);
+                              n_edge, lds, stream);
     return hipGetLastError() == hipSuccess ? DN_OK : DN_ELAUNCH;
   }
   if (qin && qin != q &&
