@@ -49,10 +49,11 @@ __device__ __forceinline__ T hist_gate(const void* hist, int64_t n_col, int64_t 
 
 // A workgroup of 256 threads owns kProjTargets targets, whose connections
 // are one contiguous range of the target-major CSR.  The range is taken in
-// chunks of kProjChunk: all threads gather a chunk's connections and leave
-// each connection's value v in LDS (kProjPer connections per thread, every
-// load unconditional so that all are in flight), then one lane per target
-// adds its own values of the chunk in ascending connection order.  The sums
+// chunks of kProjChunk: all threads read a chunk's records (in CSR order,
+// coalesced) or gather its connections' inputs, gather their history words,
+// and leave each connection's value v in LDS (kProjPer connections per
+// thread, every load unconditional so that all are in flight), then one lane
+// per target adds its own values of the chunk in ascending connection order.  The sums
 // are those of the sequential lane-per-target walk, bit for bit.
 constexpr int kProjTargets = 32;
 constexpr int kProjThreads = 256;

@@ -79,12 +79,14 @@ __global__ __launch_bounds__(256) void interp_delay_enqueue_kernel(
 // columns for all depth rows.  The tile of the input ring is staged in LDS
 // (coalesced rows) with today's row cleared.  The tile's connections (one
 // contiguous range of the target-major CSR) are then taken in chunks of
-// kRingChunk: all 256 threads gather a chunk's w, g, f and base (rounds of
-// kRingPer independent loads per thread and array in flight) and leave each
-// connection's tap values and first row in LDS; then one lane per target
-// adds its own connections of the chunk, in ascending order, to its LDS
-// column.  Every ring cell therefore receives the same values in the same
-// order as the lane-per-target form above.  Two taps need all lower taps
+// kRingChunk: all 256 threads read a chunk's records (packed in CSR order,
+// so the reads are coalesced) or, without records, gather its w, g, f and
+// base (rounds of kRingPer independent loads per thread and array in
+// flight), and leave each connection's tap values and first row in LDS;
+// then each wavefront adds one target's items of the chunk to the target's
+// LDS column in item order (ring_apply_wave).  Every ring cell therefore
+// receives the same values in the same order as the lane-per-target form
+// above.  Two taps need all lower taps
 // before all upper taps: a tile whose connections fit in one chunk applies
 // both from one gather, a longer one makes one pass over its chunks per tap.
 // The tile is written back at the end, so the returned copy of the ring is
@@ -102,9 +104,10 @@ struct RingStage {
   int32_t row[kRingChunk];  // first tap's ring row
 };
 
-// One connection's tap values and first row, packed so that the tile
-// kernel's random gather per connection is one aligned record (8 or 16 B for
-// float32, 16 or 32 B for float64) instead of four scattered words.
+// One connection's tap values and first row as one aligned record (8 or 16 B
+// for float32, 16 or 32 B for float64), written at the connection's
+// position in the target-major CSR, so the tile kernel reads a chunk's
+// records in order instead of gathering four scattered words per connection.
 template <typename T, int TAPS>
 struct alignas(sizeof(T) * TAPS + 4 <= 8 ? 8 : (sizeof(T) * TAPS + 4 <= 16 ? 16 : 32))
     RingRec {
