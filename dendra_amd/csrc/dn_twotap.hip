@@ -92,11 +92,15 @@ __global__ __launch_bounds__(256) void interp_delay_enqueue_kernel(
 // The tile is written back at the end, so the returned copy of the ring is
 // fused, and qin may equal qout (each workgroup reads its own columns whole
 // before it writes them).
-constexpr int kRingTileTargets = 16;
+// 4 targets (one per wavefront) and 1024-connection chunks keep a
+// workgroup's LDS near 20 KiB for float32 at depth 201, so 8 workgroups
+// share a CU and the apply phases of 32 wavefronts overlap (16 targets and
+// 4096-connection chunks: 75 against 47 us for 4.2M connections, two taps).
+constexpr int kRingTileTargets = 4;
 constexpr int kRingThreads = 256;
 constexpr int kRingPer = 4;                             // loads in flight per thread
 constexpr int kRingBatch = kRingThreads * kRingPer;     // connections per gather round
-constexpr int kRingChunk = 4 * kRingBatch;              // connections staged in LDS
+constexpr int kRingChunk = kRingBatch;                  // connections staged in LDS
 
 template <typename T, int TAPS>
 struct RingStage {
@@ -358,9 +362,9 @@ __global__ __launch_bounds__(256) void interp_delay_vjp_kernel(
 
 }  // namespace dn
 
-// LDS of one ring-tile workgroup: the chunk stage (48-64 KiB for float32,
-// 80-112 KiB for float64) and depth rows of the tile's columns; deeper rings
-// take the global-memory form.
+// LDS of one ring-tile workgroup: the chunk stage (8-16 KiB for float32,
+// 12-28 KiB for float64), depth rows of the tile's columns and each
+// wavefront's depth claim words; deeper rings take the global-memory form.
 constexpr size_t kRingLdsMax = 160 * 1024;
 
 template <typename T, int TAPS>
