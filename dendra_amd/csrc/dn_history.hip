@@ -150,14 +150,23 @@ __global__ __launch_bounds__(256) void history_transpose_kernel(const T* __restr
   __shared__ T tile[64][65];
   const int64_t c0 = (int64_t)blockIdx.x * 64, r0 = (int64_t)blockIdx.y * 64;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int i = ty; i < 64; i += 4) {
-    const int64_t r = r0 + i, c = c0 + tx;
-    if (r < depth && c < n_col) tile[i][tx] = hist[r * n_col + c];
+  // loads clamped into the ring (unconditional, so all 16 are in flight;
+  // entries past its edge repeat its last row or column and are not stored)
+  const int64_t c = c0 + tx < n_col ? c0 + tx : n_col - 1;
+  T v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int64_t r = r0 + ty + 4 * q < depth ? r0 + ty + 4 * q : depth - 1;
+    v[q] = hist[r * n_col + c];
   }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) tile[ty + 4 * q][tx] = v[q];
   __syncthreads();
-  for (int i = ty; i < 64; i += 4) {
-    const int64_t c = c0 + i, r = r0 + tx;
-    if (r < depth && c < n_col) ht[c * depth + r] = tile[tx][i];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int i = ty + 4 * q;
+    const int64_t cc = c0 + i, r = r0 + tx;
+    if (r < depth && cc < n_col) ht[cc * depth + r] = tile[tx][i];
   }
 }
 
